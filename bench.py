@@ -1,0 +1,185 @@
+"""Throughput of the MI355X CRKSPH step on the 2D periodic shear layer.
+
+Workload: 1024 x 1024 = 1,048,576 particles, [0,1]^2 periodic, Wendland C2
+with n_H = 32, Owen adaptive h, Cullen-Dehnen switch, Heun RK2 with the
+compatible-energy finalize, FP32 storage.  One "step" is one full RK2 step
+(two derivative evaluations + finalize) of every particle.
+
+Prints one JSON line (see the keys below).  ``roofline`` is for the dominant
+pass, timed with HIP events on the launch stream; ``cpu_baseline`` times the
+C++/OpenMP CPU oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def pass_bytes(name: str, D: int):
+    """Algorithmic (compulsory) HBM bytes of one pass launch:
+    (bytes per particle, bytes per adjacency edge)."""
+    f = 4
+    xD = f * D
+    corr = f + 2 * xD + f * D * D              # A, B, gradA, gradB
+    row = 2 * f                                 # row offset + permutation entry
+    table = {
+        # x, h in; h sorted + user out
+        "owen": (xD + f + row + 2 * f, f),
+        # volume: x, h -> V; moments: x, h, V, m, u -> A,B,gA,gB, rho, P, c + 4 user fields
+        "factors_eos": ((xD + f + f + row) + (xD + 4 * f + row + corr + 3 * f + 4 * f), 2 * f),
+        # x, h, v, V, m, rho, c, A,B,gA,gB -> J, div, vsig, drhodt(user)
+        "velocity_gradient": (xD + f + xD + 4 * f + corr + row + f * D * D + 3 * f, f),
+        # x, h, m, rho, div, vsig, div_prev, alpha0 -> alpha, R, alphas(user)
+        "switch": (xD + 7 * f + row + 3 * f, f),
+        # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt -> dvdt (sorted+user), dudt, dEdt, div; edges: j, ap, av
+        "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + 2 * xD + 3 * f, f + 2 * xD),
+        # v, dvdt, m, rho, P -> ; edges: j, ap, av in, f out
+        "balance": (2 * xD + 3 * f + row, f + 2 * xD + f),
+    }
+    return table.get(name)
+
+
+def run_gpu(nx, steps, warmup, timer=True):
+    import torch
+    from warpsph_amd import sample
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    from warpsph_amd.timing import PassTimer
+    system, config, scheme = sample.shear_layer_2d(nx=nx, device="cuda:0")
+    dt = sample.cfl_dt(system.state, config, scheme) * (2.0 / 3.0)
+    integ = getIntegrator()
+    s = system
+    for _ in range(warmup):
+        s = integ.function(s, crkSPH_step, dt, config, scheme).state
+    torch.cuda.synchronize()
+    edges = []
+    pt = PassTimer()
+    t0 = time.perf_counter()
+    if timer:
+        pt.__enter__()
+    try:
+        for _ in range(steps):
+            r = integ.function(s, crkSPH_step, dt, config, scheme)
+            s = r.state
+            edges.append(sum(st.aux[0].n_edges for st in r.stages) / len(r.stages))
+    finally:
+        if timer:
+            pt.__exit__(None, None, None)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return s, (t1 - t0), pt.summary() if timer else {}, float(sum(edges) / max(len(edges), 1)), dt
+
+
+def cpu_baseline(nx, nsteps, warm):
+    """C++/OpenMP CPU oracle (FP32, Kahan) on an nx^2 sample; returns the
+    rate and the oracle's final density for the accuracy check."""
+    import numpy as np
+    from oracle import oracle as O
+    from warpsph_amd import sample
+    from warpsph_amd.configurations import toCrkConfig
+    system, config, scheme = sample.shear_layer_2d(nx=nx, device="cpu")
+    sim = O.OracleSim(toCrkConfig(config, scheme), system.state.positions.shape[0], 32)
+    for k, name in O.STATE_MAP.items():
+        sim.set(k, getattr(system.state, name).double().numpy())
+    dt = sim.cfl_dt() * (2.0 / 3.0)
+    sim.step(dt, warm)
+    t0 = time.perf_counter()
+    sim.step(dt, nsteps)
+    t1 = time.perf_counter()
+    n = system.state.positions.shape[0]
+    return n * nsteps / (t1 - t0), sim.get("rho"), dt, O.num_threads(), (t1 - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--cpu-nx", type=int, default=384)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs the CRKSPH step on one GPU only; "
+                                   "--gpus N with N > 1 is not supported (see DESIGN.md for the sharding design)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+
+    s, secs, summary, E_mean, dt = run_gpu(args.nx, args.steps, args.warmup)
+    n = args.nx * args.nx
+    value = n * args.steps / secs
+
+    # dominant pass by total time
+    dom = max(summary.items(), key=lambda kv: kv[1][1]) if summary else None
+    roofline = None
+    if dom is not None:
+        name, (calls, total_ms, mean_ms) = dom
+        pb = pass_bytes(name, 2)
+        if pb is not None:
+            bytes_launch = pb[0] * n + pb[1] * E_mean
+            achieved = bytes_launch / (mean_ms * 1e-3)
+            roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK, "traffic": None, "kernel": name,
+                        "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": mean_ms,
+                        "bytes_per_particle": pb[0], "bytes_per_edge": pb[1], "edges_per_launch": E_mean}
+
+    cpu = None
+    l1 = None
+    if not args.no_cpu_baseline:
+        rate, rho_orc, dt_c, threads, wall = cpu_baseline(args.cpu_nx, args.cpu_steps, 1)
+        cpu = {"value": rate, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"C++/OpenMP FP32 CPU oracle, {args.cpu_nx}x{args.cpu_nx} shear layer, "
+                         f"{args.cpu_steps} RK2 steps after 1 warm-up step ({wall:.1f} s)"}
+        # density L1 of the GPU against the oracle on the same sample and steps
+        from warpsph_amd import sample
+        from warpsph_amd.integrator import getIntegrator
+        from warpsph_amd.scheme import crkSPH_step
+        sysg, config, scheme = sample.shear_layer_2d(nx=args.cpu_nx, device="cuda:0")
+        g = sysg
+        for _ in range(1 + args.cpu_steps):
+            g = getIntegrator().function(g, crkSPH_step, dt_c, config, scheme).state
+        rho_g = g.state.densities.double().cpu().numpy()
+        l1 = float(np.mean(np.abs(rho_g - rho_orc)) / np.mean(np.abs(rho_orc)))
+
+    passes = {k: {"calls": v[0], "total_ms": round(v[1], 4), "mean_ms": round(v[2], 4)} for k, v in summary.items()}
+    line = {
+        "metric": "particle-updates/s (2D CRKSPH, 1M) + density L1 vs ref; learned vs plain wall@err",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"2D periodic shear layer, {args.nx}x{args.nx} = {n} particles, Wendland C2 "
+                               f"n_H=32, Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy",
+                   "particles": n, "edges_per_stage": E_mean, "dt": dt},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "density_l1_vs_oracle": None if l1 is None else {"value": l1, "sample": f"{args.cpu_nx}^2, {1 + args.cpu_steps} steps"},
+        "passes": passes,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,313 @@
+/*
+ * crksph.h -- C-ABI of the MI355X-native CRKSPH compressible step.
+ *
+ * The library (warpsph_amd/lib/libcrksph_hip.so, gfx950 HIP) replaces the
+ * device side of warpSPH's CRKSPH step: everything below
+ *   crkSPH_step(system, dt, config, compParams)      src/warpSPH/schemes/crkSPH.py:19-366
+ * including the warpSPHCore pieces it calls (buildVerletList, computeCRKFactors,
+ * warpOperation, the SPH kernel functions) that are not part of the reference
+ * repository.
+ *
+ * Conventions
+ *  - All particle arrays are device pointers, FP32, row-major: scalars [n],
+ *    vectors [n*dim], dim x dim tensors [n*dim*dim] (row = component).
+ *  - "user order" is the caller's particle order (the order of CompSPHState).
+ *    The library works internally in Morton-sorted order ("sorted order");
+ *    an adjacency carries the permutation between the two.
+ *  - Edge arrays (ap_ij, av_ij, f_ij) are stored in the adjacency's row order:
+ *    row r (sorted particle r, user particle perm[r]) owns edges
+ *    [row_offsets[r], row_offsets[r+1]).
+ *  - Every call is asynchronous on the given stream unless documented as
+ *    blocking.  Calls return CRK_OK (0) or a negative status; the message of
+ *    the last failure on the calling thread is available from crk_last_error.
+ *  - The caller owns every buffer passed in.  The library owns only scratch
+ *    memory inside a CrkContext.  A context is not thread-safe; use one per
+ *    host thread / stream.
+ */
+#ifndef CRKSPH_H
+#define CRKSPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  CRK_OK = 0,
+  CRK_EINVAL = -1,     /* invalid argument (shape, null pointer, parameter)   */
+  CRK_EHIP = -2,       /* HIP runtime error                                   */
+  CRK_ECAPACITY = -3,  /* caller buffer too small (edge capacity)             */
+  CRK_ESTATE = -4      /* call order violated (e.g. pass before gather)       */
+};
+
+/* --------------------------------------------------------------- enums
+ * Values mirror the reference enums where one exists.                      */
+enum { /* KernelFunctions subset (warpSPHCore; values are this library's) */
+  CRK_KERNEL_WENDLAND2 = 0,
+  CRK_KERNEL_WENDLAND4 = 1,
+  CRK_KERNEL_WENDLAND6 = 2
+};
+enum { /* EnergyScheme, src/warpSPH/enumTypes.py:5-11 */
+  CRK_ENERGY_EQUALWORK = 0, CRK_ENERGY_PDV = 1, CRK_ENERGY_DIMINISHING = 2,
+  CRK_ENERGY_MONOTONIC = 3, CRK_ENERGY_HYBRID = 4, CRK_ENERGY_CRK = 5
+};
+enum { /* AdaptiveSupportScheme, src/warpSPH/enumTypes.py:14-17 */
+  CRK_SUPPORT_NONE = 0, CRK_SUPPORT_MONAGHAN = 1, CRK_SUPPORT_OWEN = 2
+};
+enum { /* ViscositySwitch, src/warpSPH/enumTypes.py:20-27 */
+  CRK_SWITCH_CULLEN_DEHNEN = 2, CRK_SWITCH_NONE = 6,
+  CRK_SWITCH_LEARNED = 100 /* no reference counterpart: per-particle MLP */
+};
+
+/* ------------------------------------------------------ parameter blocks */
+#pragma pack(push, 4)
+/* DomainDescription, src/warpSPH/utils/domain.py:7-17 */
+typedef struct CrkDomain {
+  int32_t dim;            /* 1, 2 or 3 */
+  int32_t periodic[3];
+  float   min[3];
+  float   max[3];
+} CrkDomain;
+
+/* DiffusionParameters, src/warpSPH/configurations/moduleConfigurations/diffusionParameters.py:23-36 */
+typedef struct CrkDiffusionParams {
+  float   c_s, C_l, C_q, Cu_l, Cu_q, K, thermalConductivity;
+  int32_t viscosityTerm, thermalConductivityTerm;
+  int32_t scaleBeta, monaghanSwitch, correctXi;
+} CrkDiffusionParams;
+
+/* CRKViscosity, src/warpSPH/configurations/crkSPH.py:17-25 */
+typedef struct CrkViscosityParams {
+  float   eta_fold, eta_crit;
+  int32_t enableCRKLimiter, enableVanLeerLimiter;
+  int32_t forceVanLeerOff, forceVanLeerOn;
+} CrkViscosityParams;
+
+/* ViscositySwitchConfig, .../moduleConfigurations/viscositySwitchParameters.py:12-24 */
+typedef struct CrkSwitchParams {
+  int32_t scheme;         /* CRK_SWITCH_* */
+  int32_t limitXi;
+  int32_t correctVelocityGradient; /* must be 0 (M-matrix correction not built) */
+  float   alpha_min, alpha_max, beta_c, beta_d, beta_xi;
+} CrkSwitchParams;
+
+/* Everything one derivative evaluation needs.  Mirrors SimulationConfig
+ * (src/warpSPH/configurations/simulationConfig.py:16-44) and CRKSPHConfig
+ * (src/warpSPH/configurations/crkSPH.py:71-78 + compressibleConfig.py:23-40). */
+typedef struct CrkConfig {
+  CrkDomain          domain;
+  int32_t            kernel;            /* CRK_KERNEL_* */
+  float              verletScale;       /* SimulationConfig.verletScale */
+  float              targetNeighbors;   /* SimulationConfig.targetNeighbors (n_H) */
+  float              cflFactor, minDt, maxDt, dtGrowthFactor;
+  float              gamma;             /* CompressibleSPHConfig.gamma */
+  int32_t            energyScheme;      /* CRK_ENERGY_* (only CRK is built) */
+  int32_t            adaptiveSupportScheme;   /* CRK_SUPPORT_NONE / _OWEN */
+  int32_t            adaptiveSupportIterations;
+  float              adaptiveSupportThreshold;
+  int32_t            compatibleEnergy;
+  CrkDiffusionParams diffusion;
+  CrkViscosityParams crkViscosity;
+  CrkSwitchParams    switchParams;
+  /* Extensions with no reference counterpart (0 = reference behaviour):
+   * alphaWeightedQ: scale Q_i, Q_j by (alpha_i + alpha_j)/2 in accel/dudt.
+   * evolveAlpha0:   store the updated alpha0 back into the state.          */
+  int32_t            alphaWeightedQ;
+  int32_t            evolveAlpha0;
+} CrkConfig;
+
+/* Uniform cell grid over which keys are Morton-interleaved. */
+typedef struct CrkGrid {
+  int32_t dim;
+  int32_t bits;           /* bits per dimension of the Morton code         */
+  int32_t ncell[3];       /* cells per dimension                            */
+  float   origin[3];
+  float   inv_cell[3];    /* 1 / cell width, as used by the key function   */
+  int64_t table_size;     /* 2^(dim*bits): entries of cell_start/cell_end  */
+} CrkGrid;
+#pragma pack(pop)
+
+/* ------------------------------------------------------------ context */
+typedef struct CrkContext CrkContext;
+
+int  crk_ctx_create(CrkContext** ctx);
+int  crk_ctx_destroy(CrkContext* ctx);
+/* Copies the last error message of this thread into buf (always terminated). */
+int  crk_last_error(char* buf, size_t n);
+/* Library version string, e.g. "crksph-hip 1 gfx950". */
+const char* crk_version(void);
+
+/* ------------------------------------------------------ host utilities */
+/* Kernel normalisation C_d and kernel scale H/h (Dehnen & Aly 2012). */
+int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kernel_scale);
+
+/* Owen psi_H look-up table on n_h in [n_min, n_max] (nlut entries).
+ * Restates computePsi, src/warpSPH/modules/adaptiveSupport/wp_psi.py:8-99
+ * (only the psiH column is used by the step). Host function, blocking.    */
+int crk_owen_lut(int32_t kernel, int32_t dim, int32_t nlut, double n_min, double n_max,
+                 double* n_h_out, double* psiH_out);
+/* Upload an Owen table into the context (device copy). */
+int crk_set_owen_lut(CrkContext* ctx, int32_t nlut, const double* n_h, const double* psiH,
+                     double nh_target, void* stream);
+
+/* Grid for a neighbour search of radius verletScale * h_max.  For non-periodic
+ * dimensions lo/hi are the particle bounding box (ignored when periodic).
+ * Host function.                                                            */
+int crk_grid_plan(const CrkConfig* cfg, float h_max, const float* lo, const float* hi,
+                  CrkGrid* grid);
+
+/* ------------------------------------------------- (a) adjacency build
+ * Replaces warpSPHCore.buildVerletList (called at crkSPH.py:54-58 and
+ * optimalSupportOwen.py:74): Morton keys -> stable LSD radix sort -> cell
+ * start/end table -> CSR rows of every j with |x_ij| < verletScale*max(h_i,h_j)
+ * (minimum image).  Rows are stored in sorted order; inside a row the
+ * neighbours follow the 3^dim cell scan order, then sorted order.
+ *
+ * Step 1 (blocking: returns the edge count):
+ *   keys[n] (sorted), perm[n] (sorted->user), rank[n] (user->sorted),
+ *   cell_start/cell_end[grid.table_size], row_offsets[n+1], xb[n*dim], hb[n]
+ *   (positions and supports at build time, sorted order).                  */
+int crk_adjacency_count(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid,
+                        int32_t n, const float* x, const float* h,
+                        uint32_t* keys, int32_t* perm, int32_t* rank,
+                        int32_t* cell_start, int32_t* cell_end,
+                        int32_t* row_offsets, float* xb, float* hb,
+                        int64_t* n_edges, void* stream);
+/* Step 2: fills nbr[n_edges] (sorted neighbour indices). */
+int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid,
+                       int32_t n, const int32_t* cell_start, const int32_t* cell_end,
+                       const int32_t* row_offsets, const float* xb, const float* hb,
+                       int32_t* nbr, int64_t capacity, void* stream);
+/* Blocking validity test of an adjacency built at (xb, hb) for the current
+ * (x, h): valid iff max h/hb < s and 2*max|x - xb| <= (s - max h/hb)*min hb. */
+int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n,
+                        const int32_t* perm, const float* xb, const float* hb,
+                        const float* x, const float* h, int32_t* valid, void* stream);
+/* Expand to the reference AdjacencyList layout (user indices, int64):
+ * i[E], j[E] per edge, numNeighbors[n] and neighborOffsets[n] per user particle. */
+int crk_adjacency_export(CrkContext* ctx, int32_t n, const int32_t* perm, const int32_t* row_offsets,
+                         const int32_t* nbr, int64_t n_edges, int64_t* i_out, int64_t* j_out,
+                         int64_t* num_neighbors, int64_t* neighbor_offsets, void* stream);
+
+/* ------------------------------------------------------ per-stage passes
+ * A stage binds the context to one adjacency and one user-order state; the
+ * passes below then run in sorted order on context-owned copies.            */
+typedef struct CrkAdjacency {
+  int32_t        n;
+  int64_t        n_edges;
+  const int32_t* perm;
+  const int32_t* rank;
+  const int32_t* row_offsets;
+  const int32_t* nbr;
+} CrkAdjacency;
+
+typedef struct CrkState {          /* user order, caller owned */
+  int32_t n;
+  float* positions;   /* [n*dim] */
+  float* velocities;  /* [n*dim] */
+  float* supports;    /* [n] */
+  float* masses;
+  float* densities;
+  float* internalEnergies;
+  float* entropies;
+  float* pressures;
+  float* soundspeeds;
+  float* divergence;
+  float* alpha0s;
+  float* alphas;
+} CrkState;
+
+typedef struct CrkUpdate {         /* user order, caller owned: CompressibleSystemUpdate */
+  float* dvdt;    /* [n*dim] */
+  float* dudt;    /* [n] */
+  float* drhodt;  /* [n] */
+  float* dEdt;    /* [n] */
+} CrkUpdate;
+
+typedef struct CrkEdges {          /* adjacency row order, caller owned */
+  float*  ap;     /* [E*dim] pairwise pressure acceleration */
+  float*  av;     /* [E*dim] pairwise viscous acceleration  */
+  float*  f;      /* [E]     energy partition               */
+} CrkEdges;
+
+/* Gathers the state into sorted order (x, v, h, m, u, alpha0, divergence). */
+int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj,
+                    const CrkState* st, void* stream);
+/* Re-bind to a new adjacency mid-stage (after a rebuild), keeping h. */
+int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj,
+                     const CrkState* st, void* stream);
+/* Owen adaptive support, one iteration (optimalSupportOwen.py:70-114):
+ * psi0 pass over the bound adjacency, LUT inversion, computeNewSupport,
+ * clamp to [hMin/4, 4 hMax].  hmin/hmax are the bounds of the supports at
+ * the start of the Owen loop (device scalars, see crk_minmax_h).  Writes the
+ * new supports (sorted copy and st->supports) and, when max_ratio_dev is not
+ * NULL, max |h_new/h - 1| to it (device float).                            */
+int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                       const float* hminmax_dev, float* max_ratio_dev, void* stream);
+/* hminmax_dev[0] = min supports, [1] = max supports (user array). */
+int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_dev, void* stream);
+/* computeCRKFactors (warpSPHCore; called at crkSPH.py:61): apparent volume,
+ * moments with FP64 solve -> A, B, gradA, gradB, CRK density (written to
+ * st->densities), followed by idealGasEOS (eos/idealGas.py:6-38) writing
+ * st->entropies, st->pressures, st->soundspeeds.  n_singular_dev (device
+ * int, may be NULL) receives the number of particles that used the
+ * moment-matrix fallback.                                                   */
+int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                    int32_t* n_singular_dev, void* stream);
+/* Velocity gradient (crkSPH.py:158-172): J_i = sum_j V_j (v_j - v_i) (x) gradW^R_ij,
+ * drhodt = -tr(J) rho.  Writes update->drhodt. */
+int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, void* stream);
+/* Cullen-Dehnen switch (CullenDehnen2010.py:218-331 + wp_vsig.py:11-86):
+ * writes st->alphas (and st->alpha0s when cfg->evolveAlpha0). */
+int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt, void* stream);
+/* computeCrkSPHAccelWarp + computeCrkSPHdudtWarp fused (crk/accel.py:15-250,
+ * crk/dudt.py:15-229).  Writes update->dvdt, update->dudt, update->dEdt,
+ * edges->ap, edges->av, and st->divergence = -drhodt/rho (crkSPH.py:324). */
+int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                   CrkUpdate* up, CrkEdges* edges, void* stream);
+/* computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154)
+ * with v_half = v + dt/2 dvdt (crkSPH.py:341).  Writes edges->f. */
+int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* edges, void* stream);
+/* Copy a sorted-order context field back to user order (testing / export).
+ * field: 0 V, 1 A, 2 B, 3 gradA, 4 gradB, 5 J(velocity gradient), 6 div(CD),
+ *        7 vsig, 8 R. */
+int crk_export_field(CrkContext* ctx, int32_t field, float* out_user, void* stream);
+
+/* ------------------------------------------------ integrator helpers
+ * Heun RK2 (b = 1/2, 1/2).  All user order.                                 */
+/* y = y0 + dt * d, elementwise over n*components (stage update). */
+int crk_axpy(int64_t count, const float* y0, const float* d, float dt, float* y, void* stream);
+/* y = y0 + dt * (b0*d0 + b1*d1) */
+int crk_rk_combine(int64_t count, const float* y0, const float* d0, const float* d1,
+                   float b0, float b1, float dt, float* y, void* stream);
+/* Wrap positions into the periodic domain. */
+int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream);
+/* Compatible-energy finalize (compSPH/multistep.py:52-94, systems/compSPH.py:87-129):
+ * v_half = v0 + dt/2 (b0 dvdt0 + b1 dvdt1);
+ * du_i = sum_k b_k sum_{e in row i of stage k} f_e (v_half_j - v_half_i).(ap_e + av_e);
+ * u = u0 + dt du; dudt_last = (u - u0)/dt.  Deterministic row sums. */
+int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, float dt,
+                          const float* v0, const float* u0,
+                          const float* dvdt0, const float* dvdt1, float b0, float b1,
+                          const CrkAdjacency* adj0, const CrkEdges* e0,
+                          const CrkAdjacency* adj1, const CrkEdges* e1,
+                          float* u_out, float* dudt_last, void* stream);
+
+/* ------------------------------------------------ (c) reductions
+ * computeTimestep (timestep/compressible.py:12-66): blocking, returns dt.
+ * prev_dt <= 0 means no growth limit. */
+int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* supports,
+               const float* internalEnergies, double prev_dt, double* dt_out, void* stream);
+/* FP64 conservation sums: out[0] = sum m, out[1..dim] = sum m v,
+ * out[dim+1] = sum m u, out[dim+2] = sum m |v|^2 / 2.  Blocking. */
+int crk_conservation(CrkContext* ctx, int32_t dim, int32_t n, const float* masses,
+                     const float* velocities, const float* internalEnergies,
+                     double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRKSPH_H */

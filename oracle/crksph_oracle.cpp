@@ -1,0 +1,1277 @@
+// crksph_oracle.cpp -- CPU oracle for the CRKSPH compressible step.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load this library, and only as the checker
+// or as the CPU baseline.  The product (warpsph_amd) never calls it.
+//
+// It restates, in plain C++ loops over particles in the caller's order, the
+// algorithm of warpSPH's crkSPH_step (src/warpSPH/schemes/crkSPH.py:19-366)
+// together with the pieces of the un-vendored warpSPHCore / warpSPHIntegrators
+// packages that the step calls.  Each function cites the reference lines it
+// follows.  Where the reference algorithm lives in an absent package the
+// restatement follows the literature it implements and is pinned by the
+// invariants in tests/test_oracle_*.py:
+//   - Wendland kernels: Wendland (1995), normalisation and H/h from
+//     Dehnen & Aly (2012) Table 1                      -> pinned (quadrature, FD)
+//   - CRK volume, moments, A, B, gradA, gradB, density: Frontiere, Raskin &
+//     Owen (2017), linear CRK                           -> pinned (reproducing
+//     conditions, finite differences of A and B)
+//   - neighbour search (Morton keys, stable LSD sort, cell table, CSR):
+//     closed-form keys, stable sort, brute-force neighbour sets -> pinned
+//   - RK2 tableau (Heun, b = 1/2, 1/2): parity unpinned (warpSPHIntegrators
+//     absent); total-energy and momentum conservation of the step pin the
+//     compatible-energy algebra around it.
+//   - support-mode choices (gather/scatter/super-symmetric per pass) and the
+//     CRK density form rho_i = sum_j m_j W^R_ij: parity unpinned.
+//
+// Precision: Real = float reproduces the GPU arithmetic (FP32 storage, Kahan
+// compensated FP32 pair sums, FP64 moment solve, FP64 LUT interpolation);
+// Real = double is the FP64-everywhere reference run.
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
+
+#include "../include/crksph.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ------------------------------------------------------------ kernels
+// W(r, H) = C_d / H^d * k(r/H), 0 <= q < 1 (H is the cut-off radius, the
+// reference's "support", limiter.py:107-112).
+double kernelC(int kernel, int dim) {
+  static const double C[3][3] = {
+      {1.25, 2.228169203286535, 3.3422538049298027},   // Wendland C2: 5/4, 7/pi, 21/(2pi)
+      {1.5, 2.8647889756541161, 4.9238560519055126},   // Wendland C4: 3/2, 9/pi, 495/(32pi)
+      {1.71875, 3.5468815889050962, 6.7889530412636603} // Wendland C6: 55/32, 78/(7pi), 1365/(64pi)
+  };
+  return C[kernel][dim - 1];
+}
+// H/h with h = twice the kernel standard deviation (Dehnen & Aly 2012).
+double kernelScale(int kernel, int dim) {
+  static const double S[3][3] = {
+      {1.6201851746019651, 1.8973665961010275, 1.9364916731037083},
+      {1.9364916731037087, 2.1712405933672376, 2.2079402165819615},
+      {2.2079402165819615, 2.4152294576982398, 2.4494897427831779}};
+  return S[kernel][dim - 1];
+}
+
+template <class R>
+R kval(int kernel, int dim, R q) {
+  R t = R(1) - q;
+  if (kernel == CRK_KERNEL_WENDLAND2) {
+    if (dim == 1) return t * t * t * (R(1) + R(3) * q);
+    R t2 = t * t;
+    return t2 * t2 * (R(1) + R(4) * q);
+  } else if (kernel == CRK_KERNEL_WENDLAND4) {
+    if (dim == 1) { R t2 = t * t; return t2 * t2 * t * (R(1) + R(5) * q + R(8) * q * q); }
+    R t2 = t * t; R t3 = t2 * t;
+    return t3 * t3 * (R(1) + R(6) * q + R(35.0 / 3.0) * q * q);
+  } else {
+    if (dim == 1) { R t2 = t * t; R t3 = t2 * t; return t3 * t3 * t * (R(1) + R(7) * q + R(19) * q * q + R(21) * q * q * q); }
+    R t2 = t * t; R t4 = t2 * t2;
+    return t4 * t4 * (R(1) + R(8) * q + R(25) * q * q + R(32) * q * q * q);
+  }
+}
+template <class R>
+R kder(int kernel, int dim, R q) {  // dk/dq
+  R t = R(1) - q;
+  if (kernel == CRK_KERNEL_WENDLAND2) {
+    if (dim == 1) return R(-12) * q * t * t;
+    return R(-20) * q * t * t * t;
+  } else if (kernel == CRK_KERNEL_WENDLAND4) {
+    if (dim == 1) { R t2 = t * t; return R(-14) * q * t2 * t2 * (R(1) + R(4) * q); }
+    R t2 = t * t;
+    return R(-56.0 / 3.0) * q * t2 * t2 * t * (R(1) + R(5) * q);
+  } else {
+    if (dim == 1) { R t2 = t * t; R t3 = t2 * t; return R(-6) * q * t3 * t3 * (R(3) + R(18) * q + R(35) * q * q); }
+    R t2 = t * t; R t3 = t2 * t;
+    return R(-22) * q * t3 * t3 * t * (R(1) + R(7) * q + R(16) * q * q);
+  }
+}
+
+template <class R, int D>
+struct Kern {
+  int kernel;
+  R C;
+  R hpow(R h, int p) const { R r = h; for (int k = 1; k < p; ++k) r = r * h; return r; }
+  R W(R r, R h) const {
+    R q = r / h;
+    if (!(q < R(1))) return R(0);
+    return C * kval<R>(kernel, D, q) / hpow(h, D);
+  }
+  R dWdr(R r, R h) const {
+    R q = r / h;
+    if (!(q < R(1))) return R(0);
+    return C * kder<R>(kernel, D, q) / hpow(h, D + 1);
+  }
+  // gradient with respect to x_a of W(|x_ab|, h), x_ab = x_a - x_b
+  void gradW(const R* x, R r, R h, R* g) const {
+    R fac = R(0);
+    if (r > R(0)) fac = dWdr(r, h) / r;
+    for (int d = 0; d < D; ++d) g[d] = fac * x[d];
+  }
+};
+
+// Compensated (Kahan) sum in FP32, plain sum in FP64.
+template <class R> struct Sum {
+  R s = R(0), c = R(0);
+  void add(R y) { R yy = y - c; R t = s + yy; c = (t - s) - yy; s = t; }
+  R get() const { return s; }
+};
+template <> struct Sum<double> {
+  double s = 0.0;
+  void add(double y) { s += y; }
+  double get() const { return s; }
+};
+
+// ------------------------------------------------------------ geometry
+struct Geo {
+  int D;
+  int per[3];
+  float L[3], half[3];
+};
+Geo makeGeo(const CrkConfig& c) {
+  Geo g;
+  g.D = c.domain.dim;
+  for (int d = 0; d < 3; ++d) {
+    g.per[d] = d < g.D ? c.domain.periodic[d] : 0;
+    g.L[d] = c.domain.max[d] - c.domain.min[d];
+    g.half[d] = 0.5f * g.L[d];
+  }
+  return g;
+}
+// minimum image x_i - x_j (computeDistanceVec, warpSPHCore; assumed x_i - x_j)
+template <class R>
+void minImage(const Geo& g, const R* xi, const R* xj, R* out) {
+  for (int d = 0; d < g.D; ++d) {
+    R dx = xi[d] - xj[d];
+    if (g.per[d]) {
+      R L = R(g.L[d]), h = R(g.half[d]);
+      if (dx > h) dx = dx - L;
+      else if (dx < -h) dx = dx + L;
+    }
+    out[d] = dx;
+  }
+}
+template <class R>
+R norm2(const R* x, int D) { R r = x[0] * x[0]; for (int d = 1; d < D; ++d) r = r + x[d] * x[d]; return r; }
+
+// ------------------------------------------------------------ grid / Morton
+uint32_t part1by1(uint32_t x) {
+  x &= 0x0000ffffu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+uint32_t part1by2(uint32_t x) {
+  x &= 0x000003ffu;
+  x = (x | (x << 16)) & 0xFF0000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+uint32_t morton(int D, const int* c) {
+  if (D == 1) return (uint32_t)c[0];
+  if (D == 2) return part1by1((uint32_t)c[0]) | (part1by1((uint32_t)c[1]) << 1);
+  return part1by2((uint32_t)c[0]) | (part1by2((uint32_t)c[1]) << 1) | (part1by2((uint32_t)c[2]) << 2);
+}
+int cellCoord(float x, const CrkGrid& g, int d, int periodic) {
+  float t = (x - g.origin[d]) * g.inv_cell[d];
+  int c = (int)std::floor(t);
+  int n = g.ncell[d];
+  if (periodic) { c %= n; if (c < 0) c += n; }
+  else { c = c < 0 ? 0 : (c >= n ? n - 1 : c); }
+  return c;
+}
+
+// Grid plan (same rule as crk_grid_plan in the product library).
+int planGrid(const CrkConfig& cfg, float hmax, const float* lo, const float* hi, CrkGrid& g) {
+  int D = cfg.domain.dim;
+  if (D < 1 || D > 3 || !(hmax > 0.f)) return CRK_EINVAL;
+  std::memset(&g, 0, sizeof(g));
+  g.dim = D;
+  double R = (double)cfg.verletScale * (double)hmax;
+  int maxn = 1;
+  int cap = D == 1 ? (1 << 24) : (D == 2 ? 4096 : 256);
+  for (int d = 0; d < D; ++d) {
+    double ext, o;
+    if (cfg.domain.periodic[d]) {
+      ext = (double)cfg.domain.max[d] - (double)cfg.domain.min[d];
+      o = cfg.domain.min[d];
+      if (R > 0.5 * ext) return CRK_EINVAL;
+    } else {
+      ext = (double)hi[d] - (double)lo[d];
+      o = lo[d];
+    }
+    double Reff = R * (1.0 + 1e-4) + 1e-6 * ext;
+    long n = ext > 0 ? (long)std::floor(ext / Reff) : 1;
+    if (n < 1) n = 1;
+    if (n > cap) n = cap;
+    g.ncell[d] = (int)n;
+    g.origin[d] = (float)o;
+    g.inv_cell[d] = ext > 0 ? (float)((double)n / ext) : 0.f;
+    maxn = std::max(maxn, (int)n);
+  }
+  int bits = 0;
+  while ((1 << bits) < maxn) ++bits;
+  g.bits = bits;
+  g.table_size = (int64_t)1 << (D * bits);
+  for (int d = D; d < 3; ++d) { g.ncell[d] = 1; g.origin[d] = 0.f; g.inv_cell[d] = 0.f; }
+  return CRK_OK;
+}
+
+void cellOffsets(int n, int periodic, int* offs, int& cnt) {
+  if (periodic) {
+    if (n >= 3) { offs[0] = -1; offs[1] = 0; offs[2] = 1; cnt = 3; }
+    else if (n == 2) { offs[0] = 0; offs[1] = 1; cnt = 2; }
+    else { offs[0] = 0; cnt = 1; }
+  } else { offs[0] = -1; offs[1] = 0; offs[2] = 1; cnt = 3; }
+}
+
+// ------------------------------------------------------------ Owen LUT
+// computePsi, src/warpSPH/modules/adaptiveSupport/wp_psi.py:8-99 (psiH column:
+// radial sum at lines 65-81), evaluated in FP64.
+void owenLUT(int kernel, int D, int nlut, double nmin, double nmax, double* nh, double* psiH) {
+  double C = kernelC(kernel, D);
+  for (int i = 0; i < nlut; ++i) {
+    double dn = (nmax - nmin) / ((double)nlut - 1.0);
+    double n_h = nmin + (double)i * dn;
+    double deta = 1.0 / n_h;
+    double etar = deta, result = 0.0;
+    while (etar < 1.0) {
+      double corr = D == 1 ? 2.0 : (D == 2 ? 2.0 * M_PI * etar / deta : 4.0 * M_PI * (etar * etar) / (deta * deta));
+      double val = std::fabs(kder<double>(kernel, D, etar) * C / 1.0);
+      result += val * corr;
+      etar += deta;
+    }
+    nh[i] = n_h;
+    psiH[i] = std::pow(result, 1.0 / (double)D);
+  }
+}
+// linearInterpolateLUT, src/warpSPH/modules/adaptiveSupport/owenLUT.py:6-12 (FP64)
+double lutInterp(const std::vector<double>& xs, const std::vector<double>& ys, double x) {
+  int n = (int)xs.size();
+  // searchsorted(right=True) - 1: index of last element <= x
+  int il = (int)(std::upper_bound(xs.begin(), xs.end(), x) - xs.begin()) - 1;
+  il = std::min(std::max(il, 0), n - 2);
+  int ir = il + 1;
+  double a = (x - xs[il]) / (xs[ir] - xs[il]);
+  a = std::min(std::max(a, 0.0), 1.0);
+  return ys[il] * (1.0 - a) + ys[ir] * a;
+}
+// nH_to_n_h, src/warpSPH/utils/support.py:12-15
+double nH_to_n_h(double nH, int D) {
+  double vH = D == 1 ? 2.0 : (D == 2 ? M_PI : (4.0 / 3.0) * M_PI);
+  double v = vH / nH;
+  return std::pow(1.0 / v, 1.0 / (double)D);
+}
+
+// ------------------------------------------------------------ adjacency
+struct Adj {
+  int n = 0;
+  CrkGrid grid{};
+  std::vector<uint32_t> keys;      // sorted keys
+  std::vector<int32_t> perm, rank; // sorted->user, user->sorted
+  std::vector<int32_t> cstart, cend;
+  std::vector<int64_t> off;        // user rows [n+1]
+  std::vector<int32_t> nbr;        // user neighbour ids
+  std::vector<float> xb, hb;       // build snapshot, user order (FP32 positions)
+};
+
+// ------------------------------------------------------------ simulation
+template <class R, int D>
+struct Sim {
+  CrkConfig cfg;
+  Geo geo;
+  Kern<R, D> K;
+  int n;
+  R kscale, xi;
+  double nhTarget;
+  std::vector<double> lut_nh, lut_psiH;
+  double t = 0.0;
+
+  // state (user order)
+  std::vector<R> x, v, h, m, rho, u, s, P, c, div, a0, alpha;
+  // last evaluation intermediates (user order)
+  std::vector<R> V, A, B, gA, gB, J, divcd, vsig, Rcd, psiH;
+  std::vector<R> dvdt, dudt, drhodt, dEdt;
+  std::vector<R> ap, av, f;
+  int nsingular = 0;
+  int nbuilds = 0;
+  Adj adj;
+  bool haveAdj = false;
+
+  struct Stage { Adj adj; std::vector<R> ap, av, f, dvdt, dudt, drhodt, dEdt, vel; };
+  std::vector<Stage> stages;
+
+  Sim(const CrkConfig& c_, int n_) : cfg(c_), n(n_) {
+    geo = makeGeo(cfg);
+    K.kernel = cfg.kernel;
+    K.C = R(kernelC(cfg.kernel, D));
+    kscale = R(kernelScale(cfg.kernel, D));
+    xi = kscale;
+    nhTarget = nH_to_n_h((double)cfg.targetNeighbors, D);
+    lut_nh.resize(4096); lut_psiH.resize(4096);
+    owenLUT(cfg.kernel, D, 4096, 2.0, 6.0, lut_nh.data(), lut_psiH.data());
+    auto z = [&](std::vector<R>& a, int k) { a.assign((size_t)n * k, R(0)); };
+    z(x, D); z(v, D); z(h, 1); z(m, 1); z(rho, 1); z(u, 1); z(s, 1); z(P, 1); z(c, 1);
+    z(div, 1); z(a0, 1); z(alpha, 1);
+    z(V, 1); z(A, 1); z(B, D); z(gA, D); z(gB, D * D); z(J, D * D); z(divcd, 1); z(vsig, 1);
+    z(Rcd, 1); z(psiH, 1); z(dvdt, D); z(dudt, 1); z(drhodt, 1); z(dEdt, 1);
+  }
+
+  // ---------------------------------------------------- neighbour search
+  // buildVerletList (warpSPHCore), SupportScheme.SuperSymmetric with
+  // R_ij = verletScale * max(h_i, h_j), crkSPH.py:54-58.
+  int build(Adj& a) {
+    a.n = n;
+    std::vector<float> xf((size_t)n * D), hf(n);
+    for (size_t k = 0; k < xf.size(); ++k) xf[k] = (float)x[k];
+    for (int i = 0; i < n; ++i) hf[i] = (float)h[i];
+    float hmax = hf[0];
+    for (int i = 1; i < n; ++i) hmax = std::max(hmax, hf[i]);
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+      lo[d] = hi[d] = xf[d];
+      for (int i = 1; i < n; ++i) { lo[d] = std::min(lo[d], xf[(size_t)i * D + d]); hi[d] = std::max(hi[d], xf[(size_t)i * D + d]); }
+    }
+    int st = planGrid(cfg, hmax, lo, hi, a.grid);
+    if (st) return st;
+    const CrkGrid& g = a.grid;
+    std::vector<uint32_t> key(n);
+    std::vector<int> cell((size_t)n * 3, 0);
+    for (int i = 0; i < n; ++i) {
+      int cc[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) cc[d] = cellCoord(xf[(size_t)i * D + d], g, d, cfg.domain.periodic[d]);
+      for (int d = 0; d < 3; ++d) cell[(size_t)i * 3 + d] = cc[d];
+      key[i] = morton(D, cc);
+    }
+    a.perm.resize(n);
+    for (int i = 0; i < n; ++i) a.perm[i] = i;
+    std::stable_sort(a.perm.begin(), a.perm.end(), [&](int p, int q) { return key[p] < key[q]; });
+    a.rank.resize(n);
+    a.keys.resize(n);
+    for (int k = 0; k < n; ++k) { a.rank[a.perm[k]] = k; a.keys[k] = key[a.perm[k]]; }
+    a.cstart.assign((size_t)g.table_size, 0);
+    a.cend.assign((size_t)g.table_size, 0);
+    for (int k = 0; k < n; ++k) {
+      uint32_t kk = a.keys[k];
+      if (k == 0 || a.keys[k - 1] != kk) a.cstart[kk] = k;
+      if (k == n - 1 || a.keys[k + 1] != kk) a.cend[kk] = k + 1;
+    }
+    int offs[3][3], cnt[3];
+    for (int d = 0; d < 3; ++d) {
+      if (d < D) cellOffsets(g.ncell[d], cfg.domain.periodic[d], offs[d], cnt[d]);
+      else { offs[d][0] = 0; cnt[d] = 1; }
+    }
+    a.off.assign((size_t)n + 1, 0);
+    std::vector<std::vector<int32_t>> rows(n);
+    float s = cfg.verletScale;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      const float* xi_ = &xf[(size_t)i * D];
+      for (int oz = 0; oz < cnt[2]; ++oz)
+        for (int oy = 0; oy < cnt[1]; ++oy)
+          for (int ox = 0; ox < cnt[0]; ++ox) {
+            int cc[3];
+            int o[3] = {offs[0][ox], offs[1][oy], offs[2][oz]};
+            bool ok = true;
+            for (int d = 0; d < D; ++d) {
+              int q = cell[(size_t)i * 3 + d] + o[d];
+              int nn = g.ncell[d];
+              if (cfg.domain.periodic[d]) { q %= nn; if (q < 0) q += nn; }
+              else if (q < 0 || q >= nn) { ok = false; break; }
+              cc[d] = q;
+            }
+            if (!ok) continue;
+            uint32_t code = morton(D, cc);
+            for (int k = a.cstart[code]; k < a.cend[code]; ++k) {
+              int j = a.perm[k];
+              float dx[3];
+              minImage<float>(geo, xi_, &xf[(size_t)j * D], dx);
+              float r2 = norm2<float>(dx, D);
+              float Rr = s * std::max(hf[i], hf[j]);
+              if (r2 < Rr * Rr) rows[i].push_back(j);
+            }
+          }
+    }
+    for (int i = 0; i < n; ++i) a.off[i + 1] = a.off[i] + (int64_t)rows[i].size();
+    a.nbr.resize((size_t)a.off[n]);
+    for (int i = 0; i < n; ++i) std::copy(rows[i].begin(), rows[i].end(), a.nbr.begin() + a.off[i]);
+    a.xb = xf;
+    a.hb = hf;
+    ++nbuilds;
+    return CRK_OK;
+  }
+
+  // Validity of an adjacency for the current (x, h); same rule as
+  // crk_adjacency_valid.
+  bool valid(const Adj& a) {
+    float d2max = 0.f, gmax = 0.f, hbmin = a.hb[0];
+    for (int i = 0; i < n; ++i) {
+      float xc[3], dx[3];
+      for (int d = 0; d < D; ++d) xc[d] = (float)x[(size_t)i * D + d];
+      minImage<float>(geo, xc, &a.xb[(size_t)i * D], dx);
+      d2max = std::max(d2max, norm2<float>(dx, D));
+      gmax = std::max(gmax, (float)h[i] / a.hb[i]);
+      hbmin = std::min(hbmin, a.hb[i]);
+    }
+    double sv = cfg.verletScale;
+    if (!((double)gmax < sv)) return false;
+    return 2.0 * std::sqrt((double)d2max) <= (sv - (double)gmax) * (double)hbmin;
+  }
+
+  int ensureAdj() {
+    if (haveAdj && valid(adj)) return CRK_OK;
+    int st = build(adj);
+    if (st) return st;
+    haveAdj = true;
+    return CRK_OK;
+  }
+
+  template <class F>
+  void forRows(const Adj& a, F fn) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) fn(i, a.off[i], a.off[i + 1]);
+  }
+
+  const R* X(int i) const { return &x[(size_t)i * D]; }
+
+  // ---------------------------------------------------- Owen support
+  // evaluateOptimalSupportOwen, optimalSupportOwen.py:35-127, with the psi0
+  // pass of wp_psi0.py:72-103 (gather, h_i) and computeNewSupport (:27-33).
+  // The final gather-density pass (:116-125) is not performed: its result is
+  // overwritten by computeCRKFactors at crkSPH.py:61 before any use.
+  int owen() {
+    R hmin = h[0], hmax = h[0];
+    for (int i = 1; i < n; ++i) { hmin = std::min(hmin, h[i]); hmax = std::max(hmax, h[i]); }
+    for (int it = 0; it < cfg.adaptiveSupportIterations; ++it) {
+      int st = ensureAdj();
+      if (st) return st;
+      std::vector<R> hn(n);
+      R nhT = R(nhTarget);
+      forRows(adj, [&](int i, int64_t b, int64_t e) {
+        Sum<R> acc;
+        R hi = h[i];
+        for (int64_t k = b; k < e; ++k) {
+          int j = adj.nbr[k];
+          R dx[3];
+          minImage<R>(geo, X(i), X(j), dx);
+          R r = std::sqrt(norm2<R>(dx, D));
+          R q = r / hi;
+          if (!(q < R(1))) continue;
+          acc.add(K.C * std::fabs(kder<R>(cfg.kernel, D, q)));
+        }
+        R ps = std::pow(acc.get(), R(1) / R(D));
+        psiH[i] = ps;
+        R nh = R(lutInterp(lut_psiH, lut_nh, (double)ps));
+        R sr = nhT / nh;
+        R a_ = sr >= R(1) ? R(0.4) * (R(1) + R(1) / (sr * sr * sr)) : R(0.4) * (R(1) + sr * sr);
+        R hnew = (R(1) - a_ + a_ * sr) * hi;
+        R lo = hmin * R(0.25), up = hmax * R(4.0);
+        hnew = hnew < lo ? lo : (hnew > up ? up : hnew);
+        hn[i] = hnew;
+      });
+      R maxr = R(0);
+      R nmin = hn[0], nmax = hn[0];
+      for (int i = 0; i < n; ++i) {
+        maxr = std::max(maxr, std::fabs(hn[i] / h[i] - R(1)));
+        nmin = std::min(nmin, hn[i]); nmax = std::max(nmax, hn[i]);
+      }
+      h = hn; hmin = nmin; hmax = nmax;
+      if (maxr < R(cfg.adaptiveSupportThreshold)) break;
+    }
+    return CRK_OK;
+  }
+
+  // ---------------------------------------------------- CRK factors
+  // computeCRKFactors (warpSPHCore, crkSPH.py:61), linear CRK of
+  // Frontiere, Raskin & Owen (2017):
+  //   V_i   = 1 / sum_j W(x_ij, h_i)                          (gather)
+  //   m0    = sum V_j W_ij, m1 = sum V_j x_ij W_ij, m2 = sum V_j x_ij x_ij W_ij
+  //   with W_ij = W(x_ij, h_j) (scatter, matching gradw_i(..., hj, hj) in
+  //   crk/accel.py:170-175),
+  //   B = -m2^-1 m1, A = 1 / (m0 + B.m1), and their x_i gradients;
+  //   rho_i = sum_j m_j W^R_ij.
+  void gradWR(const R* xab, R r, R hh, R Aa, const R* Ba, const R* gAa, const R* gBa, R* out) const {
+    R W = K.W(r, hh);
+    R g[3];
+    K.gradW(xab, r, hh, g);
+    R bx = R(1);
+    for (int a = 0; a < D; ++a) bx = bx + Ba[a] * xab[a];
+    for (int gm = 0; gm < D; ++gm) {
+      R gbx = Ba[gm];
+      for (int a = 0; a < D; ++a) gbx = gbx + gBa[a * D + gm] * xab[a];
+      out[gm] = gAa[gm] * bx * W + Aa * gbx * W + Aa * bx * g[gm];
+    }
+  }
+
+  void crkFactors() {
+    // apparent volume
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      Sum<R> acc;
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        if (!(r < h[i])) continue;
+        acc.add(K.W(r, h[i]));
+      }
+      V[i] = R(1) / acc.get();
+    });
+    // moments + FP64 solve
+    int nsing = 0;
+#pragma omp parallel for reduction(+ : nsing) schedule(static)
+    for (int i = 0; i < n; ++i) {
+      const int NM2 = D * (D + 1) / 2;
+      Sum<R> m0, m1[3], m2[6], dm0[3], dm1[9], dm2[18];
+      for (int64_t k = adj.off[i]; k < adj.off[i + 1]; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        R hj = h[j];
+        if (!(r < hj)) continue;
+        R W = K.W(r, hj);
+        R g[3];
+        K.gradW(dx, r, hj, g);
+        R Vj = V[j];
+        m0.add(Vj * W);
+        for (int a = 0; a < D; ++a) m1[a].add(Vj * dx[a] * W);
+        int p = 0;
+        for (int a = 0; a < D; ++a)
+          for (int bb = a; bb < D; ++bb, ++p) m2[p].add(Vj * dx[a] * dx[bb] * W);
+        for (int gm = 0; gm < D; ++gm) dm0[gm].add(Vj * g[gm]);
+        for (int a = 0; a < D; ++a)
+          for (int gm = 0; gm < D; ++gm) dm1[a * D + gm].add(Vj * (dx[a] * g[gm] + (a == gm ? W : R(0))));
+        p = 0;
+        for (int a = 0; a < D; ++a)
+          for (int bb = a; bb < D; ++bb, ++p)
+            for (int gm = 0; gm < D; ++gm) {
+              R lin = (a == gm ? dx[bb] : R(0)) + (bb == gm ? dx[a] : R(0));
+              dm2[p * D + gm].add(Vj * (dx[a] * dx[bb] * g[gm] + lin * W));
+            }
+      }
+      (void)NM2;
+      // FP64 solve
+      double M0 = (double)m0.get(), M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];
+      for (int a = 0; a < D; ++a) M1[a] = (double)m1[a].get();
+      int p = 0;
+      for (int a = 0; a < D; ++a)
+        for (int bb = a; bb < D; ++bb, ++p) { M2[a][bb] = M2[bb][a] = (double)m2[p].get(); }
+      for (int gm = 0; gm < D; ++gm) DM0[gm] = (double)dm0[gm].get();
+      for (int a = 0; a < D; ++a)
+        for (int gm = 0; gm < D; ++gm) DM1[a][gm] = (double)dm1[a * D + gm].get();
+      p = 0;
+      for (int a = 0; a < D; ++a)
+        for (int bb = a; bb < D; ++bb, ++p)
+          for (int gm = 0; gm < D; ++gm) DM2[a][bb][gm] = DM2[bb][a][gm] = (double)dm2[p * D + gm].get();
+      double inv[3][3] = {{0}};
+      bool sing = false;
+      if (D == 1) {
+        double tr = M2[0][0];
+        sing = !(tr > 0.0);
+        if (!sing) inv[0][0] = 1.0 / M2[0][0];
+      } else if (D == 2) {
+        double det = M2[0][0] * M2[1][1] - M2[0][1] * M2[1][0];
+        double tr = 0.5 * (M2[0][0] + M2[1][1]);
+        sing = !(det > 1e-10 * tr * tr);
+        if (!sing) {
+          inv[0][0] = M2[1][1] / det; inv[1][1] = M2[0][0] / det;
+          inv[0][1] = -M2[0][1] / det; inv[1][0] = -M2[1][0] / det;
+        }
+      } else {
+        double c00 = M2[1][1] * M2[2][2] - M2[1][2] * M2[2][1];
+        double c01 = M2[1][2] * M2[2][0] - M2[1][0] * M2[2][2];
+        double c02 = M2[1][0] * M2[2][1] - M2[1][1] * M2[2][0];
+        double det = M2[0][0] * c00 + M2[0][1] * c01 + M2[0][2] * c02;
+        double tr = (M2[0][0] + M2[1][1] + M2[2][2]) / 3.0;
+        sing = !(det > 1e-10 * tr * tr * tr);
+        if (!sing) {
+          inv[0][0] = c00 / det; inv[1][0] = c01 / det; inv[2][0] = c02 / det;
+          inv[0][1] = (M2[0][2] * M2[2][1] - M2[0][1] * M2[2][2]) / det;
+          inv[1][1] = (M2[0][0] * M2[2][2] - M2[0][2] * M2[2][0]) / det;
+          inv[2][1] = (M2[0][1] * M2[2][0] - M2[0][0] * M2[2][1]) / det;
+          inv[0][2] = (M2[0][1] * M2[1][2] - M2[0][2] * M2[1][1]) / det;
+          inv[1][2] = (M2[0][2] * M2[1][0] - M2[0][0] * M2[1][2]) / det;
+          inv[2][2] = (M2[0][0] * M2[1][1] - M2[0][1] * M2[1][0]) / det;
+        }
+      }
+      double Ad, Bd[3] = {0, 0, 0}, gAd[3] = {0, 0, 0}, gBd[3][3] = {{0}};
+      if (sing) {
+        // fallback: Shepard normalisation A = 1/m0, B = 0
+        ++nsing;
+        Ad = 1.0 / M0;
+        for (int gm = 0; gm < D; ++gm) gAd[gm] = -DM0[gm] / (M0 * M0);
+      } else {
+        for (int a = 0; a < D; ++a) {
+          double sacc = 0.0;
+          for (int bb = 0; bb < D; ++bb) sacc += inv[a][bb] * M1[bb];
+          Bd[a] = -sacc;
+        }
+        double den = M0;
+        for (int a = 0; a < D; ++a) den += Bd[a] * M1[a];
+        Ad = 1.0 / den;
+        for (int gm = 0; gm < D; ++gm) {
+          double dinvA = DM0[gm];
+          for (int a = 0; a < D; ++a) dinvA += 2.0 * Bd[a] * DM1[a][gm];
+          for (int a = 0; a < D; ++a)
+            for (int bb = 0; bb < D; ++bb) dinvA += Bd[a] * DM2[a][bb][gm] * Bd[bb];
+          gAd[gm] = -Ad * Ad * dinvA;
+          double tmp[3];
+          for (int bb = 0; bb < D; ++bb) {
+            double t2 = DM1[bb][gm];
+            for (int dd = 0; dd < D; ++dd) t2 += DM2[bb][dd][gm] * Bd[dd];
+            tmp[bb] = t2;
+          }
+          for (int a = 0; a < D; ++a) {
+            double sacc = 0.0;
+            for (int bb = 0; bb < D; ++bb) sacc += inv[a][bb] * tmp[bb];
+            gBd[a][gm] = -sacc;
+          }
+        }
+      }
+      A[i] = R(Ad);
+      for (int a = 0; a < D; ++a) { B[(size_t)i * D + a] = R(Bd[a]); gA[(size_t)i * D + a] = R(gAd[a]); }
+      for (int a = 0; a < D; ++a)
+        for (int gm = 0; gm < D; ++gm) gB[(size_t)i * D * D + a * D + gm] = R(gBd[a][gm]);
+    }
+    nsingular = nsing;
+    // CRK density rho_i = sum_j m_j W^R_ij = A_i (S0 + B_i . S1) with
+    // S0 = sum_j m_j W_ij and S1 = sum_j m_j x_ij W_ij (W_ij = W(x_ij, h_j))
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      Sum<R> s0, s1[3];
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        R hj = h[j];
+        if (!(r < hj)) continue;
+        R W = K.W(r, hj);
+        s0.add(m[j] * W);
+        for (int a = 0; a < D; ++a) s1[a].add(m[j] * dx[a] * W);
+      }
+      R bs = s0.get();
+      for (int a = 0; a < D; ++a) bs = bs + B[(size_t)i * D + a] * s1[a].get();
+      rho[i] = A[i] * bs;
+    });
+  }
+
+  // idealGasEOS with u given, src/warpSPH/modules/eos/idealGas.py:6-38
+  void eos() {
+    R g = R(cfg.gamma);
+    for (int i = 0; i < n; ++i) {
+      c[i] = std::sqrt(std::fabs(u[i]) * g * (g - R(1)));
+      P[i] = (g - R(1)) * rho[i] * u[i];
+      s[i] = u[i] * (g - R(1)) * std::pow(rho[i], R(1) - g);
+    }
+  }
+
+  // velocity gradient, crkSPH.py:158-172 (Gradient, Scatter, Difference, CRK,
+  // apparent volume; the final quantity after .mT):
+  //   J_i[a][g] = sum_j V_j (v_j[a] - v_i[a]) d_g W^R_ij ;  drhodt = -tr(J) rho
+  void velGrad() {
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      Sum<R> acc[9];
+      const R* vi = &v[(size_t)i * D];
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        R hj = h[j];
+        if (!(r < hj)) continue;
+        R g[3];
+        gradWR(dx, r, hj, A[i], &B[(size_t)i * D], &gA[(size_t)i * D], &gB[(size_t)i * D * D], g);
+        const R* vj = &v[(size_t)j * D];
+        for (int a = 0; a < D; ++a) {
+          R dv = vj[a] - vi[a];
+          for (int gm = 0; gm < D; ++gm) acc[a * D + gm].add(V[j] * dv * g[gm]);
+        }
+      }
+      R tr = R(0);
+      for (int a = 0; a < D; ++a)
+        for (int gm = 0; gm < D; ++gm) J[(size_t)i * D * D + a * D + gm] = acc[a * D + gm].get();
+      for (int a = 0; a < D; ++a) tr = tr + J[(size_t)i * D * D + a * D + a];
+      drhodt[i] = -tr * rho[i];
+    });
+  }
+
+  // Cullen & Dehnen (2010) switch, shockCapturing/CullenDehnen2010.py:218-331
+  // with R (:121-164), Xi (:166-191) and v_sig (wp_vsig.py:11-86).
+  void cullen(R dt) {
+    // div (Divergence, SuperSymmetric H = max(h_i, h_j), Difference, m/rho)
+    // and v_sig (max over every adjacency entry)
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      Sum<R> acc;
+      R vs = R(0);
+      const R* vi = &v[(size_t)i * D];
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        const R* vj = &v[(size_t)j * D];
+        // v_sig
+        R mu = R(0);
+        for (int a = 0; a < D; ++a) mu = mu + (vi[a] - vj[a]) * dx[a];
+        mu = mu / (r + R(1.0e-14) * h[i]);
+        R cb = R(0.5) * (c[i] + c[j]);
+        R vv = cb - mu;
+        if (mu > R(0)) vv = R(0);
+        vs = vv > vs ? vv : vs;
+        // divergence
+        R H = std::max(h[i], h[j]);
+        if (!(r < H)) continue;
+        R g[3];
+        K.gradW(dx, r, H, g);
+        R dot = R(0);
+        for (int a = 0; a < D; ++a) dot = dot + (vj[a] - vi[a]) * g[a];
+        acc.add(m[j] / rho[j] * dot);
+      }
+      divcd[i] = acc.get();
+      vsig[i] = vs;
+    });
+    // R (Interpolate, Gather) and the alpha update
+    R fkern = R(1.0 / (double)xi);
+    R amin = R(cfg.switchParams.alpha_min), amax = R(cfg.switchParams.alpha_max);
+    R beta = R(cfg.switchParams.beta_xi);
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      Sum<R> acc;
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        if (!(r < h[i])) continue;
+        R sg = divcd[j] > R(0) ? R(1) : (divcd[j] < R(0) ? R(-1) : R(0));
+        R term = rho[j] * sg;
+        acc.add(m[j] / rho[j] * term * K.W(r, h[i]));
+      }
+      R Rv = R(1) / rho[i] * acc.get();
+      Rcd[i] = Rv;
+      R t1 = R(1) - Rv;
+      R t2 = t1 * t1;
+      R nom = beta * (t2 * t2) * divcd[i];
+      R Xi;
+      if (cfg.switchParams.limitXi)
+        Xi = (nom * nom + R(1e-14) * h[i]) / (nom * nom + R(0) + R(1e-14) * h[i]);
+      else
+        Xi = (nom * nom) / (nom * nom + R(0) + R(1e-14) * h[i]);
+      R ddiv = (divcd[i] - div[i]) / dt;
+      R md = -ddiv;
+      R Ai = Xi * (md > R(0) ? md : R(0));
+      R vsg = std::fabs(vsig[i]);
+      R hi = h[i];
+      R scaling = hi * hi * Ai / (vsg * vsg + hi * hi * Ai + R(1e-14) * hi);
+      R al = amax * scaling;
+      al = al < amin ? amin : (al > amax ? amax : al);
+      R tau = hi * fkern / (R(0.1) * vsg);
+      R adot = (al - a0[i]) / tau;
+      R na = al > a0[i] ? al : a0[i];
+      na = al < na ? na + adot * dt : na;
+      na = na < amin ? amin : (na > amax ? amax : na);
+      alpha[i] = na;
+    });
+    if (cfg.evolveAlpha0) a0 = alpha;
+  }
+
+  // limiter.py:11-16 (van Leer) and :28-86 (computeVanLeer), :90-127 (crkLimiter)
+  R limiterPhi(const R* dx, R hi, R hj, const R* Ji, const R* Jj) const {
+    const CrkViscosityParams& cv = cfg.crkViscosity;
+    R factor = R(1);
+    if (cv.enableCRKLimiter) {
+      R ei = R(0), ej = R(0);
+      for (int a = 0; a < D; ++a) { R t1 = dx[a] / hi; R t2 = dx[a] / hj; ei = ei + t1 * t1; ej = ej + t2 * t2; }
+      ei = std::sqrt(ei); ej = std::sqrt(ej);
+      R eta = ei < ej ? ei : ej;
+      R ec = R(cv.eta_crit);
+      if (eta < ec) { R t = (eta - ec) / R(cv.eta_fold); factor = std::exp(-(t * t)); }
+    }
+    R phi = R(0);
+    if (cv.enableVanLeerLimiter) {
+      R xt[3];
+      for (int a = 0; a < D; ++a) xt[a] = R(0.5) * dx[a];
+      R gi = R(0), gj = R(0);
+      for (int a = 0; a < D; ++a) {
+        R ci = R(0), cj = R(0);
+        for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * xt[gm]; cj = cj + Jj[a * D + gm] * xt[gm]; }
+        gi = gi + ci * xt[a];
+        gj = gj + cj * xt[a];
+      }
+      R sj = gj >= R(0) ? R(1) : R(-1);
+      R si = gi >= R(0) ? R(1) : R(-1);
+      R ri = gi / (sj * std::fabs(gj));
+      R rj = gj / (si * std::fabs(gi));
+      if (ri != ri) ri = R(1);
+      if (rj != rj) rj = R(1);
+      R rr = ri < rj ? ri : rj;
+      R vl = R(0);
+      if (rr > R(0)) { R q = R(2) / (R(1) + rr); vl = rr * q * q; }
+      phi = vl * factor;
+    }
+    if (cv.forceVanLeerOff) phi = R(0);
+    if (cv.forceVanLeerOn) phi = R(1);
+    phi = phi < R(1) ? phi : R(1);
+    phi = phi > R(0) ? phi : R(0);
+    return phi;
+  }
+
+  // computeCrkSPHAccel (crk/accel.py:65-249) and computeCrkSPHdudt
+  // (crk/dudt.py:65-228), evaluated together per edge.
+  void accelDudt() {
+    size_t E = adj.nbr.size();
+    ap.assign(E * D, R(0)); av.assign(E * D, R(0));
+    R Cl = R(cfg.diffusion.C_l), Cq = R(cfg.diffusion.C_q);
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      Sum<R> acc[3], accu;
+      const R* vi = &v[(size_t)i * D];
+      const R* Ji = &J[(size_t)i * D * D];
+      R hi = h[i];
+      R smi = hi / kscale;
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dx[3];
+        minImage<R>(geo, X(i), X(j), dx);
+        R r = std::sqrt(norm2<R>(dx, D));
+        R hj = h[j];
+        if (!(r < std::max(hi, hj))) continue;
+        const R* vj = &v[(size_t)j * D];
+        const R* Jj = &J[(size_t)j * D * D];
+        R phi = limiterPhi(dx, hi, hj, Ji, Jj);
+        R phi2 = phi / R(2);
+        R vdi[3], vdj[3], vdjU[3];
+        for (int a = 0; a < D; ++a) {
+          R ci = R(0), cj = R(0);
+          for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * dx[gm]; cj = cj + Jj[a * D + gm] * dx[gm]; }
+          vdi[a] = vi[a] - phi2 * ci;
+          vdj[a] = vj[a] + phi2 * cj;      // accel.py:137-141
+          vdjU[a] = vj[a] - phi2 * cj;     // dudt.py:123 (J_j * (-x_ij))
+        }
+        R gi[3], gjr[3], dxn[3];
+        for (int a = 0; a < D; ++a) dxn[a] = -dx[a];
+        gradWR(dx, r, hj, A[i], &B[(size_t)i * D], &gA[(size_t)i * D], &gB[(size_t)i * D * D], gi);
+        gradWR(dxn, r, hi, A[j], &B[(size_t)j * D], &gA[(size_t)j * D], &gB[(size_t)j * D * D], gjr);
+        R gw[3];
+        for (int a = 0; a < D; ++a) gw[a] = R(0.5) * (gi[a] - gjr[a]);
+        R smj = hj / kscale;
+        R ei[3], ej[3];
+        for (int a = 0; a < D; ++a) { ei[a] = dx[a] / smi; ej[a] = dx[a] / smj; }
+        R eii = R(0), ejj = R(0);
+        for (int a = 0; a < D; ++a) { eii = eii + ei[a] * ei[a]; ejj = ejj + ej[a] * ej[a]; }
+        R di = eii + R(1.0e-7) * smi * smi, dj = ejj + R(1.0e-7) * smj * smj;
+        // accel viscosity
+        R ti = R(0), tj = R(0), tiU = R(0), tjU = R(0);
+        for (int a = 0; a < D; ++a) {
+          R w = vdi[a] - vdj[a];
+          R wU = vdi[a] - vdjU[a];
+          ti = ti + w * ei[a]; tj = tj + w * ej[a];
+          tiU = tiU + wU * ei[a]; tjU = tjU + wU * ej[a];
+        }
+        R mui = ti / di, muj = tj / dj, muiU = tiU / di, mujU = tjU / dj;
+        mui = mui < R(0) ? mui : R(0); muj = muj < R(0) ? muj : R(0);
+        muiU = muiU < R(0) ? muiU : R(0); mujU = mujU < R(0) ? mujU : R(0);
+        R Qi = rho[i] * (-Cl * c[i] * mui + Cq * mui * mui);
+        R Qj = rho[j] * (-Cl * c[j] * muj + Cq * muj * muj);
+        R QiU = rho[i] * (-Cl * c[i] * muiU + Cq * muiU * muiU);
+        R QjU = rho[j] * (-Cl * c[j] * mujU + Cq * mujU * mujU);
+        if (cfg.alphaWeightedQ) {
+          R wa = R(0.5) * (alpha[i] + alpha[j]);
+          Qi = Qi * wa; Qj = Qj * wa; QiU = QiU * wa; QjU = QjU * wa;
+        }
+        R Vi = V[i], Vj = V[j], mi = m[i];
+        R pP = -(P[i] + P[j]);
+        R pQ = -(Qi + Qj);
+        for (int a = 0; a < D; ++a) {
+          R apv = pP * gw[a] * Vi * Vj / mi;
+          R avv = pQ * Vj * gw[a] * Vi / mi;
+          ap[(size_t)k * D + a] = apv;
+          av[(size_t)k * D + a] = avv;
+          acc[a].add(apv + avv);
+        }
+        // dudt: u_ij = v_i - v_j (raw velocities), dudt.py:211-228
+        R dot = R(0);
+        for (int a = 0; a < D; ++a) dot = dot + (vi[a] - vj[a]) * gw[a];
+        R pT = P[j] * dot * Vi * Vj / mi;
+        R vT = R(0.5) * (QiU + QjU) * dot * Vi * Vj / mi;
+        accu.add(pT + vT);
+      }
+      R ke = R(0);
+      for (int a = 0; a < D; ++a) {
+        dvdt[(size_t)i * D + a] = acc[a].get();
+        ke = ke + vi[a] * acc[a].get();
+      }
+      dudt[i] = accu.get();
+      dEdt[i] = m[i] * ke + m[i] * dudt[i];
+    });
+    for (int i = 0; i < n; ++i) div[i] = -drhodt[i] / rho[i];
+  }
+
+  // computeCompSPHBalanceTerm, CRK energy scheme (compSPH/balance.py:63-154)
+  void balance(R dt) {
+    size_t E = adj.nbr.size();
+    f.assign(E, R(0));
+    R hdt = R(0.5 * (double)dt);
+    std::vector<R> vh((size_t)n * D);
+    for (size_t k = 0; k < vh.size(); ++k) vh[k] = v[k] + hdt * dvdt[k];
+    R g = R(cfg.gamma);
+    forRows(adj, [&](int i, int64_t b, int64_t e) {
+      R si = P[i] / std::pow(rho[i], g);
+      for (int64_t k = b; k < e; ++k) {
+        int j = adj.nbr[k];
+        R dot = R(0);
+        for (int a = 0; a < D; ++a)
+          dot = dot + (vh[(size_t)j * D + a] - vh[(size_t)i * D + a]) * (ap[(size_t)k * D + a] + av[(size_t)k * D + a]);
+        R dE = m[i] * dot * dt;
+        R sj = P[j] / std::pow(rho[j], g);
+        R smin = std::min(std::fabs(si), std::fabs(sj));
+        R smax = std::max(std::fabs(si), std::fabs(sj));
+        bool mA = std::fabs(si - sj) == R(0);
+        bool mB = (dE >= R(0) && si >= sj) || (dE < R(0) && si < sj);
+        bool mC = (dE >= R(0) && si < sj) || (dE < R(0) && si >= sj);
+        R ff = R(0.5);
+        if (mA) ff = R(0.5);
+        else if (mB) ff = smin / (smin + smax + R(1.0e-14));
+        else if (mC) ff = smax / (smin + smax + R(1.0e-14));
+        ff = std::max(R(0), std::min(R(1), ff));
+        f[k] = ff;
+      }
+    });
+  }
+
+  // one derivative evaluation: crkSPH_step, crkSPH.py:19-366
+  int evaluate(R dt) {
+    int st;
+    if (cfg.adaptiveSupportScheme == CRK_SUPPORT_OWEN) {
+      st = owen();
+      if (st) return st;
+    }
+    st = ensureAdj();
+    if (st) return st;
+    crkFactors();
+    eos();
+    velGrad();
+    if (cfg.switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN) cullen(dt);
+    accelDudt();
+    balance(dt);
+    return CRK_OK;
+  }
+
+  // Heun RK2 step with compatible-energy finalize
+  // (warpSPHIntegrators rungeKutta2 -- tableau unpinned; systems/compSPH.py:87-129;
+  //  compSPH/multistep.py:52-94).
+  int step(R dt) {
+    std::vector<R> x0 = x, v0 = v, u0 = u;
+    stages.clear();
+    const R bw[2] = {R(0.5), R(0.5)};
+    for (int k = 0; k < 2; ++k) {
+      if (k == 1) {
+        Stage& s0 = stages[0];
+        for (size_t q = 0; q < x.size(); ++q) x[q] = x0[q] + dt * s0.vel[q];
+        for (size_t q = 0; q < v.size(); ++q) v[q] = v0[q] + dt * s0.dvdt[q];
+        for (int i = 0; i < n; ++i) u[i] = u0[i] + dt * s0.dudt[i];
+      }
+      int st = evaluate(dt);
+      if (st) return st;
+      Stage sr;
+      sr.adj = adj; sr.ap = ap; sr.av = av; sr.f = f; sr.dvdt = dvdt; sr.dudt = dudt;
+      sr.drhodt = drhodt; sr.dEdt = dEdt; sr.vel = v;
+      stages.push_back(std::move(sr));
+    }
+    Stage& s0 = stages[0];
+    Stage& s1 = stages[1];
+    for (size_t q = 0; q < x.size(); ++q) x[q] = x0[q] + dt * (bw[0] * s0.vel[q] + bw[1] * s1.vel[q]);
+    for (size_t q = 0; q < v.size(); ++q) v[q] = v0[q] + dt * (bw[0] * s0.dvdt[q] + bw[1] * s1.dvdt[q]);
+    if (cfg.compatibleEnergy) {
+      std::vector<R> vh((size_t)n * D);
+      R hdt = R(0.5) * dt;
+      for (size_t q = 0; q < vh.size(); ++q) {
+        R fa = R(0) + bw[0] * s0.dvdt[q];
+        fa = fa + bw[1] * s1.dvdt[q];
+        vh[q] = v0[q] + R(0.5) * fa * dt;
+      }
+      (void)hdt;
+      std::vector<R> du(n, R(0));
+      for (int k = 0; k < 2; ++k) {
+        Stage& sk = stages[k];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+          Sum<R> acc;
+          for (int64_t e = sk.adj.off[i]; e < sk.adj.off[i + 1]; ++e) {
+            int j = sk.adj.nbr[e];
+            R dot = R(0);
+            for (int a = 0; a < D; ++a)
+              dot = dot + (vh[(size_t)j * D + a] - vh[(size_t)i * D + a]) * (sk.ap[(size_t)e * D + a] + sk.av[(size_t)e * D + a]);
+            acc.add(bw[k] * sk.f[e] * dot);
+          }
+          du[i] = du[i] + acc.get();
+        }
+      }
+      for (int i = 0; i < n; ++i) {
+        u[i] = u0[i] + du[i] * dt;
+        s1.dudt[i] = (u[i] - u0[i]) / dt;
+      }
+      dudt = s1.dudt;
+    } else {
+      for (int i = 0; i < n; ++i) u[i] = u0[i] + dt * (bw[0] * s0.dudt[i] + bw[1] * s1.dudt[i]);
+    }
+    // periodic wrap of positions
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < D; ++d)
+        if (cfg.domain.periodic[d]) {
+          R& xx = x[(size_t)i * D + d];
+          R lo = R(cfg.domain.min[d]), hi = R(cfg.domain.max[d]), L = R(geo.L[d]);
+          if (xx >= hi) xx = xx - L;
+          else if (xx < lo) xx = xx + L;
+        }
+    t += (double)dt;
+    return CRK_OK;
+  }
+
+  // computeTimestep, timestep/compressible.py:12-66
+  double cflDt(double prev) {
+    R g = R(cfg.gamma);
+    R hmin = h[0];
+    for (int i = 1; i < n; ++i) hmin = std::min(hmin, h[i]);
+    R best = R(0);
+    bool first = true;
+    for (int i = 0; i < n; ++i) {
+      R cs = std::sqrt(std::fabs(u[i]) * g * (g - R(1)));
+      R d = R(cfg.cflFactor) * hmin / (cs * xi);
+      if (first || d < best) { best = d; first = false; }
+    }
+    double dt = (double)best;
+    dt = std::min(std::max(dt, (double)cfg.minDt), (double)cfg.maxDt);
+    if (prev > 0 && dt > prev) dt = std::min(dt, (double)cfg.dtGrowthFactor * prev);
+    return dt;
+  }
+};
+
+// ------------------------------------------------------------ type erasure
+struct SimBase {
+  virtual ~SimBase() {}
+  virtual int precision() const = 0;
+  virtual int dim() const = 0;
+  virtual int n() const = 0;
+  virtual int set(int field, const double* in) = 0;
+  virtual int get(int field, double* out) = 0;
+  virtual int evaluate(double dt) = 0;
+  virtual int step(double dt, int nsteps) = 0;
+  virtual int64_t edges(int stage) = 0;
+  virtual int adjacency(int stage, CrkGrid* g, uint32_t* keys, int32_t* perm, int32_t* cstart,
+                        int32_t* cend, int64_t* off, int32_t* nbr) = 0;
+  virtual int edgeField(int stage, int field, double* out) = 0;
+  virtual double cflDt(double prev) = 0;
+  virtual int stats(int64_t* out) = 0;
+  virtual int buildAdjacency() = 0;
+};
+
+enum {
+  F_X = 0, F_V, F_H, F_M, F_RHO, F_U, F_S, F_P, F_C, F_DIV, F_A0, F_ALPHA,
+  F_VOL, F_A, F_B, F_GA, F_GB, F_J, F_DIVCD, F_VSIG, F_RCD, F_PSIH,
+  F_DVDT, F_DUDT, F_DRHODT, F_DEDT, F_COUNT
+};
+
+template <class R, int D>
+struct SimImpl : SimBase {
+  Sim<R, D> s;
+  SimImpl(const CrkConfig& c, int n) : s(c, n) {}
+  int precision() const override { return sizeof(R) == 4 ? 32 : 64; }
+  int dim() const override { return D; }
+  int n() const override { return s.n; }
+  std::vector<R>* field(int f, int& comps) {
+    comps = 1;
+    switch (f) {
+      case F_X: comps = D; return &s.x;
+      case F_V: comps = D; return &s.v;
+      case F_H: return &s.h;
+      case F_M: return &s.m;
+      case F_RHO: return &s.rho;
+      case F_U: return &s.u;
+      case F_S: return &s.s;
+      case F_P: return &s.P;
+      case F_C: return &s.c;
+      case F_DIV: return &s.div;
+      case F_A0: return &s.a0;
+      case F_ALPHA: return &s.alpha;
+      case F_VOL: return &s.V;
+      case F_A: return &s.A;
+      case F_B: comps = D; return &s.B;
+      case F_GA: comps = D; return &s.gA;
+      case F_GB: comps = D * D; return &s.gB;
+      case F_J: comps = D * D; return &s.J;
+      case F_DIVCD: return &s.divcd;
+      case F_VSIG: return &s.vsig;
+      case F_RCD: return &s.Rcd;
+      case F_PSIH: return &s.psiH;
+      case F_DVDT: comps = D; return &s.dvdt;
+      case F_DUDT: return &s.dudt;
+      case F_DRHODT: return &s.drhodt;
+      case F_DEDT: return &s.dEdt;
+    }
+    return nullptr;
+  }
+  int set(int f, const double* in) override {
+    int comps;
+    auto* a = field(f, comps);
+    if (!a) return CRK_EINVAL;
+    for (size_t k = 0; k < a->size(); ++k) (*a)[k] = R(in[k]);
+    if (f == F_X || f == F_H) s.haveAdj = false;
+    return CRK_OK;
+  }
+  int get(int f, double* out) override {
+    int comps;
+    auto* a = field(f, comps);
+    if (!a) return CRK_EINVAL;
+    for (size_t k = 0; k < a->size(); ++k) out[k] = (double)(*a)[k];
+    return CRK_OK;
+  }
+  int evaluate(double dt) override {
+    int st = s.evaluate(R(dt));
+    if (st) return st;
+    s.stages.clear();
+    typename Sim<R, D>::Stage sr;
+    sr.adj = s.adj; sr.ap = s.ap; sr.av = s.av; sr.f = s.f;
+    s.stages.push_back(std::move(sr));
+    return CRK_OK;
+  }
+  int step(double dt, int nsteps) override {
+    for (int k = 0; k < nsteps; ++k) {
+      int st = s.step(R(dt));
+      if (st) return st;
+    }
+    return CRK_OK;
+  }
+  const Adj* stageAdj(int stage) {
+    if (stage < 0) return s.haveAdj ? &s.adj : nullptr;
+    if (stage >= (int)s.stages.size()) return nullptr;
+    return &s.stages[stage].adj;
+  }
+  int64_t edges(int stage) override {
+    const Adj* a = stageAdj(stage);
+    return a ? (int64_t)a->nbr.size() : -1;
+  }
+  int adjacency(int stage, CrkGrid* g, uint32_t* keys, int32_t* perm, int32_t* cstart, int32_t* cend,
+                int64_t* off, int32_t* nbr) override {
+    const Adj* a = stageAdj(stage);
+    if (!a) return CRK_ESTATE;
+    if (g) *g = a->grid;
+    if (keys) std::copy(a->keys.begin(), a->keys.end(), keys);
+    if (perm) std::copy(a->perm.begin(), a->perm.end(), perm);
+    if (cstart) std::copy(a->cstart.begin(), a->cstart.end(), cstart);
+    if (cend) std::copy(a->cend.begin(), a->cend.end(), cend);
+    if (off) std::copy(a->off.begin(), a->off.end(), off);
+    if (nbr) std::copy(a->nbr.begin(), a->nbr.end(), nbr);
+    return CRK_OK;
+  }
+  int edgeField(int stage, int f, double* out) override {
+    if (stage < 0 || stage >= (int)s.stages.size()) return CRK_ESTATE;
+    auto& sr = s.stages[stage];
+    const std::vector<R>* a = f == 0 ? &sr.ap : (f == 1 ? &sr.av : &sr.f);
+    for (size_t k = 0; k < a->size(); ++k) out[k] = (double)(*a)[k];
+    return CRK_OK;
+  }
+  double cflDt(double prev) override { return s.cflDt(prev); }
+  int buildAdjacency() override {
+    s.haveAdj = false;
+    return s.ensureAdj();
+  }
+  int stats(int64_t* out) override {
+    out[0] = s.nbuilds;
+    out[1] = s.nsingular;
+    return CRK_OK;
+  }
+};
+
+template <class R>
+SimBase* makeSim(const CrkConfig& c, int n) {
+  switch (c.domain.dim) {
+    case 1: return new SimImpl<R, 1>(c, n);
+    case 2: return new SimImpl<R, 2>(c, n);
+    case 3: return new SimImpl<R, 3>(c, n);
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ C interface
+extern "C" {
+
+int orc_kernel_constants(int kernel, int dim, double* C, double* scale) {
+  if (kernel < 0 || kernel > 2 || dim < 1 || dim > 3) return CRK_EINVAL;
+  *C = kernelC(kernel, dim);
+  *scale = kernelScale(kernel, dim);
+  return CRK_OK;
+}
+int orc_kernel_eval(int kernel, int dim, int n, const double* q, double* k, double* dk) {
+  if (kernel < 0 || kernel > 2 || dim < 1 || dim > 3) return CRK_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    bool in = q[i] < 1.0;
+    k[i] = in ? kval<double>(kernel, dim, q[i]) : 0.0;
+    dk[i] = in ? kder<double>(kernel, dim, q[i]) : 0.0;
+  }
+  return CRK_OK;
+}
+int orc_owen_lut(int kernel, int dim, int nlut, double nmin, double nmax, double* nh, double* psiH) {
+  if (kernel < 0 || kernel > 2 || dim < 1 || dim > 3 || nlut < 2) return CRK_EINVAL;
+  owenLUT(kernel, dim, nlut, nmin, nmax, nh, psiH);
+  return CRK_OK;
+}
+int orc_lut_interp(int nlut, const double* xs, const double* ys, int n, const double* x, double* out) {
+  std::vector<double> X(xs, xs + nlut), Y(ys, ys + nlut);
+  for (int i = 0; i < n; ++i) out[i] = lutInterp(X, Y, x[i]);
+  return CRK_OK;
+}
+int orc_grid_plan(const CrkConfig* cfg, float hmax, const float* lo, const float* hi, CrkGrid* g) {
+  return planGrid(*cfg, hmax, lo, hi, *g);
+}
+uint32_t orc_morton(int dim, int cx, int cy, int cz) {
+  int c[3] = {cx, cy, cz};
+  return morton(dim, c);
+}
+
+void* orc_sim_create(const CrkConfig* cfg, int precision, int n) {
+  if (!cfg || n <= 0 || cfg->domain.dim < 1 || cfg->domain.dim > 3) return nullptr;
+  if (cfg->kernel < 0 || cfg->kernel > 2) return nullptr;
+  return precision == 64 ? makeSim<double>(*cfg, n) : makeSim<float>(*cfg, n);
+}
+void orc_sim_destroy(void* s) { delete (SimBase*)s; }
+int orc_sim_set(void* s, int field, const double* in) { return ((SimBase*)s)->set(field, in); }
+int orc_sim_get(void* s, int field, double* out) { return ((SimBase*)s)->get(field, out); }
+int orc_sim_evaluate(void* s, double dt) { return ((SimBase*)s)->evaluate(dt); }
+int orc_sim_step(void* s, double dt, int nsteps) { return ((SimBase*)s)->step(dt, nsteps); }
+int64_t orc_sim_edges(void* s, int stage) { return ((SimBase*)s)->edges(stage); }
+int orc_sim_adjacency(void* s, int stage, CrkGrid* g, uint32_t* keys, int32_t* perm, int32_t* cstart,
+                      int32_t* cend, int64_t* off, int32_t* nbr) {
+  return ((SimBase*)s)->adjacency(stage, g, keys, perm, cstart, cend, off, nbr);
+}
+int orc_sim_edge_field(void* s, int stage, int field, double* out) { return ((SimBase*)s)->edgeField(stage, field, out); }
+double orc_sim_cfl_dt(void* s, double prev) { return ((SimBase*)s)->cflDt(prev); }
+int orc_sim_stats(void* s, int64_t* out) { return ((SimBase*)s)->stats(out); }
+int orc_sim_build_adjacency(void* s) { return ((SimBase*)s)->buildAdjacency(); }
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

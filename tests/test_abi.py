@@ -1,0 +1,93 @@
+"""The C-ABI library loads without a GPU and exports every declared symbol;
+its host-side utilities agree with the oracle's restatement."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "crksph.h")).read()
+    return sorted(set(re.findall(r"\b(crk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_bindings_agree():
+    from warpsph_amd import abi
+    assert declared_symbols() == sorted(abi.EXPORTED)
+
+
+def test_library_exports_every_symbol():
+    from warpsph_amd import abi
+    L = abi.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert L.crk_version().decode().startswith("crksph-hip")
+
+
+def test_struct_layout():
+    from warpsph_amd import abi
+    # packed(4) POD blocks of include/crksph.h
+    assert C.sizeof(abi.CrkDomain) == 4 + 12 + 12 + 12
+    assert C.sizeof(abi.CrkGrid) == 4 + 4 + 12 + 12 + 12 + 8
+    assert C.sizeof(abi.CrkAdjacency) == 4 + 8 + 4 * 8
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_kernel_constants_match_oracle(kernel, dim):
+    from oracle import oracle as O
+    from warpsph_amd import abi
+    c, s = C.c_double(), C.c_double()
+    abi.call("crk_kernel_constants", kernel, dim, C.byref(c), C.byref(s))
+    oc, os_ = O.kernel_constants(kernel, dim)
+    assert c.value == oc and s.value == os_
+
+
+@pytest.mark.parametrize("kernel,dim", [(0, 2), (1, 1), (1, 3)])
+def test_owen_lut_matches_oracle(kernel, dim):
+    from oracle import oracle as O
+    from warpsph_amd import abi
+    nh = np.empty(4096)
+    ps = np.empty(4096)
+    abi.call("crk_owen_lut", kernel, dim, 4096, 2.0, 6.0, C.c_void_p(nh.ctypes.data), C.c_void_p(ps.ctypes.data))
+    onh, ops = O.owen_lut(kernel, dim)
+    assert np.array_equal(nh, onh)
+    assert np.array_equal(ps, ops)
+
+
+def test_grid_plan_matches_oracle():
+    from oracle import oracle as O
+    from warpsph_amd import abi, sample
+    from warpsph_amd.configurations import toCrkConfig
+    for nx in (32, 64, 1024):
+        system, config, scheme = sample.shear_layer_2d(nx=nx, device="cpu")
+        cfg = toCrkConfig(config, scheme)
+        h = float(system.state.supports.max())
+        g = abi.CrkGrid()
+        z = np.zeros(3, np.float32)
+        abi.call("crk_grid_plan", C.byref(cfg), h, C.c_void_p(z.ctypes.data), C.c_void_p(z.ctypes.data), C.byref(g))
+        og = O.grid_plan(cfg, h)
+        assert bytes(g) == bytes(og)
+
+
+def test_grid_plan_rejects_oversized_support():
+    from warpsph_amd import abi, sample
+    from warpsph_amd.configurations import toCrkConfig
+    system, config, scheme = sample.shear_layer_2d(nx=8, device="cpu")
+    cfg = toCrkConfig(config, scheme)
+    g = abi.CrkGrid()
+    z = np.zeros(3, np.float32)
+    with pytest.raises(ValueError):
+        abi.call("crk_grid_plan", C.byref(cfg), 0.5, C.c_void_p(z.ctypes.data), C.c_void_p(z.ctypes.data), C.byref(g))
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    from warpsph_amd import abi
+    monkeypatch.setattr(abi, "_lib", None)
+    monkeypatch.setattr(abi, "LIB_PATH", "/nonexistent/libcrksph_hip.so")
+    with pytest.raises(abi.CrkError):
+        abi.lib()

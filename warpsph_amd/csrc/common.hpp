@@ -1,0 +1,247 @@
+// common.hpp -- shared pieces of the gfx950 CRKSPH library: error handling,
+// the context, and device helpers (kernel functions, minimum image, Kahan
+// accumulation, Morton codes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/crksph.h"
+
+namespace crk {
+
+// ------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define CRK_HIP(call)                                                  \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) return ::crk::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define CRK_LAUNCH_CHECK(name)                                         \
+  do {                                                                 \
+    hipError_t e_ = hipGetLastError();                                 \
+    if (e_ != hipSuccess) return ::crk::hip_fail(e_, name, __FILE__, __LINE__); \
+  } while (0)
+
+#define CRK_REQUIRE(cond, ...)                                         \
+  do {                                                                 \
+    if (!(cond)) { ::crk::set_error(__VA_ARGS__); return CRK_EINVAL; } \
+  } while (0)
+
+constexpr int kBlock = 256;
+inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------ context
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+int ensure(Buf& b, size_t bytes);
+void release(Buf& b);
+
+}  // namespace crk
+
+struct CrkContext {
+  int device = 0;
+  // sorted-order working set
+  int32_t n = 0;
+  int32_t dim = 0;
+  int64_t cap_n = 0;
+  crk::Buf x, v, h, m, u, a0, divp, alpha, V, A, B, gA, gB, rho, P, c, J, divcd, vsig, Rcd, dvdt, vh, du0;
+  // bound adjacency / user state
+  CrkAdjacency adj{};
+  CrkState st{};
+  bool bound = false;
+  bool have_factors = false, have_velgrad = false, have_accel = false;
+  // Owen table
+  crk::Buf lut_nh, lut_psiH;
+  int32_t nlut = 0;
+  double nh_target = 0.0;
+  // scratch
+  crk::Buf s_keys0, s_keys1, s_vals0, s_vals1, s_hist, s_scan, s_red, s_counts;
+  crk::Buf pinned;  // host pinned buffer for blocking reads
+};
+
+namespace crk {
+
+// ------------------------------------------------------------ kernels
+// W(r, H) = C_d / H^d k(r/H) (Wendland 1995; normalisation Dehnen & Aly 2012)
+__host__ __device__ inline double kernel_C(int kernel, int dim) {
+  const double C[3][3] = {{1.25, 2.228169203286535, 3.3422538049298027},
+                          {1.5, 2.8647889756541161, 4.9238560519055126},
+                          {1.71875, 3.5468815889050962, 6.7889530412636603}};
+  return C[kernel][dim - 1];
+}
+__host__ __device__ inline double kernel_scale(int kernel, int dim) {
+  const double S[3][3] = {{1.6201851746019651, 1.8973665961010275, 1.9364916731037083},
+                          {1.9364916731037087, 2.1712405933672376, 2.2079402165819615},
+                          {2.2079402165819615, 2.4152294576982398, 2.4494897427831779}};
+  return S[kernel][dim - 1];
+}
+
+template <class T>
+__host__ __device__ inline T kval(int kernel, int dim, T q) {
+  T t = T(1) - q;
+  if (kernel == CRK_KERNEL_WENDLAND2) {
+    if (dim == 1) return t * t * t * (T(1) + T(3) * q);
+    T t2 = t * t;
+    return t2 * t2 * (T(1) + T(4) * q);
+  } else if (kernel == CRK_KERNEL_WENDLAND4) {
+    if (dim == 1) { T t2 = t * t; return t2 * t2 * t * (T(1) + T(5) * q + T(8) * q * q); }
+    T t2 = t * t; T t3 = t2 * t;
+    return t3 * t3 * (T(1) + T(6) * q + T(35.0 / 3.0) * q * q);
+  } else {
+    if (dim == 1) { T t2 = t * t; T t3 = t2 * t; return t3 * t3 * t * (T(1) + T(7) * q + T(19) * q * q + T(21) * q * q * q); }
+    T t2 = t * t; T t4 = t2 * t2;
+    return t4 * t4 * (T(1) + T(8) * q + T(25) * q * q + T(32) * q * q * q);
+  }
+}
+template <class T>
+__host__ __device__ inline T kder(int kernel, int dim, T q) {
+  T t = T(1) - q;
+  if (kernel == CRK_KERNEL_WENDLAND2) {
+    if (dim == 1) return T(-12) * q * t * t;
+    return T(-20) * q * t * t * t;
+  } else if (kernel == CRK_KERNEL_WENDLAND4) {
+    if (dim == 1) { T t2 = t * t; return T(-14) * q * t2 * t2 * (T(1) + T(4) * q); }
+    T t2 = t * t;
+    return T(-56.0 / 3.0) * q * t2 * t2 * t * (T(1) + T(5) * q);
+  } else {
+    if (dim == 1) { T t2 = t * t; T t3 = t2 * t; return T(-6) * q * t3 * t3 * (T(3) + T(18) * q + T(35) * q * q); }
+    T t2 = t * t; T t3 = t2 * t;
+    return T(-22) * q * t3 * t3 * t * (T(1) + T(7) * q + T(16) * q * q);
+  }
+}
+
+template <int D>
+struct DevKernel {
+  int kernel;
+  float C;
+  __device__ inline float hpow(float h, int p) const { float r = h; for (int k = 1; k < p; ++k) r = r * h; return r; }
+  __device__ inline float W(float r, float h) const {
+    float q = r / h;
+    if (!(q < 1.f)) return 0.f;
+    return C * kval<float>(kernel, D, q) / hpow(h, D);
+  }
+  __device__ inline float dWdr(float r, float h) const {
+    float q = r / h;
+    if (!(q < 1.f)) return 0.f;
+    return C * kder<float>(kernel, D, q) / hpow(h, D + 1);
+  }
+  __device__ inline void gradW(const float* x, float r, float h, float* g) const {
+    float fac = 0.f;
+    if (r > 0.f) fac = dWdr(r, h) / r;
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = fac * x[d];
+  }
+};
+
+// ------------------------------------------------------------ geometry
+struct Geo {
+  int per[3];
+  float L[3], half[3];
+};
+inline Geo make_geo(const CrkConfig& c) {
+  Geo g;
+  for (int d = 0; d < 3; ++d) {
+    g.per[d] = d < c.domain.dim ? c.domain.periodic[d] : 0;
+    g.L[d] = c.domain.max[d] - c.domain.min[d];
+    g.half[d] = 0.5f * g.L[d];
+  }
+  return g;
+}
+// x_i - x_j with minimum image, exactly rounded (no contraction): used where
+// an integer decision depends on the result.
+template <int D>
+__device__ inline float exact_r2(const Geo& g, const float* xi, const float* xj) {
+  float r2 = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float dx = __fsub_rn(xi[d], xj[d]);
+    if (g.per[d]) {
+      if (dx > g.half[d]) dx = __fsub_rn(dx, g.L[d]);
+      else if (dx < -g.half[d]) dx = __fadd_rn(dx, g.L[d]);
+    }
+    r2 = d == 0 ? __fmul_rn(dx, dx) : __fadd_rn(r2, __fmul_rn(dx, dx));
+  }
+  return r2;
+}
+template <int D>
+__device__ inline void min_image(const Geo& g, const float* xi, const float* xj, float* out) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float dx = xi[d] - xj[d];
+    if (g.per[d]) {
+      if (dx > g.half[d]) dx = dx - g.L[d];
+      else if (dx < -g.half[d]) dx = dx + g.L[d];
+    }
+    out[d] = dx;
+  }
+}
+template <int D>
+__device__ inline float dot(const float* a, const float* b) {
+  float r = a[0] * b[0];
+#pragma unroll
+  for (int d = 1; d < D; ++d) r = r + a[d] * b[d];
+  return r;
+}
+
+// Kahan-compensated FP32 sum
+struct Kahan {
+  float s = 0.f, c = 0.f;
+  __device__ inline void add(float y) {
+    float yy = y - c;
+    float t = s + yy;
+    c = (t - s) - yy;
+    s = t;
+  }
+};
+
+// ------------------------------------------------------------ Morton
+__host__ __device__ inline uint32_t part1by1(uint32_t x) {
+  x &= 0x0000ffffu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+__host__ __device__ inline uint32_t part1by2(uint32_t x) {
+  x &= 0x000003ffu;
+  x = (x | (x << 16)) & 0xFF0000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+template <int D>
+__host__ __device__ inline uint32_t morton(const int* c) {
+  if (D == 1) return (uint32_t)c[0];
+  if (D == 2) return part1by1((uint32_t)c[0]) | (part1by1((uint32_t)c[1]) << 1);
+  return part1by2((uint32_t)c[0]) | (part1by2((uint32_t)c[1]) << 1) | (part1by2((uint32_t)c[2]) << 2);
+}
+__device__ inline int cell_coord(float x, float origin, float inv, int n, int periodic) {
+  float t = __fmul_rn(__fsub_rn(x, origin), inv);
+  int c = (int)floorf(t);
+  if (periodic) { c %= n; if (c < 0) c += n; }
+  else { c = c < 0 ? 0 : (c >= n ? n - 1 : c); }
+  return c;
+}
+
+// atomics on non-negative floats through their bit patterns
+__device__ inline void atomic_max_pos(float* addr, float v) { atomicMax((unsigned int*)addr, __float_as_uint(v)); }
+__device__ inline void atomic_min_pos(float* addr, float v) { atomicMin((unsigned int*)addr, __float_as_uint(v)); }
+
+// ------------------------------------------------------------ small host helpers
+int scan_exclusive_i32(CrkContext* ctx, const int32_t* in, int32_t* out, int32_t n, int64_t* total_dev_or_null,
+                       hipStream_t s);
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+int fill_f32(float* p, int64_t n, float v, hipStream_t s);
+
+}  // namespace crk

@@ -1,0 +1,293 @@
+// core.hip -- context, error reporting, scans and host utilities of the
+// gfx950 CRKSPH library.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace crk {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  return CRK_EHIP;
+}
+
+int ensure(Buf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.p) return CRK_OK;
+  if (b.p) {
+    CRK_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t alloc = bytes < 256 ? 256 : bytes;
+  CRK_HIP(hipMalloc(&b.p, alloc));
+  b.bytes = alloc;
+  return CRK_OK;
+}
+
+void release(Buf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+// ------------------------------------------------------------ fills
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return CRK_OK;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
+  CRK_LAUNCH_CHECK("k_fill_i32");
+  return CRK_OK;
+}
+int fill_f32(float* p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return CRK_OK;
+  hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
+  CRK_LAUNCH_CHECK("k_fill_f32");
+  return CRK_OK;
+}
+
+// ------------------------------------------------------------ exclusive scan
+// Tiles of 2048 (256 threads x 8 items) -> per-tile exclusive scan and tile
+// sums; one block scans the tile sums (int64, overflow-checked); a last pass
+// adds the tile offsets.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ void k_scan_tiles(const int32_t* in, int32_t* out, int32_t n, int64_t* tile_sums) {
+  __shared__ int32_t sh[kBlock];
+  int t = threadIdx.x;
+  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)t * kScanItems;
+  int32_t vals[kScanItems];
+  int32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t idx = base + k;
+    int32_t v = idx < n ? in[idx] : 0;
+    vals[k] = sum;
+    sum += v;
+  }
+  sh[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int32_t x = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  int32_t excl = sh[t] - sum;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t idx = base + k;
+    if (idx < n) out[idx] = vals[k] + excl;
+  }
+  if (t == kBlock - 1) tile_sums[blockIdx.x] = (int64_t)sh[kBlock - 1];
+}
+
+__global__ void k_scan_single(int64_t* a, int32_t m, int64_t* total) {
+  __shared__ int64_t sh[kBlock];
+  int t = threadIdx.x;
+  int32_t chunk = (m + kBlock - 1) / kBlock;
+  int32_t lo = t * chunk, hi = min(m, lo + chunk);
+  int64_t sum = 0;
+  for (int32_t i = lo; i < hi; ++i) sum += a[i];
+  sh[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int64_t x = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  int64_t run = sh[t] - sum;
+  for (int32_t i = lo; i < hi; ++i) {
+    int64_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+  if (t == kBlock - 1 && total) *total = sh[kBlock - 1];
+}
+
+__global__ void k_scan_add(int32_t* out, int32_t n, const int64_t* tile_off) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int32_t)((int64_t)out[i] + tile_off[i / kScanTile]);
+}
+
+int scan_exclusive_i32(CrkContext* ctx, const int32_t* in, int32_t* out, int32_t n, int64_t* total_dev,
+                       hipStream_t s) {
+  if (n <= 0) return CRK_OK;
+  int32_t ntiles = (int32_t)((n + kScanTile - 1) / kScanTile);
+  int st = ensure(ctx->s_scan, sizeof(int64_t) * (size_t)(ntiles + 1));
+  if (st) return st;
+  int64_t* tiles = (int64_t*)ctx->s_scan.p;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, in, out, n, tiles);
+  CRK_LAUNCH_CHECK("k_scan_tiles");
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, s, tiles, ntiles, total_dev);
+  CRK_LAUNCH_CHECK("k_scan_single");
+  hipLaunchKernelGGL(k_scan_add, dim3(nblocks(n)), dim3(kBlock), 0, s, out, n, tiles);
+  CRK_LAUNCH_CHECK("k_scan_add");
+  return CRK_OK;
+}
+
+// ------------------------------------------------------------ Owen table
+// computePsi (src/warpSPH/modules/adaptiveSupport/wp_psi.py:8-99), psiH
+// column: psiH(n_h) = (sum_{eta = 1/n_h, 2/n_h, ... < 1} |k'(eta)| C_d * ring(eta))^(1/d)
+// with ring = 2 (1D), 2 pi eta n_h (2D), 4 pi eta^2 n_h^2 (3D); FP64.
+static void owen_table(int kernel, int dim, int nlut, double nmin, double nmax, double* nh, double* psiH) {
+  double C = kernel_C(kernel, dim);
+  for (int i = 0; i < nlut; ++i) {
+    double dn = (nmax - nmin) / ((double)nlut - 1.0);
+    double n_h = nmin + (double)i * dn;
+    double deta = 1.0 / n_h;
+    double etar = deta, result = 0.0;
+    while (etar < 1.0) {
+      double corr = dim == 1 ? 2.0 : (dim == 2 ? 2.0 * M_PI * etar / deta : 4.0 * M_PI * (etar * etar) / (deta * deta));
+      double val = std::fabs(kder<double>(kernel, dim, etar) * C / 1.0);
+      result += val * corr;
+      etar += deta;
+    }
+    nh[i] = n_h;
+    psiH[i] = std::pow(result, 1.0 / (double)dim);
+  }
+}
+
+}  // namespace crk
+
+using namespace crk;
+
+extern "C" {
+
+const char* crk_version(void) { return "crksph-hip 1 gfx950"; }
+
+int crk_last_error(char* buf, size_t n) {
+  if (!buf || n == 0) return CRK_EINVAL;
+  std::strncpy(buf, g_err, n - 1);
+  buf[n - 1] = 0;
+  return CRK_OK;
+}
+
+int crk_ctx_create(CrkContext** out) {
+  if (!out) { set_error("crk_ctx_create: null output"); return CRK_EINVAL; }
+  CrkContext* c = new CrkContext();
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipGetDevice", __FILE__, __LINE__); }
+  void* p = nullptr;
+  e = hipHostMalloc(&p, 4096, hipHostMallocDefault);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); }
+  c->pinned.p = p;
+  c->pinned.bytes = 4096;
+  *out = c;
+  return CRK_OK;
+}
+
+int crk_ctx_destroy(CrkContext* c) {
+  if (!c) return CRK_OK;
+  (void)hipDeviceSynchronize();
+  Buf* bufs[] = {&c->x, &c->v, &c->h, &c->m, &c->u, &c->a0, &c->divp, &c->alpha, &c->V, &c->A, &c->B, &c->gA, &c->gB,
+                 &c->rho, &c->P, &c->c, &c->J, &c->divcd, &c->vsig, &c->Rcd, &c->dvdt, &c->vh, &c->du0,
+                 &c->lut_nh, &c->lut_psiH, &c->s_keys0, &c->s_keys1, &c->s_vals0, &c->s_vals1, &c->s_hist,
+                 &c->s_scan, &c->s_red, &c->s_counts};
+  for (Buf* b : bufs) release(*b);
+  if (c->pinned.p) (void)hipHostFree(c->pinned.p);
+  delete c;
+  return CRK_OK;
+}
+
+int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kernel_scale_out) {
+  CRK_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel %d", kernel);
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3 (got %d)", dim);
+  CRK_REQUIRE(C_d && kernel_scale_out, "null output");
+  *C_d = kernel_C(kernel, dim);
+  *kernel_scale_out = kernel_scale(kernel, dim);
+  return CRK_OK;
+}
+
+int crk_owen_lut(int32_t kernel, int32_t dim, int32_t nlut, double n_min, double n_max, double* n_h_out,
+                 double* psiH_out) {
+  CRK_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel %d", kernel);
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3 (got %d)", dim);
+  CRK_REQUIRE(nlut >= 2 && n_max > n_min && n_min > 0, "invalid table range");
+  CRK_REQUIRE(n_h_out && psiH_out, "null output");
+  owen_table(kernel, dim, nlut, n_min, n_max, n_h_out, psiH_out);
+  return CRK_OK;
+}
+
+int crk_set_owen_lut(CrkContext* ctx, int32_t nlut, const double* n_h, const double* psiH, double nh_target,
+                     void* stream) {
+  CRK_REQUIRE(ctx && n_h && psiH && nlut >= 2, "crk_set_owen_lut: invalid arguments");
+  for (int i = 1; i < nlut; ++i)
+    CRK_REQUIRE(psiH[i] > psiH[i - 1], "Owen table must be strictly increasing in psiH (index %d)", i);
+  int st = ensure(ctx->lut_nh, sizeof(double) * nlut);
+  if (st) return st;
+  st = ensure(ctx->lut_psiH, sizeof(double) * nlut);
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  CRK_HIP(hipMemcpyAsync(ctx->lut_nh.p, n_h, sizeof(double) * nlut, hipMemcpyHostToDevice, s));
+  CRK_HIP(hipMemcpyAsync(ctx->lut_psiH.p, psiH, sizeof(double) * nlut, hipMemcpyHostToDevice, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  ctx->nlut = nlut;
+  ctx->nh_target = nh_target;
+  return CRK_OK;
+}
+
+// Grid for a search radius R = verletScale * h_max: ncell = floor(extent / R')
+// with R' = R (1 + 1e-4) + 1e-6 extent, so every cell is wider than R even
+// after FP32 rounding of the key computation; capped so that the Morton table
+// has at most 2^24 entries.
+int crk_grid_plan(const CrkConfig* cfg, float h_max, const float* lo, const float* hi, CrkGrid* g) {
+  CRK_REQUIRE(cfg && g, "crk_grid_plan: null argument");
+  int D = cfg->domain.dim;
+  CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3 (got %d)", D);
+  CRK_REQUIRE(h_max > 0.f && std::isfinite(h_max), "supports must be positive and finite (max %g)", (double)h_max);
+  std::memset(g, 0, sizeof(*g));
+  g->dim = D;
+  double R = (double)cfg->verletScale * (double)h_max;
+  int cap = D == 1 ? (1 << 24) : (D == 2 ? 4096 : 256);
+  int maxn = 1;
+  for (int d = 0; d < D; ++d) {
+    double ext, o;
+    if (cfg->domain.periodic[d]) {
+      ext = (double)cfg->domain.max[d] - (double)cfg->domain.min[d];
+      o = cfg->domain.min[d];
+      CRK_REQUIRE(R <= 0.5 * ext,
+                  "search radius %g exceeds half the periodic extent %g in dimension %d", R, ext, d);
+    } else {
+      CRK_REQUIRE(lo && hi, "non-periodic dimension needs the particle bounding box");
+      ext = (double)hi[d] - (double)lo[d];
+      o = lo[d];
+    }
+    double Reff = R * (1.0 + 1e-4) + 1e-6 * ext;
+    long n = ext > 0 ? (long)std::floor(ext / Reff) : 1;
+    if (n < 1) n = 1;
+    if (n > cap) n = cap;
+    g->ncell[d] = (int)n;
+    g->origin[d] = (float)o;
+    g->inv_cell[d] = ext > 0 ? (float)((double)n / ext) : 0.f;
+    if (n > maxn) maxn = (int)n;
+  }
+  int bits = 0;
+  while ((1 << bits) < maxn) ++bits;
+  g->bits = bits;
+  g->table_size = (int64_t)1 << (D * bits);
+  for (int d = D; d < 3; ++d) { g->ncell[d] = 1; g->origin[d] = 0.f; g->inv_cell[d] = 0.f; }
+  return CRK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+// integrate.hip -- RK2 helpers, the compatible-energy finalize and the
+// reductions of the CRKSPH step (CFL dt, FP64 conservation sums, min/max).
+#include <cmath>
+
+#include "common.hpp"
+
+namespace crk {
+
+__global__ void k_axpy(int64_t cnt, const float* __restrict__ y0, const float* __restrict__ d, float dt,
+                       float* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) y[i] = y0[i] + dt * d[i];
+}
+
+__global__ void k_combine(int64_t cnt, const float* __restrict__ y0, const float* __restrict__ d0,
+                          const float* __restrict__ d1, float b0, float b1, float dt, float* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) y[i] = y0[i] + dt * (b0 * d0[i] + b1 * d1[i]);
+}
+
+__global__ void k_wrap(int32_t n, int D, Geo g, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
+                       float* __restrict__ x) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+  for (int d = 0; d < D; ++d) {
+    if (!g.per[d]) continue;
+    float& xx = x[(int64_t)i * D + d];
+    if (xx >= hi[d]) xx = xx - g.L[d];
+    else if (xx < lo[d]) xx = xx + g.L[d];
+  }
+}
+
+// v_half = v0 + 0.5 * ((0 + b0 dvdt0) + b1 dvdt1) * dt  (multistep.py:63-68)
+__global__ void k_vhalf(int64_t cnt, const float* __restrict__ v0, const float* __restrict__ a0,
+                        const float* __restrict__ a1, float b0, float b1, float dt, float* __restrict__ vh) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  float fa = 0.f + b0 * a0[i];
+  fa = fa + b1 * a1[i];
+  vh[i] = v0[i] + 0.5f * fa * dt;
+}
+
+// per-stage row sums of b f (v_half_j - v_half_i).(ap + av)  (multistep.py:70-94)
+template <int D, bool LAST>
+__global__ void k_compat(const int32_t* __restrict__ perm, const int32_t* __restrict__ off,
+                         const int32_t* __restrict__ nbr, int32_t n, const float* __restrict__ vh,
+                         const float* __restrict__ ap, const float* __restrict__ av, const float* __restrict__ f,
+                         float b, float* __restrict__ du0, const float* __restrict__ u0, float dt,
+                         float* __restrict__ u, float* __restrict__ dudt_last) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t i = perm[k];
+  float vi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) vi[d] = vh[(int64_t)i * D + d];
+  Kahan acc;
+  for (int32_t e = off[k]; e < off[k + 1]; ++e) {
+    int32_t j = perm[nbr[e]];
+    float dot = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) dot = dot + (vh[(int64_t)j * D + a] - vi[a]) * (ap[(int64_t)e * D + a] + av[(int64_t)e * D + a]);
+    acc.add(b * f[e] * dot);
+  }
+  if (!LAST) {
+    du0[i] = 0.f + acc.s;
+  } else {
+    float du = du0[i] + acc.s;
+    float un = u0[i] + du * dt;
+    u[i] = un;
+    dudt_last[i] = (un - u0[i]) / dt;
+  }
+}
+
+// ------------------------------------------------------------ reductions
+__global__ void k_minmax_init(float* mm) {
+  mm[0] = 3.0e38f;
+  mm[1] = -3.0e38f;
+}
+// min/max of positive values via bit-pattern atomics
+__global__ void k_minmax(const float* __restrict__ a, int32_t n, float* mm) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  float lo = 3.0e38f, hi = 0.f;
+  if (i < n) { lo = a[i]; hi = a[i]; }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, off));
+    hi = fmaxf(hi, __shfl_xor(hi, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomic_min_pos(&mm[0], lo);
+    atomicMax((int*)&mm[1], __float_as_int(hi));
+  }
+}
+
+// dt_i = cfl * hmin / (c_i xi), c_i = sqrt(|u| g (g-1))  (timestep/compressible.py:12-66)
+__global__ void k_cfl(const float* __restrict__ u, int32_t n, float gamma, float cfl, float xi,
+                      const float* __restrict__ mm, float* out) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  float d = 3.0e38f;
+  if (i < n) {
+    float cs = sqrtf(fabsf(u[i]) * gamma * (gamma - 1.f));
+    d = cfl * mm[0] / (cs * xi);
+  }
+  for (int off = 32; off > 0; off >>= 1) d = fminf(d, __shfl_xor(d, off));
+  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, d);
+}
+
+// FP64 conservation partials: per block [m, m v (D), m u, m |v|^2 / 2]
+template <int D>
+__global__ void k_conserve(const float* __restrict__ m, const float* __restrict__ v, const float* __restrict__ u,
+                           int32_t n, double* __restrict__ partial) {
+  constexpr int K = D + 3;
+  __shared__ double sh[K][kBlock];
+  int t = threadIdx.x;
+  int32_t i = blockIdx.x * blockDim.x + t;
+  double q[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) q[c] = 0.0;
+  if (i < n) {
+    double mi = m[i];
+    q[0] = mi;
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double vd = v[(int64_t)i * D + d];
+      q[1 + d] = mi * vd;
+      v2 += vd * vd;
+    }
+    q[D + 1] = mi * (double)u[i];
+    q[D + 2] = 0.5 * mi * v2;
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) sh[c][t] = q[c];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s)
+#pragma unroll
+      for (int c = 0; c < K; ++c) sh[c][t] += sh[c][t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+#pragma unroll
+    for (int c = 0; c < K; ++c) partial[(int64_t)blockIdx.x * K + c] = sh[c][0];
+}
+
+template <int K>
+__global__ void k_conserve_final(const double* __restrict__ partial, int32_t nb, double* __restrict__ out) {
+  __shared__ double sh[K][kBlock];
+  int t = threadIdx.x;
+  double q[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) q[c] = 0.0;
+  for (int32_t b = t; b < nb; b += kBlock)
+#pragma unroll
+    for (int c = 0; c < K; ++c) q[c] += partial[(int64_t)b * K + c];
+#pragma unroll
+  for (int c = 0; c < K; ++c) sh[c][t] = q[c];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s)
+#pragma unroll
+      for (int c = 0; c < K; ++c) sh[c][t] += sh[c][t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+#pragma unroll
+    for (int c = 0; c < K; ++c) out[c] = sh[c][0];
+}
+
+}  // namespace crk
+
+using namespace crk;
+
+extern "C" {
+
+int crk_axpy(int64_t count, const float* y0, const float* d, float dt, float* y, void* stream) {
+  CRK_REQUIRE(count >= 0 && (count == 0 || (y0 && d && y)), "crk_axpy: invalid arguments");
+  if (count == 0) return CRK_OK;
+  hipLaunchKernelGGL(k_axpy, dim3(nblocks(count)), dim3(kBlock), 0, (hipStream_t)stream, count, y0, d, dt, y);
+  CRK_LAUNCH_CHECK("k_axpy");
+  return CRK_OK;
+}
+
+int crk_rk_combine(int64_t count, const float* y0, const float* d0, const float* d1, float b0, float b1, float dt,
+                   float* y, void* stream) {
+  CRK_REQUIRE(count >= 0 && (count == 0 || (y0 && d0 && d1 && y)), "crk_rk_combine: invalid arguments");
+  if (count == 0) return CRK_OK;
+  hipLaunchKernelGGL(k_combine, dim3(nblocks(count)), dim3(kBlock), 0, (hipStream_t)stream, count, y0, d0, d1, b0,
+                     b1, dt, y);
+  CRK_LAUNCH_CHECK("k_combine");
+  return CRK_OK;
+}
+
+int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream) {
+  CRK_REQUIRE(cfg && x && n > 0, "crk_wrap_positions: invalid arguments");
+  Geo g = make_geo(*cfg);
+  const CrkDomain& dm = cfg->domain;
+  hipLaunchKernelGGL(k_wrap, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, dm.dim, g, dm.min[0],
+                     dm.min[1], dm.min[2], dm.max[0], dm.max[1], dm.max[2], x);
+  CRK_LAUNCH_CHECK("k_wrap");
+  return CRK_OK;
+}
+
+int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, float dt, const float* v0,
+                          const float* u0, const float* dvdt0, const float* dvdt1, float b0, float b1,
+                          const CrkAdjacency* adj0, const CrkEdges* e0, const CrkAdjacency* adj1,
+                          const CrkEdges* e1, float* u_out, float* dudt_last, void* stream) {
+  CRK_REQUIRE(ctx && cfg && v0 && u0 && dvdt0 && dvdt1 && adj0 && e0 && adj1 && e1 && u_out && dudt_last,
+              "crk_compatible_energy: null argument");
+  CRK_REQUIRE(n > 0 && adj0->n == n && adj1->n == n, "crk_compatible_energy: particle count mismatch");
+  CRK_REQUIRE(adj0->n_edges == 0 || (e0->ap && e0->av && e0->f), "stage-0 edges missing");
+  CRK_REQUIRE(adj1->n_edges == 0 || (e1->ap && e1->av && e1->f), "stage-1 edges missing");
+  int D = cfg->domain.dim;
+  hipStream_t s = (hipStream_t)stream;
+  int st;
+  if ((st = ensure(ctx->vh, sizeof(float) * (size_t)n * D))) return st;
+  if ((st = ensure(ctx->du0, sizeof(float) * (size_t)n))) return st;
+  float* vh = (float*)ctx->vh.p;
+  float* du0 = (float*)ctx->du0.p;
+  int64_t cnt = (int64_t)n * D;
+  hipLaunchKernelGGL(k_vhalf, dim3(nblocks(cnt)), dim3(kBlock), 0, s, cnt, v0, dvdt0, dvdt1, b0, b1, dt, vh);
+  CRK_LAUNCH_CHECK("k_vhalf");
+#define COMPAT(DD)                                                                                             \
+  hipLaunchKernelGGL((k_compat<DD, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, adj0->perm, adj0->row_offsets, \
+                     adj0->nbr, n, vh, e0->ap, e0->av, e0->f, b0, du0, u0, dt, u_out, dudt_last);              \
+  hipLaunchKernelGGL((k_compat<DD, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, adj1->perm, adj1->row_offsets,  \
+                     adj1->nbr, n, vh, e1->ap, e1->av, e1->f, b1, du0, u0, dt, u_out, dudt_last);
+  switch (D) {
+    case 1: COMPAT(1); break;
+    case 2: COMPAT(2); break;
+    case 3: COMPAT(3); break;
+    default: set_error("dim must be 1..3"); return CRK_EINVAL;
+  }
+#undef COMPAT
+  CRK_LAUNCH_CHECK("k_compat");
+  return CRK_OK;
+}
+
+int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* mm, void* stream) {
+  CRK_REQUIRE(ctx && values && mm && n > 0, "crk_minmax: invalid arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, mm);
+  hipLaunchKernelGGL(k_minmax, dim3(nblocks(n)), dim3(kBlock), 0, s, values, n, mm);
+  CRK_LAUNCH_CHECK("k_minmax");
+  return CRK_OK;
+}
+
+int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* supports, const float* u,
+               double prev_dt, double* dt_out, void* stream) {
+  CRK_REQUIRE(ctx && cfg && supports && u && dt_out && n > 0, "crk_cfl_dt: invalid arguments");
+  hipStream_t s = (hipStream_t)stream;
+  int st = ensure(ctx->s_red, 64);
+  if (st) return st;
+  float* red = (float*)ctx->s_red.p;  // [0,1] min/max h, [2] dt
+  if ((st = crk_minmax(ctx, n, supports, red, stream))) return st;
+  if ((st = fill_f32(red + 2, 1, 3.0e38f, s))) return st;
+  float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
+  hipLaunchKernelGGL(k_cfl, dim3(nblocks(n)), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, red, red + 2);
+  CRK_LAUNCH_CHECK("k_cfl");
+  float* hostv = (float*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, red + 2, sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  double dt = hostv[0];
+  dt = std::fmin(std::fmax(dt, (double)cfg->minDt), (double)cfg->maxDt);
+  if (prev_dt > 0 && dt > prev_dt) dt = std::fmin(dt, (double)cfg->dtGrowthFactor * prev_dt);
+  *dt_out = dt;
+  return CRK_OK;
+}
+
+int crk_conservation(CrkContext* ctx, int32_t dim, int32_t n, const float* masses, const float* velocities,
+                     const float* u, double* out, void* stream) {
+  CRK_REQUIRE(ctx && masses && velocities && u && out && n > 0, "crk_conservation: invalid arguments");
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t nb = (int32_t)nblocks(n);
+  int K = dim + 3;
+  int st = ensure(ctx->s_counts, sizeof(double) * ((size_t)nb * K + 8));
+  if (st) return st;
+  double* partial = (double*)ctx->s_counts.p;
+  double* dout = partial + (size_t)nb * K;
+  switch (dim) {
+    case 1:
+      hipLaunchKernelGGL(k_conserve<1>, dim3(nb), dim3(kBlock), 0, s, masses, velocities, u, n, partial);
+      hipLaunchKernelGGL(k_conserve_final<4>, dim3(1), dim3(kBlock), 0, s, partial, nb, dout);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_conserve<2>, dim3(nb), dim3(kBlock), 0, s, masses, velocities, u, n, partial);
+      hipLaunchKernelGGL(k_conserve_final<5>, dim3(1), dim3(kBlock), 0, s, partial, nb, dout);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_conserve<3>, dim3(nb), dim3(kBlock), 0, s, masses, velocities, u, n, partial);
+      hipLaunchKernelGGL(k_conserve_final<6>, dim3(1), dim3(kBlock), 0, s, partial, nb, dout);
+      break;
+  }
+  CRK_LAUNCH_CHECK("k_conserve");
+  double* hostv = (double*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, dout, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c) out[c] = hostv[c];
+  return CRK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,412 @@
+// neighbors.hip -- adjacency build of the CRKSPH step (replaces
+// warpSPHCore.buildVerletList, called at src/warpSPH/schemes/crkSPH.py:54-58
+// and src/warpSPH/modules/adaptiveSupport/optimalSupportOwen.py:74).
+//
+//   1. Morton key of each particle's cell (k_keys)
+//   2. stable LSD radix sort of (key, index), 8-bit digits: LDS histograms per
+//      4096-key tile, a device-wide exclusive scan, and a scatter that ranks
+//      keys inside each wave64 with ballot match masks (k_rs_*)
+//   3. cell start/end table over the Morton code space (k_cells)
+//   4. CSR rows in sorted order: every j with |x_ij| < s * max(h_i, h_j),
+//      found by scanning the 3^d neighbouring cells (k_count / k_fill).
+// All integer decisions use exactly rounded FP32 arithmetic so that the CPU
+// oracle reproduces them bit for bit.
+#include "common.hpp"
+
+namespace crk {
+
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kBlock * kRsItems;  // 4096 keys per tile
+
+struct CellScan {
+  int cnt[3];
+  int off[3][3];
+};
+
+static CellScan make_scan(const CrkGrid& g, const CrkConfig& cfg) {
+  CellScan s;
+  for (int d = 0; d < 3; ++d) {
+    if (d >= g.dim) { s.cnt[d] = 1; s.off[d][0] = 0; continue; }
+    int n = g.ncell[d];
+    if (cfg.domain.periodic[d]) {
+      if (n >= 3) { s.off[d][0] = -1; s.off[d][1] = 0; s.off[d][2] = 1; s.cnt[d] = 3; }
+      else if (n == 2) { s.off[d][0] = 0; s.off[d][1] = 1; s.cnt[d] = 2; }
+      else { s.off[d][0] = 0; s.cnt[d] = 1; }
+    } else {
+      s.off[d][0] = -1; s.off[d][1] = 0; s.off[d][2] = 1; s.cnt[d] = 3;
+    }
+  }
+  return s;
+}
+
+struct GridDev {
+  float origin[3], inv[3];
+  int ncell[3];
+  int per[3];
+};
+static GridDev make_griddev(const CrkGrid& g, const CrkConfig& cfg) {
+  GridDev d;
+  for (int k = 0; k < 3; ++k) {
+    d.origin[k] = g.origin[k];
+    d.inv[k] = g.inv_cell[k];
+    d.ncell[k] = g.ncell[k] < 1 ? 1 : g.ncell[k];
+    d.per[k] = k < g.dim ? cfg.domain.periodic[k] : 0;
+  }
+  return d;
+}
+
+template <int D>
+__device__ inline void cell_of(const GridDev& g, const float* x, int* c) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) c[d] = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) c[d] = cell_coord(x[d], g.origin[d], g.inv[d], g.ncell[d], g.per[d]);
+}
+
+template <int D>
+__global__ void k_keys(const float* __restrict__ x, int32_t n, GridDev g, uint32_t* __restrict__ keys,
+                       int32_t* __restrict__ vals) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)i * D + d];
+  int c[3];
+  cell_of<D>(g, xi, c);
+  keys[i] = morton<D>(c);
+  vals[i] = i;
+}
+
+__global__ void k_rs_hist(const uint32_t* __restrict__ keys, int32_t n, int shift, int32_t* __restrict__ hist,
+                          int32_t ntiles) {
+  __shared__ int32_t cnt[256];
+  int t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kRsTile;
+  for (int it = 0; it < kRsItems; ++it) {
+    int64_t idx = base + it * kBlock + t;
+    if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)t * ntiles + blockIdx.x] = cnt[t];
+}
+
+__global__ void k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                             uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int32_t n, int shift,
+                             const int32_t* __restrict__ hist, int32_t ntiles) {
+  __shared__ int32_t running[256];
+  __shared__ int32_t wcnt[kBlock / 64][256];
+  const int t = threadIdx.x;
+  const int w = t >> 6;
+  const int lane = t & 63;
+  running[t] = hist[(int64_t)t * ntiles + blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; ++q) wcnt[q][t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kRsTile;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int it = 0; it < kRsItems; ++it) {
+    int64_t idx = base + it * kBlock + t;
+    bool valid = idx < n;
+    uint32_t key = valid ? kin[idx] : 0u;
+    int32_t val = valid ? vin[idx] : 0;
+    uint32_t dig = (key >> shift) & 255u;
+    unsigned long long peers = __ballot(valid ? 1 : 0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      int bit = (dig >> b) & 1u;
+      unsigned long long bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    int rank = __popcll(peers & lt);
+    if (valid && rank == 0) wcnt[w][dig] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int32_t pos = running[dig];
+      for (int q = 0; q < w; ++q) pos += wcnt[q][dig];
+      pos += rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    int32_t add = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) { add += wcnt[q][t]; wcnt[q][t] = 0; }
+    running[t] += add;
+    __syncthreads();
+  }
+}
+
+template <int D>
+__global__ void k_post_sort(const int32_t* __restrict__ perm, const float* __restrict__ x, const float* __restrict__ h,
+                            int32_t n, int32_t* __restrict__ rank, float* __restrict__ xb, float* __restrict__ hb) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t i = perm[k];
+  rank[i] = k;
+#pragma unroll
+  for (int d = 0; d < D; ++d) xb[(int64_t)k * D + d] = x[(int64_t)i * D + d];
+  hb[k] = h[i];
+}
+
+__global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* __restrict__ cs, int32_t* __restrict__ ce) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t kk = keys[k];
+  if (k == 0 || keys[k - 1] != kk) cs[kk] = k;
+  if (k == n - 1 || keys[k + 1] != kk) ce[kk] = k + 1;
+}
+
+template <int D, bool FILL>
+__global__ void k_neighbors(GridDev g, CellScan sc, Geo geo, float s, const float* __restrict__ xb,
+                            const float* __restrict__ hb, const int32_t* __restrict__ cs,
+                            const int32_t* __restrict__ ce, int32_t n, int32_t* __restrict__ counts_or_offsets,
+                            int32_t* __restrict__ nbr) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = xb[(int64_t)k * D + d];
+  float hi = hb[k];
+  int c[3];
+  cell_of<D>(g, xi, c);
+  int32_t cnt = 0;
+  int64_t out = FILL ? (int64_t)counts_or_offsets[k] : 0;
+  for (int oz = 0; oz < sc.cnt[2]; ++oz)
+    for (int oy = 0; oy < sc.cnt[1]; ++oy)
+      for (int ox = 0; ox < sc.cnt[0]; ++ox) {
+        int o[3] = {sc.off[0][ox], sc.off[1][oy], sc.off[2][oz]};
+        int cc[3] = {0, 0, 0};
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          int q = c[d] + o[d];
+          int nn = g.ncell[d];
+          if (g.per[d]) { q %= nn; if (q < 0) q += nn; }
+          else if (q < 0 || q >= nn) ok = false;
+          cc[d] = q;
+        }
+        if (!ok) continue;
+        uint32_t code = morton<D>(cc);
+        int32_t b = cs[code], e = ce[code];
+        for (int32_t kk = b; kk < e; ++kk) {
+          float r2 = exact_r2<D>(geo, xi, &xb[(int64_t)kk * D]);
+          float R = __fmul_rn(s, fmaxf(hi, hb[kk]));
+          if (r2 < __fmul_rn(R, R)) {
+            if (FILL) nbr[out + cnt] = kk;
+            ++cnt;
+          }
+        }
+      }
+  if (!FILL) counts_or_offsets[k] = cnt;
+}
+
+// validity: red[0] = max |x - xb|^2, red[1] = max h/hb, red[2] = min hb
+template <int D>
+__global__ void k_valid(Geo geo, const int32_t* __restrict__ perm, const float* __restrict__ xb,
+                        const float* __restrict__ hb, const float* __restrict__ x, const float* __restrict__ h,
+                        int32_t n, float* red) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  float d2 = 0.f, g = 0.f, hm = 3.0e38f;
+  if (k < n) {
+    int32_t i = perm[k];
+    float xi[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)i * D + d];
+    d2 = exact_r2<D>(geo, xi, &xb[(int64_t)k * D]);
+    g = __fdiv_rn(h[i], hb[k]);
+    hm = hb[k];
+  }
+  // wave reductions
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 = fmaxf(d2, __shfl_xor(d2, off));
+    g = fmaxf(g, __shfl_xor(g, off));
+    hm = fminf(hm, __shfl_xor(hm, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&red[0], d2);
+    atomic_max_pos(&red[1], g);
+    atomic_min_pos(&red[2], hm);
+  }
+}
+
+__global__ void k_valid_init(float* red) {
+  red[0] = 0.f;
+  red[1] = 0.f;
+  red[2] = 3.0e38f;
+}
+
+__global__ void k_export(const int32_t* __restrict__ perm, const int32_t* __restrict__ off,
+                         const int32_t* __restrict__ nbr, int32_t n, int64_t* __restrict__ iout,
+                         int64_t* __restrict__ jout, int64_t* __restrict__ numn, int64_t* __restrict__ noff) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t i = perm[k];
+  int32_t b = off[k], e = off[k + 1];
+  if (numn) numn[i] = e - b;
+  if (noff) noff[i] = b;
+  for (int32_t q = b; q < e; ++q) {
+    if (iout) iout[q] = i;
+    if (jout) jout[q] = perm[nbr[q]];
+  }
+}
+
+template <int D>
+static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n, const float* x,
+                      const float* h, uint32_t* keys, int32_t* perm, int32_t* rank, int32_t* cell_start,
+                      int32_t* cell_end, int32_t* row_offsets, float* xb, float* hb, int64_t* n_edges,
+                      hipStream_t s) {
+  GridDev g = make_griddev(*grid, *cfg);
+  int st;
+  // keys and the identity permutation
+  if ((st = ensure(ctx->s_keys0, sizeof(uint32_t) * (size_t)n))) return st;
+  if ((st = ensure(ctx->s_keys1, sizeof(uint32_t) * (size_t)n))) return st;
+  if ((st = ensure(ctx->s_vals0, sizeof(int32_t) * (size_t)n))) return st;
+  if ((st = ensure(ctx->s_vals1, sizeof(int32_t) * (size_t)n))) return st;
+  uint32_t* k0 = (uint32_t*)ctx->s_keys0.p;
+  uint32_t* k1 = (uint32_t*)ctx->s_keys1.p;
+  int32_t* v0 = (int32_t*)ctx->s_vals0.p;
+  int32_t* v1 = (int32_t*)ctx->s_vals1.p;
+  hipLaunchKernelGGL(k_keys<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, x, n, g, k0, v0);
+  CRK_LAUNCH_CHECK("k_keys");
+  int keybits = D * grid->bits;
+  int npass = (keybits + 7) / 8;
+  int32_t ntiles = (int32_t)((n + kRsTile - 1) / kRsTile);
+  if ((st = ensure(ctx->s_hist, sizeof(int32_t) * 256 * (size_t)ntiles))) return st;
+  int32_t* hist = (int32_t*)ctx->s_hist.p;
+  for (int p = 0; p < npass; ++p) {
+    int shift = 8 * p;
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kBlock), 0, s, k0, n, shift, hist, ntiles);
+    CRK_LAUNCH_CHECK("k_rs_hist");
+    if ((st = scan_exclusive_i32(ctx, hist, hist, 256 * ntiles, nullptr, s))) return st;
+    hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kBlock), 0, s, k0, v0, k1, v1, n, shift, hist, ntiles);
+    CRK_LAUNCH_CHECK("k_rs_scatter");
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  CRK_HIP(hipMemcpyAsync(keys, k0, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  CRK_HIP(hipMemcpyAsync(perm, v0, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_post_sort<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, perm, x, h, n, rank, xb, hb);
+  CRK_LAUNCH_CHECK("k_post_sort");
+  if ((st = fill_i32(cell_start, grid->table_size, 0, s))) return st;
+  if ((st = fill_i32(cell_end, grid->table_size, 0, s))) return st;
+  hipLaunchKernelGGL(k_cells, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, cell_start, cell_end);
+  CRK_LAUNCH_CHECK("k_cells");
+  CellScan sc = make_scan(*grid, *cfg);
+  Geo geo = make_geo(*cfg);
+  hipLaunchKernelGGL((k_neighbors<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale,
+                     xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr);
+  CRK_LAUNCH_CHECK("k_neighbors(count)");
+  int64_t* total = (int64_t*)ctx->pinned.p;
+  if ((st = ensure(ctx->s_red, 64))) return st;
+  int64_t* total_dev = (int64_t*)ctx->s_red.p;
+  if ((st = scan_exclusive_i32(ctx, row_offsets, row_offsets, n, total_dev, s))) return st;
+  CRK_HIP(hipMemcpyAsync(total, total_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  int64_t E = *total;
+  if (E > (int64_t)INT32_MAX) {
+    set_error("adjacency has %lld edges; more than 2^31-1 is not supported", (long long)E);
+    return CRK_ECAPACITY;
+  }
+  int32_t e32 = (int32_t)E;
+  CRK_HIP(hipMemcpyAsync(row_offsets + n, &e32, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  *n_edges = E;
+  return CRK_OK;
+}
+
+}  // namespace crk
+
+using namespace crk;
+
+extern "C" {
+
+int crk_adjacency_count(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n, const float* x,
+                        const float* h, uint32_t* keys, int32_t* perm, int32_t* rank, int32_t* cell_start,
+                        int32_t* cell_end, int32_t* row_offsets, float* xb, float* hb, int64_t* n_edges,
+                        void* stream) {
+  CRK_REQUIRE(ctx && cfg && grid && n_edges, "crk_adjacency_count: null argument");
+  CRK_REQUIRE(n > 0, "crk_adjacency_count: n must be positive (got %d)", n);
+  CRK_REQUIRE(x && h && keys && perm && rank && cell_start && cell_end && row_offsets && xb && hb,
+              "crk_adjacency_count: null buffer");
+  CRK_REQUIRE(grid->dim == cfg->domain.dim, "grid/config dimension mismatch");
+  CRK_REQUIRE(grid->table_size >= 1 && grid->table_size <= ((int64_t)1 << 30), "bad grid table size");
+  hipStream_t s = (hipStream_t)stream;
+  switch (cfg->domain.dim) {
+    case 1: return count_impl<1>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s);
+    case 2: return count_impl<2>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s);
+    case 3: return count_impl<3>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s);
+  }
+  set_error("dim must be 1..3");
+  return CRK_EINVAL;
+}
+
+int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n,
+                       const int32_t* cell_start, const int32_t* cell_end, const int32_t* row_offsets,
+                       const float* xb, const float* hb, int32_t* nbr, int64_t capacity, void* stream) {
+  CRK_REQUIRE(ctx && cfg && grid && cell_start && cell_end && row_offsets && xb && hb, "crk_adjacency_fill: null argument");
+  CRK_REQUIRE(n > 0, "n must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t E = 0;
+  CRK_HIP(hipMemcpyAsync(&E, row_offsets + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  if ((int64_t)E > capacity) {
+    set_error("crk_adjacency_fill: %d edges exceed the capacity %lld", E, (long long)capacity);
+    return CRK_ECAPACITY;
+  }
+  if (E > 0) CRK_REQUIRE(nbr, "crk_adjacency_fill: null neighbour buffer");
+  GridDev g = make_griddev(*grid, *cfg);
+  CellScan sc = make_scan(*grid, *cfg);
+  Geo geo = make_geo(*cfg);
+  switch (cfg->domain.dim) {
+    case 1: hipLaunchKernelGGL((k_neighbors<1, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, (int32_t*)row_offsets, nbr); break;
+    case 2: hipLaunchKernelGGL((k_neighbors<2, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, (int32_t*)row_offsets, nbr); break;
+    case 3: hipLaunchKernelGGL((k_neighbors<3, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, (int32_t*)row_offsets, nbr); break;
+    default: set_error("dim must be 1..3"); return CRK_EINVAL;
+  }
+  CRK_LAUNCH_CHECK("k_neighbors(fill)");
+  return CRK_OK;
+}
+
+int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const int32_t* perm, const float* xb,
+                        const float* hb, const float* x, const float* h, int32_t* valid, void* stream) {
+  CRK_REQUIRE(ctx && cfg && perm && xb && hb && x && h && valid, "crk_adjacency_valid: null argument");
+  CRK_REQUIRE(n > 0, "n must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  int st = ensure(ctx->s_red, 64);
+  if (st) return st;
+  float* red = (float*)ctx->s_red.p;
+  hipLaunchKernelGGL(k_valid_init, dim3(1), dim3(1), 0, s, red);
+  Geo geo = make_geo(*cfg);
+  switch (cfg->domain.dim) {
+    case 1: hipLaunchKernelGGL(k_valid<1>, dim3(nblocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
+    case 2: hipLaunchKernelGGL(k_valid<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
+    case 3: hipLaunchKernelGGL(k_valid<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
+    default: set_error("dim must be 1..3"); return CRK_EINVAL;
+  }
+  CRK_LAUNCH_CHECK("k_valid");
+  float* hostv = (float*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, red, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  double sv = cfg->verletScale;
+  double g = hostv[1];
+  int ok = g < sv && 2.0 * std::sqrt((double)hostv[0]) <= (sv - g) * (double)hostv[2];
+  *valid = ok;
+  return CRK_OK;
+}
+
+int crk_adjacency_export(CrkContext* ctx, int32_t n, const int32_t* perm, const int32_t* row_offsets,
+                         const int32_t* nbr, int64_t n_edges, int64_t* i_out, int64_t* j_out, int64_t* num_neighbors,
+                         int64_t* neighbor_offsets, void* stream) {
+  CRK_REQUIRE(ctx && perm && row_offsets, "crk_adjacency_export: null argument");
+  CRK_REQUIRE(n > 0, "n must be positive");
+  CRK_REQUIRE(n_edges == 0 || nbr || (!i_out && !j_out), "crk_adjacency_export: null neighbour list");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_export, dim3(nblocks(n)), dim3(kBlock), 0, s, perm, row_offsets, nbr, n, i_out, j_out,
+                     num_neighbors, neighbor_offsets);
+  CRK_LAUNCH_CHECK("k_export");
+  return CRK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+"""Per-device runtime: the library context, the Owen table and the
+adjacency build (the MI355X replacement of warpSPHCore.buildVerletList).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import abi, timing
+from .utils import nH_to_n_h
+
+_runtimes: Dict[int, "Runtime"] = {}
+_lock = threading.Lock()
+
+
+class Runtime:
+    def __init__(self, device: torch.device):
+        if device.type != 'cuda':
+            raise ValueError(f"the CRKSPH step runs on a GPU; got device {device}")
+        self.device = device
+        self.index = device.index if device.index is not None else torch.cuda.current_device()
+        with torch.cuda.device(self.index):
+            p = C.c_void_p()
+            abi.call("crk_ctx_create", C.byref(p))
+        self.ctx = p
+        self._lut_key: Optional[Tuple] = None
+        self.scratch = torch.empty(16, dtype=torch.float32, device=device)  # [0,1] h min/max, [2] Owen ratio
+        self.nsing = torch.zeros(1, dtype=torch.int32, device=device)
+
+    def stream(self) -> C.c_void_p:
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def ensure_lut(self, kernel: int, dim: int, target_neighbors: float, nlut: int = 4096,
+                   nmin: float = 2.0, nmax: float = 6.0):
+        """Owen table (optimalSupportOwen.py:52-54: nMin 2, nMax 6, 2^12 entries)."""
+        key = (kernel, dim, float(target_neighbors), nlut, nmin, nmax)
+        if self._lut_key == key:
+            return
+        nh = np.empty(nlut, np.float64)
+        ps = np.empty(nlut, np.float64)
+        abi.call("crk_owen_lut", kernel, dim, nlut, nmin, nmax, C.c_void_p(nh.ctypes.data), C.c_void_p(ps.ctypes.data))
+        target = float(nH_to_n_h(float(np.float32(target_neighbors)), dim))
+        abi.call("crk_set_owen_lut", self.ctx, nlut, C.c_void_p(nh.ctypes.data), C.c_void_p(ps.ctypes.data), target,
+                 self.stream())
+        self._lut_key = key
+
+
+def runtime(device) -> Runtime:
+    device = torch.device(device)
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    with _lock:
+        rt = _runtimes.get(idx)
+        if rt is None:
+            rt = Runtime(torch.device('cuda', idx))
+            _runtimes[idx] = rt
+    return rt
+
+
+def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor):
+    """Morton keys -> stable LSD radix sort -> cell table -> CSR (all on the GPU)."""
+    from .systems import AdjacencyList
+    rt = runtime(positions.device)
+    n = int(positions.shape[0])
+    D = int(cfg.domain.dim)
+    dev = positions.device
+    hmax = float(supports.max().item())
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    if not all(cfg.domain.periodic[d] for d in range(D)):
+        lo[:D] = positions.min(dim=0).values.cpu().numpy()
+        hi[:D] = positions.max(dim=0).values.cpu().numpy()
+    grid = abi.CrkGrid()
+    abi.call("crk_grid_plan", C.byref(cfg), hmax, C.c_void_p(lo.ctypes.data), C.c_void_p(hi.ctypes.data),
+             C.byref(grid))
+    T = int(grid.table_size)
+    keys = torch.empty(n, dtype=torch.int32, device=dev)  # uint32 bit patterns
+    perm = torch.empty(n, dtype=torch.int32, device=dev)
+    rank = torch.empty(n, dtype=torch.int32, device=dev)
+    cs = torch.empty(T, dtype=torch.int32, device=dev)
+    ce = torch.empty(T, dtype=torch.int32, device=dev)
+    off = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    xb = torch.empty(n * D, dtype=torch.float32, device=dev)
+    hb = torch.empty(n, dtype=torch.float32, device=dev)
+    E = C.c_int64()
+    abi.call("crk_adjacency_count", rt.ctx, C.byref(cfg), C.byref(grid), n, abi.ptr(positions), abi.ptr(supports),
+             abi.ptr(keys), abi.ptr(perm), abi.ptr(rank), abi.ptr(cs), abi.ptr(ce), abi.ptr(off), abi.ptr(xb),
+             abi.ptr(hb), C.byref(E), rt.stream())
+    nbr = torch.empty(max(E.value, 1), dtype=torch.int32, device=dev)[:E.value]
+    abi.call("crk_adjacency_fill", rt.ctx, C.byref(cfg), C.byref(grid), n, abi.ptr(cs), abi.ptr(ce), abi.ptr(off),
+             abi.ptr(xb), abi.ptr(hb), abi.ptr(nbr), E.value, rt.stream())
+    return AdjacencyList(n, grid, keys, perm, rank, cs, ce, off, nbr, xb.view(n, D), hb)
+
+
+def adjacency_valid(cfg: abi.CrkConfig, adj, positions: torch.Tensor, supports: torch.Tensor) -> bool:
+    if adj is None or adj.n != positions.shape[0]:
+        return False
+    rt = runtime(positions.device)
+    v = C.c_int32()
+    abi.call("crk_adjacency_valid", rt.ctx, C.byref(cfg), adj.n, abi.ptr(adj.perm), abi.ptr(adj.xb), abi.ptr(adj.hb),
+             abi.ptr(positions), abi.ptr(supports), C.byref(v), rt.stream())
+    return bool(v.value)
+
+
+def ensure_adjacency(cfg: abi.CrkConfig, prior, positions: torch.Tensor, supports: torch.Tensor):
+    """buildVerletList(..., priorNeighborhood=prior): reuse the prior list
+    while every pair inside the current supports is still in it."""
+    with timing.scope("adjacency_check"):
+        ok = prior is not None and adjacency_valid(cfg, prior, positions, supports)
+    if ok:
+        return prior, False
+    with timing.scope("adjacency_build"):
+        adj = build_adjacency(cfg, positions, supports)
+    return adj, True

@@ -1,0 +1,132 @@
+"""The CRKSPH derivative evaluation and the scheme builder.
+
+``crkSPH_step`` has the signature and side effects of the reference's
+``crkSPH_step`` (src/warpSPH/schemes/crkSPH.py:19-366); every pass runs in the
+gfx950 HIP library through the C-ABI of include/crksph.h.  There is no CPU
+path: without the library the call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Optional
+
+import torch
+
+from . import abi, timing
+from .configurations import (CRKSPHConfig, SimulationConfig, crkSPHConfigToDict, dictToCRKSPHConfig,
+                             toCrkConfig)
+from .enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, ViscositySwitch
+from .runtime import ensure_adjacency, runtime
+from .systems import CompressibleSystemUpdate, CompSPHState, CompSPHSystem
+
+
+def _require(st: CompSPHState, names):
+    for k in names:
+        if getattr(st, k) is None:
+            raise ValueError(f"{k} must be provided as a property of the particle state.")
+
+
+def _fill_defaults(st: CompSPHState):
+    z = st.supports
+    for k in ("densities", "entropies", "pressures", "soundspeeds"):
+        if getattr(st, k) is None:
+            setattr(st, k, torch.zeros_like(z))
+    if st.divergence is None:
+        st.divergence = torch.zeros_like(z)
+    if st.alpha0s is None:
+        st.alpha0s = torch.ones_like(z)
+    if st.alphas is None:
+        st.alphas = torch.ones_like(z)
+
+
+def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, compParams: CRKSPHConfig,
+                verbose: bool = False):
+    """One CRKSPH derivative evaluation (crkSPH.py:19-366).
+
+    Side effects on ``system.state``: supports, densities, entropies,
+    pressures, soundspeeds, alphas (alpha0s when evolveAlpha0), divergence,
+    ap_ij, av_ij, f_ij; ``system.adjacency`` is the adjacency used.
+    Returns ``(update, adjacency, state)``.
+    """
+    dt = float(dt.detach().cpu().item()) if isinstance(dt, torch.Tensor) else float(dt)
+    st = system.state
+    _require(st, ("positions", "velocities", "supports", "masses", "internalEnergies"))
+    _fill_defaults(st)
+    n = st.numParticles
+    dim = int(config.domain.dim)
+    if st.positions.dim() != 2 or st.positions.shape[1] != dim:
+        raise ValueError(f"positions must be [N, {dim}]")
+    cfg = toCrkConfig(config, compParams)
+    rt = runtime(st.positions.device)
+    s = rt.stream()
+    cst = st.cabi()
+    adj = system.adjacency
+
+    if compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen:
+        # evaluateOptimalSupportOwen, optimalSupportOwen.py:35-127
+        rt.ensure_lut(cfg.kernel, dim, cfg.targetNeighbors)
+        hmm = rt.scratch[0:2]
+        ratio = rt.scratch[2:3]
+        abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
+        iters = int(compParams.adaptiveSupportIterations)
+        for it in range(iters):
+            adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
+            A = adj.cabi()
+            abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
+            with timing.scope("owen"):
+                abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm), abi.ptr(ratio), s)
+            abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
+            if iters > 1 and float(ratio.item()) < compParams.adaptiveSupportThreshold:
+                break
+        # The reference's final gather-density pass (:116-125) is overwritten by
+        # computeCRKFactors (crkSPH.py:61) before any use; it is not evaluated.
+
+    # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
+    adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
+    system.adjacency = adj
+    A = adj.cabi()
+    abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
+    # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
+    rt.nsing.zero_()
+    with timing.scope("factors_eos"):
+        abi.call("crk_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(rt.nsing), s)
+    update = CompressibleSystemUpdate(
+        dxdt=st.velocities.clone(),
+        dvdt=torch.empty_like(st.velocities),
+        dudt=torch.empty_like(st.internalEnergies),
+        dEdt=torch.empty_like(st.internalEnergies),
+        drhodt=torch.empty_like(st.internalEnergies),
+        passive=torch.zeros(n, dtype=torch.bool, device=st.positions.device))
+    cu = update.cabi()
+    # velocity gradient (crkSPH.py:158-172)
+    with timing.scope("velocity_gradient"):
+        abi.call("crk_velocity_gradient", rt.ctx, C.byref(cfg), C.byref(cu), s)
+    # viscosity switch (crkSPH.py:195-200)
+    sw = compParams.viscositySwitchParams.scheme
+    if sw == ViscositySwitch.CullenDehnen2010:
+        with timing.scope("switch"):
+            abi.call("crk_switch", rt.ctx, C.byref(cfg), C.byref(cst), float(dt), s)
+    elif sw == ViscositySwitch.Learned:
+        from .learned import apply_learned_switch
+        apply_learned_switch(rt, cfg, st, float(dt), compParams)
+    # accel + dudt (crkSPH.py:241-284), divergence and dEdt (:324-325)
+    E = adj.n_edges
+    st.ap_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
+    st.av_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
+    st.f_ij = torch.empty(E, dtype=torch.float32, device=st.positions.device)
+    ed = abi.CrkEdges(abi.ptr(st.ap_ij).value, abi.ptr(st.av_ij).value, abi.ptr(st.f_ij).value)
+    with timing.scope("accel_dudt"):
+        abi.call("crk_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed), s)
+    # energy partition with v_half = v + dt/2 dvdt (crkSPH.py:341-364)
+    with timing.scope("balance"):
+        abi.call("crk_balance", rt.ctx, C.byref(cfg), float(dt), C.byref(ed), s)
+    return update, adj, st
+
+
+def buildScheme(schemeName):
+    """schemes/builder.py:11-36, CRKSPH branch (:22-25)."""
+    if (isinstance(schemeName, str) and schemeName == 'crkSPH') or \
+            (isinstance(schemeName, CompressibleSPHScheme) and schemeName == CompressibleSPHScheme.CRKSPH):
+        return (CompSPHSystem, CompSPHState, CRKSPHConfig, CompressibleSystemUpdate, crkSPH_step,
+                crkSPHConfigToDict, dictToCRKSPHConfig)
+    raise ValueError(f"Scheme {schemeName} not recognized.")

@@ -139,6 +139,10 @@ int  crk_ctx_destroy(CrkContext* ctx);
 int  crk_last_error(char* buf, size_t n);
 /* Library version string, e.g. "crksph-hip 1 gfx950". */
 const char* crk_version(void);
+/* ABI self-check: sizeof of CrkDomain, CrkDiffusionParams, CrkViscosityParams,
+ * CrkSwitchParams, CrkConfig, CrkGrid, CrkAdjacency, CrkState, CrkUpdate,
+ * CrkEdges (in this order) written to out[0..n-1]. */
+int crk_struct_sizes(int64_t* out, int32_t n);
 
 /* ------------------------------------------------------ host utilities */
 /* Kernel normalisation C_d and kernel scale H/h (Dehnen & Aly 2012). */

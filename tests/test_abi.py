@@ -28,12 +28,15 @@ def test_library_exports_every_symbol():
     assert L.crk_version().decode().startswith("crksph-hip")
 
 
-def test_struct_layout():
+def test_struct_layout_matches_library():
     from warpsph_amd import abi
-    # packed(4) POD blocks of include/crksph.h
-    assert C.sizeof(abi.CrkDomain) == 4 + 12 + 12 + 12
-    assert C.sizeof(abi.CrkGrid) == 4 + 4 + 12 + 12 + 12 + 8
-    assert C.sizeof(abi.CrkAdjacency) == 4 + 8 + 4 * 8
+    out = (C.c_int64 * 10)()
+    abi.call("crk_struct_sizes", out, 10)
+    mirrors = [abi.CrkDomain, abi.CrkDiffusionParams, abi.CrkViscosityParams, abi.CrkSwitchParams, abi.CrkConfig,
+               abi.CrkGrid, abi.CrkAdjacency, abi.CrkState, abi.CrkUpdate, abi.CrkEdges]
+    assert [C.sizeof(m) for m in mirrors] == list(out)
+    # pointer blocks use natural alignment: the leading int32 is padded
+    assert abi.CrkState.positions.offset == 8 and abi.CrkAdjacency.n_edges.offset == 8
 
 
 @pytest.mark.parametrize("kernel", [0, 1, 2])

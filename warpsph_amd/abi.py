@@ -81,25 +81,21 @@ class CrkGrid(C.Structure):
 
 
 class CrkAdjacency(C.Structure):
-    _pack_ = 4
     _fields_ = [("n", C.c_int32), ("n_edges", C.c_int64), ("perm", C.c_void_p), ("rank", C.c_void_p),
                 ("row_offsets", C.c_void_p), ("nbr", C.c_void_p)]
 
 
 class CrkState(C.Structure):
-    _pack_ = 4
     _fields_ = [("n", C.c_int32)] + [(k, C.c_void_p) for k in (
         "positions", "velocities", "supports", "masses", "densities", "internalEnergies",
         "entropies", "pressures", "soundspeeds", "divergence", "alpha0s", "alphas")]
 
 
 class CrkUpdate(C.Structure):
-    _pack_ = 4
     _fields_ = [(k, C.c_void_p) for k in ("dvdt", "dudt", "drhodt", "dEdt")]
 
 
 class CrkEdges(C.Structure):
-    _pack_ = 4
     _fields_ = [(k, C.c_void_p) for k in ("ap", "av", "f")]
 
 
@@ -136,6 +132,7 @@ _PROTOS = {
     "crk_compatible_energy": [_P, C.POINTER(CrkConfig), C.c_int32, C.c_float, _P, _P, _P, _P,
                               C.c_float, C.c_float, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges),
                               C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, _P, _P],
+    "crk_struct_sizes": [C.POINTER(C.c_int64), C.c_int32],
     "crk_cfl_dt": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, C.c_double,
                    C.POINTER(C.c_double), _P],
     "crk_conservation": [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P],

@@ -176,6 +176,15 @@ extern "C" {
 
 const char* crk_version(void) { return "crksph-hip 1 gfx950"; }
 
+int crk_struct_sizes(int64_t* out, int32_t n) {
+  CRK_REQUIRE(out && n >= 0, "crk_struct_sizes: invalid arguments");
+  const int64_t sz[] = {sizeof(CrkDomain), sizeof(CrkDiffusionParams), sizeof(CrkViscosityParams),
+                        sizeof(CrkSwitchParams), sizeof(CrkConfig), sizeof(CrkGrid), sizeof(CrkAdjacency),
+                        sizeof(CrkState), sizeof(CrkUpdate), sizeof(CrkEdges)};
+  for (int i = 0; i < n && i < (int)(sizeof(sz) / sizeof(sz[0])); ++i) out[i] = sz[i];
+  return CRK_OK;
+}
+
 int crk_last_error(char* buf, size_t n) {
   if (!buf || n == 0) return CRK_EINVAL;
   std::strncpy(buf, g_err, n - 1);

@@ -42,13 +42,16 @@ def pass_bytes(name: str, D: int):
         "switch": (xD + 7 * f + row + 3 * f, f),
         # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt -> dvdt (sorted+user), dudt, dEdt, div; edges: j, ap, av
         "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + 2 * xD + 3 * f, f + 2 * xD),
-        # v, dvdt, m, rho, P -> ; edges: j, ap, av in, f out
-        "balance": (2 * xD + 3 * f + row, f + 2 * xD + f),
+        # v, dvdt, m, sP -> ; edges: j, ap, av in, f out
+        "balance": (2 * xD + 2 * f + row, f + 2 * xD + f),
+        # v0, dvdt0, dvdt1 -> v_half -> sorted v_half; u0 -> u, dudt;
+        # edges: j, ap, av, f of both stages
+        "compatible_energy": (5 * xD + 2 * xD + row + 3 * f, f + 2 * (2 * xD + f)),
     }
     return table.get(name)
 
 
-def run_gpu(nx, steps, warmup, timer=True):
+def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
     import torch
     from warpsph_amd import sample
     from warpsph_amd.integrator import getIntegrator
@@ -57,6 +60,16 @@ def run_gpu(nx, steps, warmup, timer=True):
     system, config, scheme = sample.shear_layer_2d(nx=nx, device="cuda:0")
     dt = sample.cfl_dt(system.state, config, scheme) * (2.0 / 3.0)
     integ = getIntegrator()
+    groups = None
+    if tuning_file and os.path.exists(tuning_file) and not retune:
+        from warpsph_amd.autotune import load, set_groups
+        groups = load(tuning_file)
+        set_groups(system.state.positions.device, groups)
+    else:
+        from warpsph_amd.autotune import autotune, save
+        groups = autotune(system, config, scheme, dt)
+        if tuning_file:
+            save(groups, tuning_file)
     s = system
     for _ in range(warmup):
         s = integ.function(s, crkSPH_step, dt, config, scheme).state
@@ -76,7 +89,7 @@ def run_gpu(nx, steps, warmup, timer=True):
             pt.__exit__(None, None, None)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    return s, (t1 - t0), pt.summary() if timer else {}, float(sum(edges) / max(len(edges), 1)), dt
+    return s, (t1 - t0), pt.summary() if timer else {}, float(sum(edges) / max(len(edges), 1)), dt, groups
 
 
 def cpu_baseline(nx, nsteps, warm):
@@ -105,9 +118,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=1024)
-    ap.add_argument("--cpu-nx", type=int, default=384)
+    ap.add_argument("--cpu-nx", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning-file", default=None, help="load (or save after tuning) the lane-group sizes")
+    ap.add_argument("--retune", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -120,7 +135,8 @@ def main():
     import torch
     torch.cuda.set_device(0)
 
-    s, secs, summary, E_mean, dt = run_gpu(args.nx, args.steps, args.warmup)
+    s, secs, summary, E_mean, dt, groups = run_gpu(args.nx, args.steps, args.warmup, tuning_file=args.tuning_file,
+                                                   retune=args.retune)
     n = args.nx * args.nx
     value = n * args.steps / secs
 
@@ -177,6 +193,7 @@ def main():
         "cpu_baseline": cpu,
         "density_l1_vs_oracle": None if l1 is None else {"value": l1, "sample": f"{args.cpu_nx}^2, {1 + args.cpu_steps} steps"},
         "passes": passes,
+        "launch_groups": groups,
     }
     print(json.dumps(line))
 

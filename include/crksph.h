@@ -275,6 +275,15 @@ int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
 /* computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154)
  * with v_half = v + dt/2 dvdt (crkSPH.py:341).  Writes edges->f. */
 int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* edges, void* stream);
+/* Launch shape: pass ids for crk_set_group_size.  G consecutive lanes share
+ * one particle row (G in 1, 2, 4, ..., 64); chosen by the autotuner. */
+enum {
+  CRK_PASS_OWEN = 0, CRK_PASS_VOLUME = 1, CRK_PASS_MOMENTS = 2, CRK_PASS_VELGRAD = 3,
+  CRK_PASS_SWITCH = 4, CRK_PASS_ACCEL = 5, CRK_PASS_BALANCE = 6, CRK_PASS_COMPAT = 7,
+  CRK_NPASS = 8
+};
+int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group);
+
 /* Copy a sorted-order context field back to user order (testing / export).
  * field: 0 V, 1 A, 2 B, 3 gradA, 4 gradB, 5 J(velocity gradient), 6 div(CD),
  *        7 vsig, 8 R. */

@@ -90,11 +90,11 @@ def test_stage_parity(name):
     torch.cuda.synchronize()
     sim = O.from_system(cpu, config, scheme, 32)
     sim.evaluate(dt)
+    sim64 = O.from_system(cpu, config, scheme, 64)
+    sim64.evaluate(dt)
     errs = {}
     for attr, key in STATE_FIELDS:
         errs[key] = relerr(getattr(st, attr).cpu().numpy(), sim.get(key))
-    for key, t in exported.items():
-        errs[key] = relerr(t.cpu().numpy().reshape(sim.get(key).shape), sim.get(key))
     errs["dvdt"] = relerr(update.dvdt.cpu().numpy(), sim.get("dvdt"))
     errs["dudt"] = relerr(update.dudt.cpu().numpy(), sim.get("dudt"))
     errs["drhodt"] = relerr(update.drhodt.cpu().numpy(), sim.get("drhodt"))
@@ -102,17 +102,34 @@ def test_stage_parity(name):
     off, j, ep = user_rows(adj)
     ref_adj = sim.adjacency(0)
     assert np.array_equal(j, ref_adj["nbr"]), "neighbour rows differ"
-    errs["ap"] = relerr(st.ap_ij.cpu().numpy()[ep], sim.edge_field(0, "ap").reshape(-1, D))
-    errs["av"] = relerr(st.av_ij.cpu().numpy()[ep], sim.edge_field(0, "av").reshape(-1, D))
+    ap_g = st.ap_ij.cpu().numpy()[ep].reshape(-1, D)
+    av_g = st.av_ij.cpu().numpy()[ep].reshape(-1, D)
+    ap_o = sim.edge_field(0, "ap").reshape(-1, D)
+    av_o = sim.edge_field(0, "av").reshape(-1, D)
+    errs["ap"] = relerr(ap_g, ap_o)
+    errs["av"] = relerr(av_g, av_o)
+    # energy partition: compare the partitioned pair work f_ij w_ij that enters
+    # the compatible-energy update (f itself is a branch on the sign of w_ij,
+    # which is rounding noise for pairs with w_ij ~ 0)
+    rows_i = np.repeat(np.arange(n), np.diff(ref_adj["offsets"]))
+    vh = sim.get("v").reshape(n, D) + 0.5 * dt * sim.get("dvdt").reshape(n, D)
+    w = np.sum((vh[ref_adj["nbr"]] - vh[rows_i]) * (ap_o + av_o), axis=1)
     fg = st.f_ij.cpu().numpy()[ep]
     fo = sim.edge_field(0, "f")
-    mism = np.abs(fg - fo) > 1e-5
-    print({k: f"{v:.2e}" for k, v in errs.items()}, "f mismatches", int(mism.sum()), "of", fo.size)
+    errs["f*w"] = relerr(fg * w, fo * w)
+    # intermediates: as accurate as the FP32 oracle, both measured against FP64
+    inter = {}
+    for key, t in exported.items():
+        ref64 = sim64.get(key)
+        e_gpu = relerr(t.cpu().numpy().reshape(ref64.shape), ref64)
+        e_o32 = relerr(sim.get(key), ref64)
+        inter[key] = (e_gpu, e_o32)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, {k: f"{a:.1e}/{b:.1e}" for k, (a, b) in inter.items()},
+          "f mismatches", int((np.abs(fg - fo) > 1e-5).sum()), "of", fo.size)
     bad = {k: v for k, v in errs.items() if not v <= RTOL}
     assert not bad, f"fields beyond {RTOL}: {bad}"
-    # f is a branch choice on the sign of a pair work term; a sign flip of a
-    # near-zero work term is the only admissible difference
-    assert mism.mean() < 1e-3
+    bad = {k: v for k, v in inter.items() if not v[0] <= max(RTOL, 4.0 * v[1])}
+    assert not bad, f"intermediates less accurate than FP32 arithmetic allows: {bad}"
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -173,3 +190,33 @@ def test_step_conserves_energy_and_momentum():
     E1, P1 = totals(s.state)
     assert abs(E1 - E0) / abs(E0) < 1e-6
     assert np.max(np.abs(P1 - P0)) < 1e-6 * float(system.state.masses.double().sum())
+
+
+@pytest.mark.parametrize("group", [1, 2, 16, 64])
+def test_stage_parity_any_group_size(group):
+    """Every lane-group size gives the oracle's result (shear layer)."""
+    from oracle import oracle as O
+    from warpsph_amd import abi, sample
+    from warpsph_amd.autotune import set_groups
+    from warpsph_amd.scheme import crkSPH_step
+    announce(f"stage shear2d G={group}")
+    system, config, scheme = make_case("shear2d")
+    cpu = cpu_copy(system, config, scheme)
+    dt = sample.cfl_dt(system.state, config, scheme) * (2.0 / 3.0)
+    set_groups(system.state.positions.device, {p: group for p in abi.PASSES})
+    try:
+        update, adj, st = crkSPH_step(system, dt, config, scheme)
+        torch.cuda.synchronize()
+    finally:
+        set_groups(system.state.positions.device,
+                   {"owen": 8, "volume": 8, "moments": 4, "velgrad": 4, "switch": 8, "accel": 4, "balance": 8,
+                    "compat": 8})
+    sim = O.from_system(cpu, config, scheme, 32)
+    sim.evaluate(dt)
+    errs = {k: relerr(getattr(st, a).cpu().numpy(), sim.get(k)) for a, k in STATE_FIELDS}
+    errs["dvdt"] = relerr(update.dvdt.cpu().numpy(), sim.get("dvdt"))
+    errs["dudt"] = relerr(update.dudt.cpu().numpy(), sim.get("dudt"))
+    _, _, ep = user_rows(adj)
+    errs["ap"] = relerr(st.ap_ij.cpu().numpy()[ep], sim.edge_field(0, "ap").reshape(-1, 2))
+    print(errs)
+    assert all(v <= RTOL for v in errs.values()), errs

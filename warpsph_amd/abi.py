@@ -31,6 +31,7 @@ CRK_SUPPORT_OWEN = 2
 CRK_SWITCH_CULLEN_DEHNEN = 2
 CRK_SWITCH_NONE = 6
 CRK_SWITCH_LEARNED = 100
+PASSES = ("owen", "volume", "moments", "velgrad", "switch", "accel", "balance", "compat")
 
 
 class CrkDomain(C.Structure):
@@ -132,6 +133,7 @@ _PROTOS = {
     "crk_compatible_energy": [_P, C.POINTER(CrkConfig), C.c_int32, C.c_float, _P, _P, _P, _P,
                               C.c_float, C.c_float, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges),
                               C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, _P, _P],
+    "crk_set_group_size": [_P, C.c_int32, C.c_int32],
     "crk_struct_sizes": [C.POINTER(C.c_int64), C.c_int32],
     "crk_cfl_dt": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, C.c_double,
                    C.POINTER(C.c_double), _P],

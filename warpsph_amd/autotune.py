@@ -1,0 +1,94 @@
+"""Launch-shape autotuner of the CRKSPH passes.
+
+Every pair pass runs with G lanes per particle row (crk_set_group_size).
+The best G depends on the pass (register pressure, number of compensated
+accumulators to combine, edge-array traffic), the dimension and the
+neighbour count, so it is measured: for each pass in turn, every candidate G
+is timed over full RK2 steps of the caller's workload on a scratch copy of
+the state (HIP events on the launch stream), and the fastest is kept.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, Iterable, Optional
+
+import torch
+
+from . import abi
+from .runtime import runtime
+from .timing import PassTimer
+
+# pass -> timing scope that contains it
+SCOPES = {
+    "owen": "owen", "volume": "factors_eos", "moments": "factors_eos", "velgrad": "velocity_gradient",
+    "switch": "switch", "accel": "accel_dudt", "balance": "balance", "compat": "compatible_energy",
+}
+
+
+def set_groups(device, groups: Dict[str, int]):
+    rt = runtime(device)
+    for name, g in groups.items():
+        abi.call("crk_set_group_size", rt.ctx, abi.PASSES.index(name), int(g))
+    rt.groups = dict(getattr(rt, "groups", {}), **groups)
+
+
+def _copy(system):
+    s = system.initializeNewState()
+    s.adjacency = system.adjacency
+    return s
+
+
+def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 2, 4, 8, 16, 32),
+             reps: int = 2, passes: Optional[Iterable[str]] = None, verbose: bool = False) -> Dict[str, int]:
+    from .integrator import getIntegrator
+    from .scheme import crkSPH_step
+    dev = system.state.positions.device
+    rt = runtime(dev)
+    integ = getIntegrator()
+    current = dict(getattr(rt, "groups", {}))
+    defaults = {"owen": 8, "volume": 8, "moments": 4, "velgrad": 4, "switch": 8, "accel": 4, "balance": 8,
+                "compat": 8}
+    for k, v in defaults.items():
+        current.setdefault(k, v)
+    set_groups(dev, current)
+    # one untimed step so that the adjacency exists and memory is allocated
+    base = _copy(system)
+    base = integ.function(base, crkSPH_step, dt, config, scheme).state
+    torch.cuda.synchronize()
+    names = list(passes) if passes is not None else list(SCOPES)
+    if scheme.viscositySwitchParams.scheme.name != "CullenDehnen2010" and "switch" in names:
+        names.remove("switch")
+    report = {}
+    for name in names:
+        best, best_t = current[name], None
+        for g in candidates:
+            set_groups(dev, {name: g})
+            s = _copy(base)
+            integ.function(s, crkSPH_step, dt, config, scheme)  # warm this shape
+            with PassTimer() as pt:
+                for _ in range(reps):
+                    s = integ.function(_copy(base), crkSPH_step, dt, config, scheme).state
+            summ = pt.summary()
+            t = summ.get(SCOPES[name], (0, 0.0, 0.0))[1]
+            report.setdefault(name, {})[g] = t
+            if best_t is None or t < best_t:
+                best, best_t = g, t
+        current[name] = best
+        set_groups(dev, {name: best})
+        if verbose:
+            print(f"autotune {name}: " + ", ".join(f"G={g}:{t:.3f}ms" for g, t in report[name].items()) + f" -> {best}")
+    rt.tuning_report = report
+    return current
+
+
+def save(groups: Dict[str, int], path: str):
+    with open(path, "w") as f:
+        json.dump(groups, f, indent=1)
+
+
+def load(path: str) -> Optional[Dict[str, int]]:
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return {k: int(v) for k, v in json.load(f).items()}

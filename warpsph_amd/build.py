@@ -25,9 +25,15 @@ FLAGS = [
 ]
 
 
+# pair passes: fast (non correctly rounded) FP32 division and sqrt; the
+# neighbour search keeps exactly rounded arithmetic for its integer decisions
+FAST = {"passes.hip", "integrate.hip"}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(SRC, src), "-o", obj]
+    extra = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"] if src in FAST else []
+    cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

@@ -53,7 +53,9 @@ struct CrkContext {
   int32_t n = 0;
   int32_t dim = 0;
   int64_t cap_n = 0;
-  crk::Buf x, v, h, m, u, a0, divp, alpha, V, A, B, gA, gB, rho, P, c, J, divcd, vsig, Rcd, dvdt, vh, du0;
+  crk::Buf x, v, h, m, u, a0, divp, alpha, V, A, B, gA, gB, rho, P, c, sP, J, divcd, vsig, Rcd, dvdt, vh, vhs, du0;
+  // lanes per particle row for each pass (crk_set_group_size)
+  int32_t group[CRK_NPASS] = {8, 8, 4, 4, 8, 4, 8, 8};
   // bound adjacency / user state
   CrkAdjacency adj{};
   CrkState st{};
@@ -237,6 +239,29 @@ __device__ inline int cell_coord(float x, float origin, float inv, int n, int pe
 // atomics on non-negative floats through their bit patterns
 __device__ inline void atomic_max_pos(float* addr, float v) { atomicMax((unsigned int*)addr, __float_as_uint(v)); }
 __device__ inline void atomic_min_pos(float* addr, float v) { atomicMin((unsigned int*)addr, __float_as_uint(v)); }
+
+// Block-wide reductions (every thread of the block must call; result valid
+// in every thread).  Used so that grid reductions issue one atomic per block.
+template <class Op>
+__device__ inline float block_reduce(float v, Op op) {
+  __shared__ float sh_red[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float r = sh_red[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) r = op(r, sh_red[w]);
+  return r;
+}
+struct MaxOp { __device__ float operator()(float a, float b) const { return fmaxf(a, b); } };
+struct MinOp { __device__ float operator()(float a, float b) const { return fminf(a, b); } };
+// grid size of grid-stride reductions
+inline unsigned reduce_blocks(int64_t n) {
+  unsigned b = nblocks(n);
+  return b < 512u ? (b == 0 ? 1u : b) : 512u;
+}
 
 // ------------------------------------------------------------ small host helpers
 int scan_exclusive_i32(CrkContext* ctx, const int32_t* in, int32_t* out, int32_t n, int64_t* total_dev_or_null,

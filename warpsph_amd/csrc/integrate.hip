@@ -41,31 +41,90 @@ __global__ void k_vhalf(int64_t cnt, const float* __restrict__ v0, const float* 
   vh[i] = v0[i] + 0.5f * fa * dt;
 }
 
-// per-stage row sums of b f (v_half_j - v_half_i).(ap + av)  (multistep.py:70-94)
-template <int D, bool LAST>
-__global__ void k_compat(const int32_t* __restrict__ perm, const int32_t* __restrict__ off,
-                         const int32_t* __restrict__ nbr, int32_t n, const float* __restrict__ vh,
-                         const float* __restrict__ ap, const float* __restrict__ av, const float* __restrict__ f,
-                         float b, float* __restrict__ du0, const float* __restrict__ u0, float dt,
-                         float* __restrict__ u, float* __restrict__ dudt_last) {
+// v_half gathered into an adjacency's sorted order
+template <int D>
+__global__ void k_gather_vec(const int32_t* __restrict__ perm, int32_t n, const float* __restrict__ src,
+                             float* __restrict__ dst) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int32_t i = perm[k];
+#pragma unroll
+  for (int d = 0; d < D; ++d) dst[(int64_t)k * D + d] = src[(int64_t)i * D + d];
+}
+
+struct CompIn {
+  const float *ap, *av, *f;
+  float b;
+};
+
+template <int G>
+__device__ inline float group_total2(float s, float c) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    float so = __shfl_xor(s, off, G);
+    float co = __shfl_xor(c, off, G);
+    float t = s + so;
+    float bp = t - s;
+    float e = (s - (t - bp)) + (so - bp);
+    s = t;
+    c = (c + co) - e;
+  }
+  return s - c;
+}
+
+// Row sums of b_k f (v_half_j - v_half_i).(ap + av) over the rows of one
+// adjacency (multistep.py:70-94), G lanes per row.  NST = 2: both stages
+// share the adjacency and are summed in the same sweep (separate sums, added
+// in stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last
+// sweep (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep
+// of both stages.
+template <int D, int G, int NST, int MODE>
+__global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, const int32_t* __restrict__ off,
+                                                   const int32_t* __restrict__ nbr, int32_t n,
+                                                   const float* __restrict__ vhs, CompIn s0, CompIn s1,
+                                                   float* __restrict__ du0, const float* __restrict__ u0, float dt,
+                                                   float* __restrict__ u, float* __restrict__ dudt_last) {
+  const int g = threadIdx.x % G;
+  int32_t k = (int32_t)blockIdx.x * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
+  const bool valid = k < n;
+  if (!valid) k = n - 1;
   float vi[3];
 #pragma unroll
-  for (int d = 0; d < D; ++d) vi[d] = vh[(int64_t)i * D + d];
-  Kahan acc;
-  for (int32_t e = off[k]; e < off[k + 1]; ++e) {
-    int32_t j = perm[nbr[e]];
-    float dot = 0.f;
+  for (int d = 0; d < D; ++d) vi[d] = vhs[(int64_t)k * D + d];
+  float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
+  const int32_t b = off[k], e = valid ? off[k + 1] : b;
+  for (int32_t q = b + g; q < e; q += G) {
+    int32_t j = nbr[q];
+    float dv[3];
 #pragma unroll
-    for (int a = 0; a < D; ++a) dot = dot + (vh[(int64_t)j * D + a] - vi[a]) * (ap[(int64_t)e * D + a] + av[(int64_t)e * D + a]);
-    acc.add(b * f[e] * dot);
+    for (int a = 0; a < D; ++a) dv[a] = vhs[(int64_t)j * D + a] - vi[a];
+    float d0 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) d0 = d0 + dv[a] * (s0.ap[(int64_t)q * D + a] + s0.av[(int64_t)q * D + a]);
+    {
+      float y = s0.b * s0.f[q] * d0 - ca;
+      float t = sa + y;
+      ca = (t - sa) - y;
+      sa = t;
+    }
+    if (NST == 2) {
+      float d1 = 0.f;
+#pragma unroll
+      for (int a = 0; a < D; ++a) d1 = d1 + dv[a] * (s1.ap[(int64_t)q * D + a] + s1.av[(int64_t)q * D + a]);
+      float y = s1.b * s1.f[q] * d1 - cb;
+      float t = sb + y;
+      cb = (t - sb) - y;
+      sb = t;
+    }
   }
-  if (!LAST) {
-    du0[i] = 0.f + acc.s;
+  float ta = group_total2<G>(sa, ca);
+  float tb = NST == 2 ? group_total2<G>(sb, cb) : 0.f;
+  if (!valid || g != 0) return;
+  int32_t i = perm[k];
+  if (MODE == 0) {
+    du0[i] = 0.f + ta;
   } else {
-    float du = du0[i] + acc.s;
+    float du = MODE == 1 ? du0[i] + ta : (0.f + ta) + tb;
     float un = u0[i] + du * dt;
     u[i] = un;
     dudt_last[i] = (un - u0[i]) / dt;
@@ -77,16 +136,17 @@ __global__ void k_minmax_init(float* mm) {
   mm[0] = 3.0e38f;
   mm[1] = -3.0e38f;
 }
-// min/max of positive values via bit-pattern atomics
+// min/max of positive values via bit-pattern atomics (grid-stride, one
+// atomic per block)
 __global__ void k_minmax(const float* __restrict__ a, int32_t n, float* mm) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float lo = 3.0e38f, hi = 0.f;
-  if (i < n) { lo = a[i]; hi = a[i]; }
-  for (int off = 32; off > 0; off >>= 1) {
-    lo = fminf(lo, __shfl_xor(lo, off));
-    hi = fmaxf(hi, __shfl_xor(hi, off));
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    lo = fminf(lo, a[i]);
+    hi = fmaxf(hi, a[i]);
   }
-  if ((threadIdx.x & 63) == 0) {
+  lo = block_reduce(lo, MinOp());
+  hi = block_reduce(hi, MaxOp());
+  if (threadIdx.x == 0) {
     atomic_min_pos(&mm[0], lo);
     atomicMax((int*)&mm[1], __float_as_int(hi));
   }
@@ -95,14 +155,13 @@ __global__ void k_minmax(const float* __restrict__ a, int32_t n, float* mm) {
 // dt_i = cfl * hmin / (c_i xi), c_i = sqrt(|u| g (g-1))  (timestep/compressible.py:12-66)
 __global__ void k_cfl(const float* __restrict__ u, int32_t n, float gamma, float cfl, float xi,
                       const float* __restrict__ mm, float* out) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float d = 3.0e38f;
-  if (i < n) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float cs = sqrtf(fabsf(u[i]) * gamma * (gamma - 1.f));
-    d = cfl * mm[0] / (cs * xi);
+    d = fminf(d, cfl * mm[0] / (cs * xi));
   }
-  for (int off = 32; off > 0; off >>= 1) d = fminf(d, __shfl_xor(d, off));
-  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, d);
+  d = block_reduce(d, MinOp());
+  if (threadIdx.x == 0) atomic_min_pos(out, d);
 }
 
 // FP64 conservation partials: per block [m, m v (D), m u, m |v|^2 / 2]
@@ -211,27 +270,57 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   CRK_REQUIRE(adj0->n_edges == 0 || (e0->ap && e0->av && e0->f), "stage-0 edges missing");
   CRK_REQUIRE(adj1->n_edges == 0 || (e1->ap && e1->av && e1->f), "stage-1 edges missing");
   int D = cfg->domain.dim;
+  CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
   hipStream_t s = (hipStream_t)stream;
   int st;
   if ((st = ensure(ctx->vh, sizeof(float) * (size_t)n * D))) return st;
+  if ((st = ensure(ctx->vhs, sizeof(float) * (size_t)n * D))) return st;
   if ((st = ensure(ctx->du0, sizeof(float) * (size_t)n))) return st;
   float* vh = (float*)ctx->vh.p;
+  float* vhs = (float*)ctx->vhs.p;
   float* du0 = (float*)ctx->du0.p;
   int64_t cnt = (int64_t)n * D;
   hipLaunchKernelGGL(k_vhalf, dim3(nblocks(cnt)), dim3(kBlock), 0, s, cnt, v0, dvdt0, dvdt1, b0, b1, dt, vh);
   CRK_LAUNCH_CHECK("k_vhalf");
-#define COMPAT(DD)                                                                                             \
-  hipLaunchKernelGGL((k_compat<DD, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, adj0->perm, adj0->row_offsets, \
-                     adj0->nbr, n, vh, e0->ap, e0->av, e0->f, b0, du0, u0, dt, u_out, dudt_last);              \
-  hipLaunchKernelGGL((k_compat<DD, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, adj1->perm, adj1->row_offsets,  \
-                     adj1->nbr, n, vh, e1->ap, e1->av, e1->f, b1, du0, u0, dt, u_out, dudt_last);
-  switch (D) {
-    case 1: COMPAT(1); break;
-    case 2: COMPAT(2); break;
-    case 3: COMPAT(3); break;
-    default: set_error("dim must be 1..3"); return CRK_EINVAL;
+  const bool shared = adj0->perm == adj1->perm && adj0->row_offsets == adj1->row_offsets && adj0->nbr == adj1->nbr &&
+                      adj0->n_edges == adj1->n_edges;
+  CompIn c0{e0->ap, e0->av, e0->f, b0}, c1{e1->ap, e1->av, e1->f, b1};
+  const int GS = ctx->group[CRK_PASS_COMPAT];
+  unsigned nb = nblocks((int64_t)n * GS);
+#define GATHER(DD, A) hipLaunchKernelGGL(k_gather_vec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, (A)->perm, n, vh, vhs)
+#define COMPAT(DD, GG, NST, MODE, A, X, Y)                                                                       \
+  hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, (A)->row_offsets,    \
+                     (A)->nbr, n, vhs, X, Y, du0, u0, dt, u_out, dudt_last)
+#define RUN(DD, GG)                                          \
+  if (shared) {                                              \
+    GATHER(DD, adj0);                                        \
+    COMPAT(DD, GG, 2, 2, adj0, c0, c1);                      \
+  } else {                                                   \
+    GATHER(DD, adj0);                                        \
+    COMPAT(DD, GG, 1, 0, adj0, c0, c0);                      \
+    GATHER(DD, adj1);                                        \
+    COMPAT(DD, GG, 1, 1, adj1, c1, c1);                      \
   }
+#define RUNG(DD)                                  \
+  switch (GS) {                                   \
+    case 1: RUN(DD, 1) break;                     \
+    case 2: RUN(DD, 2) break;                     \
+    case 4: RUN(DD, 4) break;                     \
+    case 8: RUN(DD, 8) break;                     \
+    case 16: RUN(DD, 16) break;                   \
+    case 32: RUN(DD, 32) break;                   \
+    case 64: RUN(DD, 64) break;                   \
+    default: set_error("bad group size %d", GS); return CRK_EINVAL; \
+  }
+  switch (D) {
+    case 1: RUNG(1) break;
+    case 2: RUNG(2) break;
+    case 3: RUNG(3) break;
+  }
+#undef RUNG
+#undef RUN
 #undef COMPAT
+#undef GATHER
   CRK_LAUNCH_CHECK("k_compat");
   return CRK_OK;
 }
@@ -240,7 +329,7 @@ int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* mm, void*
   CRK_REQUIRE(ctx && values && mm && n > 0, "crk_minmax: invalid arguments");
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, mm);
-  hipLaunchKernelGGL(k_minmax, dim3(nblocks(n)), dim3(kBlock), 0, s, values, n, mm);
+  hipLaunchKernelGGL(k_minmax, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, values, n, mm);
   CRK_LAUNCH_CHECK("k_minmax");
   return CRK_OK;
 }
@@ -255,7 +344,7 @@ int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* su
   if ((st = crk_minmax(ctx, n, supports, red, stream))) return st;
   if ((st = fill_f32(red + 2, 1, 3.0e38f, s))) return st;
   float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
-  hipLaunchKernelGGL(k_cfl, dim3(nblocks(n)), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, red, red + 2);
+  hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, red, red + 2);
   CRK_LAUNCH_CHECK("k_cfl");
   float* hostv = (float*)ctx->pinned.p;
   CRK_HIP(hipMemcpyAsync(hostv, red + 2, sizeof(float), hipMemcpyDeviceToHost, s));

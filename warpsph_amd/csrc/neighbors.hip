@@ -203,28 +203,25 @@ __global__ void k_neighbors(GridDev g, CellScan sc, Geo geo, float s, const floa
 }
 
 // validity: red[0] = max |x - xb|^2, red[1] = max h/hb, red[2] = min hb
+// (grid-stride, one atomic per block)
 template <int D>
 __global__ void k_valid(Geo geo, const int32_t* __restrict__ perm, const float* __restrict__ xb,
                         const float* __restrict__ hb, const float* __restrict__ x, const float* __restrict__ h,
                         int32_t n, float* red) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f, g = 0.f, hm = 3.0e38f;
-  if (k < n) {
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     int32_t i = perm[k];
     float xi[3];
 #pragma unroll
     for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)i * D + d];
-    d2 = exact_r2<D>(geo, xi, &xb[(int64_t)k * D]);
-    g = __fdiv_rn(h[i], hb[k]);
-    hm = hb[k];
+    d2 = fmaxf(d2, exact_r2<D>(geo, xi, &xb[(int64_t)k * D]));
+    g = fmaxf(g, __fdiv_rn(h[i], hb[k]));
+    hm = fminf(hm, hb[k]);
   }
-  // wave reductions
-  for (int off = 32; off > 0; off >>= 1) {
-    d2 = fmaxf(d2, __shfl_xor(d2, off));
-    g = fmaxf(g, __shfl_xor(g, off));
-    hm = fminf(hm, __shfl_xor(hm, off));
-  }
-  if ((threadIdx.x & 63) == 0) {
+  d2 = block_reduce(d2, MaxOp());
+  g = block_reduce(g, MaxOp());
+  hm = block_reduce(hm, MinOp());
+  if (threadIdx.x == 0) {
     atomic_max_pos(&red[0], d2);
     atomic_max_pos(&red[1], g);
     atomic_min_pos(&red[2], hm);
@@ -380,9 +377,9 @@ int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const 
   hipLaunchKernelGGL(k_valid_init, dim3(1), dim3(1), 0, s, red);
   Geo geo = make_geo(*cfg);
   switch (cfg->domain.dim) {
-    case 1: hipLaunchKernelGGL(k_valid<1>, dim3(nblocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
-    case 2: hipLaunchKernelGGL(k_valid<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
-    case 3: hipLaunchKernelGGL(k_valid<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
+    case 1: hipLaunchKernelGGL(k_valid<1>, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
+    case 2: hipLaunchKernelGGL(k_valid<2>, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
+    case 3: hipLaunchKernelGGL(k_valid<3>, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
     default: set_error("dim must be 1..3"); return CRK_EINVAL;
   }
   CRK_LAUNCH_CHECK("k_valid");

@@ -1,10 +1,17 @@
 // passes.hip -- the pair passes of one CRKSPH derivative evaluation
 // (src/warpSPH/schemes/crkSPH.py:19-366), run on Morton-sorted copies of the
-// state held by the context.  One thread owns one sorted particle row; pair
-// sums are Kahan-compensated FP32, the CRK moment solve is FP64.
+// state held by the context.
+//
+// Work mapping: a group of G consecutive lanes of a wave64 owns one sorted
+// particle row of the CSR adjacency; lane g handles edges off[k]+g, +G, ...
+// so edge arrays (neighbour ids, ap, av, f) stream coalesced.  Each lane
+// keeps Kahan-compensated FP32 partial sums; the G partials are combined by
+// an exact TwoSum butterfly (deterministic, every lane ends with the same
+// value).  The CRK moment solve runs in FP64.  G per pass is chosen by the
+// autotuner (crk_set_group_size).
 //
 // Pass order inside a stage:
-//   crk_stage_begin      gather x, v, h, m, u, alpha0, divergence to sorted order
+//   crk_stage_begin      gather x, v, h, m, u, alpha0, alpha, divergence to sorted order
 //   crk_owen_iteration   psi0 -> n_h -> new h           (optimalSupportOwen.py:70-114)
 //   crk_factors_eos      V, then moments -> A, B, gradA, gradB (FP64 solve),
 //                        rho = A (S0 + B.S1), EOS      (computeCRKFactors; idealGas.py)
@@ -19,10 +26,135 @@
 
 namespace crk {
 
-struct Rows {
-  const int32_t* off;
-  const int32_t* nbr;
+// ------------------------------------------------------------ group helpers
+struct Comp {  // compensated value s - c
+  float s = 0.f, c = 0.f;
+  __device__ inline void add(float y) {
+    float yy = y - c;
+    float t = s + yy;
+    c = (t - s) - yy;
+    s = t;
+  }
 };
+
+template <int G>
+__device__ inline float group_total(Comp a) {
+  float s = a.s, c = a.c;
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    float so = __shfl_xor(s, off, G);
+    float co = __shfl_xor(c, off, G);
+    float t = s + so;
+    float bp = t - s;
+    float e = (s - (t - bp)) + (so - bp);
+    s = t;
+    c = (c + co) - e;
+  }
+  return s - c;
+}
+
+template <int G>
+__device__ inline float group_max(float v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, G));
+  return v;
+}
+
+struct Lane {
+  int g;       // lane inside the group
+  int32_t k;   // row (sorted particle)
+  bool valid;
+};
+template <int G>
+__device__ inline Lane lane_of(int32_t n) {
+  Lane l;
+  l.g = threadIdx.x % G;
+  l.k = (int32_t)blockIdx.x * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
+  l.valid = l.k < n;
+  if (!l.valid) l.k = n - 1;  // safe index for loads; results discarded
+  return l;
+}
+
+// ------------------------------------------------------------ kernel functions
+// k(q) and dk/dq for 0 <= q < 1 with shared subexpressions.
+template <int D>
+__device__ inline void kwd(int kernel, float q, float& k, float& dk) {
+  float t = 1.f - q;
+  float t2 = t * t;
+  if (kernel == CRK_KERNEL_WENDLAND2) {
+    if (D == 1) { k = t2 * t * (1.f + 3.f * q); dk = -12.f * q * t2; }
+    else { float t3 = t2 * t; k = t3 * t * (1.f + 4.f * q); dk = -20.f * q * t3; }
+  } else if (kernel == CRK_KERNEL_WENDLAND4) {
+    float t4 = t2 * t2;
+    if (D == 1) { k = t4 * t * (1.f + 5.f * q + 8.f * q * q); dk = -14.f * q * t4 * (1.f + 4.f * q); }
+    else { float t5 = t4 * t; k = t5 * t * (1.f + 6.f * q + (35.f / 3.f) * q * q); dk = (-56.f / 3.f) * q * t5 * (1.f + 5.f * q); }
+  } else {
+    float t4 = t2 * t2;
+    if (D == 1) { float t6 = t4 * t2; k = t6 * t * (1.f + 7.f * q + 19.f * q * q + 21.f * q * q * q); dk = -6.f * q * t6 * (3.f + 18.f * q + 35.f * q * q); }
+    else { float t7 = t4 * t2 * t; k = t7 * t * (1.f + 8.f * q + 25.f * q * q + 32.f * q * q * q); dk = -22.f * q * t7 * (1.f + 7.f * q + 16.f * q * q); }
+  }
+}
+
+template <int D>
+__device__ inline float ipow(float x) {
+  float r = x;
+#pragma unroll
+  for (int k = 1; k < D; ++k) r = r * x;
+  return r;
+}
+
+// W(r, h) and dW/dr / r (for the gradient W' x / r) with 1/h given.
+template <int D>
+struct WdW {
+  float W, fac;  // gradient = fac * x_ab
+};
+template <int D>
+__device__ inline WdW<D> wdw(int kernel, float C, float r, float invr, float invh) {
+  WdW<D> o;
+  float q = r * invh;
+  if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
+  float k, dk;
+  kwd<D>(kernel, q, k, dk);
+  float ihD = ipow<D>(invh);
+  o.W = C * k * ihD;
+  o.fac = C * dk * ihD * invh * invr;  // r > 0 implied by dk(0) = 0 and invr finite
+  return o;
+}
+
+__device__ inline float safe_inv(float r) { return r > 0.f ? 1.f / r : 0.f; }
+
+// CRK-corrected gradient d_g W^R_ab (Frontiere, Raskin & Owen 2017):
+// gA (1 + B.x) W + A (B_g + gB^T x) W + A (1 + B.x) gradW
+template <int D>
+struct CorrD {
+  float A, B[D], gA[D], gB[D * D];
+};
+template <int D>
+__device__ inline CorrD<D> load_corr(const float* Ap, const float* Bp, const float* gAp, const float* gBp, int32_t k) {
+  CorrD<D> c;
+  c.A = Ap[k];
+#pragma unroll
+  for (int a = 0; a < D; ++a) { c.B[a] = Bp[(int64_t)k * D + a]; c.gA[a] = gAp[(int64_t)k * D + a]; }
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) c.gB[a] = gBp[(int64_t)k * D * D + a];
+  return c;
+}
+template <int D>
+__device__ inline void grad_wr(const CorrD<D>& cr, const float* x, const WdW<D>& w, float* out) {
+  float bx = 1.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) bx = bx + cr.B[a] * x[a];
+  float AW = cr.A * w.W;
+  float ABf = cr.A * bx * w.fac;
+  float bxW = bx * w.W;
+#pragma unroll
+  for (int gm = 0; gm < D; ++gm) {
+    float gbx = cr.B[gm];
+#pragma unroll
+    for (int a = 0; a < D; ++a) gbx = gbx + cr.gB[a * D + gm] * x[a];
+    out[gm] = cr.gA[gm] * bxW + AW * gbx + ABf * x[gm];
+  }
+}
 
 // ------------------------------------------------------------ gather
 template <int D>
@@ -47,6 +179,11 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
   alpha[k] = AL ? AL[i] : 1.f;
 }
 
+struct Rows {
+  const int32_t* off;
+  const int32_t* nbr;
+};
+
 // ------------------------------------------------------------ Owen
 __device__ inline double lut_interp(const double* __restrict__ xs, const double* __restrict__ ys, int n, double x) {
   int lo = 0, hi = n;  // upper_bound
@@ -63,59 +200,77 @@ __device__ inline double lut_interp(const double* __restrict__ xs, const double*
   return ys[il] * (1.0 - a) + ys[ir] * a;
 }
 
-template <int D>
-__global__ void k_owen(Rows rows, Geo geo, int kernel, float C, int32_t n, const float* __restrict__ x,
-                       float* __restrict__ h, const int32_t* __restrict__ perm, float* __restrict__ supports,
-                       const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh, int nlut, float nhT,
-                       const float* __restrict__ hminmax, float* __restrict__ ratio) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, int kernel, float C, int32_t n,
+                                                 const float* __restrict__ x, float* __restrict__ h,
+                                                 const int32_t* __restrict__ perm, float* __restrict__ supports,
+                                                 const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh,
+                                                 int nlut, float nhT, const float* __restrict__ hminmax,
+                                                 float* __restrict__ ratio) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float xi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
-  float hi = h[k];
-  Kahan acc;
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  const float hi = h[k];
+  const float ih = 1.f / hi;
+  Comp acc;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float dx[3];
     min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float r = sqrtf(dot<D>(dx, dx));
-    float q = r / hi;
-    if (!(q < 1.f)) continue;
-    acc.add(C * fabsf(kder<float>(kernel, D, q)));
+    float qq = sqrtf(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float kk, dk;
+    kwd<D>(kernel, qq, kk, dk);
+    acc.add(C * fabsf(dk));
   }
-  float ps = powf(acc.s, 1.f / (float)D);
-  float nh = (float)lut_interp(lut_psiH, lut_nh, nlut, (double)ps);
-  float sr = nhT / nh;
-  float a = sr >= 1.f ? 0.4f * (1.f + 1.f / (sr * sr * sr)) : 0.4f * (1.f + sr * sr);
-  float hn = (1.f - a + a * sr) * hi;
-  float lo = hminmax[0] * 0.25f, up = hminmax[1] * 4.0f;
-  hn = hn < lo ? lo : (hn > up ? up : hn);
-  h[k] = hn;
-  supports[perm[k]] = hn;
-  if (ratio) atomic_max_pos(ratio, fabsf(hn / hi - 1.f));
+  float S = group_total<G>(acc);
+  float rmax = 0.f;
+  if (L.valid && L.g == 0) {
+    float ps = powf(S, 1.f / (float)D);
+    float nh = (float)lut_interp(lut_psiH, lut_nh, nlut, (double)ps);
+    float sr = nhT / nh;
+    float a = sr >= 1.f ? 0.4f * (1.f + 1.f / (sr * sr * sr)) : 0.4f * (1.f + sr * sr);
+    float hn = (1.f - a + a * sr) * hi;
+    float lo = hminmax[0] * 0.25f, up = hminmax[1] * 4.0f;
+    hn = hn < lo ? lo : (hn > up ? up : hn);
+    h[k] = hn;
+    supports[perm[k]] = hn;
+    rmax = fabsf(hn / hi - 1.f);
+  }
+  if (ratio) {  // max |h_new / h - 1|, one atomic per block
+    rmax = block_reduce(rmax, MaxOp());
+    if (threadIdx.x == 0) atomic_max_pos(ratio, rmax);
+  }
 }
 
 // ------------------------------------------------------------ CRK volume
-template <int D>
-__global__ void k_volume(Rows rows, Geo geo, DevKernel<D> K, int32_t n, const float* __restrict__ x,
-                         const float* __restrict__ h, float* __restrict__ V) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_volume(Rows rows, Geo geo, int kernel, float C, int32_t n,
+                                                   const float* __restrict__ x, const float* __restrict__ h,
+                                                   float* __restrict__ V) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float xi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
-  float hi = h[k];
-  Kahan acc;
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  const float ih = 1.f / h[k];
+  Comp acc;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float dx[3];
     min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float r = sqrtf(dot<D>(dx, dx));
-    if (!(r < hi)) continue;
-    acc.add(K.W(r, hi));
+    float qq = sqrtf(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float kk, dk;
+    kwd<D>(kernel, qq, kk, dk);
+    acc.add(C * kk * ipow<D>(ih));
   }
-  V[k] = 1.f / acc.s;
+  float S = group_total<G>(acc);
+  if (L.valid && L.g == 0) V[k] = 1.f / S;
 }
 
 // ------------------------------------------------------------ CRK moments
@@ -124,100 +279,111 @@ __global__ void k_volume(Rows rows, Geo geo, DevKernel<D> K, int32_t n, const fl
 //   B = -m2^-1 m1, A = 1/(m0 + B.m1), grad A = -A^2 (dm0 + 2 B.dm1 + B.dm2.B),
 //   grad B = -m2^-1 (dm1 + dm2 B); singular m2 -> A = 1/m0, B = 0.
 // CRK density rho = A (S0 + B.S1), S0 = sum m_j W, S1 = sum m_j x W; then the
-// ideal-gas EOS with u given (eos/idealGas.py:6-38).
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, DevKernel<D> K, int32_t n,
+// ideal-gas EOS with u given (eos/idealGas.py:6-38) and s = P / rho^gamma
+// (the entropy function the energy partition compares, compSPH/balance.py:136-137).
+struct MomOut {
+  float *A, *B, *gA, *gB, *rho, *P, *c, *sP;
+  float *dens_u, *ent_u, *pres_u, *cs_u;
+  int32_t* nsing;
+};
+
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, int kernel, float C, int32_t n,
                                                     const float* __restrict__ x, const float* __restrict__ h,
                                                     const float* __restrict__ V, const float* __restrict__ m,
-                                                    const float* __restrict__ u, float gamma, float* __restrict__ Ao,
-                                                    float* __restrict__ Bo, float* __restrict__ gAo,
-                                                    float* __restrict__ gBo, float* __restrict__ rho,
-                                                    float* __restrict__ Po, float* __restrict__ co,
-                                                    const int32_t* __restrict__ perm, float* __restrict__ dens_u,
-                                                    float* __restrict__ ent_u, float* __restrict__ pres_u,
-                                                    float* __restrict__ cs_u, int32_t* __restrict__ nsing) {
+                                                    const float* __restrict__ u, float gamma,
+                                                    const int32_t* __restrict__ perm, MomOut o) {
   constexpr int NM2 = D * (D + 1) / 2;
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float xi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
-  Kahan m0, m1[D], m2[NM2], dm0[D], dm1[D * D], dm2[NM2 * D], S0, S1[D];
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  Comp m0, m1[D], m2[NM2], dm0[D], dm1[D * D], dm2[NM2 * D], S0, S1[D];
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float dx[3];
     min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
     float r = sqrtf(dot<D>(dx, dx));
     float hj = h[j];
     if (!(r < hj)) continue;
-    float W = K.W(r, hj);
+    WdW<D> w = wdw<D>(kernel, C, r, safe_inv(r), 1.f / hj);
     float g[3];
-    K.gradW(dx, r, hj, g);
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = w.fac * dx[d];
     float Vj = V[j];
     float mj = m[j];
-    m0.add(Vj * W);
+    float VW = Vj * w.W;
+    m0.add(VW);
 #pragma unroll
-    for (int a = 0; a < D; ++a) m1[a].add(Vj * dx[a] * W);
+    for (int a = 0; a < D; ++a) m1[a].add(VW * dx[a]);
     {
       int p = 0;
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int b = a; b < D; ++b, ++p) m2[p].add(Vj * dx[a] * dx[b] * W);
+        for (int bb = a; bb < D; ++bb, ++p) m2[p].add(VW * dx[a] * dx[bb]);
     }
 #pragma unroll
     for (int gm = 0; gm < D; ++gm) dm0[gm].add(Vj * g[gm]);
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int gm = 0; gm < D; ++gm) dm1[a * D + gm].add(Vj * (dx[a] * g[gm] + (a == gm ? W : 0.f)));
+      for (int gm = 0; gm < D; ++gm) dm1[a * D + gm].add(Vj * (dx[a] * g[gm] + (a == gm ? w.W : 0.f)));
     {
       int p = 0;
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int b = a; b < D; ++b, ++p)
+        for (int bb = a; bb < D; ++bb, ++p)
 #pragma unroll
           for (int gm = 0; gm < D; ++gm) {
-            float lin = (a == gm ? dx[b] : 0.f) + (b == gm ? dx[a] : 0.f);
-            dm2[p * D + gm].add(Vj * (dx[a] * dx[b] * g[gm] + lin * W));
+            float lin = (a == gm ? dx[bb] : 0.f) + (bb == gm ? dx[a] : 0.f);
+            dm2[p * D + gm].add(Vj * (dx[a] * dx[bb] * g[gm] + lin * w.W));
           }
     }
-    S0.add(mj * W);
+    float mW = mj * w.W;
+    S0.add(mW);
 #pragma unroll
-    for (int a = 0; a < D; ++a) S1[a].add(mj * dx[a] * W);
+    for (int a = 0; a < D; ++a) S1[a].add(mW * dx[a]);
   }
-  // FP64 solve
-  double M0 = m0.s, M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];
+  // combine partial sums across the group
+  double M0 = group_total<G>(m0), M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];
 #pragma unroll
-  for (int a = 0; a < D; ++a) M1[a] = m1[a].s;
+  for (int a = 0; a < D; ++a) M1[a] = group_total<G>(m1[a]);
   {
     int p = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int b = a; b < D; ++b, ++p) { M2[a][b] = m2[p].s; M2[b][a] = m2[p].s; }
+      for (int bb = a; bb < D; ++bb, ++p) { double t = group_total<G>(m2[p]); M2[a][bb] = t; M2[bb][a] = t; }
   }
 #pragma unroll
-  for (int gm = 0; gm < D; ++gm) DM0[gm] = dm0[gm].s;
+  for (int gm = 0; gm < D; ++gm) DM0[gm] = group_total<G>(dm0[gm]);
 #pragma unroll
   for (int a = 0; a < D; ++a)
 #pragma unroll
-    for (int gm = 0; gm < D; ++gm) DM1[a][gm] = dm1[a * D + gm].s;
+    for (int gm = 0; gm < D; ++gm) DM1[a][gm] = group_total<G>(dm1[a * D + gm]);
   {
     int p = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int b = a; b < D; ++b, ++p)
+      for (int bb = a; bb < D; ++bb, ++p)
 #pragma unroll
-        for (int gm = 0; gm < D; ++gm) { DM2[a][b][gm] = dm2[p * D + gm].s; DM2[b][a][gm] = dm2[p * D + gm].s; }
+        for (int gm = 0; gm < D; ++gm) { double t = group_total<G>(dm2[p * D + gm]); DM2[a][bb][gm] = t; DM2[bb][a][gm] = t; }
   }
+  float s0 = group_total<G>(S0), s1[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) s1[a] = group_total<G>(S1[a]);
+  if (!L.valid || L.g != 0) return;
+  // FP64 solve
   double inv[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) inv[a][b] = 0.0;
+    for (int bb = 0; bb < 3; ++bb) inv[a][bb] = 0.0;
   bool sing;
   if (D == 1) {
     sing = !(M2[0][0] > 0.0);
@@ -227,8 +393,9 @@ __global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, DevKerne
     double tr = 0.5 * (M2[0][0] + M2[1][1]);
     sing = !(det > 1e-10 * tr * tr);
     if (!sing) {
-      inv[0][0] = M2[1][1] / det; inv[1][1] = M2[0][0] / det;
-      inv[0][1] = -M2[0][1] / det; inv[1][0] = -M2[1][0] / det;
+      double id = 1.0 / det;
+      inv[0][0] = M2[1][1] * id; inv[1][1] = M2[0][0] * id;
+      inv[0][1] = -M2[0][1] * id; inv[1][0] = -M2[1][0] * id;
     }
   } else {
     double c00 = M2[1][1] * M2[2][2] - M2[1][2] * M2[2][1];
@@ -238,22 +405,23 @@ __global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, DevKerne
     double tr = (M2[0][0] + M2[1][1] + M2[2][2]) / 3.0;
     sing = !(det > 1e-10 * tr * tr * tr);
     if (!sing) {
-      inv[0][0] = c00 / det; inv[1][0] = c01 / det; inv[2][0] = c02 / det;
-      inv[0][1] = (M2[0][2] * M2[2][1] - M2[0][1] * M2[2][2]) / det;
-      inv[1][1] = (M2[0][0] * M2[2][2] - M2[0][2] * M2[2][0]) / det;
-      inv[2][1] = (M2[0][1] * M2[2][0] - M2[0][0] * M2[2][1]) / det;
-      inv[0][2] = (M2[0][1] * M2[1][2] - M2[0][2] * M2[1][1]) / det;
-      inv[1][2] = (M2[0][2] * M2[1][0] - M2[0][0] * M2[1][2]) / det;
-      inv[2][2] = (M2[0][0] * M2[1][1] - M2[0][1] * M2[1][0]) / det;
+      double id = 1.0 / det;
+      inv[0][0] = c00 * id; inv[1][0] = c01 * id; inv[2][0] = c02 * id;
+      inv[0][1] = (M2[0][2] * M2[2][1] - M2[0][1] * M2[2][2]) * id;
+      inv[1][1] = (M2[0][0] * M2[2][2] - M2[0][2] * M2[2][0]) * id;
+      inv[2][1] = (M2[0][1] * M2[2][0] - M2[0][0] * M2[2][1]) * id;
+      inv[0][2] = (M2[0][1] * M2[1][2] - M2[0][2] * M2[1][1]) * id;
+      inv[1][2] = (M2[0][2] * M2[1][0] - M2[0][0] * M2[1][2]) * id;
+      inv[2][2] = (M2[0][0] * M2[1][1] - M2[0][1] * M2[1][0]) * id;
     }
   }
   double Ad, Bd[3] = {0, 0, 0}, gAd[3] = {0, 0, 0}, gBd[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) gBd[a][b] = 0.0;
+    for (int bb = 0; bb < 3; ++bb) gBd[a][bb] = 0.0;
   if (sing) {
-    if (nsing) atomicAdd(nsing, 1);
+    if (o.nsing) atomicAdd(o.nsing, 1);
     Ad = 1.0 / M0;
 #pragma unroll
     for (int gm = 0; gm < D; ++gm) gAd[gm] = -DM0[gm] / (M0 * M0);
@@ -262,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, DevKerne
     for (int a = 0; a < D; ++a) {
       double s = 0.0;
 #pragma unroll
-      for (int b = 0; b < D; ++b) s += inv[a][b] * M1[b];
+      for (int bb = 0; bb < D; ++bb) s += inv[a][bb] * M1[bb];
       Bd[a] = -s;
     }
     double den = M0;
@@ -277,256 +445,239 @@ __global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, DevKerne
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int b = 0; b < D; ++b) dinvA += Bd[a] * DM2[a][b][gm] * Bd[b];
+        for (int bb = 0; bb < D; ++bb) dinvA += Bd[a] * DM2[a][bb][gm] * Bd[bb];
       gAd[gm] = -Ad * Ad * dinvA;
       double tmp[3];
 #pragma unroll
-      for (int b = 0; b < D; ++b) {
-        double t2 = DM1[b][gm];
+      for (int bb = 0; bb < D; ++bb) {
+        double t2 = DM1[bb][gm];
 #pragma unroll
-        for (int dd = 0; dd < D; ++dd) t2 += DM2[b][dd][gm] * Bd[dd];
-        tmp[b] = t2;
+        for (int dd = 0; dd < D; ++dd) t2 += DM2[bb][dd][gm] * Bd[dd];
+        tmp[bb] = t2;
       }
 #pragma unroll
       for (int a = 0; a < D; ++a) {
         double s = 0.0;
 #pragma unroll
-        for (int b = 0; b < D; ++b) s += inv[a][b] * tmp[b];
+        for (int bb = 0; bb < D; ++bb) s += inv[a][bb] * tmp[bb];
         gBd[a][gm] = -s;
       }
     }
   }
   float Af = (float)Ad;
   float Bf[3];
-  Ao[k] = Af;
+  o.A[k] = Af;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     Bf[a] = (float)Bd[a];
-    Bo[(int64_t)k * D + a] = Bf[a];
-    gAo[(int64_t)k * D + a] = (float)gAd[a];
+    o.B[(int64_t)k * D + a] = Bf[a];
+    o.gA[(int64_t)k * D + a] = (float)gAd[a];
   }
 #pragma unroll
   for (int a = 0; a < D; ++a)
 #pragma unroll
-    for (int gm = 0; gm < D; ++gm) gBo[(int64_t)k * D * D + a * D + gm] = (float)gBd[a][gm];
-  float bs = S0.s;
+    for (int gm = 0; gm < D; ++gm) o.gB[(int64_t)k * D * D + a * D + gm] = (float)gBd[a][gm];
+  float bs = s0;
 #pragma unroll
-  for (int a = 0; a < D; ++a) bs = bs + Bf[a] * S1[a].s;
+  for (int a = 0; a < D; ++a) bs = bs + Bf[a] * s1[a];
   float rh = Af * bs;
-  rho[k] = rh;
-  // EOS
+  o.rho[k] = rh;
   float ui = u[k];
   float cc = sqrtf(fabsf(ui) * gamma * (gamma - 1.f));
   float P = (gamma - 1.f) * rh * ui;
   float s = ui * (gamma - 1.f) * powf(rh, 1.f - gamma);
-  Po[k] = P;
-  co[k] = cc;
+  o.P[k] = P;
+  o.c[k] = cc;
+  o.sP[k] = P / powf(rh, gamma);
   int32_t i = perm[k];
-  dens_u[i] = rh;
-  pres_u[i] = P;
-  cs_u[i] = cc;
-  ent_u[i] = s;
+  o.dens_u[i] = rh;
+  o.pres_u[i] = P;
+  o.cs_u[i] = cc;
+  o.ent_u[i] = s;
 }
 
-// ------------------------------------------------------------ CRK gradient
-template <int D>
-__device__ inline void grad_wr(const DevKernel<D>& K, const float* xab, float r, float hh, float Aa,
-                               const float* Ba, const float* gAa, const float* gBa, float* out) {
-  float W = K.W(r, hh);
-  float g[3];
-  K.gradW(xab, r, hh, g);
-  float bx = 1.f;
-#pragma unroll
-  for (int a = 0; a < D; ++a) bx = bx + Ba[a] * xab[a];
-#pragma unroll
-  for (int gm = 0; gm < D; ++gm) {
-    float gbx = Ba[gm];
-#pragma unroll
-    for (int a = 0; a < D; ++a) gbx = gbx + gBa[a * D + gm] * xab[a];
-    out[gm] = gAa[gm] * bx * W + Aa * gbx * W + Aa * bx * g[gm];
-  }
-}
-
-template <int D>
-struct Corr {
-  float A, B[D], gA[D], gB[D * D];
-  __device__ inline void load(const float* Ap, const float* Bp, const float* gAp, const float* gBp, int32_t k) {
-    A = Ap[k];
-#pragma unroll
-    for (int a = 0; a < D; ++a) { B[a] = Bp[(int64_t)k * D + a]; gA[a] = gAp[(int64_t)k * D + a]; }
-#pragma unroll
-    for (int a = 0; a < D * D; ++a) gB[a] = gBp[(int64_t)k * D * D + a];
-  }
+// ------------------------------------------------------------ velocity gradient
+// J_i[a][g] = sum_j V_j (v_j - v_i)[a] d_g W^R_ij (scatter), fused with the
+// Cullen-Dehnen divergence (super-symmetric support max(h_i, h_j), m/rho
+// volumes) and the signal speed (every adjacency entry).
+struct VgIn {
+  const float *x, *h, *v, *V, *m, *rho, *c, *A, *B, *gA, *gB;
+};
+struct VgOut {
+  float *J, *divcd, *vsig, *drhodt_u;
 };
 
-// velocity gradient J_i[a][g] = sum_j V_j (v_j - v_i)[a] d_g W^R_ij (scatter),
-// fused with the Cullen-Dehnen divergence (super-symmetric support
-// max(h_i, h_j), m/rho volumes) and the signal speed (every adjacency entry).
-template <int D, bool CULLEN>
-__global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, DevKernel<D> K, int32_t n,
-                                                    const float* __restrict__ x, const float* __restrict__ h,
-                                                    const float* __restrict__ v, const float* __restrict__ V,
-                                                    const float* __restrict__ m, const float* __restrict__ rho,
-                                                    const float* __restrict__ c, const float* __restrict__ Ap,
-                                                    const float* __restrict__ Bp, const float* __restrict__ gAp,
-                                                    const float* __restrict__ gBp, float* __restrict__ J,
-                                                    float* __restrict__ divcd, float* __restrict__ vsig,
-                                                    const int32_t* __restrict__ perm, float* __restrict__ drhodt_u) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+template <int D, int G, bool CULLEN>
+__global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, int kernel, float C, int32_t n, VgIn in,
+                                                    const int32_t* __restrict__ perm, VgOut o) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float xi[3], vi[3];
 #pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = x[(int64_t)k * D + d]; vi[d] = v[(int64_t)k * D + d]; }
-  float hi = h[k];
-  float ci = c[k];
-  Corr<D> cr;
-  cr.load(Ap, Bp, gAp, gBp, k);
-  Kahan acc[D * D];
-  Kahan dacc;
+  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
+  const float hi = in.h[k];
+  const float ci = in.c[k];
+  CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
+  Comp acc[D * D];
+  Comp dacc;
   float vs = 0.f;
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float dx[3];
-    min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
+    min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     float r = sqrtf(dot<D>(dx, dx));
+    float ir = safe_inv(r);
     float vj[3];
 #pragma unroll
-    for (int d = 0; d < D; ++d) vj[d] = v[(int64_t)j * D + d];
-    float hj = h[j];
+    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
+    float hj = in.h[j];
     if (CULLEN) {
       float mu = 0.f;
 #pragma unroll
       for (int a = 0; a < D; ++a) mu = mu + (vi[a] - vj[a]) * dx[a];
       mu = mu / (r + 1.0e-14f * hi);
-      float cb = 0.5f * (ci + c[j]);
+      float cb = 0.5f * (ci + in.c[j]);
       float vv = cb - mu;
       if (mu > 0.f) vv = 0.f;
       vs = vv > vs ? vv : vs;
       float H = fmaxf(hi, hj);
       if (r < H) {
-        float g[3];
-        K.gradW(dx, r, H, g);
+        WdW<D> w = wdw<D>(kernel, C, r, ir, 1.f / H);
         float dt_ = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * g[a];
-        dacc.add(m[j] / rho[j] * dt_);
+        for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * dx[a];
+        dacc.add(in.m[j] / in.rho[j] * (w.fac * dt_));
       }
     }
     if (!(r < hj)) continue;
+    WdW<D> w = wdw<D>(kernel, C, r, ir, 1.f / hj);
     float g[3];
-    grad_wr<D>(K, dx, r, hj, cr.A, cr.B, cr.gA, cr.gB, g);
-    float Vj = V[j];
+    grad_wr<D>(cr, dx, w, g);
+    float Vj = in.V[j];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float dv = vj[a] - vi[a];
+      float dv = Vj * (vj[a] - vi[a]);
 #pragma unroll
-      for (int gm = 0; gm < D; ++gm) acc[a * D + gm].add(Vj * dv * g[gm]);
+      for (int gm = 0; gm < D; ++gm) acc[a * D + gm].add(dv * g[gm]);
     }
   }
+  float Jt[D * D];
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) Jt[a] = group_total<G>(acc[a]);
+  float dv = 0.f;
+  if (CULLEN) {
+    dv = group_total<G>(dacc);
+    vs = group_max<G>(vs);
+  }
+  if (!L.valid || L.g != 0) return;
   float tr = 0.f;
 #pragma unroll
-  for (int a = 0; a < D * D; ++a) J[(int64_t)k * D * D + a] = acc[a].s;
+  for (int a = 0; a < D * D; ++a) o.J[(int64_t)k * D * D + a] = Jt[a];
 #pragma unroll
-  for (int a = 0; a < D; ++a) tr = tr + acc[a * D + a].s;
-  drhodt_u[perm[k]] = -tr * rho[k];
+  for (int a = 0; a < D; ++a) tr = tr + Jt[a * D + a];
+  o.drhodt_u[perm[k]] = -tr * in.rho[k];
   if (CULLEN) {
-    divcd[k] = dacc.s;
-    vsig[k] = vs;
+    o.divcd[k] = dv;
+    o.vsig[k] = vs;
   }
 }
 
-// Cullen & Dehnen (2010) as in CullenDehnen2010.py:218-331: R (gather
-// interpolation of rho sign(div)), Xi, A = Xi max(0, -ddiv/dt), local alpha,
-// and the alpha0 relaxation.
-template <int D>
-__global__ void k_cullen(Rows rows, Geo geo, DevKernel<D> K, int32_t n, const float* __restrict__ x,
-                         const float* __restrict__ h, const float* __restrict__ m, const float* __restrict__ rho,
-                         const float* __restrict__ divcd, const float* __restrict__ vsig,
-                         const float* __restrict__ divp, float* __restrict__ a0, float* __restrict__ alpha,
-                         float* __restrict__ Rcd, float dt, float fkern, float amin, float amax, float beta,
-                         int limitXi, int evolve, const int32_t* __restrict__ perm, float* __restrict__ alphas_u,
-                         float* __restrict__ alpha0s_u) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+// ------------------------------------------------------------ Cullen-Dehnen
+// CullenDehnen2010.py:218-331: R (gather interpolation of rho sign(div)), Xi,
+// A = Xi max(0, -ddiv/dt), local alpha, and the alpha0 relaxation.
+struct CdIn {
+  const float *x, *h, *m, *rho, *divcd, *vsig, *divp;
+};
+struct CdOut {
+  float *a0, *alpha, *Rcd, *alphas_u, *alpha0s_u;
+};
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_cullen(Rows rows, Geo geo, int kernel, float C, int32_t n, CdIn in,
+                                                   float dt, float fkern, float amin, float amax, float beta,
+                                                   int limitXi, int evolve, const int32_t* __restrict__ perm,
+                                                   CdOut o) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float xi[3];
 #pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
-  float hi = h[k];
-  Kahan acc;
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  for (int d = 0; d < D; ++d) xi[d] = in.x[(int64_t)k * D + d];
+  const float hi = in.h[k];
+  const float ih = 1.f / hi;
+  Comp acc;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float dx[3];
-    min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float r = sqrtf(dot<D>(dx, dx));
-    if (!(r < hi)) continue;
-    float dj = divcd[j];
+    min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    float qq = sqrtf(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float dj = in.divcd[j];
     float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);
-    float term = rho[j] * sg;
-    acc.add(m[j] / rho[j] * term * K.W(r, hi));
+    float kk, dk;
+    kwd<D>(kernel, qq, kk, dk);
+    acc.add(in.m[j] * sg * (C * kk * ipow<D>(ih)));
   }
-  float Rv = 1.f / rho[k] * acc.s;
-  Rcd[k] = Rv;
+  float S = group_total<G>(acc);
+  if (!L.valid || L.g != 0) return;
+  float Rv = S / in.rho[k];
+  o.Rcd[k] = Rv;
+  float dk_ = in.divcd[k];
   float t1 = 1.f - Rv;
   float t2 = t1 * t1;
-  float nom = beta * (t2 * t2) * divcd[k];
+  float nom = beta * (t2 * t2) * dk_;
   float Xi = limitXi ? (nom * nom + 1e-14f * hi) / (nom * nom + 0.f + 1e-14f * hi)
                      : (nom * nom) / (nom * nom + 0.f + 1e-14f * hi);
-  float ddiv = (divcd[k] - divp[k]) / dt;
+  float ddiv = (dk_ - in.divp[k]) / dt;
   float md = -ddiv;
   float Ai = Xi * (md > 0.f ? md : 0.f);
-  float vsg = fabsf(vsig[k]);
+  float vsg = fabsf(in.vsig[k]);
   float scaling = hi * hi * Ai / (vsg * vsg + hi * hi * Ai + 1e-14f * hi);
   float al = amax * scaling;
   al = al < amin ? amin : (al > amax ? amax : al);
   float tau = hi * fkern / (0.1f * vsg);
-  float a0k = a0[k];
+  float a0k = o.a0[k];
   float adot = (al - a0k) / tau;
   float na = al > a0k ? al : a0k;
   na = al < na ? na + adot * dt : na;
   na = na < amin ? amin : (na > amax ? amax : na);
-  alpha[k] = na;
+  o.alpha[k] = na;
   int32_t i = perm[k];
-  alphas_u[i] = na;
-  if (evolve) { a0[k] = na; if (alpha0s_u) alpha0s_u[i] = na; }
+  o.alphas_u[i] = na;
+  if (evolve) { o.a0[k] = na; if (o.alpha0s_u) o.alpha0s_u[i] = na; }
 }
 
 // ------------------------------------------------------------ accel + dudt
 struct LimParams {
-  float eta_fold, eta_crit;
+  float eta_fold, inv_eta_fold, eta_crit;
   int crk, vl, off, on;
 };
 
+// limiter.py:11-16 (van Leer), :28-86 (computeVanLeer), :90-127 (crkLimiter)
 template <int D>
-__device__ inline float limiter_phi(const LimParams& lp, const float* dx, float hi, float hj, const float* Ji,
-                                    const float* Jj) {
+__device__ inline float limiter_phi(const LimParams& lp, const float* dx, float r, float ihi, float ihj,
+                                    const float* Ji, const float* Jj) {
   float factor = 1.f;
   if (lp.crk) {
-    float ei = 0.f, ej = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) { float t1 = dx[a] / hi; float t2 = dx[a] / hj; ei = ei + t1 * t1; ej = ej + t2 * t2; }
-    ei = sqrtf(ei);
-    ej = sqrtf(ej);
-    float eta = ei < ej ? ei : ej;
-    if (eta < lp.eta_crit) { float t = (eta - lp.eta_crit) / lp.eta_fold; factor = expf(-(t * t)); }
+    float eta = r * fminf(ihi, ihj);
+    if (eta < lp.eta_crit) { float t = (eta - lp.eta_crit) * lp.inv_eta_fold; factor = __expf(-(t * t)); }
   }
   float phi = 0.f;
   if (lp.vl) {
-    float xt[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) xt[a] = 0.5f * dx[a];
+    // g_i = (J_i x/2) . x/2
     float gi = 0.f, gj = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       float ci = 0.f, cj = 0.f;
 #pragma unroll
-      for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * xt[gm]; cj = cj + Jj[a * D + gm] * xt[gm]; }
-      gi = gi + ci * xt[a];
-      gj = gj + cj * xt[a];
+      for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * dx[gm]; cj = cj + Jj[a * D + gm] * dx[gm]; }
+      gi = gi + ci * dx[a];
+      gj = gj + cj * dx[a];
     }
-    float sj = gj >= 0.f ? 1.f : -1.f;
-    float si = gi >= 0.f ? 1.f : -1.f;
-    float ri = gi / (sj * fabsf(gj));
-    float rj = gj / (si * fabsf(gi));
+    gi = 0.25f * gi;
+    gj = 0.25f * gj;
+    float ri = gi / gj;  // sgn(g)|g| = g, sgn(0) = +1
+    float rj = gj / gi;
     if (ri != ri) ri = 1.f;
     if (rj != rj) rj = 1.f;
     float rr = ri < rj ? ri : rj;
@@ -541,48 +692,55 @@ __device__ inline float limiter_phi(const LimParams& lp, const float* dx, float 
   return phi;
 }
 
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_accel_dudt(
-    Rows rows, Geo geo, DevKernel<D> K, LimParams lp, float kscale, float Cl, float Cq, int alphaQ, int32_t n,
-    const float* __restrict__ x, const float* __restrict__ h, const float* __restrict__ v,
-    const float* __restrict__ m, const float* __restrict__ V, const float* __restrict__ rho,
-    const float* __restrict__ P, const float* __restrict__ c, const float* __restrict__ J,
-    const float* __restrict__ Ap, const float* __restrict__ Bp, const float* __restrict__ gAp,
-    const float* __restrict__ gBp, const float* __restrict__ alpha, float* __restrict__ ap,
-    float* __restrict__ av, float* __restrict__ dvdt_s, const int32_t* __restrict__ perm,
-    float* __restrict__ dvdt_u, float* __restrict__ dudt_u, float* __restrict__ dEdt_u,
-    const float* __restrict__ drhodt_u, float* __restrict__ div_u) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+struct AdIn {
+  const float *x, *h, *v, *m, *V, *rho, *P, *c, *J, *A, *B, *gA, *gB, *alpha;
+  const float* drhodt_u;
+};
+struct AdOut {
+  float *ap, *av, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
+};
+
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_accel_dudt(Rows rows, Geo geo, int kernel, float C, LimParams lp,
+                                                       float inv_kscale, float Cl, float Cq, int alphaQ, int32_t n,
+                                                       AdIn in, const int32_t* __restrict__ perm, AdOut o) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float xi[3], vi[3], Ji[D * D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = x[(int64_t)k * D + d]; vi[d] = v[(int64_t)k * D + d]; }
+  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
 #pragma unroll
-  for (int a = 0; a < D * D; ++a) Ji[a] = J[(int64_t)k * D * D + a];
-  float hi = h[k], rhoi = rho[k], Pi = P[k], ci = c[k], Vi = V[k], mi = m[k];
-  float ali = alphaQ ? alpha[k] : 1.f;
-  Corr<D> cri;
-  cri.load(Ap, Bp, gAp, gBp, k);
-  float smi = hi / kscale;
-  Kahan acc[D], accu;
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  for (int a = 0; a < D * D; ++a) Ji[a] = in.J[(int64_t)k * D * D + a];
+  const float hi = in.h[k], rhoi = in.rho[k], Pi = in.P[k], ci = in.c[k], Vi = in.V[k], mi = in.m[k];
+  const float ihi = 1.f / hi;
+  const float ali = alphaQ ? in.alpha[k] : 1.f;
+  const CorrD<D> cri = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
+  const float smi = hi * inv_kscale;
+  const float ismi = 1.f / smi;
+  const float epsi = 1.0e-7f * smi * smi;
+  const float ViOm = Vi / mi;
+  Comp acc[D], accu;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float dx[3];
-    min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
+    min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     float r = sqrtf(dot<D>(dx, dx));
-    float hj = h[j];
+    float hj = in.h[j];
     if (!(r < fmaxf(hi, hj))) {
 #pragma unroll
-      for (int a = 0; a < D; ++a) { ap[(int64_t)e * D + a] = 0.f; av[(int64_t)e * D + a] = 0.f; }
+      for (int a = 0; a < D; ++a) { o.ap[(int64_t)q * D + a] = 0.f; o.av[(int64_t)q * D + a] = 0.f; }
       continue;
     }
+    const float ihj = 1.f / hj;
+    const float ir = safe_inv(r);
     float vj[3], Jj[D * D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) vj[d] = v[(int64_t)j * D + d];
+    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
 #pragma unroll
-    for (int a = 0; a < D * D; ++a) Jj[a] = J[(int64_t)j * D * D + a];
-    float phi = limiter_phi<D>(lp, dx, hi, hj, Ji, Jj);
-    float phi2 = phi / 2.f;
+    for (int a = 0; a < D * D; ++a) Jj[a] = in.J[(int64_t)j * D * D + a];
+    float phi = limiter_phi<D>(lp, dx, r, ihi, ihj, Ji, Jj);
+    float phi2 = 0.5f * phi;
     float vdi[3], vdj[3], vdjU[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -590,115 +748,114 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(
 #pragma unroll
       for (int gm = 0; gm < D; ++gm) { cI = cI + Ji[a * D + gm] * dx[gm]; cJ = cJ + Jj[a * D + gm] * dx[gm]; }
       vdi[a] = vi[a] - phi2 * cI;
-      vdj[a] = vj[a] + phi2 * cJ;
-      vdjU[a] = vj[a] - phi2 * cJ;
+      vdj[a] = vj[a] + phi2 * cJ;   // accel.py:137-141
+      vdjU[a] = vj[a] - phi2 * cJ;  // dudt.py:123 (J_j (-x_ij))
     }
-    Corr<D> crj;
-    crj.load(Ap, Bp, gAp, gBp, j);
+    const CorrD<D> crj = load_corr<D>(in.A, in.B, in.gA, in.gB, j);
+    WdW<D> wj = wdw<D>(kernel, C, r, ir, ihj);  // W(x_ij, h_j) for the i-side gradient
+    WdW<D> wi = wdw<D>(kernel, C, r, ir, ihi);  // W(x_ji, h_i) for the j-side gradient
     float gi[3], gjr[3], dxn[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) dxn[a] = -dx[a];
-    grad_wr<D>(K, dx, r, hj, cri.A, cri.B, cri.gA, cri.gB, gi);
-    grad_wr<D>(K, dxn, r, hi, crj.A, crj.B, crj.gA, crj.gB, gjr);
+    grad_wr<D>(cri, dx, wj, gi);
+    grad_wr<D>(crj, dxn, wi, gjr);
     float gw[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) gw[a] = 0.5f * (gi[a] - gjr[a]);
-    float smj = hj / kscale;
-    float ei[3], ej[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) { ei[a] = dx[a] / smi; ej[a] = dx[a] / smj; }
-    float eii = dot<D>(ei, ei), ejj = dot<D>(ej, ej);
-    float di = eii + 1.0e-7f * smi * smi, dj = ejj + 1.0e-7f * smj * smj;
-    float ti = 0.f, tj = 0.f, tiU = 0.f, tjU = 0.f;
+    // mu_i = (vdot_i - vdot_j).eta_i / (eta_i.eta_i + 1e-7 s_i^2), eta_i = x / s_i
+    //      = s_i (w.x) / (r^2 + 1e-7 s_i^4) ... evaluated in the reference's form
+    const float smj = hj * inv_kscale;
+    const float ismj = 1.f / smj;
+    float r2 = r * r;
+    float di = r2 * ismi * ismi + epsi;
+    float dj = r2 * ismj * ismj + 1.0e-7f * smj * smj;
+    float wx = 0.f, wxU = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float w = vdi[a] - vdj[a];
-      float wU = vdi[a] - vdjU[a];
-      ti = ti + w * ei[a];
-      tj = tj + w * ej[a];
-      tiU = tiU + wU * ei[a];
-      tjU = tjU + wU * ej[a];
+      wx = wx + (vdi[a] - vdj[a]) * dx[a];
+      wxU = wxU + (vdi[a] - vdjU[a]) * dx[a];
     }
-    float mui = ti / di, muj = tj / dj, muiU = tiU / di, mujU = tjU / dj;
-    mui = mui < 0.f ? mui : 0.f;
-    muj = muj < 0.f ? muj : 0.f;
-    muiU = muiU < 0.f ? muiU : 0.f;
-    mujU = mujU < 0.f ? mujU : 0.f;
-    float rhoj = rho[j], cj = c[j];
+    float mui = fminf(wx * ismi / di, 0.f), muj = fminf(wx * ismj / dj, 0.f);
+    float muiU = fminf(wxU * ismi / di, 0.f), mujU = fminf(wxU * ismj / dj, 0.f);
+    float rhoj = in.rho[j], cj = in.c[j];
     float Qi = rhoi * (-Cl * ci * mui + Cq * mui * mui);
     float Qj = rhoj * (-Cl * cj * muj + Cq * muj * muj);
     float QiU = rhoi * (-Cl * ci * muiU + Cq * muiU * muiU);
     float QjU = rhoj * (-Cl * cj * mujU + Cq * mujU * mujU);
     if (alphaQ) {
-      float wa = 0.5f * (ali + alpha[j]);
+      float wa = 0.5f * (ali + in.alpha[j]);
       Qi = Qi * wa; Qj = Qj * wa; QiU = QiU * wa; QjU = QjU * wa;
     }
-    float Vj = V[j], Pj = P[j];
-    float pP = -(Pi + Pj);
-    float pQ = -(Qi + Qj);
+    const float VV = ViOm * in.V[j];
+    const float Pj = in.P[j];
+    const float pP = -(Pi + Pj) * VV;
+    const float pQ = -(Qi + Qj) * VV;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float apv = pP * gw[a] * Vi * Vj / mi;
-      float avv = pQ * Vj * gw[a] * Vi / mi;
-      ap[(int64_t)e * D + a] = apv;
-      av[(int64_t)e * D + a] = avv;
+      float apv = pP * gw[a];
+      float avv = pQ * gw[a];
+      o.ap[(int64_t)q * D + a] = apv;
+      o.av[(int64_t)q * D + a] = avv;
       acc[a].add(apv + avv);
     }
     float dt_ = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];
-    float pT = Pj * dt_ * Vi * Vj / mi;
-    float vT = 0.5f * (QiU + QjU) * dt_ * Vi * Vj / mi;
-    accu.add(pT + vT);
+    accu.add((Pj + 0.5f * (QiU + QjU)) * dt_ * VV);
   }
+  float dv[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) dv[a] = group_total<G>(acc[a]);
+  float du = group_total<G>(accu);
+  if (!L.valid || L.g != 0) return;
   int32_t i = perm[k];
   float ke = 0.f;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    dvdt_s[(int64_t)k * D + a] = acc[a].s;
-    dvdt_u[(int64_t)i * D + a] = acc[a].s;
-    ke = ke + vi[a] * acc[a].s;
+    o.dvdt_s[(int64_t)k * D + a] = dv[a];
+    o.dvdt_u[(int64_t)i * D + a] = dv[a];
+    ke = ke + vi[a] * dv[a];
   }
-  float du = accu.s;
-  dudt_u[i] = du;
-  dEdt_u[i] = mi * ke + mi * du;
-  div_u[i] = -drhodt_u[i] / rhoi;
+  o.dudt_u[i] = du;
+  o.dEdt_u[i] = mi * ke + mi * du;
+  o.div_u[i] = -in.drhodt_u[i] / rhoi;
 }
 
 // ------------------------------------------------------------ balance
-template <int D>
-__global__ void k_balance(Rows rows, int32_t n, const float* __restrict__ v, const float* __restrict__ dvdt,
-                          const float* __restrict__ m, const float* __restrict__ rho, const float* __restrict__ P,
-                          float gamma, float dt, float hdt, const float* __restrict__ ap,
-                          const float* __restrict__ av, float* __restrict__ f) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_balance(Rows rows, int32_t n, const float* __restrict__ v,
+                                                    const float* __restrict__ dvdt, const float* __restrict__ m,
+                                                    const float* __restrict__ sP, float dt, float hdt,
+                                                    const float* __restrict__ ap, const float* __restrict__ av,
+                                                    float* __restrict__ f) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
   float vhi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) vhi[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
-  float si = P[k] / powf(rho[k], gamma);
-  float mi = m[k];
-  for (int32_t e = rows.off[k]; e < rows.off[k + 1]; ++e) {
-    int32_t j = rows.nbr[e];
+  const float si = sP[k];
+  const float mdt = m[k] * dt;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
     float d_ = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       float vhj = v[(int64_t)j * D + a] + hdt * dvdt[(int64_t)j * D + a];
-      d_ = d_ + (vhj - vhi[a]) * (ap[(int64_t)e * D + a] + av[(int64_t)e * D + a]);
+      d_ = d_ + (vhj - vhi[a]) * (ap[(int64_t)q * D + a] + av[(int64_t)q * D + a]);
     }
-    float dE = mi * d_ * dt;
-    float sj = P[j] / powf(rho[j], gamma);
-    float smin = fminf(fabsf(si), fabsf(sj));
-    float smax = fmaxf(fabsf(si), fabsf(sj));
-    bool mA = fabsf(si - sj) == 0.f;
+    float dE = mdt * d_;
+    float sj = sP[j];
+    float asi = fabsf(si), asj = fabsf(sj);
+    float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
     bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
     bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
     float ff = 0.5f;
-    if (mA) ff = 0.5f;
-    else if (mB) ff = smin / (smin + smax + 1.0e-14f);
-    else if (mC) ff = smax / (smin + smax + 1.0e-14f);
-    ff = fmaxf(0.f, fminf(1.f, ff));
-    f[e] = ff;
+    if (!(si - sj == 0.f)) {
+      if (mB) ff = smin / (smin + smax + 1.0e-14f);
+      else if (mC) ff = smax / (smin + smax + 1.0e-14f);
+    }
+    f[q] = fmaxf(0.f, fminf(1.f, ff));
   }
 }
 
@@ -723,7 +880,7 @@ static int require_bound(CrkContext* ctx, const CrkConfig* cfg) {
 static int alloc_stage(CrkContext* ctx, int32_t n, int D) {
   size_t f = sizeof(float) * (size_t)n;
   Buf* s1[] = {&ctx->h, &ctx->m, &ctx->u, &ctx->a0, &ctx->divp, &ctx->V, &ctx->A, &ctx->rho, &ctx->P, &ctx->c,
-               &ctx->divcd, &ctx->vsig, &ctx->Rcd, &ctx->du0, &ctx->alpha};
+               &ctx->divcd, &ctx->vsig, &ctx->Rcd, &ctx->du0, &ctx->alpha, &ctx->sP};
   for (Buf* b : s1) { int st = ensure(*b, f); if (st) return st; }
   Buf* sd[] = {&ctx->x, &ctx->v, &ctx->B, &ctx->gA, &ctx->dvdt, &ctx->vh};
   for (Buf* b : sd) { int st = ensure(*b, f * D); if (st) return st; }
@@ -737,17 +894,21 @@ static T* F(Buf& b) { return (T*)b.p; }
 
 static Rows rows_of(const CrkContext* ctx) { return Rows{ctx->adj.row_offsets, ctx->adj.nbr}; }
 
-template <int D>
-static DevKernel<D> devkernel(const CrkConfig* cfg) {
-  DevKernel<D> K;
-  K.kernel = cfg->kernel;
-  K.C = (float)kernel_C(cfg->kernel, D);
-  return K;
-}
+inline unsigned group_blocks(int32_t n, int G) { return nblocks((int64_t)n * G); }
 
 }  // namespace crk
 
 using namespace crk;
+
+// dispatch over dimension and group size (1, 2, 4, 8, 16, 32, 64)
+#define CRK_G_CASE(GG, ...) case GG: { constexpr int G = GG; __VA_ARGS__; } break;
+#define CRK_DIM_G_DISPATCH(DIM, GS, ...)                                                      \
+  switch (DIM) {                                                                              \
+    case 1: { constexpr int DD = 1; switch (GS) { CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__) CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__) CRK_G_CASE(64, __VA_ARGS__) default: set_error("bad group size %d", GS); return CRK_EINVAL; } } break; \
+    case 2: { constexpr int DD = 2; switch (GS) { CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__) CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__) CRK_G_CASE(64, __VA_ARGS__) default: set_error("bad group size %d", GS); return CRK_EINVAL; } } break; \
+    case 3: { constexpr int DD = 3; switch (GS) { CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__) CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__) CRK_G_CASE(64, __VA_ARGS__) default: set_error("bad group size %d", GS); return CRK_EINVAL; } } break; \
+    default: set_error("dim must be 1..3"); return CRK_EINVAL;                                \
+  }
 
 #define CRK_DIM_DISPATCH(D, ...)                       \
   switch (D) {                                         \
@@ -765,6 +926,15 @@ static int validate_adj(const CrkAdjacency* adj, const CrkState* st) {
   CRK_REQUIRE(adj->perm && adj->row_offsets && (adj->n_edges == 0 || adj->nbr), "adjacency buffers missing");
   CRK_REQUIRE(st->positions && st->velocities && st->supports && st->masses && st->internalEnergies,
               "state buffers missing");
+  return CRK_OK;
+}
+
+int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group) {
+  CRK_REQUIRE(ctx, "crk_set_group_size: null context");
+  CRK_REQUIRE(pass >= 0 && pass < CRK_NPASS, "unknown pass %d", pass);
+  CRK_REQUIRE(group == 1 || group == 2 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
+              "group size must be a power of two in [1, 64] (got %d)", group);
+  ctx->group[pass] = group;
   return CRK_OK;
 }
 
@@ -810,11 +980,12 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   Geo geo = make_geo(*cfg);
   float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   if (max_ratio_dev) { if ((r = fill_f32(max_ratio_dev, 1, 0.f, s))) return r; }
-  CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_owen<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo,
-                                                cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h),
-                                                ctx->adj.perm, st->supports, F<double>(ctx->lut_psiH),
-                                                F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                                                hminmax_dev, max_ratio_dev));
+  int GS = ctx->group[CRK_PASS_OWEN];
+  CRK_DIM_G_DISPATCH(ctx->dim, GS,
+      hipLaunchKernelGGL((k_owen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
+                         F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
+                         hminmax_dev, max_ratio_dev));
   CRK_LAUNCH_CHECK("k_owen");
   return CRK_OK;
 }
@@ -827,18 +998,21 @@ int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, i
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   Geo geo = make_geo(*cfg);
-  CRK_DIM_DISPATCH(ctx->dim, {
-    DevKernel<DD> K = devkernel<DD>(cfg);
-    hipLaunchKernelGGL(k_volume<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo, K, n, F<float>(ctx->x),
-                       F<float>(ctx->h), F<float>(ctx->V));
-    CRK_LAUNCH_CHECK("k_volume");
-    hipLaunchKernelGGL(k_moments<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo, K, n,
-                       F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V), F<float>(ctx->m), F<float>(ctx->u),
-                       cfg->gamma, F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB),
-                       F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), ctx->adj.perm, st->densities,
-                       st->entropies, st->pressures, st->soundspeeds, n_singular_dev);
-    CRK_LAUNCH_CHECK("k_moments");
-  });
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  int GV = ctx->group[CRK_PASS_VOLUME];
+  CRK_DIM_G_DISPATCH(ctx->dim, GV,
+      hipLaunchKernelGGL((k_volume<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V)));
+  CRK_LAUNCH_CHECK("k_volume");
+  MomOut mo{F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->rho),
+            F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->sP), st->densities, st->entropies, st->pressures,
+            st->soundspeeds, n_singular_dev};
+  int GM = ctx->group[CRK_PASS_MOMENTS];
+  CRK_DIM_G_DISPATCH(ctx->dim, GM,
+      hipLaunchKernelGGL((k_moments<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V), F<float>(ctx->m),
+                         F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo));
+  CRK_LAUNCH_CHECK("k_moments");
   ctx->have_factors = true;
   return CRK_OK;
 }
@@ -851,22 +1025,22 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   bool cullen = cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED;
-  CRK_DIM_DISPATCH(ctx->dim, {
-    DevKernel<DD> K = devkernel<DD>(cfg);
-    if (cullen)
-      hipLaunchKernelGGL((k_velgrad<DD, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo, K, n,
-                         F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
-                         F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
-                         F<float>(ctx->gB), F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig),
-                         ctx->adj.perm, up->drhodt);
-    else
-      hipLaunchKernelGGL((k_velgrad<DD, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo, K, n,
-                         F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
-                         F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
-                         F<float>(ctx->gB), F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig),
-                         ctx->adj.perm, up->drhodt);
-  });
+  VgIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
+          F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
+          F<float>(ctx->gB)};
+  VgOut out{F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig), up->drhodt};
+  int GS = ctx->group[CRK_PASS_VELGRAD];
+  if (cullen) {
+    CRK_DIM_G_DISPATCH(ctx->dim, GS,
+        hipLaunchKernelGGL((k_velgrad<DD, G, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+                           geo, cfg->kernel, C, n, in, ctx->adj.perm, out));
+  } else {
+    CRK_DIM_G_DISPATCH(ctx->dim, GS,
+        hipLaunchKernelGGL((k_velgrad<DD, G, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+                           geo, cfg->kernel, C, n, in, ctx->adj.perm, out));
+  }
   CRK_LAUNCH_CHECK("k_velgrad");
   ctx->have_velgrad = true;
   return CRK_OK;
@@ -886,16 +1060,17 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   float fkern = (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim));
   const CrkSwitchParams& sp = cfg->switchParams;
-  CRK_DIM_DISPATCH(ctx->dim, {
-    DevKernel<DD> K = devkernel<DD>(cfg);
-    hipLaunchKernelGGL(k_cullen<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo, K, n, F<float>(ctx->x),
-                       F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->rho), F<float>(ctx->divcd),
-                       F<float>(ctx->vsig), F<float>(ctx->divp), F<float>(ctx->a0), F<float>(ctx->alpha),
-                       F<float>(ctx->Rcd), dt, fkern, sp.alpha_min, sp.alpha_max, sp.beta_xi, sp.limitXi,
-                       cfg->evolveAlpha0, ctx->adj.perm, st->alphas, st->alpha0s);
-  });
+  CdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->rho), F<float>(ctx->divcd),
+          F<float>(ctx->vsig), F<float>(ctx->divp)};
+  CdOut out{F<float>(ctx->a0), F<float>(ctx->alpha), F<float>(ctx->Rcd), st->alphas, st->alpha0s};
+  int GS = ctx->group[CRK_PASS_SWITCH];
+  CRK_DIM_G_DISPATCH(ctx->dim, GS,
+      hipLaunchKernelGGL((k_cullen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         cfg->kernel, C, n, in, dt, fkern, sp.alpha_min, sp.alpha_max, sp.beta_xi, sp.limitXi,
+                         cfg->evolveAlpha0, ctx->adj.perm, out));
   CRK_LAUNCH_CHECK("k_cullen");
   return CRK_OK;
 }
@@ -911,20 +1086,20 @@ int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, Cr
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   Geo geo = make_geo(*cfg);
-  LimParams lp{cfg->crkViscosity.eta_fold, cfg->crkViscosity.eta_crit, cfg->crkViscosity.enableCRKLimiter,
-               cfg->crkViscosity.enableVanLeerLimiter, cfg->crkViscosity.forceVanLeerOff,
-               cfg->crkViscosity.forceVanLeerOn};
-  float kscale = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
-  const float* alpha_s = F<float>(ctx->alpha);
-  CRK_DIM_DISPATCH(ctx->dim, {
-    DevKernel<DD> K = devkernel<DD>(cfg);
-    hipLaunchKernelGGL(k_accel_dudt<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), geo, K, lp, kscale,
-                       cfg->diffusion.C_l, cfg->diffusion.C_q, cfg->alphaWeightedQ, n, F<float>(ctx->x),
-                       F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V), F<float>(ctx->rho),
-                       F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A), F<float>(ctx->B),
-                       F<float>(ctx->gA), F<float>(ctx->gB), alpha_s, ed->ap, ed->av, F<float>(ctx->dvdt),
-                       ctx->adj.perm, up->dvdt, up->dudt, up->dEdt, (const float*)up->drhodt, st->divergence);
-  });
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  LimParams lp{cfg->crkViscosity.eta_fold, 1.f / cfg->crkViscosity.eta_fold, cfg->crkViscosity.eta_crit,
+               cfg->crkViscosity.enableCRKLimiter, cfg->crkViscosity.enableVanLeerLimiter,
+               cfg->crkViscosity.forceVanLeerOff, cfg->crkViscosity.forceVanLeerOn};
+  float inv_kscale = (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim));
+  AdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V),
+          F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A),
+          F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
+  AdOut out{ed->ap, ed->av, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
+  int GS = ctx->group[CRK_PASS_ACCEL];
+  CRK_DIM_G_DISPATCH(ctx->dim, GS,
+      hipLaunchKernelGGL((k_accel_dudt<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         cfg->kernel, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q, cfg->alphaWeightedQ,
+                         n, in, ctx->adj.perm, out));
   CRK_LAUNCH_CHECK("k_accel_dudt");
   ctx->have_accel = true;
   return CRK_OK;
@@ -938,10 +1113,11 @@ int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, v
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   float hdt = (float)(0.5 * (double)dt);
-  CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_balance<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, rows_of(ctx), n,
-                                                F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->m),
-                                                F<float>(ctx->rho), F<float>(ctx->P), cfg->gamma, dt, hdt, ed->ap,
-                                                ed->av, ed->f));
+  int GS = ctx->group[CRK_PASS_BALANCE];
+  CRK_DIM_G_DISPATCH(ctx->dim, GS,
+      hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), n,
+                         F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->m), F<float>(ctx->sP), dt, hdt, ed->ap,
+                         ed->av, ed->f));
   CRK_LAUNCH_CHECK("k_balance");
   return CRK_OK;
 }

@@ -74,7 +74,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
             A = adj.cabi()
             abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
             with timing.scope("owen"):
-                abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm), abi.ptr(ratio), s)
+                abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm),
+                         abi.ptr(ratio if iters > 1 else None), s)
             abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
             if iters > 1 and float(ratio.item()) < compParams.adaptiveSupportThreshold:
                 break
@@ -82,10 +83,14 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # computeCRKFactors (crkSPH.py:61) before any use; it is not evaluated.
 
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
-    adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
+    owen = compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen
+    adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports)
     system.adjacency = adj
     A = adj.cabi()
-    abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
+    if rebuilt or not owen:
+        # (with Owen and a reused adjacency the sorted copies are current:
+        # the Owen pass updated the sorted supports in place)
+        abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
     # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
     rt.nsing.zero_()
     with timing.scope("factors_eos"):

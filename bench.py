@@ -34,14 +34,19 @@ def pass_bytes(name: str, D: int):
     table = {
         # x, h in; h sorted + user out
         "owen": (xD + f + row + 2 * f, f),
-        # volume: x, h -> V; moments: x, h, V, m, u -> A,B,gA,gB, rho, P, c + 4 user fields
-        "factors_eos": ((xD + f + f + row) + (xD + 4 * f + row + corr + 3 * f + 4 * f), 2 * f),
-        # x, h, v, V, m, rho, c, A,B,gA,gB -> J, div, vsig, drhodt(user)
-        "velocity_gradient": (xD + f + xD + 4 * f + corr + row + f * D * D + 3 * f, f),
-        # x, h, m, rho, div, vsig, div_prev, alpha0 -> alpha, R, alphas(user)
-        "switch": (xD + 7 * f + row + 3 * f, f),
-        # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt -> dvdt (sorted+user), dudt, dEdt, div; edges: j, ap, av
-        "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + 2 * xD + 3 * f, f + 2 * xD),
+        # x, h, v, u -> V, vsig, in-support count; edges: j in; the in-support half: (j, edge) out;
+        # the skin half: ap, av zeroed
+        "support": (2 * xD + 2 * f + row + 3 * f, f + 0.5 * (2 * f) + 0.5 * (2 * xD)),
+        # moments: x, h, V, m, u -> A,B,gA,gB, rho, P, c, P/rho^gamma + 4 user fields; edges: the
+        # in-support half of the compacted neighbour list
+        "factors_eos": (xD + 4 * f + row + corr + 4 * f + 4 * f, 0.5 * f),
+        # x, h, v, V, m, rho, A,B,gA,gB -> J, div, drhodt(user); edges: compacted j
+        "velocity_gradient": (xD + f + xD + 3 * f + corr + row + f * D * D + 2 * f, 0.5 * f),
+        # x, h, m, rho, div, vsig, div_prev, alpha0 -> alpha, R, alphas(user); edges: compacted j
+        "switch": (xD + 7 * f + row + 3 * f, 0.5 * f),
+        # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt -> dvdt (sorted+user), dudt, dEdt, div;
+        # edges: compacted (j, edge) and ap, av of the in-support half
+        "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + 2 * xD + 3 * f, 0.5 * (2 * f + 4 * xD)),
         # v, dvdt, m, sP -> ; edges: j, ap, av in, f out
         "balance": (2 * xD + 2 * f + row, f + 2 * xD + f),
         # v0, dvdt0, dvdt1 -> v_half -> sorted v_half; u0 -> u, dudt;
@@ -123,7 +128,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning-file", default=None, help="load (or save after tuning) the lane-group sizes")
     ap.add_argument("--retune", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 256 MiB buffer 4 times (calibrates the HBM traffic counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "copies": 4}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -253,7 +253,16 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
                        const float* hminmax_dev, float* max_ratio_dev, void* stream);
 /* hminmax_dev[0] = min supports, [1] = max supports (user array). */
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_dev, void* stream);
-/* computeCRKFactors (warpSPHCore; called at crkSPH.py:61): apparent volume,
+/* Support sweep of the bound adjacency for the current positions and supports
+ * (after the Owen update, before crk_factors_eos).  One pass over every edge:
+ *  - the CRK apparent volume V_i = 1 / sum_j W(x_ij, h_i) (computeCRKFactors);
+ *  - the edges with |x_ij| < max(h_i, h_j) are compacted per row (row order
+ *    kept) for the kernel-weighted passes, and the other (Verlet skin) edges
+ *    get ap = av = 0 in *edges (both CRK gradients vanish there);
+ *  - with the Cullen-Dehnen switch, the signal speed
+ *    max_j (c_ij - mu_ij) (wp_vsig.py:11-86) over every adjacency entry.   */
+int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* edges, void* stream);
+/* computeCRKFactors (warpSPHCore; called at crkSPH.py:61), after crk_support:
  * moments with FP64 solve -> A, B, gradA, gradB, CRK density (written to
  * st->densities), followed by idealGasEOS (eos/idealGas.py:6-38) writing
  * st->entropies, st->pressures, st->soundspeeds.  n_singular_dev (device
@@ -280,7 +289,7 @@ int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* edges
 enum {
   CRK_PASS_OWEN = 0, CRK_PASS_VOLUME = 1, CRK_PASS_MOMENTS = 2, CRK_PASS_VELGRAD = 3,
   CRK_PASS_SWITCH = 4, CRK_PASS_ACCEL = 5, CRK_PASS_BALANCE = 6, CRK_PASS_COMPAT = 7,
-  CRK_NPASS = 8
+  CRK_PASS_SUPPORT = 8, CRK_NPASS = 9
 };
 int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group);
 

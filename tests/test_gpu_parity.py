@@ -209,8 +209,8 @@ def test_stage_parity_any_group_size(group):
         torch.cuda.synchronize()
     finally:
         set_groups(system.state.positions.device,
-                   {"owen": 8, "volume": 8, "moments": 4, "velgrad": 4, "switch": 8, "accel": 4, "balance": 8,
-                    "compat": 8})
+                   {"owen": 4, "support": 32, "moments": 2, "velgrad": 8, "switch": 8, "accel": 16, "balance": 32,
+                    "compat": 32})
     sim = O.from_system(cpu, config, scheme, 32)
     sim.evaluate(dt)
     errs = {k: relerr(getattr(st, a).cpu().numpy(), sim.get(k)) for a, k in STATE_FIELDS}

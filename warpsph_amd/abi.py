@@ -31,7 +31,7 @@ CRK_SUPPORT_OWEN = 2
 CRK_SWITCH_CULLEN_DEHNEN = 2
 CRK_SWITCH_NONE = 6
 CRK_SWITCH_LEARNED = 100
-PASSES = ("owen", "volume", "moments", "velgrad", "switch", "accel", "balance", "compat")
+PASSES = ("owen", "volume", "moments", "velgrad", "switch", "accel", "balance", "compat", "support")
 
 
 class CrkDomain(C.Structure):
@@ -120,6 +120,7 @@ _PROTOS = {
     "crk_stage_rebind": [_P, C.POINTER(CrkConfig), C.POINTER(CrkAdjacency), C.POINTER(CrkState), _P],
     "crk_owen_iteration": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
     "crk_minmax": [_P, C.c_int32, _P, _P, _P],
+    "crk_support": [_P, C.POINTER(CrkConfig), C.POINTER(CrkEdges), _P],
     "crk_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P],
     "crk_velocity_gradient": [_P, C.POINTER(CrkConfig), C.POINTER(CrkUpdate), _P],
     "crk_switch": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.c_float, _P],

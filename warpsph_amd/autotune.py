@@ -21,7 +21,7 @@ from .timing import PassTimer
 
 # pass -> timing scope that contains it
 SCOPES = {
-    "owen": "owen", "volume": "factors_eos", "moments": "factors_eos", "velgrad": "velocity_gradient",
+    "owen": "owen", "support": "support", "moments": "factors_eos", "velgrad": "velocity_gradient",
     "switch": "switch", "accel": "accel_dudt", "balance": "balance", "compat": "compatible_energy",
 }
 
@@ -47,8 +47,8 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
     rt = runtime(dev)
     integ = getIntegrator()
     current = dict(getattr(rt, "groups", {}))
-    defaults = {"owen": 8, "volume": 8, "moments": 4, "velgrad": 4, "switch": 8, "accel": 4, "balance": 8,
-                "compat": 8}
+    defaults = {"owen": 4, "support": 32, "moments": 2, "velgrad": 8, "switch": 8, "accel": 16, "balance": 32,
+                "compat": 32}
     for k, v in defaults.items():
         current.setdefault(k, v)
     set_groups(dev, current)

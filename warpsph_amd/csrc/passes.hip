@@ -65,11 +65,20 @@ struct Lane {
   int32_t k;   // row (sorted particle)
   bool valid;
 };
+// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8).
+// Renumber them so that every XCD works on one contiguous range of sorted
+// rows: its L2 then holds one compact region of the domain.
+__device__ inline int32_t xcd_block() {
+  const int32_t nb = gridDim.x, b = blockIdx.x;
+  const int32_t q = nb / 8, r = nb % 8, x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
+}
+
 template <int G>
 __device__ inline Lane lane_of(int32_t n) {
   Lane l;
   l.g = threadIdx.x % G;
-  l.k = (int32_t)blockIdx.x * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
+  l.k = xcd_block() * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
   l.valid = l.k < n;
   if (!l.valid) l.k = n - 1;  // safe index for loads; results discarded
   return l;
@@ -184,6 +193,97 @@ struct Rows {
   const int32_t* nbr;
 };
 
+// In-support edges of each row (|x_ij| < max(h_i, h_j)), compacted in row
+// order into [off[k], off[k] + cnt[k]) of nbr (neighbour) and pos (edge index
+// in the full adjacency).  About half of a Verlet row is skin; the
+// kernel-weighted passes iterate only these edges, so every lane of a group
+// works on a contributing pair.
+struct ARows {
+  const int32_t* off;
+  const int32_t* cnt;
+  const int32_t* nbr;
+  const int32_t* pos;
+};
+
+// ------------------------------------------------------------ support sweep
+// One pass over the full adjacency after the Owen update: the CRK apparent
+// volume V_i = 1 / sum_j W(x_ij, h_i); the in-support edges compacted per row
+// with group ballots; ap = av = 0 for the skin edges; and the signal speed
+// v_sig = max_j (0.5 (c_i + c_j) - mu_ij if mu_ij <= 0 else 0),
+// mu_ij = (v_i - v_j).x_ij / (r_ij + 1e-14 h_i), over every entry
+// (wp_vsig.py:55-86), c = sqrt(|u| gamma (gamma - 1)).
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, int kernel, float C, int32_t n,
+                                                    const float* __restrict__ x, const float* __restrict__ h,
+                                                    const float* __restrict__ v, const float* __restrict__ u,
+                                                    float cfac, int32_t* __restrict__ anbr,
+                                                    int32_t* __restrict__ apos, int32_t* __restrict__ acnt,
+                                                    float* __restrict__ vsig, float* __restrict__ V,
+                                                    float* __restrict__ ap, float* __restrict__ av) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3], vi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) { xi[d] = x[(int64_t)k * D + d]; vi[d] = v[(int64_t)k * D + d]; }
+  const float hi = h[k];
+  const float ih = 1.f / hi;
+  const float ci = sqrtf(fabsf(u[k]) * cfac);
+  const float eps = 1.0e-14f * hi;
+  const int lane = threadIdx.x & 63;
+  const int gbase = lane & ~(G - 1);
+  const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  int32_t run = 0;
+  float vs = 0.f;
+  Comp vol;
+  for (int32_t base = b; base < e; base += G) {
+    const int32_t q = base + L.g;
+    const bool inside = q < e;
+    bool act = false;
+    int32_t j = 0;
+    if (inside) {
+      j = rows.nbr[q];
+      float dx[3];
+      min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
+      float r = sqrtf(dot<D>(dx, dx));
+      if (vsig) {
+        float mu = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) mu = mu + (vi[a] - v[(int64_t)j * D + a]) * dx[a];
+        mu = mu / (r + eps);
+        float vv = 0.5f * (ci + sqrtf(fabsf(u[j]) * cfac)) - mu;
+        if (mu > 0.f) vv = 0.f;
+        vs = vv > vs ? vv : vs;
+      }
+      float qq = r * ih;
+      if (qq < 1.f) {
+        float kk, dk;
+        kwd<D>(kernel, qq, kk, dk);
+        vol.add(C * kk * ipow<D>(ih));
+      }
+      act = r < fmaxf(hi, h[j]);
+      if (ap) {  // whole rows zeroed with coalesced stores; accel overwrites the in-support edges
+#pragma unroll
+        for (int a = 0; a < D; ++a) { ap[(int64_t)q * D + a] = 0.f; av[(int64_t)q * D + a] = 0.f; }
+      }
+    }
+    const unsigned long long m = __ballot(act ? 1 : 0) & gmask;
+    if (act) {
+      const int32_t slot = b + run + __popcll(m & lt);
+      anbr[slot] = j;
+      apos[slot] = q;
+    }
+    run += __popcll(m);
+  }
+  vs = group_max<G>(vs);
+  float S = group_total<G>(vol);
+  if (!L.valid || L.g != 0) return;
+  acnt[k] = run;
+  V[k] = 1.f / S;
+  if (vsig) vsig[k] = vs;
+}
+
 // ------------------------------------------------------------ Owen
 __device__ inline double lut_interp(const double* __restrict__ xs, const double* __restrict__ ys, int n, double x) {
   int lo = 0, hi = n;  // upper_bound
@@ -246,33 +346,6 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, int kernel,
   }
 }
 
-// ------------------------------------------------------------ CRK volume
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_volume(Rows rows, Geo geo, int kernel, float C, int32_t n,
-                                                   const float* __restrict__ x, const float* __restrict__ h,
-                                                   float* __restrict__ V) {
-  Lane L = lane_of<G>(n);
-  const int32_t k = L.k;
-  float xi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
-  const float ih = 1.f / h[k];
-  Comp acc;
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
-    float dx[3];
-    min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float qq = sqrtf(dot<D>(dx, dx)) * ih;
-    if (!(qq < 1.f)) continue;
-    float kk, dk;
-    kwd<D>(kernel, qq, kk, dk);
-    acc.add(C * kk * ipow<D>(ih));
-  }
-  float S = group_total<G>(acc);
-  if (L.valid && L.g == 0) V[k] = 1.f / S;
-}
-
 // ------------------------------------------------------------ CRK moments
 // Linear CRK (Frontiere, Raskin & Owen 2017) with W_ij = W(x_ij, h_j):
 //   m0 = sum V_j W, m1 = sum V_j x W, m2 = sum V_j x x W and x_i-gradients,
@@ -288,7 +361,7 @@ struct MomOut {
 };
 
 template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, int kernel, float C, int32_t n,
+__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, int kernel, float C, int32_t n,
                                                     const float* __restrict__ x, const float* __restrict__ h,
                                                     const float* __restrict__ V, const float* __restrict__ m,
                                                     const float* __restrict__ u, float gamma,
@@ -300,7 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_moments(Rows rows, Geo geo, int kern
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
   Comp m0, m1[D], m2[NM2], dm0[D], dm1[D * D], dm2[NM2 * D], S0, S1[D];
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   for (int32_t q = b + L.g; q < e; q += G) {
     int32_t j = rows.nbr[q];
     float dx[3];
@@ -508,7 +581,7 @@ struct VgOut {
 };
 
 template <int D, int G, bool CULLEN>
-__global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, int kernel, float C, int32_t n, VgIn in,
+__global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, int kernel, float C, int32_t n, VgIn in,
                                                     const int32_t* __restrict__ perm, VgOut o) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
@@ -516,12 +589,10 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, int kern
 #pragma unroll
   for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
   const float hi = in.h[k];
-  const float ci = in.c[k];
   CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
   Comp acc[D * D];
   Comp dacc;
-  float vs = 0.f;
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   for (int32_t q = b + L.g; q < e; q += G) {
     int32_t j = rows.nbr[q];
     float dx[3];
@@ -533,14 +604,6 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, int kern
     for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
     float hj = in.h[j];
     if (CULLEN) {
-      float mu = 0.f;
-#pragma unroll
-      for (int a = 0; a < D; ++a) mu = mu + (vi[a] - vj[a]) * dx[a];
-      mu = mu / (r + 1.0e-14f * hi);
-      float cb = 0.5f * (ci + in.c[j]);
-      float vv = cb - mu;
-      if (mu > 0.f) vv = 0.f;
-      vs = vv > vs ? vv : vs;
       float H = fmaxf(hi, hj);
       if (r < H) {
         WdW<D> w = wdw<D>(kernel, C, r, ir, 1.f / H);
@@ -566,10 +629,7 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, int kern
 #pragma unroll
   for (int a = 0; a < D * D; ++a) Jt[a] = group_total<G>(acc[a]);
   float dv = 0.f;
-  if (CULLEN) {
-    dv = group_total<G>(dacc);
-    vs = group_max<G>(vs);
-  }
+  if (CULLEN) dv = group_total<G>(dacc);
   if (!L.valid || L.g != 0) return;
   float tr = 0.f;
 #pragma unroll
@@ -577,10 +637,7 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(Rows rows, Geo geo, int kern
 #pragma unroll
   for (int a = 0; a < D; ++a) tr = tr + Jt[a * D + a];
   o.drhodt_u[perm[k]] = -tr * in.rho[k];
-  if (CULLEN) {
-    o.divcd[k] = dv;
-    o.vsig[k] = vs;
-  }
+  if (CULLEN) o.divcd[k] = dv;
 }
 
 // ------------------------------------------------------------ Cullen-Dehnen
@@ -593,7 +650,7 @@ struct CdOut {
   float *a0, *alpha, *Rcd, *alphas_u, *alpha0s_u;
 };
 template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_cullen(Rows rows, Geo geo, int kernel, float C, int32_t n, CdIn in,
+__global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, int kernel, float C, int32_t n, CdIn in,
                                                    float dt, float fkern, float amin, float amax, float beta,
                                                    int limitXi, int evolve, const int32_t* __restrict__ perm,
                                                    CdOut o) {
@@ -605,7 +662,7 @@ __global__ void __launch_bounds__(kBlock) k_cullen(Rows rows, Geo geo, int kerne
   const float hi = in.h[k];
   const float ih = 1.f / hi;
   Comp acc;
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   for (int32_t q = b + L.g; q < e; q += G) {
     int32_t j = rows.nbr[q];
     float dx[3];
@@ -701,7 +758,7 @@ struct AdOut {
 };
 
 template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_accel_dudt(Rows rows, Geo geo, int kernel, float C, LimParams lp,
+__global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, int kernel, float C, LimParams lp,
                                                        float inv_kscale, float Cl, float Cq, int alphaQ, int32_t n,
                                                        AdIn in, const int32_t* __restrict__ perm, AdOut o) {
   Lane L = lane_of<G>(n);
@@ -720,18 +777,14 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(Rows rows, Geo geo, int k
   const float epsi = 1.0e-7f * smi * smi;
   const float ViOm = Vi / mi;
   Comp acc[D], accu;
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
+    const int32_t j = rows.nbr[q0];
+    const int64_t q = rows.pos[q0];  // edge index in the full adjacency
     float dx[3];
     min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     float r = sqrtf(dot<D>(dx, dx));
     float hj = in.h[j];
-    if (!(r < fmaxf(hi, hj))) {
-#pragma unroll
-      for (int a = 0; a < D; ++a) { o.ap[(int64_t)q * D + a] = 0.f; o.av[(int64_t)q * D + a] = 0.f; }
-      continue;
-    }
     const float ihj = 1.f / hj;
     const float ir = safe_inv(r);
     float vj[3], Jj[D * D];
@@ -893,6 +946,9 @@ template <class T>
 static T* F(Buf& b) { return (T*)b.p; }
 
 static Rows rows_of(const CrkContext* ctx) { return Rows{ctx->adj.row_offsets, ctx->adj.nbr}; }
+static ARows arows_of(CrkContext* ctx) {
+  return ARows{ctx->adj.row_offsets, F<int32_t>(ctx->act_cnt), F<int32_t>(ctx->act_nbr), F<int32_t>(ctx->act_pos)};
+}
 
 inline unsigned group_blocks(int32_t n, int G) { return nblocks((int64_t)n * G); }
 
@@ -951,7 +1007,7 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   ctx->adj = *adj;
   ctx->st = *st;
   ctx->bound = true;
-  ctx->have_factors = ctx->have_velgrad = ctx->have_accel = false;
+  ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = false;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->perm, n,
@@ -980,6 +1036,7 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   Geo geo = make_geo(*cfg);
   float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   if (max_ratio_dev) { if ((r = fill_f32(max_ratio_dev, 1, 0.f, s))) return r; }
+  ctx->have_support = false;
   int GS = ctx->group[CRK_PASS_OWEN];
   CRK_DIM_G_DISPATCH(ctx->dim, GS,
       hipLaunchKernelGGL((k_owen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
@@ -990,26 +1047,48 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   return CRK_OK;
 }
 
+int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(!ed || ctx->adj.n_edges == 0 || (ed->ap && ed->av), "crk_support: edge buffers missing");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = ctx->n;
+  int64_t E = ctx->adj.n_edges;
+  if ((r = ensure(ctx->act_nbr, sizeof(int32_t) * (size_t)(E > 0 ? E : 1)))) return r;
+  if ((r = ensure(ctx->act_pos, sizeof(int32_t) * (size_t)(E > 0 ? E : 1)))) return r;
+  if ((r = ensure(ctx->act_cnt, sizeof(int32_t) * (size_t)n))) return r;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  float cfac = cfg->gamma * (cfg->gamma - 1.f);
+  float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)
+                  ? F<float>(ctx->vsig) : nullptr;
+  int GS = ctx->group[CRK_PASS_SUPPORT];
+  CRK_DIM_G_DISPATCH(ctx->dim, GS,
+      hipLaunchKernelGGL((k_support<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u),
+                         cfac, F<int32_t>(ctx->act_nbr), F<int32_t>(ctx->act_pos), F<int32_t>(ctx->act_cnt), vs,
+                         F<float>(ctx->V), ed ? ed->ap : nullptr, ed ? ed->av : nullptr));
+  CRK_LAUNCH_CHECK("k_support");
+  ctx->have_support = true;
+  return CRK_OK;
+}
+
 int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, int32_t* n_singular_dev, void* stream) {
   int r = require_bound(ctx, cfg);
   if (r) return r;
+  if (!ctx->have_support) { set_error("crk_factors_eos before crk_support"); return CRK_ESTATE; }
   CRK_REQUIRE(st && st->densities && st->entropies && st->pressures && st->soundspeeds,
               "crk_factors_eos: state output buffers missing");
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   Geo geo = make_geo(*cfg);
   float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
-  int GV = ctx->group[CRK_PASS_VOLUME];
-  CRK_DIM_G_DISPATCH(ctx->dim, GV,
-      hipLaunchKernelGGL((k_volume<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
-                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V)));
-  CRK_LAUNCH_CHECK("k_volume");
   MomOut mo{F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->rho),
             F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->sP), st->densities, st->entropies, st->pressures,
             st->soundspeeds, n_singular_dev};
   int GM = ctx->group[CRK_PASS_MOMENTS];
   CRK_DIM_G_DISPATCH(ctx->dim, GM,
-      hipLaunchKernelGGL((k_moments<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+      hipLaunchKernelGGL((k_moments<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
                          cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V), F<float>(ctx->m),
                          F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo));
   CRK_LAUNCH_CHECK("k_moments");
@@ -1034,11 +1113,11 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
   int GS = ctx->group[CRK_PASS_VELGRAD];
   if (cullen) {
     CRK_DIM_G_DISPATCH(ctx->dim, GS,
-        hipLaunchKernelGGL((k_velgrad<DD, G, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+        hipLaunchKernelGGL((k_velgrad<DD, G, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx),
                            geo, cfg->kernel, C, n, in, ctx->adj.perm, out));
   } else {
     CRK_DIM_G_DISPATCH(ctx->dim, GS,
-        hipLaunchKernelGGL((k_velgrad<DD, G, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+        hipLaunchKernelGGL((k_velgrad<DD, G, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx),
                            geo, cfg->kernel, C, n, in, ctx->adj.perm, out));
   }
   CRK_LAUNCH_CHECK("k_velgrad");
@@ -1068,7 +1147,7 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
   CdOut out{F<float>(ctx->a0), F<float>(ctx->alpha), F<float>(ctx->Rcd), st->alphas, st->alpha0s};
   int GS = ctx->group[CRK_PASS_SWITCH];
   CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_cullen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+      hipLaunchKernelGGL((k_cullen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
                          cfg->kernel, C, n, in, dt, fkern, sp.alpha_min, sp.alpha_max, sp.beta_xi, sp.limitXi,
                          cfg->evolveAlpha0, ctx->adj.perm, out));
   CRK_LAUNCH_CHECK("k_cullen");
@@ -1097,7 +1176,7 @@ int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, Cr
   AdOut out{ed->ap, ed->av, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
   int GS = ctx->group[CRK_PASS_ACCEL];
   CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_accel_dudt<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+      hipLaunchKernelGGL((k_accel_dudt<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
                          cfg->kernel, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q, cfg->alphaWeightedQ,
                          n, in, ctx->adj.perm, out));
   CRK_LAUNCH_CHECK("k_accel_dudt");

@@ -91,6 +91,15 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # (with Owen and a reused adjacency the sorted copies are current:
         # the Owen pass updated the sorted supports in place)
         abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
+    # edge arrays of this stage; the support sweep zeroes the skin edges and
+    # computes the apparent volume
+    E = adj.n_edges
+    st.ap_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
+    st.av_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
+    st.f_ij = torch.empty(E, dtype=torch.float32, device=st.positions.device)
+    ed = abi.CrkEdges(abi.ptr(st.ap_ij).value, abi.ptr(st.av_ij).value, abi.ptr(st.f_ij).value)
+    with timing.scope("support"):
+        abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
     # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
     rt.nsing.zero_()
     with timing.scope("factors_eos"):
@@ -115,11 +124,6 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         from .learned import apply_learned_switch
         apply_learned_switch(rt, cfg, st, float(dt), compParams)
     # accel + dudt (crkSPH.py:241-284), divergence and dEdt (:324-325)
-    E = adj.n_edges
-    st.ap_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
-    st.av_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
-    st.f_ij = torch.empty(E, dtype=torch.float32, device=st.positions.device)
-    ed = abi.CrkEdges(abi.ptr(st.ap_ij).value, abi.ptr(st.av_ij).value, abi.ptr(st.f_ij).value)
     with timing.scope("accel_dudt"):
         abi.call("crk_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed), s)
     # energy partition with v_half = v + dt/2 dvdt (crkSPH.py:341-364)

@@ -25,33 +25,33 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
 def pass_bytes(name: str, D: int):
-    """Algorithmic (compulsory) HBM bytes of one pass launch:
-    (bytes per particle, bytes per adjacency edge)."""
+    """Algorithmic (compulsory) HBM bytes of one pass launch as
+    (bytes per particle, bytes per adjacency edge, bytes per in-support pair).
+    Neighbour-side gathers are not counted: every particle's data is read
+    once from HBM, the other touches are cache reuse."""
     f = 4
     xD = f * D
     corr = f + 2 * xD + f * D * D              # A, B, gradA, gradB
     row = 2 * f                                 # row offset + permutation entry
     table = {
-        # x, h in; h sorted + user out
-        "owen": (xD + f + row + 2 * f, f),
-        # x, h, v, u -> V, vsig, in-support count; edges: j in; the in-support half: (j, edge) out;
-        # the skin half: ap, av zeroed
-        "support": (2 * xD + 2 * f + row + 3 * f, f + 0.5 * (2 * f) + 0.5 * (2 * xD)),
-        # moments: x, h, V, m, u -> A,B,gA,gB, rho, P, c, P/rho^gamma + 4 user fields; edges: the
-        # in-support half of the compacted neighbour list
-        "factors_eos": (xD + 4 * f + row + corr + 4 * f + 4 * f, 0.5 * f),
-        # x, h, v, V, m, rho, A,B,gA,gB -> J, div, drhodt(user); edges: compacted j
-        "velocity_gradient": (xD + f + xD + 3 * f + corr + row + f * D * D + 2 * f, 0.5 * f),
-        # x, h, m, rho, div, vsig, div_prev, alpha0 -> alpha, R, alphas(user); edges: compacted j
-        "switch": (xD + 7 * f + row + 3 * f, 0.5 * f),
-        # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt -> dvdt (sorted+user), dudt, dEdt, div;
-        # edges: compacted (j, edge) and ap, av of the in-support half
-        "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + 2 * xD + 3 * f, 0.5 * (2 * f + 4 * xD)),
-        # v, dvdt, m, sP -> ; edges: j, ap, av in, f out
-        "balance": (2 * xD + 2 * f + row, f + 2 * xD + f),
-        # v0, dvdt0, dvdt1 -> v_half -> sorted v_half; u0 -> u, dudt;
-        # edges: j, ap, av, f of both stages
-        "compatible_energy": (5 * xD + 2 * xD + row + 3 * f, f + 2 * (2 * xD + f)),
+        # x, h in; h sorted + user out; edges: j
+        "owen": (xD + f + row + 2 * f, f, 0),
+        # x, h, v, u -> V, vsig, pair count; edges: j; pairs: j out
+        "support": (2 * xD + 2 * f + row + 3 * f, f, f),
+        # x, h, V, m, u, pair count -> A,B,gA,gB, rho, P, c, P/rho^gamma + 4 user fields; pairs: j
+        "factors_eos": (xD + 4 * f + row + f + corr + 4 * f + 4 * f, 0, f),
+        # x, h, v, V, m, rho, A,B,gA,gB, count -> J, div, drhodt(user); pairs: j
+        "velocity_gradient": (xD + f + xD + 3 * f + corr + row + f + f * D * D + 2 * f, 0, f),
+        # x, h, m, rho, div, vsig, div_prev, alpha0, count -> alpha, R, alphas(user); pairs: j
+        "switch": (xD + 7 * f + row + f + 3 * f, 0, f),
+        # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt, count -> dvdt (sorted+user), dudt, dEdt, div;
+        # pairs: j in, ap, av out
+        "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + f + 2 * xD + 3 * f, 0, f + 2 * xD),
+        # v, dvdt, m, sP, count; pairs: j, ap, av in, f out
+        "balance": (2 * xD + 2 * f + row + f, 0, f + 2 * xD + f),
+        # v0, dvdt0, dvdt1 (gathered) -> sorted v_half; u0 -> u, dudt; counts of both stages;
+        # pairs of both stages: j, ap, av, f (the per-pair figure is per stage)
+        "compatible_energy": (3 * xD + 2 * xD + row + 3 * f + 2 * f, 0, f + 2 * xD + f),
     }
     return table.get(name)
 
@@ -79,7 +79,7 @@ def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
     for _ in range(warmup):
         s = integ.function(s, crkSPH_step, dt, config, scheme).state
     torch.cuda.synchronize()
-    edges = []
+    edges, pairs = [], []
     pt = PassTimer()
     t0 = time.perf_counter()
     if timer:
@@ -89,12 +89,16 @@ def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
             r = integ.function(s, crkSPH_step, dt, config, scheme)
             s = r.state
             edges.append(sum(st.aux[0].n_edges for st in r.stages) / len(r.stages))
+            last = r
     finally:
         if timer:
             pt.__exit__(None, None, None)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    return s, (t1 - t0), pt.summary() if timer else {}, float(sum(edges) / max(len(edges), 1)), dt, groups
+    # in-support pairs per stage of the last step (read after the timed region)
+    P_mean = sum(st.aux[1].pairs.n_pairs for st in last.stages) / len(last.stages)
+    return (s, (t1 - t0), pt.summary() if timer else {}, float(sum(edges) / max(len(edges), 1)), float(P_mean), dt,
+            groups)
 
 
 def cpu_baseline(nx, nsteps, warm):
@@ -151,7 +155,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
 
-    s, secs, summary, E_mean, dt, groups = run_gpu(args.nx, args.steps, args.warmup, tuning_file=args.tuning_file,
+    s, secs, summary, E_mean, P_mean, dt, groups = run_gpu(args.nx, args.steps, args.warmup, tuning_file=args.tuning_file,
                                                    retune=args.retune)
     n = args.nx * args.nx
     value = n * args.steps / secs
@@ -163,12 +167,14 @@ def main():
         name, (calls, total_ms, mean_ms) = dom
         pb = pass_bytes(name, 2)
         if pb is not None:
-            bytes_launch = pb[0] * n + pb[1] * E_mean
+            stages = 2 if name == "compatible_energy" else 1
+            bytes_launch = pb[0] * n + pb[1] * E_mean + pb[2] * P_mean * stages
             achieved = bytes_launch / (mean_ms * 1e-3)
             roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK, "traffic": None, "kernel": name,
                         "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": mean_ms,
-                        "bytes_per_particle": pb[0], "bytes_per_edge": pb[1], "edges_per_launch": E_mean}
+                        "bytes_per_particle": pb[0], "bytes_per_edge": pb[1], "bytes_per_pair": pb[2],
+                        "edges_per_launch": E_mean, "pairs_per_launch": P_mean * stages}
 
     cpu = None
     l1 = None
@@ -204,7 +210,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"2D periodic shear layer, {args.nx}x{args.nx} = {n} particles, Wendland C2 "
                                f"n_H=32, Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy",
-                   "particles": n, "edges_per_stage": E_mean, "dt": dt},
+                   "particles": n, "edges_per_stage": E_mean, "pairs_per_stage": P_mean, "dt": dt},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "density_l1_vs_oracle": None if l1 is None else {"value": l1, "sample": f"{args.cpu_nx}^2, {1 + args.cpu_steps} steps"},

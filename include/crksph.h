@@ -231,10 +231,14 @@ typedef struct CrkUpdate {         /* user order, caller owned: CompressibleSyst
   float* dEdt;    /* [n] */
 } CrkUpdate;
 
-typedef struct CrkEdges {          /* adjacency row order, caller owned */
-  float*  ap;     /* [E*dim] pairwise pressure acceleration */
-  float*  av;     /* [E*dim] pairwise viscous acceleration  */
-  float*  f;      /* [E]     energy partition               */
+typedef struct CrkEdges {          /* one stage's interacting pairs, caller owned */
+  int32_t* count; /* [n]     pairs of sorted row k with |x_ij| < max(h_i, h_j) */
+  int32_t* nbr;   /* [E]     sorted neighbour j of each pair: row k's pairs fill the
+                             slots row_offsets[k] .. row_offsets[k] + count[k] - 1
+                             (adjacency row order kept); the other slots are unused */
+  float*   ap;    /* [E*dim] pairwise pressure acceleration, same slots      */
+  float*   av;    /* [E*dim] pairwise viscous acceleration, same slots       */
+  float*   f;     /* [E]     energy partition, same slots                    */
 } CrkEdges;
 
 /* Gathers the state into sorted order (x, v, h, m, u, alpha0, divergence). */
@@ -256,11 +260,14 @@ int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_de
 /* Support sweep of the bound adjacency for the current positions and supports
  * (after the Owen update, before crk_factors_eos).  One pass over every edge:
  *  - the CRK apparent volume V_i = 1 / sum_j W(x_ij, h_i) (computeCRKFactors);
- *  - the edges with |x_ij| < max(h_i, h_j) are compacted per row (row order
- *    kept) for the kernel-weighted passes, and the other (Verlet skin) edges
- *    get ap = av = 0 in *edges (both CRK gradients vanish there);
+ *  - the pairs with |x_ij| < max(h_i, h_j) are compacted per row (row order
+ *    kept) into edges->count / edges->nbr; every later pass of the stage, and
+ *    the pair data ap, av, f, use these slots.  The other (Verlet skin) edges
+ *    have ap = av = 0 (both CRK gradients vanish there) and are not stored;
+ *    crk_edges_export expands to the full adjacency order;
  *  - with the Cullen-Dehnen switch, the signal speed
- *    max_j (c_ij - mu_ij) (wp_vsig.py:11-86) over every adjacency entry.   */
+ *    max_j (c_ij - mu_ij) (wp_vsig.py:11-86) over every adjacency entry.
+ * The context keeps edges->count and edges->nbr for the passes that follow. */
 int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* edges, void* stream);
 /* computeCRKFactors (warpSPHCore; called at crkSPH.py:61), after crk_support:
  * moments with FP64 solve -> A, B, gradA, gradB, CRK density (written to
@@ -293,6 +300,13 @@ enum {
 };
 int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group);
 
+/* Pair data of one stage in the reference layout (ap_ij, av_ij, f_ij over the
+ * adjacency edges, src/warpSPH/systems/compSPH.py:30-32): slot data of
+ * *edges scattered to adjacency order, zero on the pairs out of support.
+ * Any output may be NULL.                                                   */
+int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, const CrkEdges* edges,
+                     float* ap_out, float* av_out, float* f_out, void* stream);
+
 /* Copy a sorted-order context field back to user order (testing / export).
  * field: 0 V, 1 A, 2 B, 3 gradA, 4 gradB, 5 J(velocity gradient), 6 div(CD),
  *        7 vsig, 8 R. */
@@ -309,7 +323,7 @@ int crk_rk_combine(int64_t count, const float* y0, const float* d0, const float*
 int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream);
 /* Compatible-energy finalize (compSPH/multistep.py:52-94, systems/compSPH.py:87-129):
  * v_half = v0 + dt/2 (b0 dvdt0 + b1 dvdt1);
- * du_i = sum_k b_k sum_{e in row i of stage k} f_e (v_half_j - v_half_i).(ap_e + av_e);
+ * du_i = sum_k b_k sum_{pairs e of row i in stage k} f_e (v_half_j - v_half_i).(ap_e + av_e);
  * u = u0 + dt du; dudt_last = (u - u0)/dt.  Deterministic row sums. */
 int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, float dt,
                           const float* v0, const float* u0,

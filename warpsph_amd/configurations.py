@@ -251,14 +251,32 @@ def dictToCRKSPHConfig(d: Dict[str, Any]) -> CRKSPHConfig:
 
 
 # ------------------------------------------------------------------ C-ABI packing
+def _host_values(t) -> list:
+    """Host copy of a small (domain) tensor, cached on the tensor while its
+    storage and version are unchanged, so that packing the configuration on
+    every call does not wait for the GPU."""
+    if not isinstance(t, torch.Tensor):
+        return list(t)
+    key = (t.data_ptr(), t._version, tuple(t.shape), t.dtype)
+    cached = getattr(t, "_crk_host_values", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    vals = t.detach().cpu().tolist()
+    try:
+        t._crk_host_values = (key, vals)
+    except (AttributeError, RuntimeError):
+        pass
+    return vals
+
+
 def toCrkConfig(config: SimulationConfig, scheme: CRKSPHConfig) -> abi.CrkConfig:
     """Pack the dataclasses into the POD parameter block of include/crksph.h."""
     c = abi.CrkConfig()
     dom = config.domain
     c.domain.dim = int(dom.dim)
-    mn = dom.min.detach().cpu().tolist()
-    mx = dom.max.detach().cpu().tolist()
-    per = dom.periodic.detach().cpu().tolist()
+    mn = _host_values(dom.min)
+    mx = _host_values(dom.max)
+    per = _host_values(dom.periodic)
     for d in range(dom.dim):
         c.domain.min[d] = float(mn[d])
         c.domain.max[d] = float(mx[d])

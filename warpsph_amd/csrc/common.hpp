@@ -54,8 +54,10 @@ struct CrkContext {
   int32_t dim = 0;
   int64_t cap_n = 0;
   crk::Buf x, v, h, m, u, a0, divp, alpha, V, A, B, gA, gB, rho, P, c, sP, J, divcd, vsig, Rcd, dvdt, vh, vhs, du0;
-  // in-support edges of each row, compacted in row order (crk_support)
-  crk::Buf act_nbr, act_pos, act_cnt;
+  // in-support pairs of each row, compacted in row order (caller-owned
+  // buffers of the stage, bound by crk_support)
+  int32_t* pair_cnt = nullptr;
+  int32_t* pair_nbr = nullptr;
   bool have_support = false;
   // lanes per particle row for each pass (crk_set_group_size)
   int32_t group[CRK_NPASS] = {4, 8, 2, 8, 8, 16, 32, 32, 32};

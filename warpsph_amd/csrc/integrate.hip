@@ -31,28 +31,26 @@ __global__ void k_wrap(int32_t n, int D, Geo g, float lo0, float lo1, float lo2,
   }
 }
 
-// v_half = v0 + 0.5 * ((0 + b0 dvdt0) + b1 dvdt1) * dt  (multistep.py:63-68)
-__global__ void k_vhalf(int64_t cnt, const float* __restrict__ v0, const float* __restrict__ a0,
-                        const float* __restrict__ a1, float b0, float b1, float dt, float* __restrict__ vh) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cnt) return;
-  float fa = 0.f + b0 * a0[i];
-  fa = fa + b1 * a1[i];
-  vh[i] = v0[i] + 0.5f * fa * dt;
-}
-
-// v_half gathered into an adjacency's sorted order
+// v_half = v0 + 0.5 * ((0 + b0 dvdt0) + b1 dvdt1) * dt  (multistep.py:63-68),
+// written in an adjacency's sorted order
 template <int D>
-__global__ void k_gather_vec(const int32_t* __restrict__ perm, int32_t n, const float* __restrict__ src,
-                             float* __restrict__ dst) {
+__global__ void k_vhalf_sorted(const int32_t* __restrict__ perm, int32_t n, const float* __restrict__ v0,
+                               const float* __restrict__ a0, const float* __restrict__ a1, float b0, float b1,
+                               float dt, float* __restrict__ vhs) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  int32_t i = perm[k];
+  int64_t i = perm[k];
 #pragma unroll
-  for (int d = 0; d < D; ++d) dst[(int64_t)k * D + d] = src[(int64_t)i * D + d];
+  for (int d = 0; d < D; ++d) {
+    float fa = 0.f + b0 * a0[i * D + d];
+    fa = fa + b1 * a1[i * D + d];
+    vhs[(int64_t)k * D + d] = v0[i * D + d] + 0.5f * fa * dt;
+  }
 }
 
+// one stage's in-support pairs (slots [off[k], off[k] + cnt[k])) and pair data
 struct CompIn {
+  const int32_t *off, *cnt, *nbr;
   const float *ap, *av, *f;
   float b;
 };
@@ -72,15 +70,32 @@ __device__ inline float group_total2(float s, float c) {
   return s - c;
 }
 
-// Row sums of b_k f (v_half_j - v_half_i).(ap + av) over the rows of one
-// adjacency (multistep.py:70-94), G lanes per row.  NST = 2: both stages
-// share the adjacency and are summed in the same sweep (separate sums, added
-// in stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last
-// sweep (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep
-// of both stages.
+// Row sums of b_k f (v_half_j - v_half_i).(ap + av) over the in-support
+// pairs of each row (multistep.py:70-94; the skin pairs of the reference's
+// Verlet list carry ap = av = 0), G lanes per row.  NST = 2: both stages use
+// the same sorted order and are summed in one sweep (separate sums, added in
+// stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last sweep
+// (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep of
+// both stages.
+template <int D, int G>
+__device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g, const float* __restrict__ vhs,
+                                  const float* vi, float& sa, float& ca) {
+  const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
+  for (int32_t q = b + g; q < e; q += G) {
+    const int32_t j = s.nbr[q];
+    float d0 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      d0 = d0 + (vhs[(int64_t)j * D + a] - vi[a]) * (s.ap[(int64_t)q * D + a] + s.av[(int64_t)q * D + a]);
+    float y = s.b * s.f[q] * d0 - ca;
+    float t = sa + y;
+    ca = (t - sa) - y;
+    sa = t;
+  }
+}
+
 template <int D, int G, int NST, int MODE>
-__global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, const int32_t* __restrict__ off,
-                                                   const int32_t* __restrict__ nbr, int32_t n,
+__global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, int32_t n,
                                                    const float* __restrict__ vhs, CompIn s0, CompIn s1,
                                                    float* __restrict__ du0, const float* __restrict__ u0, float dt,
                                                    float* __restrict__ u, float* __restrict__ dudt_last) {
@@ -92,31 +107,8 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
 #pragma unroll
   for (int d = 0; d < D; ++d) vi[d] = vhs[(int64_t)k * D + d];
   float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
-  const int32_t b = off[k], e = valid ? off[k + 1] : b;
-  for (int32_t q = b + g; q < e; q += G) {
-    int32_t j = nbr[q];
-    float dv[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) dv[a] = vhs[(int64_t)j * D + a] - vi[a];
-    float d0 = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) d0 = d0 + dv[a] * (s0.ap[(int64_t)q * D + a] + s0.av[(int64_t)q * D + a]);
-    {
-      float y = s0.b * s0.f[q] * d0 - ca;
-      float t = sa + y;
-      ca = (t - sa) - y;
-      sa = t;
-    }
-    if (NST == 2) {
-      float d1 = 0.f;
-#pragma unroll
-      for (int a = 0; a < D; ++a) d1 = d1 + dv[a] * (s1.ap[(int64_t)q * D + a] + s1.av[(int64_t)q * D + a]);
-      float y = s1.b * s1.f[q] * d1 - cb;
-      float t = sb + y;
-      cb = (t - sb) - y;
-      sb = t;
-    }
-  }
+  compat_sum<D, G>(s0, k, valid, g, vhs, vi, sa, ca);
+  if (NST == 2) compat_sum<D, G>(s1, k, valid, g, vhs, vi, sb, cb);
   float ta = group_total2<G>(sa, ca);
   float tb = NST == 2 ? group_total2<G>(sb, cb) : 0.f;
   if (!valid || g != 0) return;
@@ -267,39 +259,39 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   CRK_REQUIRE(ctx && cfg && v0 && u0 && dvdt0 && dvdt1 && adj0 && e0 && adj1 && e1 && u_out && dudt_last,
               "crk_compatible_energy: null argument");
   CRK_REQUIRE(n > 0 && adj0->n == n && adj1->n == n, "crk_compatible_energy: particle count mismatch");
-  CRK_REQUIRE(adj0->n_edges == 0 || (e0->ap && e0->av && e0->f), "stage-0 edges missing");
-  CRK_REQUIRE(adj1->n_edges == 0 || (e1->ap && e1->av && e1->f), "stage-1 edges missing");
+  CRK_REQUIRE(adj0->perm && adj1->perm && adj0->row_offsets && adj1->row_offsets && e0->count && e1->count,
+              "crk_compatible_energy: adjacency/pair counts missing");
+  CRK_REQUIRE(adj0->n_edges == 0 || (e0->nbr && e0->ap && e0->av && e0->f), "stage-0 pairs missing");
+  CRK_REQUIRE(adj1->n_edges == 0 || (e1->nbr && e1->ap && e1->av && e1->f), "stage-1 pairs missing");
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
   hipStream_t s = (hipStream_t)stream;
   int st;
-  if ((st = ensure(ctx->vh, sizeof(float) * (size_t)n * D))) return st;
   if ((st = ensure(ctx->vhs, sizeof(float) * (size_t)n * D))) return st;
   if ((st = ensure(ctx->du0, sizeof(float) * (size_t)n))) return st;
-  float* vh = (float*)ctx->vh.p;
   float* vhs = (float*)ctx->vhs.p;
   float* du0 = (float*)ctx->du0.p;
-  int64_t cnt = (int64_t)n * D;
-  hipLaunchKernelGGL(k_vhalf, dim3(nblocks(cnt)), dim3(kBlock), 0, s, cnt, v0, dvdt0, dvdt1, b0, b1, dt, vh);
-  CRK_LAUNCH_CHECK("k_vhalf");
-  const bool shared = adj0->perm == adj1->perm && adj0->row_offsets == adj1->row_offsets && adj0->nbr == adj1->nbr &&
-                      adj0->n_edges == adj1->n_edges;
-  CompIn c0{e0->ap, e0->av, e0->f, b0}, c1{e1->ap, e1->av, e1->f, b1};
+  // both stages in one sweep when they share the sorted order
+  const bool shared = adj0->perm == adj1->perm;
+  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->ap, e0->av, e0->f, b0};
+  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->ap, e1->av, e1->f, b1};
   const int GS = ctx->group[CRK_PASS_COMPAT];
   unsigned nb = nblocks((int64_t)n * GS);
-#define GATHER(DD, A) hipLaunchKernelGGL(k_gather_vec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, (A)->perm, n, vh, vhs)
-#define COMPAT(DD, GG, NST, MODE, A, X, Y)                                                                       \
-  hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, (A)->row_offsets,    \
-                     (A)->nbr, n, vhs, X, Y, du0, u0, dt, u_out, dudt_last)
-#define RUN(DD, GG)                                          \
-  if (shared) {                                              \
-    GATHER(DD, adj0);                                        \
-    COMPAT(DD, GG, 2, 2, adj0, c0, c1);                      \
-  } else {                                                   \
-    GATHER(DD, adj0);                                        \
-    COMPAT(DD, GG, 1, 0, adj0, c0, c0);                      \
-    GATHER(DD, adj1);                                        \
-    COMPAT(DD, GG, 1, 1, adj1, c1, c1);                      \
+#define VHALF(DD, A)                                                                                          \
+  hipLaunchKernelGGL(k_vhalf_sorted<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, (A)->perm, n, v0, dvdt0, dvdt1, b0, \
+                     b1, dt, vhs)
+#define COMPAT(DD, GG, NST, MODE, A, X, Y)                                                                     \
+  hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, n, vhs, X, Y, du0, \
+                     u0, dt, u_out, dudt_last)
+#define RUN(DD, GG)                     \
+  if (shared) {                         \
+    VHALF(DD, adj0);                    \
+    COMPAT(DD, GG, 2, 2, adj0, c0, c1); \
+  } else {                              \
+    VHALF(DD, adj0);                    \
+    COMPAT(DD, GG, 1, 0, adj0, c0, c0); \
+    VHALF(DD, adj1);                    \
+    COMPAT(DD, GG, 1, 1, adj1, c1, c1); \
   }
 #define RUNG(DD)                                  \
   switch (GS) {                                   \
@@ -320,7 +312,7 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
 #undef RUNG
 #undef RUN
 #undef COMPAT
-#undef GATHER
+#undef VHALF
   CRK_LAUNCH_CHECK("k_compat");
   return CRK_OK;
 }

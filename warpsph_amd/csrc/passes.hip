@@ -193,22 +193,21 @@ struct Rows {
   const int32_t* nbr;
 };
 
-// In-support edges of each row (|x_ij| < max(h_i, h_j)), compacted in row
-// order into [off[k], off[k] + cnt[k]) of nbr (neighbour) and pos (edge index
-// in the full adjacency).  About half of a Verlet row is skin; the
-// kernel-weighted passes iterate only these edges, so every lane of a group
+// In-support pairs of each row (|x_ij| < max(h_i, h_j)), compacted in row
+// order into the slots [off[k], off[k] + cnt[k]) of nbr; the pair data (ap,
+// av, f) live in the same slots.  About half of a Verlet row is skin; the
+// kernel-weighted passes iterate only these pairs, so every lane of a group
 // works on a contributing pair.
 struct ARows {
   const int32_t* off;
   const int32_t* cnt;
   const int32_t* nbr;
-  const int32_t* pos;
 };
 
 // ------------------------------------------------------------ support sweep
 // One pass over the full adjacency after the Owen update: the CRK apparent
-// volume V_i = 1 / sum_j W(x_ij, h_i); the in-support edges compacted per row
-// with group ballots; ap = av = 0 for the skin edges; and the signal speed
+// volume V_i = 1 / sum_j W(x_ij, h_i); the in-support pairs compacted per row
+// with group ballots; and the signal speed
 // v_sig = max_j (0.5 (c_i + c_j) - mu_ij if mu_ij <= 0 else 0),
 // mu_ij = (v_i - v_j).x_ij / (r_ij + 1e-14 h_i), over every entry
 // (wp_vsig.py:55-86), c = sqrt(|u| gamma (gamma - 1)).
@@ -217,9 +216,8 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, int kern
                                                     const float* __restrict__ x, const float* __restrict__ h,
                                                     const float* __restrict__ v, const float* __restrict__ u,
                                                     float cfac, int32_t* __restrict__ anbr,
-                                                    int32_t* __restrict__ apos, int32_t* __restrict__ acnt,
-                                                    float* __restrict__ vsig, float* __restrict__ V,
-                                                    float* __restrict__ ap, float* __restrict__ av) {
+                                                    int32_t* __restrict__ acnt, float* __restrict__ vsig,
+                                                    float* __restrict__ V) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3], vi[3];
@@ -263,16 +261,10 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, int kern
         vol.add(C * kk * ipow<D>(ih));
       }
       act = r < fmaxf(hi, h[j]);
-      if (ap) {  // whole rows zeroed with coalesced stores; accel overwrites the in-support edges
-#pragma unroll
-        for (int a = 0; a < D; ++a) { ap[(int64_t)q * D + a] = 0.f; av[(int64_t)q * D + a] = 0.f; }
-      }
     }
     const unsigned long long m = __ballot(act ? 1 : 0) & gmask;
     if (act) {
-      const int32_t slot = b + run + __popcll(m & lt);
-      anbr[slot] = j;
-      apos[slot] = q;
+      anbr[b + run + __popcll(m & lt)] = j;
     }
     run += __popcll(m);
   }
@@ -780,7 +772,6 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, int 
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
     const int32_t j = rows.nbr[q0];
-    const int64_t q = rows.pos[q0];  // edge index in the full adjacency
     float dx[3];
     min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     float r = sqrtf(dot<D>(dx, dx));
@@ -847,8 +838,8 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, int 
     for (int a = 0; a < D; ++a) {
       float apv = pP * gw[a];
       float avv = pQ * gw[a];
-      o.ap[(int64_t)q * D + a] = apv;
-      o.av[(int64_t)q * D + a] = avv;
+      o.ap[(int64_t)q0 * D + a] = apv;
+      o.av[(int64_t)q0 * D + a] = avv;
       acc[a].add(apv + avv);
     }
     float dt_ = 0.f;
@@ -876,7 +867,7 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, int 
 
 // ------------------------------------------------------------ balance
 template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_balance(Rows rows, int32_t n, const float* __restrict__ v,
+__global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float* __restrict__ v,
                                                     const float* __restrict__ dvdt, const float* __restrict__ m,
                                                     const float* __restrict__ sP, float dt, float hdt,
                                                     const float* __restrict__ ap, const float* __restrict__ av,
@@ -888,7 +879,7 @@ __global__ void __launch_bounds__(kBlock) k_balance(Rows rows, int32_t n, const 
   for (int d = 0; d < D; ++d) vhi[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
   const float si = sP[k];
   const float mdt = m[k] * dt;
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   for (int32_t q = b + L.g; q < e; q += G) {
     int32_t j = rows.nbr[q];
     float d_ = 0.f;
@@ -913,6 +904,32 @@ __global__ void __launch_bounds__(kBlock) k_balance(Rows rows, int32_t n, const 
 }
 
 // ------------------------------------------------------------ export
+// Slot data of a row to adjacency order: the pairs are the row's edges with
+// r < max(h_i, h_j) in row order, so one merge walk over the row matches
+// them (neighbours are distinct within a row).
+template <int D>
+__global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* __restrict__ nbr,
+                               const int32_t* __restrict__ cnt, const int32_t* __restrict__ pnbr, int32_t n,
+                               const float* __restrict__ ap, const float* __restrict__ av,
+                               const float* __restrict__ f, float* __restrict__ ap_o, float* __restrict__ av_o,
+                               float* __restrict__ f_o) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t b = off[k], e = off[k + 1], c = cnt[k];
+  int32_t p = 0;
+  for (int32_t q = b; q < e; ++q) {
+    const bool hit = p < c && pnbr[b + p] == nbr[q];
+    const int64_t sl = b + p;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      if (ap_o) ap_o[(int64_t)q * D + a] = hit ? ap[sl * D + a] : 0.f;
+      if (av_o) av_o[(int64_t)q * D + a] = hit ? av[sl * D + a] : 0.f;
+    }
+    if (f_o) f_o[q] = hit ? f[sl] : 0.f;
+    p += hit ? 1 : 0;
+  }
+}
+
 template <int C>
 __global__ void k_scatter_field(const float* __restrict__ src, const int32_t* __restrict__ perm, int32_t n,
                                 float* __restrict__ dst) {
@@ -946,9 +963,7 @@ template <class T>
 static T* F(Buf& b) { return (T*)b.p; }
 
 static Rows rows_of(const CrkContext* ctx) { return Rows{ctx->adj.row_offsets, ctx->adj.nbr}; }
-static ARows arows_of(CrkContext* ctx) {
-  return ARows{ctx->adj.row_offsets, F<int32_t>(ctx->act_cnt), F<int32_t>(ctx->act_nbr), F<int32_t>(ctx->act_pos)};
-}
+static ARows arows_of(CrkContext* ctx) { return ARows{ctx->adj.row_offsets, ctx->pair_cnt, ctx->pair_nbr}; }
 
 inline unsigned group_blocks(int32_t n, int G) { return nblocks((int64_t)n * G); }
 
@@ -1050,13 +1065,9 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
 int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* stream) {
   int r = require_bound(ctx, cfg);
   if (r) return r;
-  CRK_REQUIRE(!ed || ctx->adj.n_edges == 0 || (ed->ap && ed->av), "crk_support: edge buffers missing");
+  CRK_REQUIRE(ed && ed->count && (ctx->adj.n_edges == 0 || ed->nbr), "crk_support: pair buffers (count, nbr) missing");
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
-  int64_t E = ctx->adj.n_edges;
-  if ((r = ensure(ctx->act_nbr, sizeof(int32_t) * (size_t)(E > 0 ? E : 1)))) return r;
-  if ((r = ensure(ctx->act_pos, sizeof(int32_t) * (size_t)(E > 0 ? E : 1)))) return r;
-  if ((r = ensure(ctx->act_cnt, sizeof(int32_t) * (size_t)n))) return r;
   Geo geo = make_geo(*cfg);
   float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   float cfac = cfg->gamma * (cfg->gamma - 1.f);
@@ -1066,9 +1077,10 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   CRK_DIM_G_DISPATCH(ctx->dim, GS,
       hipLaunchKernelGGL((k_support<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
                          cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u),
-                         cfac, F<int32_t>(ctx->act_nbr), F<int32_t>(ctx->act_pos), F<int32_t>(ctx->act_cnt), vs,
-                         F<float>(ctx->V), ed ? ed->ap : nullptr, ed ? ed->av : nullptr));
+                         cfac, ed->nbr, ed->count, vs, F<float>(ctx->V)));
   CRK_LAUNCH_CHECK("k_support");
+  ctx->pair_cnt = ed->count;
+  ctx->pair_nbr = ed->nbr;
   ctx->have_support = true;
   return CRK_OK;
 }
@@ -1161,6 +1173,8 @@ int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, Cr
   CRK_REQUIRE(st && st->divergence, "crk_accel_dudt: state->divergence missing");
   CRK_REQUIRE(up && up->dvdt && up->dudt && up->dEdt && up->drhodt, "crk_accel_dudt: update buffers missing");
   CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av)), "crk_accel_dudt: edge buffers missing");
+  CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
+              "crk_accel_dudt: edges differ from the pairs of crk_support");
   if (!ctx->have_velgrad) { set_error("crk_accel_dudt before crk_velocity_gradient"); return CRK_ESTATE; }
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
@@ -1188,16 +1202,35 @@ int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, v
   int r = require_bound(ctx, cfg);
   if (r) return r;
   CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av && ed->f)), "crk_balance: edge buffers missing");
+  CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
+              "crk_balance: edges differ from the pairs of crk_support");
   if (!ctx->have_accel) { set_error("crk_balance before crk_accel_dudt"); return CRK_ESTATE; }
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   float hdt = (float)(0.5 * (double)dt);
   int GS = ctx->group[CRK_PASS_BALANCE];
   CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), n,
+      hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n,
                          F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->m), F<float>(ctx->sP), dt, hdt, ed->ap,
                          ed->av, ed->f));
   CRK_LAUNCH_CHECK("k_balance");
+  return CRK_OK;
+}
+
+int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, const CrkEdges* ed, float* ap_out,
+                     float* av_out, float* f_out, void* stream) {
+  CRK_REQUIRE(ctx && adj && ed, "crk_edges_export: null argument");
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3");
+  CRK_REQUIRE(adj->n > 0 && adj->row_offsets && ed->count, "crk_edges_export: adjacency/pairs missing");
+  if (adj->n_edges == 0) return CRK_OK;
+  CRK_REQUIRE(adj->nbr && ed->nbr, "crk_edges_export: neighbour lists missing");
+  CRK_REQUIRE((!ap_out || ed->ap) && (!av_out || ed->av) && (!f_out || ed->f), "crk_edges_export: pair data missing");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = adj->n;
+  CRK_DIM_DISPATCH(dim, hipLaunchKernelGGL(k_edges_export<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->row_offsets,
+                                           adj->nbr, ed->count, ed->nbr, n, ed->ap, ed->av, ed->f, ap_out, av_out,
+                                           f_out));
+  CRK_LAUNCH_CHECK("k_edges_export");
   return CRK_OK;
 }
 

@@ -44,11 +44,11 @@ class RungeKutta2:
         dt = float(dt)
         rt = runtime(st0.positions.device)
         s = rt.stream()
-        # stage 1
-        sys1 = initial.initializeNewState()
+        # stage 1 reads the initial positions, velocities and energies
+        sys1 = initial.stageSystem("share")
         upd1, adj1, st1 = f(sys1, dt, config, compParams, verbose=verbose)
         # stage 2 starts from stage 1's state with y0 + dt k1
-        sys2 = sys1.initializeNewState()
+        sys2 = sys1.stageSystem("alloc")
         sys2.adjacency = adj1
         s2 = sys2.state
         nD = st0.positions.numel()

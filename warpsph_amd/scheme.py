@@ -17,7 +17,7 @@ from .configurations import (CRKSPHConfig, SimulationConfig, crkSPHConfigToDict,
                              toCrkConfig)
 from .enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, ViscositySwitch
 from .runtime import ensure_adjacency, runtime
-from .systems import CompressibleSystemUpdate, CompSPHState, CompSPHSystem
+from .systems import CompressibleSystemUpdate, CompSPHState, CompSPHSystem, StagePairs
 
 
 def _require(st: CompSPHState, names):
@@ -45,7 +45,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
 
     Side effects on ``system.state``: supports, densities, entropies,
     pressures, soundspeeds, alphas (alpha0s when evolveAlpha0), divergence,
-    ap_ij, av_ij, f_ij; ``system.adjacency`` is the adjacency used.
+    pairs (read as ap_ij, av_ij, f_ij); ``system.adjacency`` is the adjacency used.
     Returns ``(update, adjacency, state)``.
     """
     dt = float(dt.detach().cpu().item()) if isinstance(dt, torch.Tensor) else float(dt)
@@ -76,8 +76,11 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
             with timing.scope("owen"):
                 abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm),
                          abi.ptr(ratio if iters > 1 else None), s)
+            if it + 1 == iters:
+                break
+            # hMin, hMax of the new supports bound the next iteration (optimalSupportOwen.py:103-104)
             abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
-            if iters > 1 and float(ratio.item()) < compParams.adaptiveSupportThreshold:
+            if float(ratio.item()) < compParams.adaptiveSupportThreshold:
                 break
         # The reference's final gather-density pass (:116-125) is overwritten by
         # computeCRKFactors (crkSPH.py:61) before any use; it is not evaluated.
@@ -91,13 +94,10 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # (with Owen and a reused adjacency the sorted copies are current:
         # the Owen pass updated the sorted supports in place)
         abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
-    # edge arrays of this stage; the support sweep zeroes the skin edges and
-    # computes the apparent volume
-    E = adj.n_edges
-    st.ap_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
-    st.av_ij = torch.empty((E, dim), dtype=torch.float32, device=st.positions.device)
-    st.f_ij = torch.empty(E, dtype=torch.float32, device=st.positions.device)
-    ed = abi.CrkEdges(abi.ptr(st.ap_ij).value, abi.ptr(st.av_ij).value, abi.ptr(st.f_ij).value)
+    # the interacting pairs of this stage (compacted by the support sweep,
+    # which also computes the apparent volume) and their pair data
+    st.pairs = StagePairs(adj, dim)
+    ed = st.pairs.cabi()
     with timing.scope("support"):
         abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
     # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
@@ -105,7 +105,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     with timing.scope("factors_eos"):
         abi.call("crk_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(rt.nsing), s)
     update = CompressibleSystemUpdate(
-        dxdt=st.velocities.clone(),
+        dxdt=st.velocities,  # dx/dt = v (the stage's velocity tensor, read only)
         dvdt=torch.empty_like(st.velocities),
         dudt=torch.empty_like(st.internalEnergies),
         dEdt=torch.empty_like(st.internalEnergies),

@@ -10,6 +10,7 @@ user particle order).
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 from dataclasses import dataclass
 from typing import Any, List, Optional
@@ -21,6 +22,54 @@ from . import abi
 
 def _clone(v):
     return v.clone() if isinstance(v, torch.Tensor) else v
+
+
+class StagePairs:
+    """The interacting pairs of one derivative evaluation and their data.
+
+    ``count[k]`` pairs of sorted row k (|x_ij| < max(h_i, h_j)) fill the
+    slots ``row_offsets[k] ..`` of ``nbr``; ``ap``, ``av`` and ``f`` use the
+    same slots.  The reference keeps ap_ij, av_ij and f_ij over every edge of
+    its Verlet list (src/warpSPH/systems/compSPH.py:30-32), with zeros on the
+    pairs out of support; ``full()`` produces that layout on demand.
+    """
+
+    def __init__(self, adjacency, dim: int):
+        dev = adjacency.perm.device
+        E = max(adjacency.n_edges, 1)
+        self.adjacency = adjacency
+        self.dim = int(dim)
+        self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)
+        self.nbr = torch.empty(E, dtype=torch.int32, device=dev)
+        self.ap = torch.empty((E, dim), dtype=torch.float32, device=dev)
+        self.av = torch.empty((E, dim), dtype=torch.float32, device=dev)
+        self.f = torch.empty(E, dtype=torch.float32, device=dev)
+        self._full = None
+
+    def cabi(self) -> abi.CrkEdges:
+        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.ap, self.av, self.f)))
+
+    @property
+    def n_pairs(self) -> int:
+        return int(self.count.sum().item())
+
+    def full(self):
+        if self._full is None:
+            from .runtime import runtime
+            a = self.adjacency
+            E = a.n_edges
+            dev = self.count.device
+            ap = torch.empty((E, self.dim), dtype=torch.float32, device=dev)
+            av = torch.empty((E, self.dim), dtype=torch.float32, device=dev)
+            f = torch.empty(E, dtype=torch.float32, device=dev)
+            if E > 0:
+                rt = runtime(dev)
+                A = a.cabi()
+                ed = self.cabi()
+                abi.call("crk_edges_export", rt.ctx, self.dim, C.byref(A), C.byref(ed), abi.ptr(ap), abi.ptr(av),
+                         abi.ptr(f), rt.stream())
+            self._full = (ap, av, f)
+        return self._full
 
 
 @dataclass
@@ -42,20 +91,64 @@ class CompSPHState:
     divergence: Optional[torch.Tensor] = None
     alpha0s: Optional[torch.Tensor] = None
     alphas: Optional[torch.Tensor] = None
-    ap_ij: Optional[torch.Tensor] = None
-    av_ij: Optional[torch.Tensor] = None
-    f_ij: Optional[torch.Tensor] = None
+    # pair data of the last derivative evaluation (ap_ij, av_ij, f_ij below)
+    pairs: Optional[StagePairs] = None
 
     # fields integrated by the RK driver (integrated(...) in the reference)
     INTEGRATED = ("positions", "velocities", "internalEnergies")
 
-    EDGE_FIELDS = ("ap_ij", "av_ij", "f_ij")
+    EDGE_FIELDS = ("pairs",)
 
     def initializeNewState(self) -> "CompSPHState":
         """Copy of the per-particle fields; the edge arrays are rebuilt by the
         next derivative evaluation and are not copied."""
         return CompSPHState(**{f.name: (None if f.name in self.EDGE_FIELDS else _clone(getattr(self, f.name)))
                                for f in dataclasses.fields(self)})
+
+    # How one derivative evaluation uses the per-particle fields: read only;
+    # overwritten before any read; read and then updated in place.
+    READ_ONLY = ("masses", "kinds", "materials", "UIDs", "totalEnergies")
+    OUTPUTS = ("densities", "entropies", "pressures", "soundspeeds")
+    CARRIED = ("supports", "divergence", "alpha0s", "alphas")
+
+    def stageState(self, integrated: str) -> "CompSPHState":
+        """State of one stage inside the integrator: read-only fields are
+        shared, carried fields copied, outputs freshly allocated.  The
+        integrated fields (positions, velocities, internalEnergies) are shared
+        with this state when ``integrated == "share"`` (the stage only reads
+        them) or freshly allocated when ``integrated == "alloc"`` (the
+        integrator writes the stage values)."""
+        if integrated not in ("share", "alloc"):
+            raise ValueError(f"integrated must be 'share' or 'alloc', got {integrated!r}")
+        kw = {}
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if f.name in self.EDGE_FIELDS:
+                v = None
+            elif isinstance(v, torch.Tensor):
+                if f.name in self.INTEGRATED:
+                    v = v if integrated == "share" else torch.empty_like(v)
+                elif f.name in self.OUTPUTS:
+                    v = torch.empty_like(v)
+                elif f.name not in self.READ_ONLY:
+                    v = v.clone()
+            kw[f.name] = v
+        return CompSPHState(**kw)
+
+    @property
+    def ap_ij(self) -> Optional[torch.Tensor]:
+        """[E, dim] pairwise pressure acceleration over the adjacency edges."""
+        return None if self.pairs is None else self.pairs.full()[0]
+
+    @property
+    def av_ij(self) -> Optional[torch.Tensor]:
+        """[E, dim] pairwise viscous acceleration over the adjacency edges."""
+        return None if self.pairs is None else self.pairs.full()[1]
+
+    @property
+    def f_ij(self) -> Optional[torch.Tensor]:
+        """[E] energy partition over the adjacency edges."""
+        return None if self.pairs is None else self.pairs.full()[2]
 
     @property
     def numParticles(self) -> int:
@@ -184,6 +277,11 @@ class CompSPHSystem:
     domain: Any = None
     t: float = 0.0
 
+    def stageSystem(self, integrated: str) -> "CompSPHSystem":
+        """System of one integrator stage (see CompSPHState.stageState)."""
+        return CompSPHSystem(state=self.state.stageState(integrated), adjacency=self.adjacency, t=self.t,
+                             domain=self.domain)
+
     def initializeNewState(self, *args, verbose=False, **kwargs) -> "CompSPHSystem":
         return CompSPHSystem(state=self.state.initializeNewState(), adjacency=self.adjacency, t=self.t,
                              domain=self.domain)
@@ -204,8 +302,8 @@ class CompSPHSystem:
             rt = runtime(st0.positions.device)
             cfg = toCrkConfig(config, compParams)
             (a0, s0), (a1, s1) = returnValues
-            e0 = abi.CrkEdges(abi.ptr(s0.ap_ij).value, abi.ptr(s0.av_ij).value, abi.ptr(s0.f_ij).value)
-            e1 = abi.CrkEdges(abi.ptr(s1.ap_ij).value, abi.ptr(s1.av_ij).value, abi.ptr(s1.f_ij).value)
+            e0 = s0.pairs.cabi()
+            e1 = s1.pairs.cabi()
             u_new = torch.empty_like(st0.internalEnergies)
             A0, A1 = a0.cabi(), a1.cabi()
             from . import timing

@@ -288,6 +288,14 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
  * edges->ap, edges->av, and st->divergence = -drhodt/rho (crkSPH.py:324). */
 int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                    CrkUpdate* up, CrkEdges* edges, void* stream);
+/* crk_switch followed by crk_accel_dudt in one sweep over the pairs: the
+ * switch's kernel sum (CullenDehnen2010.py:218-331) is accumulated while the
+ * accelerations are computed.  Same results as the two calls; only valid
+ * when the viscosity does not read alpha (cfg->alphaWeightedQ == 0), since
+ * the reference evaluates the switch first (crkSPH.py:195-200 before
+ * :241-284).  Writes everything both calls write.                          */
+int crk_switch_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                          CrkUpdate* up, CrkEdges* edges, float dt, void* stream);
 /* computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154)
  * with v_half = v + dt/2 dvdt (crkSPH.py:341).  Writes edges->f. */
 int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* edges, void* stream);

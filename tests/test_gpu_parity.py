@@ -220,3 +220,35 @@ def test_stage_parity_any_group_size(group):
     errs["ap"] = relerr(st.ap_ij.cpu().numpy()[ep], sim.edge_field(0, "ap").reshape(-1, 2))
     print(errs)
     assert all(v <= RTOL for v in errs.values()), errs
+
+
+@pytest.mark.parametrize("fuse", [False, True])
+def test_switch_fused_or_separate(fuse):
+    """The Cullen-Dehnen switch evaluated in its own pass (crk_switch) and
+    inside the acceleration sweep (crk_switch_accel_dudt) both give the
+    oracle's alphas and R (shear layer, evolveAlpha0 off)."""
+    from oracle import oracle as O
+    from warpsph_amd import abi, sample
+    from warpsph_amd.runtime import runtime
+    from warpsph_amd.scheme import crkSPH_step
+    announce(f"stage shear2d fuse_switch={fuse}")
+    system, config, scheme = make_case("shear2d")
+    cpu = cpu_copy(system, config, scheme)
+    dt = sample.cfl_dt(system.state, config, scheme) * (2.0 / 3.0)
+    rt = runtime(system.state.positions.device)
+    rt.fuse_switch = fuse
+    try:
+        update, adj, st = crkSPH_step(system, dt, config, scheme)
+        n = st.positions.shape[0]
+        R = torch.empty(n, dtype=torch.float32, device=st.positions.device)
+        abi.call("crk_export_field", rt.ctx, 8, abi.ptr(R), rt.stream())
+        torch.cuda.synchronize()
+    finally:
+        rt.fuse_switch = True
+    sim = O.from_system(cpu, config, scheme, 32)
+    sim.evaluate(dt)
+    errs = {"alpha": relerr(st.alphas.cpu().numpy(), sim.get("alpha")),
+            "R": relerr(R.cpu().numpy(), sim.get("Rcd")),
+            "dvdt": relerr(update.dvdt.cpu().numpy(), sim.get("dvdt"))}
+    print(errs)
+    assert all(v <= RTOL for v in errs.values()), errs

@@ -126,6 +126,8 @@ _PROTOS = {
     "crk_switch": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.c_float, _P],
     "crk_accel_dudt": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.POINTER(CrkUpdate),
                        C.POINTER(CrkEdges), _P],
+    "crk_switch_accel_dudt": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.POINTER(CrkUpdate),
+                              C.POINTER(CrkEdges), C.c_float, _P],
     "crk_balance": [_P, C.POINTER(CrkConfig), C.c_float, C.POINTER(CrkEdges), _P],
     "crk_export_field": [_P, C.c_int32, _P, _P],
     "crk_edges_export": [_P, C.c_int32, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, _P, _P, _P],

@@ -57,7 +57,9 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
     base = integ.function(base, crkSPH_step, dt, config, scheme).state
     torch.cuda.synchronize()
     names = list(passes) if passes is not None else list(SCOPES)
-    if scheme.viscositySwitchParams.scheme.name != "CullenDehnen2010" and "switch" in names:
+    separate_switch = (scheme.viscositySwitchParams.scheme.name == "CullenDehnen2010" and
+                       (not rt.fuse_switch or bool(getattr(scheme, "alphaWeightedViscosity", False))))
+    if not separate_switch and "switch" in names:
         names.remove("switch")
     report = {}
     for name in names:

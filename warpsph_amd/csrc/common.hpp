@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -153,6 +154,7 @@ struct DevKernel {
 struct Geo {
   int per[3];
   float L[3], half[3];
+  float wrap[3];  // half for periodic dimensions, +inf otherwise (branch-free minimum image)
 };
 inline Geo make_geo(const CrkConfig& c) {
   Geo g;
@@ -160,6 +162,7 @@ inline Geo make_geo(const CrkConfig& c) {
     g.per[d] = d < c.domain.dim ? c.domain.periodic[d] : 0;
     g.L[d] = c.domain.max[d] - c.domain.min[d];
     g.half[d] = 0.5f * g.L[d];
+    g.wrap[d] = g.per[d] ? g.half[d] : INFINITY;
   }
   return g;
 }

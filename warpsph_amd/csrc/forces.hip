@@ -1,0 +1,523 @@
+// forces.hip -- the pair passes after the CRK factors: velocity gradient
+// (with the Cullen-Dehnen divergence), the Cullen-Dehnen switch, the
+// acceleration / energy pass (optionally with the switch fused into the same
+// sweep) and the energy partition (src/warpSPH/schemes/crkSPH.py:158-364).
+#include <cmath>
+
+#include "pairs.hpp"
+
+namespace crk {
+
+// ------------------------------------------------------------ velocity gradient
+// J_i[a][g] = sum_j V_j (v_j - v_i)[a] d_g W^R_ij (scatter), fused with the
+// Cullen-Dehnen divergence (super-symmetric support max(h_i, h_j), m/rho
+// volumes).
+struct VgIn {
+  const float *x, *h, *v, *V, *m, *rho, *c, *A, *B, *gA, *gB;
+};
+struct VgOut {
+  float *J, *divcd, *vsig, *drhodt_u;
+};
+
+template <int D, int G, int K, bool CULLEN>
+__global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C, int32_t n, VgIn in,
+                                                    const int32_t* __restrict__ perm, VgOut o) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3], vi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
+  const float hi = in.h[k];
+  CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
+  Comp acc[D * D];
+  Comp dacc;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
+    float dx[3];
+    pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    float r = fsqrt(dot<D>(dx, dx));
+    float ir = inv_or_zero(r);
+    float vj[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
+    float hj = in.h[j];
+    if (CULLEN) {
+      float H = fmaxf(hi, hj);
+      if (r < H) {
+        WdW w = wdw<K, D>(C, r, ir, frcp(H));
+        float dt_ = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * dx[a];
+        dacc.add(in.m[j] * frcp(in.rho[j]) * (w.fac * dt_));
+      }
+    }
+    if (!(r < hj)) continue;
+    WdW w = wdw<K, D>(C, r, ir, frcp(hj));
+    float g[3];
+    grad_wr<D>(cr, dx, w, g);
+    float Vj = in.V[j];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      float dv = Vj * (vj[a] - vi[a]);
+#pragma unroll
+      for (int gm = 0; gm < D; ++gm) acc[a * D + gm].add(dv * g[gm]);
+    }
+  }
+  float Jt[D * D];
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) Jt[a] = group_total<G>(acc[a]);
+  float dv = 0.f;
+  if (CULLEN) dv = group_total<G>(dacc);
+  if (!L.valid || L.g != 0) return;
+  float tr = 0.f;
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) o.J[(int64_t)k * D * D + a] = Jt[a];
+#pragma unroll
+  for (int a = 0; a < D; ++a) tr = tr + Jt[a * D + a];
+  o.drhodt_u[perm[k]] = -tr * in.rho[k];
+  if (CULLEN) o.divcd[k] = dv;
+}
+
+// ------------------------------------------------------------ Cullen-Dehnen
+// CullenDehnen2010.py:218-331: R (gather interpolation of rho sign(div)), Xi,
+// A = Xi max(0, -ddiv/dt), local alpha, and the alpha0 relaxation.
+struct CdIn {
+  const float *x, *h, *m, *rho, *divcd, *vsig, *divp;
+};
+struct CdOut {
+  float *a0, *alpha, *Rcd, *alphas_u, *alpha0s_u;
+};
+struct CdParams {
+  float dt, fkern, amin, amax, beta;
+  int limitXi, evolve;
+};
+
+// per-row update from S = sum_j m_j sgn(div_j) W(x_ij, h_i)
+__device__ inline void cullen_row(const CdIn& in, const CdParams& p, int32_t k, int32_t i, float S, float hi,
+                                  const CdOut& o) {
+  float Rv = S / in.rho[k];
+  o.Rcd[k] = Rv;
+  float dk_ = in.divcd[k];
+  float t1 = 1.f - Rv;
+  float t2 = t1 * t1;
+  float nom = p.beta * (t2 * t2) * dk_;
+  float Xi = p.limitXi ? (nom * nom + 1e-14f * hi) / (nom * nom + 0.f + 1e-14f * hi)
+                       : (nom * nom) / (nom * nom + 0.f + 1e-14f * hi);
+  float ddiv = (dk_ - in.divp[k]) / p.dt;
+  float md = -ddiv;
+  float Ai = Xi * (md > 0.f ? md : 0.f);
+  float vsg = fabsf(in.vsig[k]);
+  float scaling = hi * hi * Ai / (vsg * vsg + hi * hi * Ai + 1e-14f * hi);
+  float al = p.amax * scaling;
+  al = al < p.amin ? p.amin : (al > p.amax ? p.amax : al);
+  float tau = hi * p.fkern / (0.1f * vsg);
+  float a0k = o.a0[k];
+  float adot = (al - a0k) / tau;
+  float na = al > a0k ? al : a0k;
+  na = al < na ? na + adot * p.dt : na;
+  na = na < p.amin ? p.amin : (na > p.amax ? p.amax : na);
+  o.alpha[k] = na;
+  o.alphas_u[i] = na;
+  if (p.evolve) { o.a0[k] = na; if (o.alpha0s_u) o.alpha0s_u[i] = na; }
+}
+
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, float C, int32_t n, CdIn in, CdParams p,
+                                                   const int32_t* __restrict__ perm, CdOut o) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = in.x[(int64_t)k * D + d];
+  const float hi = in.h[k];
+  const float ih = 1.f / hi;
+  Comp acc;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
+    float dx[3];
+    pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float dj = in.divcd[j];
+    float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);
+    float kk, dk;
+    kwd<K, D>(qq, kk, dk);
+    acc.add(in.m[j] * sg * (C * kk * ipow<D>(ih)));
+  }
+  float S = group_total<G>(acc);
+  if (!L.valid || L.g != 0) return;
+  cullen_row(in, p, k, perm[k], S, hi, o);
+}
+
+// ------------------------------------------------------------ accel + dudt
+struct LimParams {
+  float eta_fold, inv_eta_fold, eta_crit;
+  int crk, vl, off, on;
+};
+
+// limiter.py:11-16 (van Leer), :28-86 (computeVanLeer), :90-127 (crkLimiter)
+template <int D>
+__device__ inline float limiter_phi(const LimParams& lp, const float* dx, float r, float ihi, float ihj,
+                                    const float* Ji, const float* Jj) {
+  float factor = 1.f;
+  if (lp.crk) {
+    float eta = r * fminf(ihi, ihj);
+    if (eta < lp.eta_crit) { float t = (eta - lp.eta_crit) * lp.inv_eta_fold; factor = __expf(-(t * t)); }
+  }
+  float phi = 0.f;
+  if (lp.vl) {
+    // g_i = (J_i x/2) . x/2
+    float gi = 0.f, gj = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      float ci = 0.f, cj = 0.f;
+#pragma unroll
+      for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * dx[gm]; cj = cj + Jj[a * D + gm] * dx[gm]; }
+      gi = gi + ci * dx[a];
+      gj = gj + cj * dx[a];
+    }
+    gi = 0.25f * gi;
+    gj = 0.25f * gj;
+    float ri = gi * frcp(gj);  // sgn(g)|g| = g, sgn(0) = +1
+    float rj = gj * frcp(gi);
+    if (ri != ri) ri = 1.f;
+    if (rj != rj) rj = 1.f;
+    float rr = ri < rj ? ri : rj;
+    float vl = 0.f;
+    if (rr > 0.f) { float q = 2.f * frcp(1.f + rr); vl = rr * q * q; }
+    phi = vl * factor;
+  }
+  if (lp.off) phi = 0.f;
+  if (lp.on) phi = 1.f;
+  phi = phi < 1.f ? phi : 1.f;
+  phi = phi > 0.f ? phi : 0.f;
+  return phi;
+}
+
+struct AdIn {
+  const float *x, *h, *v, *m, *V, *rho, *P, *c, *J, *A, *B, *gA, *gB, *alpha;
+  const float* drhodt_u;
+};
+struct AdOut {
+  float *ap, *av, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
+};
+
+// SW: the Cullen-Dehnen switch of the next step is evaluated in the same
+// sweep (valid when the viscosity does not read alpha, i.e. without
+// alpha-weighted Q): S_i = sum_j m_j sgn(div_j) W(x_ij, h_i) over r < h_i.
+template <int D, int G, int K, bool SW>
+__global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, float C, LimParams lp, float inv_kscale,
+                                                       float Cl, float Cq, int alphaQ, int32_t n, AdIn in,
+                                                       const int32_t* __restrict__ perm, AdOut o, CdIn cin,
+                                                       CdParams cp, CdOut cout) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3], vi[3], Ji[D * D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) Ji[a] = in.J[(int64_t)k * D * D + a];
+  const float hi = in.h[k], rhoi = in.rho[k], Pi = in.P[k], ci = in.c[k], Vi = in.V[k], mi = in.m[k];
+  const float ihi = 1.f / hi;
+  const float ali = alphaQ ? in.alpha[k] : 1.f;
+  const CorrD<D> cri = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
+  const float smi = hi * inv_kscale;
+  const float ismi = 1.f / smi;
+  const float epsi = 1.0e-7f * smi * smi;
+  const float ViOm = Vi / mi;
+  Comp acc[D], accu, accs;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
+    const int32_t j = rows.nbr[q0];
+    float dx[3];
+    pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    const float r2 = dot<D>(dx, dx);
+    const float r = fsqrt(r2);
+    const float hj = in.h[j];
+    const float ihj = frcp(hj);
+    const float ir = inv_or_zero(r);
+    float vj[3], Jj[D * D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
+#pragma unroll
+    for (int a = 0; a < D * D; ++a) Jj[a] = in.J[(int64_t)j * D * D + a];
+    float phi = limiter_phi<D>(lp, dx, r, ihi, ihj, Ji, Jj);
+    float phi2 = 0.5f * phi;
+    float vdi[3], vdj[3], vdjU[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      float cI = 0.f, cJ = 0.f;
+#pragma unroll
+      for (int gm = 0; gm < D; ++gm) { cI = cI + Ji[a * D + gm] * dx[gm]; cJ = cJ + Jj[a * D + gm] * dx[gm]; }
+      vdi[a] = vi[a] - phi2 * cI;
+      vdj[a] = vj[a] + phi2 * cJ;   // accel.py:137-141
+      vdjU[a] = vj[a] - phi2 * cJ;  // dudt.py:123 (J_j (-x_ij))
+    }
+    const CorrD<D> crj = load_corr<D>(in.A, in.B, in.gA, in.gB, j);
+    WdW wj = wdw<K, D>(C, r, ir, ihj);  // W(x_ij, h_j) for the i-side gradient
+    WdW wi = wdw<K, D>(C, r, ir, ihi);  // W(x_ji, h_i) for the j-side gradient
+    if (SW && wi.W != 0.f) {
+      float dj = cin.divcd[j];
+      float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);
+      accs.add(cin.m[j] * sg * wi.W);
+    }
+    float gi[3], gjr[3], dxn[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) dxn[a] = -dx[a];
+    grad_wr<D>(cri, dx, wj, gi);
+    grad_wr<D>(crj, dxn, wi, gjr);
+    float gw[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) gw[a] = 0.5f * (gi[a] - gjr[a]);
+    // mu_i = (vdot_i - vdot_j).eta_i / (eta_i.eta_i + 1e-7 s_i^2), eta_i = x / s_i
+    //      = s_i (w.x) / (r^2 + 1e-7 s_i^4) ... evaluated in the reference's form
+    const float smj = hj * inv_kscale;
+    const float ismj = frcp(smj);
+    const float rdi = frcp(r2 * ismi * ismi + epsi);
+    const float rdj = frcp(r2 * ismj * ismj + 1.0e-7f * smj * smj);
+    float wx = 0.f, wxU = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      wx = wx + (vdi[a] - vdj[a]) * dx[a];
+      wxU = wxU + (vdi[a] - vdjU[a]) * dx[a];
+    }
+    const float fi = ismi * rdi, fj = ismj * rdj;
+    float mui = fminf(wx * fi, 0.f), muj = fminf(wx * fj, 0.f);
+    float muiU = fminf(wxU * fi, 0.f), mujU = fminf(wxU * fj, 0.f);
+    float rhoj = in.rho[j], cj = in.c[j];
+    float Qi = rhoi * (-Cl * ci * mui + Cq * mui * mui);
+    float Qj = rhoj * (-Cl * cj * muj + Cq * muj * muj);
+    float QiU = rhoi * (-Cl * ci * muiU + Cq * muiU * muiU);
+    float QjU = rhoj * (-Cl * cj * mujU + Cq * mujU * mujU);
+    if (alphaQ) {
+      float wa = 0.5f * (ali + in.alpha[j]);
+      Qi = Qi * wa; Qj = Qj * wa; QiU = QiU * wa; QjU = QjU * wa;
+    }
+    const float VV = ViOm * in.V[j];
+    const float Pj = in.P[j];
+    const float pP = -(Pi + Pj) * VV;
+    const float pQ = -(Qi + Qj) * VV;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      float apv = pP * gw[a];
+      float avv = pQ * gw[a];
+      o.ap[(int64_t)q0 * D + a] = apv;
+      o.av[(int64_t)q0 * D + a] = avv;
+      acc[a].add(apv + avv);
+    }
+    float dt_ = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];
+    accu.add((Pj + 0.5f * (QiU + QjU)) * dt_ * VV);
+  }
+  float dv[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) dv[a] = group_total<G>(acc[a]);
+  float du = group_total<G>(accu);
+  float S = SW ? group_total<G>(accs) : 0.f;
+  if (!L.valid || L.g != 0) return;
+  int32_t i = perm[k];
+  float ke = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    o.dvdt_s[(int64_t)k * D + a] = dv[a];
+    o.dvdt_u[(int64_t)i * D + a] = dv[a];
+    ke = ke + vi[a] * dv[a];
+  }
+  o.dudt_u[i] = du;
+  o.dEdt_u[i] = mi * ke + mi * du;
+  o.div_u[i] = -in.drhodt_u[i] / rhoi;
+  if (SW) cullen_row(cin, cp, k, i, S, hi, cout);
+}
+
+// ------------------------------------------------------------ balance
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float* __restrict__ v,
+                                                    const float* __restrict__ dvdt, const float* __restrict__ m,
+                                                    const float* __restrict__ sP, float dt, float hdt,
+                                                    const float* __restrict__ ap, const float* __restrict__ av,
+                                                    float* __restrict__ f) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float vhi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) vhi[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
+  const float si = sP[k];
+  const float mdt = m[k] * dt;
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    int32_t j = rows.nbr[q];
+    float d_ = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      float vhj = v[(int64_t)j * D + a] + hdt * dvdt[(int64_t)j * D + a];
+      d_ = d_ + (vhj - vhi[a]) * (ap[(int64_t)q * D + a] + av[(int64_t)q * D + a]);
+    }
+    float dE = mdt * d_;
+    float sj = sP[j];
+    float asi = fabsf(si), asj = fabsf(sj);
+    float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
+    bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
+    bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
+    float ff = 0.5f;
+    if (!(si - sj == 0.f)) {
+      float rs = frcp(smin + smax + 1.0e-14f);
+      if (mB) ff = smin * rs;
+      else if (mC) ff = smax * rs;
+    }
+    f[q] = fmaxf(0.f, fminf(1.f, ff));
+  }
+}
+
+}  // namespace crk
+
+using namespace crk;
+
+static CdParams cd_params(const CrkConfig* cfg, float dt) {
+  const CrkSwitchParams& sp = cfg->switchParams;
+  return CdParams{dt, (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim)), sp.alpha_min, sp.alpha_max,
+                  sp.beta_xi, sp.limitXi, cfg->evolveAlpha0};
+}
+static CdIn cd_in(CrkContext* ctx) {
+  return CdIn{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->rho), F<float>(ctx->divcd),
+              F<float>(ctx->vsig), F<float>(ctx->divp)};
+}
+static CdOut cd_out(CrkContext* ctx, const CrkState* st) {
+  return CdOut{F<float>(ctx->a0), F<float>(ctx->alpha), F<float>(ctx->Rcd), st->alphas, st->alpha0s};
+}
+
+extern "C" {
+
+int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(up && up->drhodt, "crk_velocity_gradient: update->drhodt missing");
+  if (!ctx->have_factors) { set_error("crk_velocity_gradient before crk_factors_eos"); return CRK_ESTATE; }
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = ctx->n;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  bool cullen = cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED;
+  VgIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
+          F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
+          F<float>(ctx->gB)};
+  VgOut out{F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig), up->drhodt};
+  int GS = ctx->group[CRK_PASS_VELGRAD];
+  if (cullen) {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_velgrad<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
+                           arows_of(ctx), geo, C, n, in, ctx->adj.perm, out));
+  } else {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_velgrad<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
+                           arows_of(ctx), geo, C, n, in, ctx->adj.perm, out));
+  }
+  CRK_LAUNCH_CHECK("k_velgrad");
+  ctx->have_velgrad = true;
+  return CRK_OK;
+}
+
+int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(st && st->alphas, "crk_switch: state->alphas missing");
+  CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch: state->alpha0s missing");
+  CRK_REQUIRE(dt > 0.f, "crk_switch: dt must be positive");
+  if (!ctx->have_velgrad) { set_error("crk_switch before crk_velocity_gradient"); return CRK_ESTATE; }
+  if (cfg->switchParams.scheme != CRK_SWITCH_CULLEN_DEHNEN) {
+    set_error("crk_switch: switch scheme %d is not the Cullen-Dehnen switch", cfg->switchParams.scheme);
+    return CRK_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = ctx->n;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  int GS = ctx->group[CRK_PASS_SWITCH];
+  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+      hipLaunchKernelGGL((k_cullen<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo, C,
+                         n, cd_in(ctx), cd_params(cfg, dt), ctx->adj.perm, cd_out(ctx, st)));
+  CRK_LAUNCH_CHECK("k_cullen");
+  return CRK_OK;
+}
+
+static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, CrkUpdate* up, CrkEdges* ed,
+                      bool sw, float dt, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(st && st->divergence, "crk_accel_dudt: state->divergence missing");
+  CRK_REQUIRE(up && up->dvdt && up->dudt && up->dEdt && up->drhodt, "crk_accel_dudt: update buffers missing");
+  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av)), "crk_accel_dudt: edge buffers missing");
+  CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
+              "crk_accel_dudt: edges differ from the pairs of crk_support");
+  if (!ctx->have_velgrad) { set_error("crk_accel_dudt before crk_velocity_gradient"); return CRK_ESTATE; }
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = ctx->n;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  LimParams lp{cfg->crkViscosity.eta_fold, 1.f / cfg->crkViscosity.eta_fold, cfg->crkViscosity.eta_crit,
+               cfg->crkViscosity.enableCRKLimiter, cfg->crkViscosity.enableVanLeerLimiter,
+               cfg->crkViscosity.forceVanLeerOff, cfg->crkViscosity.forceVanLeerOn};
+  float inv_kscale = (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim));
+  AdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V),
+          F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A),
+          F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
+  AdOut out{ed->ap, ed->av, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
+  int GS = ctx->group[CRK_PASS_ACCEL];
+  if (sw) {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
+                           arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
+                           cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, cd_in(ctx), cd_params(cfg, dt),
+                           cd_out(ctx, st)));
+  } else {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
+                           arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
+                           cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, CdIn{}, CdParams{}, CdOut{}));
+  }
+  CRK_LAUNCH_CHECK("k_accel_dudt");
+  ctx->have_accel = true;
+  return CRK_OK;
+}
+
+int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, CrkUpdate* up, CrkEdges* ed,
+                   void* stream) {
+  return accel_dudt(ctx, cfg, st, up, ed, false, 0.f, stream);
+}
+
+int crk_switch_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, CrkUpdate* up, CrkEdges* ed,
+                          float dt, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN,
+              "crk_switch_accel_dudt: switch scheme %d is not the Cullen-Dehnen switch", cfg->switchParams.scheme);
+  CRK_REQUIRE(!cfg->alphaWeightedQ, "crk_switch_accel_dudt: alpha-weighted Q reads the switch; call crk_switch "
+                                    "then crk_accel_dudt");
+  CRK_REQUIRE(st && st->alphas, "crk_switch_accel_dudt: state->alphas missing");
+  CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch_accel_dudt: state->alpha0s missing");
+  CRK_REQUIRE(dt > 0.f, "crk_switch_accel_dudt: dt must be positive");
+  return accel_dudt(ctx, cfg, st, up, ed, true, dt, stream);
+}
+
+int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av && ed->f)), "crk_balance: edge buffers missing");
+  CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
+              "crk_balance: edges differ from the pairs of crk_support");
+  if (!ctx->have_accel) { set_error("crk_balance before crk_accel_dudt"); return CRK_ESTATE; }
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = ctx->n;
+  float hdt = (float)(0.5 * (double)dt);
+  int GS = ctx->group[CRK_PASS_BALANCE];
+  CRK_DIM_G_DISPATCH(ctx->dim, GS,
+      hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n,
+                         F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->m), F<float>(ctx->sP), dt, hdt, ed->ap,
+                         ed->av, ed->f));
+  CRK_LAUNCH_CHECK("k_balance");
+  return CRK_OK;
+}
+
+}  // extern "C"

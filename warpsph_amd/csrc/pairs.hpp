@@ -1,0 +1,232 @@
+// pairs.hpp -- device machinery shared by the pair passes of one CRKSPH
+// derivative evaluation: lane groups, compensated sums, kernel functions,
+// the CRK-corrected gradient, the minimum image, and the launch dispatch.
+//
+// Work mapping: a group of G consecutive lanes of a wave64 owns one sorted
+// particle row; lane g handles the row's pairs g, g + G, ... so pair arrays
+// (neighbour ids, ap, av, f) stream coalesced.  Each lane keeps
+// Kahan-compensated FP32 partial sums; the G partials are combined by an
+// exact TwoSum butterfly (deterministic, every lane ends with the same
+// value).
+#pragma once
+
+#include "common.hpp"
+
+namespace crk {
+
+// ------------------------------------------------------------ fast math
+// v_rcp_f32 / v_sqrt_f32 (about 1 ulp, no denormal rescaling) for the pair
+// terms; none of their operands (distances, supports, limiter ratios) is
+// denormal in a physical configuration.
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float inv_or_zero(float r) { return r > 0.f ? frcp(r) : 0.f; }
+
+// x_i - x_j with the minimum image (the neighbour search's convention: a
+// periodic component beyond +-L/2 is shifted by L), branch free.
+template <int D>
+__device__ inline void pair_dx(const Geo& g, const float* xi, const float* xj, float* dx) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float t = xi[d] - xj[d];
+    float s = t > g.wrap[d] ? g.L[d] : 0.f;  // two selects (a nested one compiles to branches)
+    s = t < -g.wrap[d] ? -g.L[d] : s;
+    dx[d] = t - s;
+  }
+}
+
+// ------------------------------------------------------------ lane groups
+struct Comp {  // compensated value s - c
+  float s = 0.f, c = 0.f;
+  __device__ inline void add(float y) {
+    float yy = y - c;
+    float t = s + yy;
+    c = (t - s) - yy;
+    s = t;
+  }
+};
+
+template <int G>
+__device__ inline float group_total(Comp a) {
+  float s = a.s, c = a.c;
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    float so = __shfl_xor(s, off, G);
+    float co = __shfl_xor(c, off, G);
+    float t = s + so;
+    float bp = t - s;
+    float e = (s - (t - bp)) + (so - bp);
+    s = t;
+    c = (c + co) - e;
+  }
+  return s - c;
+}
+
+template <int G>
+__device__ inline float group_max(float v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, G));
+  return v;
+}
+
+struct Lane {
+  int g;       // lane inside the group
+  int32_t k;   // row (sorted particle)
+  bool valid;
+};
+// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8).
+// Renumber them so that every XCD works on one contiguous range of sorted
+// rows: its L2 then holds one compact region of the domain.
+__device__ inline int32_t xcd_block() {
+  const int32_t nb = gridDim.x, b = blockIdx.x;
+  const int32_t q = nb / 8, r = nb % 8, x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
+}
+
+template <int G>
+__device__ inline Lane lane_of(int32_t n) {
+  Lane l;
+  l.g = threadIdx.x % G;
+  l.k = xcd_block() * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
+  l.valid = l.k < n;
+  if (!l.valid) l.k = n - 1;  // safe index for loads; results discarded
+  return l;
+}
+
+// ------------------------------------------------------------ kernel functions
+// k(q) and dk/dq for 0 <= q < 1 (Wendland C2, C4, C6 as K = 0, 1, 2).
+template <int K, int D>
+__device__ inline void kwd(float q, float& k, float& dk) {
+  float t = 1.f - q;
+  float t2 = t * t;
+  if (K == CRK_KERNEL_WENDLAND2) {
+    if (D == 1) { k = t2 * t * (1.f + 3.f * q); dk = -12.f * q * t2; }
+    else { float t3 = t2 * t; k = t3 * t * (1.f + 4.f * q); dk = -20.f * q * t3; }
+  } else if (K == CRK_KERNEL_WENDLAND4) {
+    float t4 = t2 * t2;
+    if (D == 1) { k = t4 * t * (1.f + 5.f * q + 8.f * q * q); dk = -14.f * q * t4 * (1.f + 4.f * q); }
+    else { float t5 = t4 * t; k = t5 * t * (1.f + 6.f * q + (35.f / 3.f) * q * q); dk = (-56.f / 3.f) * q * t5 * (1.f + 5.f * q); }
+  } else {
+    float t4 = t2 * t2;
+    if (D == 1) { float t6 = t4 * t2; k = t6 * t * (1.f + 7.f * q + 19.f * q * q + 21.f * q * q * q); dk = -6.f * q * t6 * (3.f + 18.f * q + 35.f * q * q); }
+    else { float t7 = t4 * t2 * t; k = t7 * t * (1.f + 8.f * q + 25.f * q * q + 32.f * q * q * q); dk = -22.f * q * t7 * (1.f + 7.f * q + 16.f * q * q); }
+  }
+}
+
+template <int D>
+__device__ inline float ipow(float x) {
+  float r = x;
+#pragma unroll
+  for (int k = 1; k < D; ++k) r = r * x;
+  return r;
+}
+
+// W(r, h) and dW/dr / r (gradient = fac * x_ab) with 1/h and 1/r given.
+struct WdW {
+  float W, fac;
+};
+template <int K, int D>
+__device__ inline WdW wdw(float C, float r, float invr, float invh) {
+  WdW o;
+  float q = r * invh;
+  if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
+  float k, dk;
+  kwd<K, D>(q, k, dk);
+  float ihD = ipow<D>(invh);
+  o.W = C * k * ihD;
+  o.fac = C * dk * ihD * invh * invr;  // r > 0 implied by dk(0) = 0 and invr finite
+  return o;
+}
+
+// CRK-corrected gradient d_g W^R_ab (Frontiere, Raskin & Owen 2017):
+// gA (1 + B.x) W + A (B_g + gB^T x) W + A (1 + B.x) gradW
+template <int D>
+struct CorrD {
+  float A, B[D], gA[D], gB[D * D];
+};
+template <int D>
+__device__ inline CorrD<D> load_corr(const float* Ap, const float* Bp, const float* gAp, const float* gBp, int32_t k) {
+  CorrD<D> c;
+  c.A = Ap[k];
+#pragma unroll
+  for (int a = 0; a < D; ++a) { c.B[a] = Bp[(int64_t)k * D + a]; c.gA[a] = gAp[(int64_t)k * D + a]; }
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) c.gB[a] = gBp[(int64_t)k * D * D + a];
+  return c;
+}
+template <int D>
+__device__ inline void grad_wr(const CorrD<D>& cr, const float* x, const WdW& w, float* out) {
+  float bx = 1.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) bx = bx + cr.B[a] * x[a];
+  float AW = cr.A * w.W;
+  float ABf = cr.A * bx * w.fac;
+  float bxW = bx * w.W;
+#pragma unroll
+  for (int gm = 0; gm < D; ++gm) {
+    float gbx = cr.B[gm];
+#pragma unroll
+    for (int a = 0; a < D; ++a) gbx = gbx + cr.gB[a * D + gm] * x[a];
+    out[gm] = cr.gA[gm] * bxW + AW * gbx + ABf * x[gm];
+  }
+}
+
+// ------------------------------------------------------------ rows
+// Full Verlet rows of the bound adjacency.
+struct Rows {
+  const int32_t* off;
+  const int32_t* nbr;
+};
+// In-support pairs of each row (|x_ij| < max(h_i, h_j)), compacted in row
+// order into the slots [off[k], off[k] + cnt[k]) of nbr; the pair data (ap,
+// av, f) live in the same slots.  About half of a Verlet row is skin; the
+// kernel-weighted passes iterate only these pairs, so every lane of a group
+// works on a contributing pair.
+struct ARows {
+  const int32_t* off;
+  const int32_t* cnt;
+  const int32_t* nbr;
+};
+
+inline Rows rows_of(const CrkContext* ctx) { return Rows{ctx->adj.row_offsets, ctx->adj.nbr}; }
+inline ARows arows_of(const CrkContext* ctx) { return ARows{ctx->adj.row_offsets, ctx->pair_cnt, ctx->pair_nbr}; }
+inline unsigned group_blocks(int32_t n, int G) { return nblocks((int64_t)n * G); }
+
+template <class T>
+inline T* F(Buf& b) { return (T*)b.p; }
+
+int require_bound(CrkContext* ctx, const CrkConfig* cfg);
+
+}  // namespace crk
+
+// ------------------------------------------------------------ dispatch
+// over the kernel function (KK), the dimension (DD) and the group size (G)
+#define CRK_G_CASE(GG, ...) case GG: { constexpr int G = GG; __VA_ARGS__; } break;
+#define CRK_G_SWITCH(GS, ...)                                                                  \
+  switch (GS) {                                                                                \
+    CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__)           \
+    CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__)         \
+    CRK_G_CASE(64, __VA_ARGS__)                                                                \
+    default: ::crk::set_error("bad group size %d", GS); return CRK_EINVAL;                     \
+  }
+#define CRK_D_CASE(DV, ...) case DV: { constexpr int DD = DV; __VA_ARGS__; } break;
+#define CRK_DIM_G_DISPATCH(DIM, GS, ...)                                                       \
+  switch (DIM) {                                                                               \
+    CRK_D_CASE(1, CRK_G_SWITCH(GS, __VA_ARGS__))                                               \
+    CRK_D_CASE(2, CRK_G_SWITCH(GS, __VA_ARGS__))                                               \
+    CRK_D_CASE(3, CRK_G_SWITCH(GS, __VA_ARGS__))                                               \
+    default: ::crk::set_error("dim must be 1..3"); return CRK_EINVAL;                          \
+  }
+#define CRK_K_CASE(KV, ...) case KV: { constexpr int KK = KV; __VA_ARGS__; } break;
+#define CRK_KDG_DISPATCH(KERN, DIM, GS, ...)                                                   \
+  switch (KERN) {                                                                              \
+    CRK_K_CASE(0, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
+    CRK_K_CASE(1, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
+    CRK_K_CASE(2, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
+    default: ::crk::set_error("unknown kernel %d", KERN); return CRK_EINVAL;                   \
+  }
+#define CRK_DIM_DISPATCH(D, ...)                                                               \
+  switch (D) {                                                                                 \
+    CRK_D_CASE(1, __VA_ARGS__) CRK_D_CASE(2, __VA_ARGS__) CRK_D_CASE(3, __VA_ARGS__)           \
+    default: ::crk::set_error("dim must be 1..3"); return CRK_EINVAL;                          \
+  }

@@ -2,18 +2,11 @@
 // (src/warpSPH/schemes/crkSPH.py:19-366), run on Morton-sorted copies of the
 // state held by the context.
 //
-// Work mapping: a group of G consecutive lanes of a wave64 owns one sorted
-// particle row of the CSR adjacency; lane g handles edges off[k]+g, +G, ...
-// so edge arrays (neighbour ids, ap, av, f) stream coalesced.  Each lane
-// keeps Kahan-compensated FP32 partial sums; the G partials are combined by
-// an exact TwoSum butterfly (deterministic, every lane ends with the same
-// value).  The CRK moment solve runs in FP64.  G per pass is chosen by the
-// autotuner (crk_set_group_size).
-//
 // Pass order inside a stage:
 //   crk_stage_begin      gather x, v, h, m, u, alpha0, alpha, divergence to sorted order
 //   crk_owen_iteration   psi0 -> n_h -> new h           (optimalSupportOwen.py:70-114)
-//   crk_factors_eos      V, then moments -> A, B, gradA, gradB (FP64 solve),
+//   crk_support          V, signal speed, in-support pairs compacted per row
+//   crk_factors_eos      moments -> A, B, gradA, gradB (FP64 solve),
 //                        rho = A (S0 + B.S1), EOS      (computeCRKFactors; idealGas.py)
 //   crk_velocity_gradient J (CRK, scatter), drhodt, and the Cullen divergence
 //                        and signal speed              (crkSPH.py:158-172; wp_vsig.py)
@@ -22,148 +15,9 @@
 //   crk_balance          f_ij                          (compSPH/balance.py)
 #include <cmath>
 
-#include "common.hpp"
+#include "pairs.hpp"
 
 namespace crk {
-
-// ------------------------------------------------------------ group helpers
-struct Comp {  // compensated value s - c
-  float s = 0.f, c = 0.f;
-  __device__ inline void add(float y) {
-    float yy = y - c;
-    float t = s + yy;
-    c = (t - s) - yy;
-    s = t;
-  }
-};
-
-template <int G>
-__device__ inline float group_total(Comp a) {
-  float s = a.s, c = a.c;
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    float so = __shfl_xor(s, off, G);
-    float co = __shfl_xor(c, off, G);
-    float t = s + so;
-    float bp = t - s;
-    float e = (s - (t - bp)) + (so - bp);
-    s = t;
-    c = (c + co) - e;
-  }
-  return s - c;
-}
-
-template <int G>
-__device__ inline float group_max(float v) {
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, G));
-  return v;
-}
-
-struct Lane {
-  int g;       // lane inside the group
-  int32_t k;   // row (sorted particle)
-  bool valid;
-};
-// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8).
-// Renumber them so that every XCD works on one contiguous range of sorted
-// rows: its L2 then holds one compact region of the domain.
-__device__ inline int32_t xcd_block() {
-  const int32_t nb = gridDim.x, b = blockIdx.x;
-  const int32_t q = nb / 8, r = nb % 8, x = b % 8;
-  return x * q + (x < r ? x : r) + b / 8;
-}
-
-template <int G>
-__device__ inline Lane lane_of(int32_t n) {
-  Lane l;
-  l.g = threadIdx.x % G;
-  l.k = xcd_block() * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
-  l.valid = l.k < n;
-  if (!l.valid) l.k = n - 1;  // safe index for loads; results discarded
-  return l;
-}
-
-// ------------------------------------------------------------ kernel functions
-// k(q) and dk/dq for 0 <= q < 1 with shared subexpressions.
-template <int D>
-__device__ inline void kwd(int kernel, float q, float& k, float& dk) {
-  float t = 1.f - q;
-  float t2 = t * t;
-  if (kernel == CRK_KERNEL_WENDLAND2) {
-    if (D == 1) { k = t2 * t * (1.f + 3.f * q); dk = -12.f * q * t2; }
-    else { float t3 = t2 * t; k = t3 * t * (1.f + 4.f * q); dk = -20.f * q * t3; }
-  } else if (kernel == CRK_KERNEL_WENDLAND4) {
-    float t4 = t2 * t2;
-    if (D == 1) { k = t4 * t * (1.f + 5.f * q + 8.f * q * q); dk = -14.f * q * t4 * (1.f + 4.f * q); }
-    else { float t5 = t4 * t; k = t5 * t * (1.f + 6.f * q + (35.f / 3.f) * q * q); dk = (-56.f / 3.f) * q * t5 * (1.f + 5.f * q); }
-  } else {
-    float t4 = t2 * t2;
-    if (D == 1) { float t6 = t4 * t2; k = t6 * t * (1.f + 7.f * q + 19.f * q * q + 21.f * q * q * q); dk = -6.f * q * t6 * (3.f + 18.f * q + 35.f * q * q); }
-    else { float t7 = t4 * t2 * t; k = t7 * t * (1.f + 8.f * q + 25.f * q * q + 32.f * q * q * q); dk = -22.f * q * t7 * (1.f + 7.f * q + 16.f * q * q); }
-  }
-}
-
-template <int D>
-__device__ inline float ipow(float x) {
-  float r = x;
-#pragma unroll
-  for (int k = 1; k < D; ++k) r = r * x;
-  return r;
-}
-
-// W(r, h) and dW/dr / r (for the gradient W' x / r) with 1/h given.
-template <int D>
-struct WdW {
-  float W, fac;  // gradient = fac * x_ab
-};
-template <int D>
-__device__ inline WdW<D> wdw(int kernel, float C, float r, float invr, float invh) {
-  WdW<D> o;
-  float q = r * invh;
-  if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
-  float k, dk;
-  kwd<D>(kernel, q, k, dk);
-  float ihD = ipow<D>(invh);
-  o.W = C * k * ihD;
-  o.fac = C * dk * ihD * invh * invr;  // r > 0 implied by dk(0) = 0 and invr finite
-  return o;
-}
-
-__device__ inline float safe_inv(float r) { return r > 0.f ? 1.f / r : 0.f; }
-
-// CRK-corrected gradient d_g W^R_ab (Frontiere, Raskin & Owen 2017):
-// gA (1 + B.x) W + A (B_g + gB^T x) W + A (1 + B.x) gradW
-template <int D>
-struct CorrD {
-  float A, B[D], gA[D], gB[D * D];
-};
-template <int D>
-__device__ inline CorrD<D> load_corr(const float* Ap, const float* Bp, const float* gAp, const float* gBp, int32_t k) {
-  CorrD<D> c;
-  c.A = Ap[k];
-#pragma unroll
-  for (int a = 0; a < D; ++a) { c.B[a] = Bp[(int64_t)k * D + a]; c.gA[a] = gAp[(int64_t)k * D + a]; }
-#pragma unroll
-  for (int a = 0; a < D * D; ++a) c.gB[a] = gBp[(int64_t)k * D * D + a];
-  return c;
-}
-template <int D>
-__device__ inline void grad_wr(const CorrD<D>& cr, const float* x, const WdW<D>& w, float* out) {
-  float bx = 1.f;
-#pragma unroll
-  for (int a = 0; a < D; ++a) bx = bx + cr.B[a] * x[a];
-  float AW = cr.A * w.W;
-  float ABf = cr.A * bx * w.fac;
-  float bxW = bx * w.W;
-#pragma unroll
-  for (int gm = 0; gm < D; ++gm) {
-    float gbx = cr.B[gm];
-#pragma unroll
-    for (int a = 0; a < D; ++a) gbx = gbx + cr.gB[a * D + gm] * x[a];
-    out[gm] = cr.gA[gm] * bxW + AW * gbx + ABf * x[gm];
-  }
-}
 
 // ------------------------------------------------------------ gather
 template <int D>
@@ -188,22 +42,6 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
   alpha[k] = AL ? AL[i] : 1.f;
 }
 
-struct Rows {
-  const int32_t* off;
-  const int32_t* nbr;
-};
-
-// In-support pairs of each row (|x_ij| < max(h_i, h_j)), compacted in row
-// order into the slots [off[k], off[k] + cnt[k]) of nbr; the pair data (ap,
-// av, f) live in the same slots.  About half of a Verlet row is skin; the
-// kernel-weighted passes iterate only these pairs, so every lane of a group
-// works on a contributing pair.
-struct ARows {
-  const int32_t* off;
-  const int32_t* cnt;
-  const int32_t* nbr;
-};
-
 // ------------------------------------------------------------ support sweep
 // One pass over the full adjacency after the Owen update: the CRK apparent
 // volume V_i = 1 / sum_j W(x_ij, h_i); the in-support pairs compacted per row
@@ -211,8 +49,8 @@ struct ARows {
 // v_sig = max_j (0.5 (c_i + c_j) - mu_ij if mu_ij <= 0 else 0),
 // mu_ij = (v_i - v_j).x_ij / (r_ij + 1e-14 h_i), over every entry
 // (wp_vsig.py:55-86), c = sqrt(|u| gamma (gamma - 1)).
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, int kernel, float C, int32_t n,
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C, int32_t n,
                                                     const float* __restrict__ x, const float* __restrict__ h,
                                                     const float* __restrict__ v, const float* __restrict__ u,
                                                     float cfac, int32_t* __restrict__ anbr,
@@ -243,21 +81,21 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, int kern
     if (inside) {
       j = rows.nbr[q];
       float dx[3];
-      min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-      float r = sqrtf(dot<D>(dx, dx));
+      pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+      float r = fsqrt(dot<D>(dx, dx));
       if (vsig) {
         float mu = 0.f;
 #pragma unroll
         for (int a = 0; a < D; ++a) mu = mu + (vi[a] - v[(int64_t)j * D + a]) * dx[a];
-        mu = mu / (r + eps);
-        float vv = 0.5f * (ci + sqrtf(fabsf(u[j]) * cfac)) - mu;
+        mu = mu * frcp(r + eps);
+        float vv = 0.5f * (ci + fsqrt(fabsf(u[j]) * cfac)) - mu;
         if (mu > 0.f) vv = 0.f;
         vs = vv > vs ? vv : vs;
       }
       float qq = r * ih;
       if (qq < 1.f) {
         float kk, dk;
-        kwd<D>(kernel, qq, kk, dk);
+        kwd<K, D>(qq, kk, dk);
         vol.add(C * kk * ipow<D>(ih));
       }
       act = r < fmaxf(hi, h[j]);
@@ -292,8 +130,8 @@ __device__ inline double lut_interp(const double* __restrict__ xs, const double*
   return ys[il] * (1.0 - a) + ys[ir] * a;
 }
 
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, int kernel, float C, int32_t n,
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, int32_t n,
                                                  const float* __restrict__ x, float* __restrict__ h,
                                                  const int32_t* __restrict__ perm, float* __restrict__ supports,
                                                  const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh,
@@ -311,11 +149,11 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, int kernel,
   for (int32_t q = b + L.g; q < e; q += G) {
     int32_t j = rows.nbr[q];
     float dx[3];
-    min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float qq = sqrtf(dot<D>(dx, dx)) * ih;
+    pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+    float qq = fsqrt(dot<D>(dx, dx)) * ih;
     if (!(qq < 1.f)) continue;
     float kk, dk;
-    kwd<D>(kernel, qq, kk, dk);
+    kwd<K, D>(qq, kk, dk);
     acc.add(C * fabsf(dk));
   }
   float S = group_total<G>(acc);
@@ -352,8 +190,8 @@ struct MomOut {
   int32_t* nsing;
 };
 
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, int kernel, float C, int32_t n,
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, float C, int32_t n,
                                                     const float* __restrict__ x, const float* __restrict__ h,
                                                     const float* __restrict__ V, const float* __restrict__ m,
                                                     const float* __restrict__ u, float gamma,
@@ -369,11 +207,11 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, int ker
   for (int32_t q = b + L.g; q < e; q += G) {
     int32_t j = rows.nbr[q];
     float dx[3];
-    min_image<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float r = sqrtf(dot<D>(dx, dx));
+    pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+    float r = fsqrt(dot<D>(dx, dx));
     float hj = h[j];
     if (!(r < hj)) continue;
-    WdW<D> w = wdw<D>(kernel, C, r, safe_inv(r), 1.f / hj);
+    WdW w = wdw<K, D>(C, r, inv_or_zero(r), frcp(hj));
     float g[3];
 #pragma unroll
     for (int d = 0; d < D; ++d) g[d] = w.fac * dx[d];
@@ -561,348 +399,6 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, int ker
   o.ent_u[i] = s;
 }
 
-// ------------------------------------------------------------ velocity gradient
-// J_i[a][g] = sum_j V_j (v_j - v_i)[a] d_g W^R_ij (scatter), fused with the
-// Cullen-Dehnen divergence (super-symmetric support max(h_i, h_j), m/rho
-// volumes) and the signal speed (every adjacency entry).
-struct VgIn {
-  const float *x, *h, *v, *V, *m, *rho, *c, *A, *B, *gA, *gB;
-};
-struct VgOut {
-  float *J, *divcd, *vsig, *drhodt_u;
-};
-
-template <int D, int G, bool CULLEN>
-__global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, int kernel, float C, int32_t n, VgIn in,
-                                                    const int32_t* __restrict__ perm, VgOut o) {
-  Lane L = lane_of<G>(n);
-  const int32_t k = L.k;
-  float xi[3], vi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
-  const float hi = in.h[k];
-  CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
-  Comp acc[D * D];
-  Comp dacc;
-  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
-    float dx[3];
-    min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
-    float r = sqrtf(dot<D>(dx, dx));
-    float ir = safe_inv(r);
-    float vj[3];
-#pragma unroll
-    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
-    float hj = in.h[j];
-    if (CULLEN) {
-      float H = fmaxf(hi, hj);
-      if (r < H) {
-        WdW<D> w = wdw<D>(kernel, C, r, ir, 1.f / H);
-        float dt_ = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * dx[a];
-        dacc.add(in.m[j] / in.rho[j] * (w.fac * dt_));
-      }
-    }
-    if (!(r < hj)) continue;
-    WdW<D> w = wdw<D>(kernel, C, r, ir, 1.f / hj);
-    float g[3];
-    grad_wr<D>(cr, dx, w, g);
-    float Vj = in.V[j];
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      float dv = Vj * (vj[a] - vi[a]);
-#pragma unroll
-      for (int gm = 0; gm < D; ++gm) acc[a * D + gm].add(dv * g[gm]);
-    }
-  }
-  float Jt[D * D];
-#pragma unroll
-  for (int a = 0; a < D * D; ++a) Jt[a] = group_total<G>(acc[a]);
-  float dv = 0.f;
-  if (CULLEN) dv = group_total<G>(dacc);
-  if (!L.valid || L.g != 0) return;
-  float tr = 0.f;
-#pragma unroll
-  for (int a = 0; a < D * D; ++a) o.J[(int64_t)k * D * D + a] = Jt[a];
-#pragma unroll
-  for (int a = 0; a < D; ++a) tr = tr + Jt[a * D + a];
-  o.drhodt_u[perm[k]] = -tr * in.rho[k];
-  if (CULLEN) o.divcd[k] = dv;
-}
-
-// ------------------------------------------------------------ Cullen-Dehnen
-// CullenDehnen2010.py:218-331: R (gather interpolation of rho sign(div)), Xi,
-// A = Xi max(0, -ddiv/dt), local alpha, and the alpha0 relaxation.
-struct CdIn {
-  const float *x, *h, *m, *rho, *divcd, *vsig, *divp;
-};
-struct CdOut {
-  float *a0, *alpha, *Rcd, *alphas_u, *alpha0s_u;
-};
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, int kernel, float C, int32_t n, CdIn in,
-                                                   float dt, float fkern, float amin, float amax, float beta,
-                                                   int limitXi, int evolve, const int32_t* __restrict__ perm,
-                                                   CdOut o) {
-  Lane L = lane_of<G>(n);
-  const int32_t k = L.k;
-  float xi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = in.x[(int64_t)k * D + d];
-  const float hi = in.h[k];
-  const float ih = 1.f / hi;
-  Comp acc;
-  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
-    float dx[3];
-    min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
-    float qq = sqrtf(dot<D>(dx, dx)) * ih;
-    if (!(qq < 1.f)) continue;
-    float dj = in.divcd[j];
-    float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);
-    float kk, dk;
-    kwd<D>(kernel, qq, kk, dk);
-    acc.add(in.m[j] * sg * (C * kk * ipow<D>(ih)));
-  }
-  float S = group_total<G>(acc);
-  if (!L.valid || L.g != 0) return;
-  float Rv = S / in.rho[k];
-  o.Rcd[k] = Rv;
-  float dk_ = in.divcd[k];
-  float t1 = 1.f - Rv;
-  float t2 = t1 * t1;
-  float nom = beta * (t2 * t2) * dk_;
-  float Xi = limitXi ? (nom * nom + 1e-14f * hi) / (nom * nom + 0.f + 1e-14f * hi)
-                     : (nom * nom) / (nom * nom + 0.f + 1e-14f * hi);
-  float ddiv = (dk_ - in.divp[k]) / dt;
-  float md = -ddiv;
-  float Ai = Xi * (md > 0.f ? md : 0.f);
-  float vsg = fabsf(in.vsig[k]);
-  float scaling = hi * hi * Ai / (vsg * vsg + hi * hi * Ai + 1e-14f * hi);
-  float al = amax * scaling;
-  al = al < amin ? amin : (al > amax ? amax : al);
-  float tau = hi * fkern / (0.1f * vsg);
-  float a0k = o.a0[k];
-  float adot = (al - a0k) / tau;
-  float na = al > a0k ? al : a0k;
-  na = al < na ? na + adot * dt : na;
-  na = na < amin ? amin : (na > amax ? amax : na);
-  o.alpha[k] = na;
-  int32_t i = perm[k];
-  o.alphas_u[i] = na;
-  if (evolve) { o.a0[k] = na; if (o.alpha0s_u) o.alpha0s_u[i] = na; }
-}
-
-// ------------------------------------------------------------ accel + dudt
-struct LimParams {
-  float eta_fold, inv_eta_fold, eta_crit;
-  int crk, vl, off, on;
-};
-
-// limiter.py:11-16 (van Leer), :28-86 (computeVanLeer), :90-127 (crkLimiter)
-template <int D>
-__device__ inline float limiter_phi(const LimParams& lp, const float* dx, float r, float ihi, float ihj,
-                                    const float* Ji, const float* Jj) {
-  float factor = 1.f;
-  if (lp.crk) {
-    float eta = r * fminf(ihi, ihj);
-    if (eta < lp.eta_crit) { float t = (eta - lp.eta_crit) * lp.inv_eta_fold; factor = __expf(-(t * t)); }
-  }
-  float phi = 0.f;
-  if (lp.vl) {
-    // g_i = (J_i x/2) . x/2
-    float gi = 0.f, gj = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      float ci = 0.f, cj = 0.f;
-#pragma unroll
-      for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * dx[gm]; cj = cj + Jj[a * D + gm] * dx[gm]; }
-      gi = gi + ci * dx[a];
-      gj = gj + cj * dx[a];
-    }
-    gi = 0.25f * gi;
-    gj = 0.25f * gj;
-    float ri = gi / gj;  // sgn(g)|g| = g, sgn(0) = +1
-    float rj = gj / gi;
-    if (ri != ri) ri = 1.f;
-    if (rj != rj) rj = 1.f;
-    float rr = ri < rj ? ri : rj;
-    float vl = 0.f;
-    if (rr > 0.f) { float q = 2.f / (1.f + rr); vl = rr * q * q; }
-    phi = vl * factor;
-  }
-  if (lp.off) phi = 0.f;
-  if (lp.on) phi = 1.f;
-  phi = phi < 1.f ? phi : 1.f;
-  phi = phi > 0.f ? phi : 0.f;
-  return phi;
-}
-
-struct AdIn {
-  const float *x, *h, *v, *m, *V, *rho, *P, *c, *J, *A, *B, *gA, *gB, *alpha;
-  const float* drhodt_u;
-};
-struct AdOut {
-  float *ap, *av, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
-};
-
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, int kernel, float C, LimParams lp,
-                                                       float inv_kscale, float Cl, float Cq, int alphaQ, int32_t n,
-                                                       AdIn in, const int32_t* __restrict__ perm, AdOut o) {
-  Lane L = lane_of<G>(n);
-  const int32_t k = L.k;
-  float xi[3], vi[3], Ji[D * D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
-#pragma unroll
-  for (int a = 0; a < D * D; ++a) Ji[a] = in.J[(int64_t)k * D * D + a];
-  const float hi = in.h[k], rhoi = in.rho[k], Pi = in.P[k], ci = in.c[k], Vi = in.V[k], mi = in.m[k];
-  const float ihi = 1.f / hi;
-  const float ali = alphaQ ? in.alpha[k] : 1.f;
-  const CorrD<D> cri = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
-  const float smi = hi * inv_kscale;
-  const float ismi = 1.f / smi;
-  const float epsi = 1.0e-7f * smi * smi;
-  const float ViOm = Vi / mi;
-  Comp acc[D], accu;
-  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
-    const int32_t j = rows.nbr[q0];
-    float dx[3];
-    min_image<D>(geo, xi, &in.x[(int64_t)j * D], dx);
-    float r = sqrtf(dot<D>(dx, dx));
-    float hj = in.h[j];
-    const float ihj = 1.f / hj;
-    const float ir = safe_inv(r);
-    float vj[3], Jj[D * D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
-#pragma unroll
-    for (int a = 0; a < D * D; ++a) Jj[a] = in.J[(int64_t)j * D * D + a];
-    float phi = limiter_phi<D>(lp, dx, r, ihi, ihj, Ji, Jj);
-    float phi2 = 0.5f * phi;
-    float vdi[3], vdj[3], vdjU[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      float cI = 0.f, cJ = 0.f;
-#pragma unroll
-      for (int gm = 0; gm < D; ++gm) { cI = cI + Ji[a * D + gm] * dx[gm]; cJ = cJ + Jj[a * D + gm] * dx[gm]; }
-      vdi[a] = vi[a] - phi2 * cI;
-      vdj[a] = vj[a] + phi2 * cJ;   // accel.py:137-141
-      vdjU[a] = vj[a] - phi2 * cJ;  // dudt.py:123 (J_j (-x_ij))
-    }
-    const CorrD<D> crj = load_corr<D>(in.A, in.B, in.gA, in.gB, j);
-    WdW<D> wj = wdw<D>(kernel, C, r, ir, ihj);  // W(x_ij, h_j) for the i-side gradient
-    WdW<D> wi = wdw<D>(kernel, C, r, ir, ihi);  // W(x_ji, h_i) for the j-side gradient
-    float gi[3], gjr[3], dxn[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) dxn[a] = -dx[a];
-    grad_wr<D>(cri, dx, wj, gi);
-    grad_wr<D>(crj, dxn, wi, gjr);
-    float gw[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) gw[a] = 0.5f * (gi[a] - gjr[a]);
-    // mu_i = (vdot_i - vdot_j).eta_i / (eta_i.eta_i + 1e-7 s_i^2), eta_i = x / s_i
-    //      = s_i (w.x) / (r^2 + 1e-7 s_i^4) ... evaluated in the reference's form
-    const float smj = hj * inv_kscale;
-    const float ismj = 1.f / smj;
-    float r2 = r * r;
-    float di = r2 * ismi * ismi + epsi;
-    float dj = r2 * ismj * ismj + 1.0e-7f * smj * smj;
-    float wx = 0.f, wxU = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      wx = wx + (vdi[a] - vdj[a]) * dx[a];
-      wxU = wxU + (vdi[a] - vdjU[a]) * dx[a];
-    }
-    float mui = fminf(wx * ismi / di, 0.f), muj = fminf(wx * ismj / dj, 0.f);
-    float muiU = fminf(wxU * ismi / di, 0.f), mujU = fminf(wxU * ismj / dj, 0.f);
-    float rhoj = in.rho[j], cj = in.c[j];
-    float Qi = rhoi * (-Cl * ci * mui + Cq * mui * mui);
-    float Qj = rhoj * (-Cl * cj * muj + Cq * muj * muj);
-    float QiU = rhoi * (-Cl * ci * muiU + Cq * muiU * muiU);
-    float QjU = rhoj * (-Cl * cj * mujU + Cq * mujU * mujU);
-    if (alphaQ) {
-      float wa = 0.5f * (ali + in.alpha[j]);
-      Qi = Qi * wa; Qj = Qj * wa; QiU = QiU * wa; QjU = QjU * wa;
-    }
-    const float VV = ViOm * in.V[j];
-    const float Pj = in.P[j];
-    const float pP = -(Pi + Pj) * VV;
-    const float pQ = -(Qi + Qj) * VV;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      float apv = pP * gw[a];
-      float avv = pQ * gw[a];
-      o.ap[(int64_t)q0 * D + a] = apv;
-      o.av[(int64_t)q0 * D + a] = avv;
-      acc[a].add(apv + avv);
-    }
-    float dt_ = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];
-    accu.add((Pj + 0.5f * (QiU + QjU)) * dt_ * VV);
-  }
-  float dv[D];
-#pragma unroll
-  for (int a = 0; a < D; ++a) dv[a] = group_total<G>(acc[a]);
-  float du = group_total<G>(accu);
-  if (!L.valid || L.g != 0) return;
-  int32_t i = perm[k];
-  float ke = 0.f;
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    o.dvdt_s[(int64_t)k * D + a] = dv[a];
-    o.dvdt_u[(int64_t)i * D + a] = dv[a];
-    ke = ke + vi[a] * dv[a];
-  }
-  o.dudt_u[i] = du;
-  o.dEdt_u[i] = mi * ke + mi * du;
-  o.div_u[i] = -in.drhodt_u[i] / rhoi;
-}
-
-// ------------------------------------------------------------ balance
-template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float* __restrict__ v,
-                                                    const float* __restrict__ dvdt, const float* __restrict__ m,
-                                                    const float* __restrict__ sP, float dt, float hdt,
-                                                    const float* __restrict__ ap, const float* __restrict__ av,
-                                                    float* __restrict__ f) {
-  Lane L = lane_of<G>(n);
-  const int32_t k = L.k;
-  float vhi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) vhi[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
-  const float si = sP[k];
-  const float mdt = m[k] * dt;
-  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
-    float d_ = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      float vhj = v[(int64_t)j * D + a] + hdt * dvdt[(int64_t)j * D + a];
-      d_ = d_ + (vhj - vhi[a]) * (ap[(int64_t)q * D + a] + av[(int64_t)q * D + a]);
-    }
-    float dE = mdt * d_;
-    float sj = sP[j];
-    float asi = fabsf(si), asj = fabsf(sj);
-    float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
-    bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
-    bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
-    float ff = 0.5f;
-    if (!(si - sj == 0.f)) {
-      if (mB) ff = smin / (smin + smax + 1.0e-14f);
-      else if (mC) ff = smax / (smin + smax + 1.0e-14f);
-    }
-    f[q] = fmaxf(0.f, fminf(1.f, ff));
-  }
-}
-
 // ------------------------------------------------------------ export
 // Slot data of a row to adjacency order: the pairs are the row's edges with
 // r < max(h_i, h_j) in row order, so one merge walk over the row matches
@@ -940,12 +436,14 @@ __global__ void k_scatter_field(const float* __restrict__ src, const int32_t* __
   for (int c = 0; c < C; ++c) dst[(int64_t)i * C + c] = src[(int64_t)k * C + c];
 }
 
-static int require_bound(CrkContext* ctx, const CrkConfig* cfg) {
+
+int require_bound(CrkContext* ctx, const CrkConfig* cfg) {
   if (!ctx || !cfg) { set_error("null context/config"); return CRK_EINVAL; }
   if (!ctx->bound) { set_error("no stage bound: call crk_stage_begin first"); return CRK_ESTATE; }
   if (cfg->domain.dim != ctx->dim) { set_error("config dimension %d differs from the bound stage (%d)", cfg->domain.dim, ctx->dim); return CRK_EINVAL; }
   return CRK_OK;
 }
+
 
 static int alloc_stage(CrkContext* ctx, int32_t n, int D) {
   size_t f = sizeof(float) * (size_t)n;
@@ -959,35 +457,9 @@ static int alloc_stage(CrkContext* ctx, int32_t n, int D) {
   return CRK_OK;
 }
 
-template <class T>
-static T* F(Buf& b) { return (T*)b.p; }
-
-static Rows rows_of(const CrkContext* ctx) { return Rows{ctx->adj.row_offsets, ctx->adj.nbr}; }
-static ARows arows_of(CrkContext* ctx) { return ARows{ctx->adj.row_offsets, ctx->pair_cnt, ctx->pair_nbr}; }
-
-inline unsigned group_blocks(int32_t n, int G) { return nblocks((int64_t)n * G); }
-
 }  // namespace crk
 
 using namespace crk;
-
-// dispatch over dimension and group size (1, 2, 4, 8, 16, 32, 64)
-#define CRK_G_CASE(GG, ...) case GG: { constexpr int G = GG; __VA_ARGS__; } break;
-#define CRK_DIM_G_DISPATCH(DIM, GS, ...)                                                      \
-  switch (DIM) {                                                                              \
-    case 1: { constexpr int DD = 1; switch (GS) { CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__) CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__) CRK_G_CASE(64, __VA_ARGS__) default: set_error("bad group size %d", GS); return CRK_EINVAL; } } break; \
-    case 2: { constexpr int DD = 2; switch (GS) { CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__) CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__) CRK_G_CASE(64, __VA_ARGS__) default: set_error("bad group size %d", GS); return CRK_EINVAL; } } break; \
-    case 3: { constexpr int DD = 3; switch (GS) { CRK_G_CASE(1, __VA_ARGS__) CRK_G_CASE(2, __VA_ARGS__) CRK_G_CASE(4, __VA_ARGS__) CRK_G_CASE(8, __VA_ARGS__) CRK_G_CASE(16, __VA_ARGS__) CRK_G_CASE(32, __VA_ARGS__) CRK_G_CASE(64, __VA_ARGS__) default: set_error("bad group size %d", GS); return CRK_EINVAL; } } break; \
-    default: set_error("dim must be 1..3"); return CRK_EINVAL;                                \
-  }
-
-#define CRK_DIM_DISPATCH(D, ...)                       \
-  switch (D) {                                         \
-    case 1: { constexpr int DD = 1; __VA_ARGS__; } break; \
-    case 2: { constexpr int DD = 2; __VA_ARGS__; } break; \
-    case 3: { constexpr int DD = 3; __VA_ARGS__; } break; \
-    default: set_error("dim must be 1..3"); return CRK_EINVAL; \
-  }
 
 extern "C" {
 
@@ -1053,9 +525,9 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   if (max_ratio_dev) { if ((r = fill_f32(max_ratio_dev, 1, 0.f, s))) return r; }
   ctx->have_support = false;
   int GS = ctx->group[CRK_PASS_OWEN];
-  CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_owen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
-                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
+  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+      hipLaunchKernelGGL((k_owen<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                          F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
                          hminmax_dev, max_ratio_dev));
   CRK_LAUNCH_CHECK("k_owen");
@@ -1074,9 +546,9 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)
                   ? F<float>(ctx->vsig) : nullptr;
   int GS = ctx->group[CRK_PASS_SUPPORT];
-  CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_support<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
-                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u),
+  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+      hipLaunchKernelGGL((k_support<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
+                         C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u),
                          cfac, ed->nbr, ed->count, vs, F<float>(ctx->V)));
   CRK_LAUNCH_CHECK("k_support");
   ctx->pair_cnt = ed->count;
@@ -1099,121 +571,12 @@ int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, i
             F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->sP), st->densities, st->entropies, st->pressures,
             st->soundspeeds, n_singular_dev};
   int GM = ctx->group[CRK_PASS_MOMENTS];
-  CRK_DIM_G_DISPATCH(ctx->dim, GM,
-      hipLaunchKernelGGL((k_moments<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
-                         cfg->kernel, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V), F<float>(ctx->m),
+  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
+      hipLaunchKernelGGL((k_moments<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
+                         C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V), F<float>(ctx->m),
                          F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo));
   CRK_LAUNCH_CHECK("k_moments");
   ctx->have_factors = true;
-  return CRK_OK;
-}
-
-int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, void* stream) {
-  int r = require_bound(ctx, cfg);
-  if (r) return r;
-  CRK_REQUIRE(up && up->drhodt, "crk_velocity_gradient: update->drhodt missing");
-  if (!ctx->have_factors) { set_error("crk_velocity_gradient before crk_factors_eos"); return CRK_ESTATE; }
-  hipStream_t s = (hipStream_t)stream;
-  int32_t n = ctx->n;
-  Geo geo = make_geo(*cfg);
-  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
-  bool cullen = cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED;
-  VgIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
-          F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
-          F<float>(ctx->gB)};
-  VgOut out{F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig), up->drhodt};
-  int GS = ctx->group[CRK_PASS_VELGRAD];
-  if (cullen) {
-    CRK_DIM_G_DISPATCH(ctx->dim, GS,
-        hipLaunchKernelGGL((k_velgrad<DD, G, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx),
-                           geo, cfg->kernel, C, n, in, ctx->adj.perm, out));
-  } else {
-    CRK_DIM_G_DISPATCH(ctx->dim, GS,
-        hipLaunchKernelGGL((k_velgrad<DD, G, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx),
-                           geo, cfg->kernel, C, n, in, ctx->adj.perm, out));
-  }
-  CRK_LAUNCH_CHECK("k_velgrad");
-  ctx->have_velgrad = true;
-  return CRK_OK;
-}
-
-int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt, void* stream) {
-  int r = require_bound(ctx, cfg);
-  if (r) return r;
-  CRK_REQUIRE(st && st->alphas, "crk_switch: state->alphas missing");
-  CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch: state->alpha0s missing");
-  CRK_REQUIRE(dt > 0.f, "crk_switch: dt must be positive");
-  if (!ctx->have_velgrad) { set_error("crk_switch before crk_velocity_gradient"); return CRK_ESTATE; }
-  if (cfg->switchParams.scheme != CRK_SWITCH_CULLEN_DEHNEN) {
-    set_error("crk_switch: switch scheme %d is not the Cullen-Dehnen switch", cfg->switchParams.scheme);
-    return CRK_EINVAL;
-  }
-  hipStream_t s = (hipStream_t)stream;
-  int32_t n = ctx->n;
-  Geo geo = make_geo(*cfg);
-  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
-  float fkern = (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim));
-  const CrkSwitchParams& sp = cfg->switchParams;
-  CdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->rho), F<float>(ctx->divcd),
-          F<float>(ctx->vsig), F<float>(ctx->divp)};
-  CdOut out{F<float>(ctx->a0), F<float>(ctx->alpha), F<float>(ctx->Rcd), st->alphas, st->alpha0s};
-  int GS = ctx->group[CRK_PASS_SWITCH];
-  CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_cullen<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
-                         cfg->kernel, C, n, in, dt, fkern, sp.alpha_min, sp.alpha_max, sp.beta_xi, sp.limitXi,
-                         cfg->evolveAlpha0, ctx->adj.perm, out));
-  CRK_LAUNCH_CHECK("k_cullen");
-  return CRK_OK;
-}
-
-int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, CrkUpdate* up, CrkEdges* ed,
-                   void* stream) {
-  int r = require_bound(ctx, cfg);
-  if (r) return r;
-  CRK_REQUIRE(st && st->divergence, "crk_accel_dudt: state->divergence missing");
-  CRK_REQUIRE(up && up->dvdt && up->dudt && up->dEdt && up->drhodt, "crk_accel_dudt: update buffers missing");
-  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av)), "crk_accel_dudt: edge buffers missing");
-  CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
-              "crk_accel_dudt: edges differ from the pairs of crk_support");
-  if (!ctx->have_velgrad) { set_error("crk_accel_dudt before crk_velocity_gradient"); return CRK_ESTATE; }
-  hipStream_t s = (hipStream_t)stream;
-  int32_t n = ctx->n;
-  Geo geo = make_geo(*cfg);
-  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
-  LimParams lp{cfg->crkViscosity.eta_fold, 1.f / cfg->crkViscosity.eta_fold, cfg->crkViscosity.eta_crit,
-               cfg->crkViscosity.enableCRKLimiter, cfg->crkViscosity.enableVanLeerLimiter,
-               cfg->crkViscosity.forceVanLeerOff, cfg->crkViscosity.forceVanLeerOn};
-  float inv_kscale = (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim));
-  AdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V),
-          F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A),
-          F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
-  AdOut out{ed->ap, ed->av, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
-  int GS = ctx->group[CRK_PASS_ACCEL];
-  CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_accel_dudt<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
-                         cfg->kernel, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q, cfg->alphaWeightedQ,
-                         n, in, ctx->adj.perm, out));
-  CRK_LAUNCH_CHECK("k_accel_dudt");
-  ctx->have_accel = true;
-  return CRK_OK;
-}
-
-int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, void* stream) {
-  int r = require_bound(ctx, cfg);
-  if (r) return r;
-  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av && ed->f)), "crk_balance: edge buffers missing");
-  CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
-              "crk_balance: edges differ from the pairs of crk_support");
-  if (!ctx->have_accel) { set_error("crk_balance before crk_accel_dudt"); return CRK_ESTATE; }
-  hipStream_t s = (hipStream_t)stream;
-  int32_t n = ctx->n;
-  float hdt = (float)(0.5 * (double)dt);
-  int GS = ctx->group[CRK_PASS_BALANCE];
-  CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n,
-                         F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->m), F<float>(ctx->sP), dt, hdt, ed->ap,
-                         ed->av, ed->f));
-  CRK_LAUNCH_CHECK("k_balance");
   return CRK_OK;
 }
 

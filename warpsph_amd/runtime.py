@@ -30,6 +30,9 @@ class Runtime:
         self._lut_key: Optional[Tuple] = None
         self.scratch = torch.empty(16, dtype=torch.float32, device=device)  # [0,1] h min/max, [2] Owen ratio
         self.nsing = torch.zeros(1, dtype=torch.int32, device=device)
+        # evaluate the Cullen-Dehnen switch inside the acceleration sweep when
+        # the viscosity does not read alpha (False: separate crk_switch pass)
+        self.fuse_switch = True
 
     def stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)

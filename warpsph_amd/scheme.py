@@ -115,9 +115,12 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     # velocity gradient (crkSPH.py:158-172)
     with timing.scope("velocity_gradient"):
         abi.call("crk_velocity_gradient", rt.ctx, C.byref(cfg), C.byref(cu), s)
-    # viscosity switch (crkSPH.py:195-200)
+    # viscosity switch (crkSPH.py:195-200).  Without alpha-weighted Q the
+    # accelerations do not read alpha, and the Cullen-Dehnen sum is evaluated
+    # in the acceleration sweep (crk_switch_accel_dudt).
     sw = compParams.viscositySwitchParams.scheme
-    if sw == ViscositySwitch.CullenDehnen2010:
+    fused = sw == ViscositySwitch.CullenDehnen2010 and not cfg.alphaWeightedQ and rt.fuse_switch
+    if sw == ViscositySwitch.CullenDehnen2010 and not fused:
         with timing.scope("switch"):
             abi.call("crk_switch", rt.ctx, C.byref(cfg), C.byref(cst), float(dt), s)
     elif sw == ViscositySwitch.Learned:
@@ -125,7 +128,11 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         apply_learned_switch(rt, cfg, st, float(dt), compParams)
     # accel + dudt (crkSPH.py:241-284), divergence and dEdt (:324-325)
     with timing.scope("accel_dudt"):
-        abi.call("crk_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed), s)
+        if fused:
+            abi.call("crk_switch_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed),
+                     float(dt), s)
+        else:
+            abi.call("crk_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed), s)
     # energy partition with v_half = v + dt/2 dvdt (crkSPH.py:341-364)
     with timing.scope("balance"):
         abi.call("crk_balance", rt.ctx, C.byref(cfg), float(dt), C.byref(ed), s)

@@ -12,6 +12,7 @@ C++/OpenMP CPU oracle on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -79,15 +80,24 @@ def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
     for _ in range(warmup):
         s = integ.function(s, crkSPH_step, dt, config, scheme).state
     torch.cuda.synchronize()
+    # Move the objects that exist now (torch's modules, the system) out of the
+    # collector's view: a full collection over them would stall the host for
+    # tens of ms in the middle of the timed steps (as timeit avoids by
+    # disabling the collector; here it stays enabled for new objects).
+    gc.collect()
+    gc.freeze()
     edges, pairs = [], []
     pt = PassTimer()
     t0 = time.perf_counter()
     if timer:
         pt.__enter__()
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     try:
-        for _ in range(steps):
+        marks[0].record()
+        for it in range(steps):
             r = integ.function(s, crkSPH_step, dt, config, scheme)
             s = r.state
+            marks[it + 1].record()
             edges.append(sum(st.aux[0].n_edges for st in r.stages) / len(r.stages))
             last = r
     finally:
@@ -97,8 +107,13 @@ def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
     t1 = time.perf_counter()
     # in-support pairs per stage of the last step (read after the timed region)
     P_mean = sum(st.aux[1].pairs.n_pairs for st in last.stages) / len(last.stages)
+    seq = [marks[i].elapsed_time(marks[i + 1]) for i in range(steps)]
+    step_ms = sorted(seq)
+    spread = {"min": step_ms[0], "median": step_ms[len(step_ms) // 2], "max": step_ms[-1]}
+    if os.environ.get("CRK_BENCH_STEP_TIMES"):
+        spread["all"] = [round(t, 3) for t in seq]
     return (s, (t1 - t0), pt.summary() if timer else {}, float(sum(edges) / max(len(edges), 1)), float(P_mean), dt,
-            groups)
+            groups, spread)
 
 
 def cpu_baseline(nx, nsteps, warm):
@@ -155,7 +170,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
 
-    s, secs, summary, E_mean, P_mean, dt, groups = run_gpu(args.nx, args.steps, args.warmup, tuning_file=args.tuning_file,
+    s, secs, summary, E_mean, P_mean, dt, groups, spread = run_gpu(args.nx, args.steps, args.warmup, tuning_file=args.tuning_file,
                                                    retune=args.retune)
     n = args.nx * args.nx
     value = n * args.steps / secs
@@ -216,6 +231,7 @@ def main():
         "density_l1_vs_oracle": None if l1 is None else {"value": l1, "sample": f"{args.cpu_nx}^2, {1 + args.cpu_steps} steps"},
         "passes": passes,
         "launch_groups": groups,
+        "step_ms": spread,
     }
     print(json.dumps(line))
 

@@ -333,6 +333,35 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
 }
 
 // ------------------------------------------------------------ balance
+// computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154).
+// Memory-latency bound (two dependent loads per pair): every lane keeps two
+// pairs in flight.
+template <int D>
+__device__ inline float balance_f(const float* __restrict__ v, const float* __restrict__ dvdt,
+                                  const float* __restrict__ sP, const float* __restrict__ ap,
+                                  const float* __restrict__ av, float hdt, const float* vhi, float si, float mdt,
+                                  int32_t j, int64_t q) {
+  float d_ = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float vhj = v[(int64_t)j * D + a] + hdt * dvdt[(int64_t)j * D + a];
+    d_ = d_ + (vhj - vhi[a]) * (ap[q * D + a] + av[q * D + a]);
+  }
+  float dE = mdt * d_;
+  float sj = sP[j];
+  float asi = fabsf(si), asj = fabsf(sj);
+  float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
+  bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
+  bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
+  float ff = 0.5f;
+  if (!(si - sj == 0.f)) {
+    float rs = frcp(smin + smax + 1.0e-14f);
+    if (mB) ff = smin * rs;
+    else if (mC) ff = smax * rs;
+  }
+  return fmaxf(0.f, fminf(1.f, ff));
+}
+
 template <int D, int G>
 __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float* __restrict__ v,
                                                     const float* __restrict__ dvdt, const float* __restrict__ m,
@@ -347,27 +376,15 @@ __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const
   const float si = sP[k];
   const float mdt = m[k] * dt;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
-    float d_ = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      float vhj = v[(int64_t)j * D + a] + hdt * dvdt[(int64_t)j * D + a];
-      d_ = d_ + (vhj - vhi[a]) * (ap[(int64_t)q * D + a] + av[(int64_t)q * D + a]);
-    }
-    float dE = mdt * d_;
-    float sj = sP[j];
-    float asi = fabsf(si), asj = fabsf(sj);
-    float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
-    bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
-    bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
-    float ff = 0.5f;
-    if (!(si - sj == 0.f)) {
-      float rs = frcp(smin + smax + 1.0e-14f);
-      if (mB) ff = smin * rs;
-      else if (mC) ff = smax * rs;
-    }
-    f[q] = fmaxf(0.f, fminf(1.f, ff));
+  for (int32_t q = b + L.g; q < e; q += 2 * G) {
+    const int32_t q1 = q + G;
+    const bool two = q1 < e;
+    const int32_t j0 = rows.nbr[q];
+    const int32_t j1 = rows.nbr[two ? q1 : q];
+    const float f0 = balance_f<D>(v, dvdt, sP, ap, av, hdt, vhi, si, mdt, j0, q);
+    const float f1 = balance_f<D>(v, dvdt, sP, ap, av, hdt, vhi, si, mdt, j1, two ? q1 : q);
+    f[q] = f0;
+    if (two) f[q1] = f1;
   }
 }
 

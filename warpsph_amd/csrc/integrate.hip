@@ -77,20 +77,35 @@ __device__ inline float group_total2(float s, float c) {
 // stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last sweep
 // (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep of
 // both stages.
+template <int D>
+__device__ inline float compat_term(const CompIn& s, const float* __restrict__ vhs, const float* vi, int32_t q) {
+  const int32_t j = s.nbr[q];
+  float d0 = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    d0 = d0 + (vhs[(int64_t)j * D + a] - vi[a]) * (s.ap[(int64_t)q * D + a] + s.av[(int64_t)q * D + a]);
+  return s.b * s.f[q] * d0;
+}
+__device__ inline void kahan_add(float& s, float& c, float x) {
+  float y = x - c;
+  float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// The sum over one stage's pairs of row k, two pairs in flight per lane
+// (memory-latency bound).
 template <int D, int G>
 __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g, const float* __restrict__ vhs,
                                   const float* vi, float& sa, float& ca) {
   const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
-  for (int32_t q = b + g; q < e; q += G) {
-    const int32_t j = s.nbr[q];
-    float d0 = 0.f;
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-      d0 = d0 + (vhs[(int64_t)j * D + a] - vi[a]) * (s.ap[(int64_t)q * D + a] + s.av[(int64_t)q * D + a]);
-    float y = s.b * s.f[q] * d0 - ca;
-    float t = sa + y;
-    ca = (t - sa) - y;
-    sa = t;
+  for (int32_t q = b + g; q < e; q += 2 * G) {
+    const int32_t q1 = q + G;
+    const bool two = q1 < e;
+    const float t0 = compat_term<D>(s, vhs, vi, q);
+    const float t1 = compat_term<D>(s, vhs, vi, two ? q1 : q);
+    kahan_add(sa, ca, t0);
+    if (two) kahan_add(sa, ca, t1);
   }
 }
 

@@ -65,7 +65,7 @@ def runtime(device) -> Runtime:
 
 def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor):
     """Morton keys -> stable LSD radix sort -> cell table -> CSR (all on the GPU)."""
-    from .systems import AdjacencyList
+    from .systems import AdjacencyList, capacity
     rt = runtime(positions.device)
     n = int(positions.shape[0])
     D = int(cfg.domain.dim)
@@ -92,7 +92,7 @@ def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch
     abi.call("crk_adjacency_count", rt.ctx, C.byref(cfg), C.byref(grid), n, abi.ptr(positions), abi.ptr(supports),
              abi.ptr(keys), abi.ptr(perm), abi.ptr(rank), abi.ptr(cs), abi.ptr(ce), abi.ptr(off), abi.ptr(xb),
              abi.ptr(hb), C.byref(E), rt.stream())
-    nbr = torch.empty(max(E.value, 1), dtype=torch.int32, device=dev)[:E.value]
+    nbr = torch.empty(capacity(E.value), dtype=torch.int32, device=dev)[:E.value]
     abi.call("crk_adjacency_fill", rt.ctx, C.byref(cfg), C.byref(grid), n, abi.ptr(cs), abi.ptr(ce), abi.ptr(off),
              abi.ptr(xb), abi.ptr(hb), abi.ptr(nbr), E.value, rt.stream())
     return AdjacencyList(n, grid, keys, perm, rank, cs, ce, off, nbr, xb.view(n, D), hb)

@@ -24,6 +24,18 @@ def _clone(v):
     return v.clone() if isinstance(v, torch.Tensor) else v
 
 
+def capacity(count: int) -> int:
+    """Allocation size for a per-edge array: rounded up to a multiple of 1/16
+    of the largest power of two below it (at least 2^20 entries), so that the
+    edge arrays of successive steps reuse the same cached device blocks while
+    the adjacency size drifts by a few percent."""
+    count = int(count)
+    if count <= (1 << 20):
+        return max(1, count)
+    g = max(1 << 20, (1 << (count.bit_length() - 1)) // 16)
+    return ((count + g - 1) // g) * g
+
+
 class StagePairs:
     """The interacting pairs of one derivative evaluation and their data.
 
@@ -36,7 +48,7 @@ class StagePairs:
 
     def __init__(self, adjacency, dim: int):
         dev = adjacency.perm.device
-        E = max(adjacency.n_edges, 1)
+        E = capacity(adjacency.n_edges)
         self.adjacency = adjacency
         self.dim = int(dim)
         self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)

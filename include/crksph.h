@@ -238,7 +238,12 @@ typedef struct CrkEdges {          /* one stage's interacting pairs, caller owne
                              (adjacency row order kept); the other slots are unused */
   float*   ap;    /* [E*dim] pairwise pressure acceleration, same slots      */
   float*   av;    /* [E*dim] pairwise viscous acceleration, same slots       */
-  float*   f;     /* [E]     energy partition, same slots                    */
+  float*   rec;   /* [n*4]   energy-partition record of sorted particle k:
+                             v_half = v + dt/2 dvdt (dim components, zero padded
+                             to 3) and s = P/rho^gamma (crk_balance)           */
+  float*   f;     /* [E]     energy partition f_ij in the pair slots; optional:
+                             NULL keeps it implicit (evaluated from rec where
+                             it is used)                                       */
 } CrkEdges;
 
 /* Gathers the state into sorted order (x, v, h, m, u, alpha0, divergence). */
@@ -297,7 +302,10 @@ int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
 int crk_switch_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                           CrkUpdate* up, CrkEdges* edges, float dt, void* stream);
 /* computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154)
- * with v_half = v + dt/2 dvdt (crkSPH.py:341).  Writes edges->f. */
+ * with v_half = v + dt/2 dvdt (crkSPH.py:341).  Writes edges->rec; f_ij is a
+ * function of rec_i, rec_j, m_i dt and ap + av of the pair, evaluated by
+ * crk_compatible_energy and crk_edges_export.  When edges->f is not NULL it
+ * is also written for every pair slot.                                      */
 int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* edges, void* stream);
 /* Launch shape: pass ids for crk_set_group_size.  G consecutive lanes share
  * one particle row (G in 1, 2, 4, ..., 64); chosen by the autotuner. */
@@ -311,13 +319,12 @@ int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group);
 /* Pair data of one stage in the reference layout (ap_ij, av_ij, f_ij over the
  * adjacency edges, src/warpSPH/systems/compSPH.py:30-32): slot data of
  * *edges scattered to adjacency order, zero on the pairs out of support.
- * Any output may be NULL.                                                   */
+ * f_ij is read from edges->f when stored, else evaluated from edges->rec with
+ * the (user-order) masses and the stage's dt.  Any output may be NULL.     */
 int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, const CrkEdges* edges,
-                     float* ap_out, float* av_out, float* f_out, void* stream);
+                     const float* masses, float dt, float* ap_out, float* av_out, float* f_out,
+                     void* stream);
 
-/* Copy a sorted-order context field back to user order (testing / export).
- * field: 0 V, 1 A, 2 B, 3 gradA, 4 gradB, 5 J(velocity gradient), 6 div(CD),
- *        7 vsig, 8 R. */
 int crk_export_field(CrkContext* ctx, int32_t field, float* out_user, void* stream);
 
 /* ------------------------------------------------ integrator helpers
@@ -332,9 +339,11 @@ int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream);
 /* Compatible-energy finalize (compSPH/multistep.py:52-94, systems/compSPH.py:87-129):
  * v_half = v0 + dt/2 (b0 dvdt0 + b1 dvdt1);
  * du_i = sum_k b_k sum_{pairs e of row i in stage k} f_e (v_half_j - v_half_i).(ap_e + av_e);
- * u = u0 + dt du; dudt_last = (u - u0)/dt.  Deterministic row sums. */
+ * u = u0 + dt du; dudt_last = (u - u0)/dt.  Deterministic row sums.  f_e is
+ * evaluated from the stage's records (crk_balance) with m_i dt; masses are
+ * in user order.                                                           */
 int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, float dt,
-                          const float* v0, const float* u0,
+                          const float* masses, const float* v0, const float* u0,
                           const float* dvdt0, const float* dvdt1, float b0, float b1,
                           const CrkAdjacency* adj0, const CrkEdges* e0,
                           const CrkAdjacency* adj1, const CrkEdges* e1,

@@ -97,7 +97,7 @@ class CrkUpdate(C.Structure):
 
 
 class CrkEdges(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "ap", "av", "f")]
+    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "ap", "av", "rec", "f")]
 
 
 _P = C.c_void_p
@@ -130,11 +130,12 @@ _PROTOS = {
                               C.POINTER(CrkEdges), C.c_float, _P],
     "crk_balance": [_P, C.POINTER(CrkConfig), C.c_float, C.POINTER(CrkEdges), _P],
     "crk_export_field": [_P, C.c_int32, _P, _P],
-    "crk_edges_export": [_P, C.c_int32, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, _P, _P, _P],
+    "crk_edges_export": [_P, C.c_int32, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, C.c_float, _P, _P, _P,
+                         _P],
     "crk_axpy": [C.c_int64, _P, _P, C.c_float, _P, _P],
     "crk_rk_combine": [C.c_int64, _P, _P, _P, C.c_float, C.c_float, C.c_float, _P, _P],
     "crk_wrap_positions": [C.POINTER(CrkConfig), C.c_int32, _P, _P],
-    "crk_compatible_energy": [_P, C.POINTER(CrkConfig), C.c_int32, C.c_float, _P, _P, _P, _P,
+    "crk_compatible_energy": [_P, C.POINTER(CrkConfig), C.c_int32, C.c_float, _P, _P, _P, _P, _P,
                               C.c_float, C.c_float, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges),
                               C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, _P, _P],
     "crk_set_group_size": [_P, C.c_int32, C.c_int32],

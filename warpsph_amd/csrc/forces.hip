@@ -333,58 +333,40 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
 }
 
 // ------------------------------------------------------------ balance
-// computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154).
-// Memory-latency bound (two dependent loads per pair): every lane keeps two
-// pairs in flight.
+// computeCompSPHBalanceTermWarp, CRK energy scheme (compSPH/balance.py:63-154)
+// with v_half = v + dt/2 dvdt (crkSPH.py:341).  The per-particle inputs of the
+// partition are packed into one record per sorted particle,
+// rec = {v_half (D components, zero padded), s = P/rho^gamma}; f_ij itself is
+// a function of rec_i, rec_j, m_i dt and a_ij = ap + av and is evaluated
+// where it is consumed (the compatible-energy finalize, the export).
 template <int D>
-__device__ inline float balance_f(const float* __restrict__ v, const float* __restrict__ dvdt,
-                                  const float* __restrict__ sP, const float* __restrict__ ap,
-                                  const float* __restrict__ av, float hdt, const float* vhi, float si, float mdt,
-                                  int32_t j, int64_t q) {
-  float d_ = 0.f;
+__global__ void k_balance_rec(int32_t n, const float* __restrict__ v, const float* __restrict__ dvdt,
+                              const float* __restrict__ sP, float hdt, float4* __restrict__ rec) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float vh[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    float vhj = v[(int64_t)j * D + a] + hdt * dvdt[(int64_t)j * D + a];
-    d_ = d_ + (vhj - vhi[a]) * (ap[q * D + a] + av[q * D + a]);
-  }
-  float dE = mdt * d_;
-  float sj = sP[j];
-  float asi = fabsf(si), asj = fabsf(sj);
-  float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
-  bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
-  bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
-  float ff = 0.5f;
-  if (!(si - sj == 0.f)) {
-    float rs = frcp(smin + smax + 1.0e-14f);
-    if (mB) ff = smin * rs;
-    else if (mC) ff = smax * rs;
-  }
-  return fmaxf(0.f, fminf(1.f, ff));
+  for (int d = 0; d < D; ++d) vh[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
+  rec[k] = make_float4(vh[0], vh[1], vh[2], sP[k]);
 }
 
+// f_ij over the pair slots (optional output of crk_balance)
 template <int D, int G>
-__global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float* __restrict__ v,
-                                                    const float* __restrict__ dvdt, const float* __restrict__ m,
-                                                    const float* __restrict__ sP, float dt, float hdt,
+__global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float4* __restrict__ rec,
+                                                    const float* __restrict__ m, float dt,
                                                     const float* __restrict__ ap, const float* __restrict__ av,
                                                     float* __restrict__ f) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
-  float vhi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) vhi[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
-  const float si = sP[k];
+  const float4 ri = rec[k];
   const float mdt = m[k] * dt;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q = b + L.g; q < e; q += 2 * G) {
-    const int32_t q1 = q + G;
-    const bool two = q1 < e;
-    const int32_t j0 = rows.nbr[q];
-    const int32_t j1 = rows.nbr[two ? q1 : q];
-    const float f0 = balance_f<D>(v, dvdt, sP, ap, av, hdt, vhi, si, mdt, j0, q);
-    const float f1 = balance_f<D>(v, dvdt, sP, ap, av, hdt, vhi, si, mdt, j1, two ? q1 : q);
-    f[q] = f0;
-    if (two) f[q1] = f1;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    const int32_t j = rows.nbr[q];
+    float a[3];
+#pragma unroll
+    for (int c = 0; c < D; ++c) a[c] = ap[(int64_t)q * D + c] + av[(int64_t)q * D + c];
+    f[q] = pair_fraction<D>(ri, rec[j], mdt, a);
   }
 }
 
@@ -521,19 +503,26 @@ int crk_switch_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
 int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, void* stream) {
   int r = require_bound(ctx, cfg);
   if (r) return r;
-  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av && ed->f)), "crk_balance: edge buffers missing");
+  CRK_REQUIRE(ed && ed->rec, "crk_balance: edges->rec missing");
+  CRK_REQUIRE(!ed->f || ctx->adj.n_edges == 0 || (ed->ap && ed->av), "crk_balance: edge buffers missing");
   CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
               "crk_balance: edges differ from the pairs of crk_support");
   if (!ctx->have_accel) { set_error("crk_balance before crk_accel_dudt"); return CRK_ESTATE; }
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
   float hdt = (float)(0.5 * (double)dt);
-  int GS = ctx->group[CRK_PASS_BALANCE];
-  CRK_DIM_G_DISPATCH(ctx->dim, GS,
-      hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n,
-                         F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->m), F<float>(ctx->sP), dt, hdt, ed->ap,
-                         ed->av, ed->f));
-  CRK_LAUNCH_CHECK("k_balance");
+  float4* rec = (float4*)ed->rec;
+  CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_balance_rec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, n,
+                                                F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->sP), hdt,
+                                                rec));
+  CRK_LAUNCH_CHECK("k_balance_rec");
+  if (ed->f && ctx->adj.n_edges > 0) {
+    int GS = ctx->group[CRK_PASS_BALANCE];
+    CRK_DIM_G_DISPATCH(ctx->dim, GS,
+        hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n, rec,
+                           F<float>(ctx->m), dt, ed->ap, ed->av, ed->f));
+    CRK_LAUNCH_CHECK("k_balance");
+  }
   return CRK_OK;
 }
 

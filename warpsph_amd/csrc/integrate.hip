@@ -2,7 +2,7 @@
 // reductions of the CRKSPH step (CFL dt, FP64 conservation sums, min/max).
 #include <cmath>
 
-#include "common.hpp"
+#include "pairs.hpp"
 
 namespace crk {
 
@@ -48,10 +48,12 @@ __global__ void k_vhalf_sorted(const int32_t* __restrict__ perm, int32_t n, cons
   }
 }
 
-// one stage's in-support pairs (slots [off[k], off[k] + cnt[k])) and pair data
+// one stage's in-support pairs (slots [off[k], off[k] + cnt[k])), pair data
+// and energy-partition records
 struct CompIn {
   const int32_t *off, *cnt, *nbr;
-  const float *ap, *av, *f;
+  const float *ap, *av;
+  const float4* rec;
   float b;
 };
 
@@ -77,15 +79,6 @@ __device__ inline float group_total2(float s, float c) {
 // stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last sweep
 // (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep of
 // both stages.
-template <int D>
-__device__ inline float compat_term(const CompIn& s, const float* __restrict__ vhs, const float* vi, int32_t q) {
-  const int32_t j = s.nbr[q];
-  float d0 = 0.f;
-#pragma unroll
-  for (int a = 0; a < D; ++a)
-    d0 = d0 + (vhs[(int64_t)j * D + a] - vi[a]) * (s.ap[(int64_t)q * D + a] + s.av[(int64_t)q * D + a]);
-  return s.b * s.f[q] * d0;
-}
 __device__ inline void kahan_add(float& s, float& c, float x) {
   float y = x - c;
   float t = s + y;
@@ -93,27 +86,32 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
   s = t;
 }
 
-// The sum over one stage's pairs of row k, two pairs in flight per lane
-// (memory-latency bound).
+// The sum over one stage's pairs of row k: b f_ij (v_half_j - v_half_i).a_ij
+// with f_ij from the stage's records (the energy partition of crk_balance).
 template <int D, int G>
 __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g, const float* __restrict__ vhs,
-                                  const float* vi, float& sa, float& ca) {
+                                  const float* vi, float mdt, float& sa, float& ca) {
   const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
-  for (int32_t q = b + g; q < e; q += 2 * G) {
-    const int32_t q1 = q + G;
-    const bool two = q1 < e;
-    const float t0 = compat_term<D>(s, vhs, vi, q);
-    const float t1 = compat_term<D>(s, vhs, vi, two ? q1 : q);
-    kahan_add(sa, ca, t0);
-    if (two) kahan_add(sa, ca, t1);
+  const float4 ri = s.rec[k];
+  for (int32_t q = b + g; q < e; q += G) {
+    const int32_t j = s.nbr[q];
+    float a[3], d0 = 0.f;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      a[c] = s.ap[(int64_t)q * D + c] + s.av[(int64_t)q * D + c];
+      d0 = d0 + (vhs[(int64_t)j * D + c] - vi[c]) * a[c];
+    }
+    const float f = pair_fraction<D>(ri, s.rec[j], mdt, a);
+    kahan_add(sa, ca, s.b * f * d0);
   }
 }
 
 template <int D, int G, int NST, int MODE>
 __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, int32_t n,
                                                    const float* __restrict__ vhs, CompIn s0, CompIn s1,
-                                                   float* __restrict__ du0, const float* __restrict__ u0, float dt,
-                                                   float* __restrict__ u, float* __restrict__ dudt_last) {
+                                                   const float* __restrict__ m_u, float* __restrict__ du0,
+                                                   const float* __restrict__ u0, float dt, float* __restrict__ u,
+                                                   float* __restrict__ dudt_last) {
   const int g = threadIdx.x % G;
   int32_t k = (int32_t)blockIdx.x * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
   const bool valid = k < n;
@@ -121,13 +119,14 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
   float vi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) vi[d] = vhs[(int64_t)k * D + d];
+  const int32_t i = perm[k];
+  const float mdt = m_u[i] * dt;
   float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
-  compat_sum<D, G>(s0, k, valid, g, vhs, vi, sa, ca);
-  if (NST == 2) compat_sum<D, G>(s1, k, valid, g, vhs, vi, sb, cb);
+  compat_sum<D, G>(s0, k, valid, g, vhs, vi, mdt, sa, ca);
+  if (NST == 2) compat_sum<D, G>(s1, k, valid, g, vhs, vi, mdt, sb, cb);
   float ta = group_total2<G>(sa, ca);
   float tb = NST == 2 ? group_total2<G>(sb, cb) : 0.f;
   if (!valid || g != 0) return;
-  int32_t i = perm[k];
   if (MODE == 0) {
     du0[i] = 0.f + ta;
   } else {
@@ -267,17 +266,18 @@ int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream) 
   return CRK_OK;
 }
 
-int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, float dt, const float* v0,
-                          const float* u0, const float* dvdt0, const float* dvdt1, float b0, float b1,
-                          const CrkAdjacency* adj0, const CrkEdges* e0, const CrkAdjacency* adj1,
+int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, float dt, const float* masses,
+                          const float* v0, const float* u0, const float* dvdt0, const float* dvdt1, float b0,
+                          float b1, const CrkAdjacency* adj0, const CrkEdges* e0, const CrkAdjacency* adj1,
                           const CrkEdges* e1, float* u_out, float* dudt_last, void* stream) {
-  CRK_REQUIRE(ctx && cfg && v0 && u0 && dvdt0 && dvdt1 && adj0 && e0 && adj1 && e1 && u_out && dudt_last,
+  CRK_REQUIRE(ctx && cfg && masses && v0 && u0 && dvdt0 && dvdt1 && adj0 && e0 && adj1 && e1 && u_out && dudt_last,
               "crk_compatible_energy: null argument");
+  CRK_REQUIRE(e0->rec && e1->rec, "crk_compatible_energy: energy-partition records missing (crk_balance)");
   CRK_REQUIRE(n > 0 && adj0->n == n && adj1->n == n, "crk_compatible_energy: particle count mismatch");
   CRK_REQUIRE(adj0->perm && adj1->perm && adj0->row_offsets && adj1->row_offsets && e0->count && e1->count,
               "crk_compatible_energy: adjacency/pair counts missing");
-  CRK_REQUIRE(adj0->n_edges == 0 || (e0->nbr && e0->ap && e0->av && e0->f), "stage-0 pairs missing");
-  CRK_REQUIRE(adj1->n_edges == 0 || (e1->nbr && e1->ap && e1->av && e1->f), "stage-1 pairs missing");
+  CRK_REQUIRE(adj0->n_edges == 0 || (e0->nbr && e0->ap && e0->av), "stage-0 pairs missing");
+  CRK_REQUIRE(adj1->n_edges == 0 || (e1->nbr && e1->ap && e1->av), "stage-1 pairs missing");
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
   hipStream_t s = (hipStream_t)stream;
@@ -288,16 +288,16 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   float* du0 = (float*)ctx->du0.p;
   // both stages in one sweep when they share the sorted order
   const bool shared = adj0->perm == adj1->perm;
-  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->ap, e0->av, e0->f, b0};
-  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->ap, e1->av, e1->f, b1};
+  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->ap, e0->av, (const float4*)e0->rec, b0};
+  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->ap, e1->av, (const float4*)e1->rec, b1};
   const int GS = ctx->group[CRK_PASS_COMPAT];
   unsigned nb = nblocks((int64_t)n * GS);
 #define VHALF(DD, A)                                                                                          \
   hipLaunchKernelGGL(k_vhalf_sorted<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, (A)->perm, n, v0, dvdt0, dvdt1, b0, \
                      b1, dt, vhs)
 #define COMPAT(DD, GG, NST, MODE, A, X, Y)                                                                     \
-  hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, n, vhs, X, Y, du0, \
-                     u0, dt, u_out, dudt_last)
+  hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, n, vhs, X, Y, masses, \
+                     du0, u0, dt, u_out, dudt_last)
 #define RUN(DD, GG)                     \
   if (shared) {                         \
     VHALF(DD, adj0);                    \

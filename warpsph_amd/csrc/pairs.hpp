@@ -171,6 +171,34 @@ __device__ inline void grad_wr(const CorrD<D>& cr, const float* x, const WdW& w,
   }
 }
 
+// ------------------------------------------------------------ energy partition
+// Fraction f_ij of the pair work dE = m_i dt (v_half_j - v_half_i).a_ij given
+// to particle i, CRK energy scheme (compSPH/balance.py:126-154): s = P/rho^gamma;
+// equal s -> 1/2; otherwise the pair work goes preferentially to the particle
+// that moves s towards the other's.
+__device__ inline float energy_fraction(float dE, float si, float sj) {
+  float asi = fabsf(si), asj = fabsf(sj);
+  float smin = fminf(asi, asj), smax = fmaxf(asi, asj);
+  bool mB = (dE >= 0.f && si >= sj) || (dE < 0.f && si < sj);
+  bool mC = (dE >= 0.f && si < sj) || (dE < 0.f && si >= sj);
+  float ff = 0.5f;
+  if (!(si - sj == 0.f)) {
+    float rs = frcp(smin + smax + 1.0e-14f);
+    if (mB) ff = smin * rs;
+    else if (mC) ff = smax * rs;
+  }
+  return fmaxf(0.f, fminf(1.f, ff));
+}
+// f_ij from the stage record rec = {v_half (D components), ..., s} of i and j
+template <int D>
+__device__ inline float pair_fraction(const float4& ri, const float4& rj, float mdt, const float* a) {
+  const float vi[3] = {ri.x, ri.y, ri.z}, vj[3] = {rj.x, rj.y, rj.z};
+  float d = 0.f;
+#pragma unroll
+  for (int c = 0; c < D; ++c) d = d + (vj[c] - vi[c]) * a[c];
+  return energy_fraction(mdt * d, ri.w, rj.w);
+}
+
 // ------------------------------------------------------------ rows
 // Full Verlet rows of the bound adjacency.
 struct Rows {

@@ -402,16 +402,20 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, float C
 // ------------------------------------------------------------ export
 // Slot data of a row to adjacency order: the pairs are the row's edges with
 // r < max(h_i, h_j) in row order, so one merge walk over the row matches
-// them (neighbours are distinct within a row).
+// them (neighbours are distinct within a row).  f_ij is evaluated from the
+// stage's energy-partition records (or read from the slots when stored).
 template <int D>
 __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* __restrict__ nbr,
                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ pnbr, int32_t n,
+                               const int32_t* __restrict__ perm, const float* __restrict__ m_u, float dt,
                                const float* __restrict__ ap, const float* __restrict__ av,
-                               const float* __restrict__ f, float* __restrict__ ap_o, float* __restrict__ av_o,
-                               float* __restrict__ f_o) {
+                               const float4* __restrict__ rec, const float* __restrict__ f,
+                               float* __restrict__ ap_o, float* __restrict__ av_o, float* __restrict__ f_o) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int32_t b = off[k], e = off[k + 1], c = cnt[k];
+  const float mdt = (f_o && !f) ? m_u[perm[k]] * dt : 0.f;
+  const float4 ri = (f_o && !f) ? rec[k] : make_float4(0.f, 0.f, 0.f, 0.f);
   int32_t p = 0;
   for (int32_t q = b; q < e; ++q) {
     const bool hit = p < c && pnbr[b + p] == nbr[q];
@@ -421,7 +425,20 @@ __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* _
       if (ap_o) ap_o[(int64_t)q * D + a] = hit ? ap[sl * D + a] : 0.f;
       if (av_o) av_o[(int64_t)q * D + a] = hit ? av[sl * D + a] : 0.f;
     }
-    if (f_o) f_o[q] = hit ? f[sl] : 0.f;
+    if (f_o) {
+      float fv = 0.f;
+      if (hit) {
+        if (f) {
+          fv = f[sl];
+        } else {
+          float a[3];
+#pragma unroll
+          for (int d = 0; d < D; ++d) a[d] = ap[sl * D + d] + av[sl * D + d];
+          fv = pair_fraction<D>(ri, rec[nbr[q]], mdt, a);
+        }
+      }
+      f_o[q] = fv;
+    }
     p += hit ? 1 : 0;
   }
 }
@@ -580,19 +597,21 @@ int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, i
   return CRK_OK;
 }
 
-int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, const CrkEdges* ed, float* ap_out,
-                     float* av_out, float* f_out, void* stream) {
+int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, const CrkEdges* ed,
+                     const float* masses, float dt, float* ap_out, float* av_out, float* f_out, void* stream) {
   CRK_REQUIRE(ctx && adj && ed, "crk_edges_export: null argument");
   CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3");
-  CRK_REQUIRE(adj->n > 0 && adj->row_offsets && ed->count, "crk_edges_export: adjacency/pairs missing");
+  CRK_REQUIRE(adj->n > 0 && adj->row_offsets && adj->perm && ed->count, "crk_edges_export: adjacency/pairs missing");
   if (adj->n_edges == 0) return CRK_OK;
   CRK_REQUIRE(adj->nbr && ed->nbr, "crk_edges_export: neighbour lists missing");
-  CRK_REQUIRE((!ap_out || ed->ap) && (!av_out || ed->av) && (!f_out || ed->f), "crk_edges_export: pair data missing");
+  CRK_REQUIRE((!ap_out || ed->ap) && (!av_out || ed->av), "crk_edges_export: pair data missing");
+  CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->ap && ed->av && dt > 0.f),
+              "crk_edges_export: f needs edges->f, or edges->rec with masses, ap, av and dt > 0");
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(dim, hipLaunchKernelGGL(k_edges_export<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->row_offsets,
-                                           adj->nbr, ed->count, ed->nbr, n, ed->ap, ed->av, ed->f, ap_out, av_out,
-                                           f_out));
+                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dt, ed->ap, ed->av,
+                                           (const float4*)ed->rec, ed->f, ap_out, av_out, f_out));
   CRK_LAUNCH_CHECK("k_edges_export");
   return CRK_OK;
 }

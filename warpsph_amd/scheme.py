@@ -96,7 +96,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
     # the interacting pairs of this stage (compacted by the support sweep,
     # which also computes the apparent volume) and their pair data
-    st.pairs = StagePairs(adj, dim)
+    st.pairs = StagePairs(adj, dim, st.masses)
     ed = st.pairs.cabi()
     with timing.scope("support"):
         abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
@@ -133,7 +133,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
                      float(dt), s)
         else:
             abi.call("crk_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed), s)
-    # energy partition with v_half = v + dt/2 dvdt (crkSPH.py:341-364)
+    # energy partition with v_half = v + dt/2 dvdt (crkSPH.py:341-364): the
+    # per-particle records; f_ij is evaluated where it is used
+    st.pairs.dt = float(dt)
     with timing.scope("balance"):
         abi.call("crk_balance", rt.ctx, C.byref(cfg), float(dt), C.byref(ed), s)
     return update, adj, st

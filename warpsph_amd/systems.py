@@ -46,20 +46,23 @@ class StagePairs:
     pairs out of support; ``full()`` produces that layout on demand.
     """
 
-    def __init__(self, adjacency, dim: int):
+    def __init__(self, adjacency, dim: int, masses: torch.Tensor):
         dev = adjacency.perm.device
         E = capacity(adjacency.n_edges)
         self.adjacency = adjacency
         self.dim = int(dim)
+        self.masses = masses
+        self.dt = 0.0  # set by the derivative evaluation (energy partition)
         self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)
         self.nbr = torch.empty(E, dtype=torch.int32, device=dev)
         self.ap = torch.empty((E, dim), dtype=torch.float32, device=dev)
         self.av = torch.empty((E, dim), dtype=torch.float32, device=dev)
-        self.f = torch.empty(E, dtype=torch.float32, device=dev)
+        self.rec = torch.empty((adjacency.n, 4), dtype=torch.float32, device=dev)
         self._full = None
 
     def cabi(self) -> abi.CrkEdges:
-        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.ap, self.av, self.f)))
+        """Pointers for the C-ABI; f_ij stays implicit (evaluated from rec)."""
+        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.ap, self.av, self.rec)), None)
 
     @property
     def n_pairs(self) -> int:
@@ -78,8 +81,8 @@ class StagePairs:
                 rt = runtime(dev)
                 A = a.cabi()
                 ed = self.cabi()
-                abi.call("crk_edges_export", rt.ctx, self.dim, C.byref(A), C.byref(ed), abi.ptr(ap), abi.ptr(av),
-                         abi.ptr(f), rt.stream())
+                abi.call("crk_edges_export", rt.ctx, self.dim, C.byref(A), C.byref(ed), abi.ptr(self.masses),
+                         float(self.dt), abi.ptr(ap), abi.ptr(av), abi.ptr(f), rt.stream())
             self._full = (ap, av, f)
         return self._full
 
@@ -320,7 +323,8 @@ class CompSPHSystem:
             A0, A1 = a0.cabi(), a1.cabi()
             from . import timing
             with timing.scope("compatible_energy"):
-                    abi.call("crk_compatible_energy", rt.ctx, C.byref(cfg), n, float(dt), abi.ptr(st0.velocities),
+                    abi.call("crk_compatible_energy", rt.ctx, C.byref(cfg), n, float(dt), abi.ptr(st0.masses),
+                         abi.ptr(st0.velocities),
                          abi.ptr(st0.internalEnergies), abi.ptr(updateValues[0].dvdt), abi.ptr(updateValues[1].dvdt),
                          float(weights[0]), float(weights[1]), C.byref(A0), C.byref(e0), C.byref(A1), C.byref(e1),
                          abi.ptr(u_new), abi.ptr(updateValues[-1].dudt), rt.stream())

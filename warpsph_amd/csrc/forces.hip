@@ -299,14 +299,15 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
     const float Pj = in.P[j];
     const float pP = -(Pi + Pj) * VV;
     const float pQ = -(Qi + Qj) * VV;
+    float apv[3], avv[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float apv = pP * gw[a];
-      float avv = pQ * gw[a];
-      o.ap[(int64_t)q0 * D + a] = apv;
-      o.av[(int64_t)q0 * D + a] = avv;
-      acc[a].add(apv + avv);
+      apv[a] = pP * gw[a];
+      avv[a] = pQ * gw[a];
+      acc[a].add(apv[a] + avv[a]);
     }
+    store_vec<D>(&o.ap[(int64_t)q0 * D], apv);
+    store_vec<D>(&o.av[(int64_t)q0 * D], avv);
     float dt_ = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];

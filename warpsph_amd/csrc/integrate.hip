@@ -59,17 +59,10 @@ struct CompIn {
 
 template <int G>
 __device__ inline float group_total2(float s, float c) {
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    float so = __shfl_xor(s, off, G);
-    float co = __shfl_xor(c, off, G);
-    float t = s + so;
-    float bp = t - s;
-    float e = (s - (t - bp)) + (so - bp);
-    s = t;
-    c = (c + co) - e;
-  }
-  return s - c;
+  Comp a;
+  a.s = s;
+  a.c = c;
+  return group_total<G>(a);
 }
 
 // Row sums of b_k f (v_half_j - v_half_i).(ap + av) over the in-support

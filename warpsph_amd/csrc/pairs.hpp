@@ -4,10 +4,9 @@
 //
 // Work mapping: a group of G consecutive lanes of a wave64 owns one sorted
 // particle row; lane g handles the row's pairs g, g + G, ... so pair arrays
-// (neighbour ids, ap, av, f) stream coalesced.  Each lane keeps
-// Kahan-compensated FP32 partial sums; the G partials are combined by an
-// exact TwoSum butterfly (deterministic, every lane ends with the same
-// value).
+// (neighbour ids, ap, av) stream coalesced.  Each lane keeps
+// Kahan-compensated FP32 partial sums; the G partials are combined in FP64
+// (deterministic, every lane ends with the same value).
 #pragma once
 
 #include "common.hpp"
@@ -35,6 +34,17 @@ __device__ inline void pair_dx(const Geo& g, const float* xi, const float* xj, f
   }
 }
 
+// D-vector store with one instruction for D = 2 (8-byte aligned rows)
+template <int D>
+__device__ inline void store_vec(float* p, const float* v) {
+  if (D == 2) {
+    *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < D; ++a) p[a] = v[a];
+  }
+}
+
 // ------------------------------------------------------------ lane groups
 struct Comp {  // compensated value s - c
   float s = 0.f, c = 0.f;
@@ -46,20 +56,19 @@ struct Comp {  // compensated value s - c
   }
 };
 
+// Sum of the G lanes' compensated partials: each partial s - c is formed
+// exactly in FP64 and the G values are added by an FP64 xor butterfly (the
+// order is the same for every lane, so all lanes end with the same value).
+template <int G>
+__device__ inline double group_total_d(Comp a) {
+  double t = (double)a.s - (double)a.c;
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, G);
+  return t;
+}
 template <int G>
 __device__ inline float group_total(Comp a) {
-  float s = a.s, c = a.c;
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    float so = __shfl_xor(s, off, G);
-    float co = __shfl_xor(c, off, G);
-    float t = s + so;
-    float bp = t - s;
-    float e = (s - (t - bp)) + (so - bp);
-    s = t;
-    c = (c + co) - e;
-  }
-  return s - c;
+  return (float)group_total_d<G>(a);
 }
 
 template <int G>

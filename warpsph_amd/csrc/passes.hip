@@ -252,22 +252,22 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, float C
     for (int a = 0; a < D; ++a) S1[a].add(mW * dx[a]);
   }
   // combine partial sums across the group
-  double M0 = group_total<G>(m0), M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];
+  double M0 = group_total_d<G>(m0), M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];
 #pragma unroll
-  for (int a = 0; a < D; ++a) M1[a] = group_total<G>(m1[a]);
+  for (int a = 0; a < D; ++a) M1[a] = group_total_d<G>(m1[a]);
   {
     int p = 0;
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int bb = a; bb < D; ++bb, ++p) { double t = group_total<G>(m2[p]); M2[a][bb] = t; M2[bb][a] = t; }
+      for (int bb = a; bb < D; ++bb, ++p) { double t = group_total_d<G>(m2[p]); M2[a][bb] = t; M2[bb][a] = t; }
   }
 #pragma unroll
-  for (int gm = 0; gm < D; ++gm) DM0[gm] = group_total<G>(dm0[gm]);
+  for (int gm = 0; gm < D; ++gm) DM0[gm] = group_total_d<G>(dm0[gm]);
 #pragma unroll
   for (int a = 0; a < D; ++a)
 #pragma unroll
-    for (int gm = 0; gm < D; ++gm) DM1[a][gm] = group_total<G>(dm1[a * D + gm]);
+    for (int gm = 0; gm < D; ++gm) DM1[a][gm] = group_total_d<G>(dm1[a * D + gm]);
   {
     int p = 0;
 #pragma unroll
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, float C
 #pragma unroll
       for (int bb = a; bb < D; ++bb, ++p)
 #pragma unroll
-        for (int gm = 0; gm < D; ++gm) { double t = group_total<G>(dm2[p * D + gm]); DM2[a][bb][gm] = t; DM2[bb][a][gm] = t; }
+        for (int gm = 0; gm < D; ++gm) { double t = group_total_d<G>(dm2[p * D + gm]); DM2[a][bb][gm] = t; DM2[bb][a][gm] = t; }
   }
   float s0 = group_total<G>(S0), s1[3];
 #pragma unroll

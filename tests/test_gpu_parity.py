@@ -281,3 +281,39 @@ def test_balance_stored_f_equals_implicit():
     torch.cuda.synchronize()
     assert torch.equal(f_stored, f_implicit)
     assert float(f_implicit.min()) >= 0.0 and float(f_implicit.max()) <= 1.0
+
+
+def test_sod_t_end_density_l1():
+    """Sod shock tube to t_end = 0.2 (about 650 RK2 steps) on the GPU and on
+    the FP32 oracle with the same dt sequence: density L1 within 1e-4, and
+    both close to the exact Riemann solution."""
+    import os
+    from oracle import oracle as O
+    from warpsph_amd import sample
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import sod_exact
+    announce("sod1d to t = 0.2")
+    system, config, scheme = make_case("sod1d")
+    cpu = cpu_copy(system, config, scheme)
+    sim = O.from_system(cpu, config, scheme, 32)
+    integ = getIntegrator()
+    s, t, dt, steps = system, 0.0, -1.0, 0
+    while t < 0.2 - 1e-9:
+        dt = min(sim.cfl_dt(dt), 0.2 - t)
+        sim.step(dt, 1)
+        s = integ.function(s, crkSPH_step, dt, config, scheme).state
+        t += dt
+        steps += 1
+    torch.cuda.synchronize()
+    rho_g = s.state.densities.double().cpu().numpy()
+    rho_o = sim.get("rho")
+    l1 = float(np.mean(np.abs(rho_g - rho_o)) / np.mean(np.abs(rho_o)))
+    x = s.state.positions.double().cpu().numpy().reshape(-1)
+    m = cpu.state.masses.double().numpy()
+    exact = sod_exact.doubled_tube(x, 0.2)[:, 0]
+    l1_exact = float(np.sum(m / rho_g * np.abs(rho_g - exact)) / np.sum(m / rho_g * exact))
+    print(f"steps {steps}: density L1 gpu vs oracle {l1:.2e}, gpu vs exact {l1_exact:.4f}")
+    assert l1 <= 1e-4
+    assert l1_exact < 0.01

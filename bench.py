@@ -23,38 +23,48 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# lane-group sizes chosen by the autotuner for the 1M shear layer on MI355X
+DEFAULT_TUNING = os.path.join(ROOT, "warpsph_amd", "tuning", "mi355x_shear2d_1M.json")
 
 
-def pass_bytes(name: str, D: int):
-    """Algorithmic (compulsory) HBM bytes of one pass launch as
-    (bytes per particle, bytes per adjacency edge, bytes per in-support pair).
-    Neighbour-side gathers are not counted: every particle's data is read
-    once from HBM, the other touches are cache reuse."""
-    f = 4
-    xD = f * D
-    corr = f + 2 * xD + f * D * D              # A, B, gradA, gradB
-    row = 2 * f                                 # row offset + permutation entry
-    table = {
-        # x, h in; h sorted + user out; edges: j
-        "owen": (xD + f + row + 2 * f, f, 0),
-        # x, h, v, u -> V, vsig, pair count; edges: j; pairs: j out
-        "support": (2 * xD + 2 * f + row + 3 * f, f, f),
-        # x, h, V, m, u, pair count -> A,B,gA,gB, rho, P, c, P/rho^gamma + 4 user fields; pairs: j
-        "factors_eos": (xD + 4 * f + row + f + corr + 4 * f + 4 * f, 0, f),
-        # x, h, v, V, m, rho, A,B,gA,gB, count -> J, div, drhodt(user); pairs: j
-        "velocity_gradient": (xD + f + xD + 3 * f + corr + row + f + f * D * D + 2 * f, 0, f),
-        # x, h, m, rho, div, vsig, div_prev, alpha0, count -> alpha, R, alphas(user); pairs: j
-        "switch": (xD + 7 * f + row + f + 3 * f, 0, f),
-        # x, h, v, m, V, rho, P, c, J, A,B,gA,gB, drhodt, count -> dvdt (sorted+user), dudt, dEdt, div;
-        # pairs: j in, ap, av out
-        "accel_dudt": (xD + f + xD + 5 * f + f * D * D + corr + f + row + f + 2 * xD + 3 * f, 0, f + 2 * xD),
-        # v, dvdt, m, sP, count; pairs: j, ap, av in, f out
-        "balance": (2 * xD + 2 * f + row + f, 0, f + 2 * xD + f),
-        # v0, dvdt0, dvdt1 (gathered) -> sorted v_half; u0 -> u, dudt; counts of both stages;
-        # pairs of both stages: j, ap, av, f (the per-pair figure is per stage)
-        "compatible_energy": (3 * xD + 2 * xD + row + 3 * f + 2 * f, 0, f + 2 * xD + f),
-    }
-    return table.get(name)
+# Algorithmic bytes per launch, from the reference dataflow (each per-particle
+# array of a reference pass read once and written once; neighbour gathers
+# cache resident): per-particle bytes of the reference passes that one launch
+# performs, 2D FP32, plus bytes per pair entry the launch processes (ap, av
+# written: 8 + 8 B).  Reference passes: Owen 16; CRK volume 16, v_sig 28;
+# moments + solve 52, CRK density 32, EOS 20; grad v 76, CD divergence 32;
+# R 28, alpha update 32; accel 100, dudt 96; balance 32; finalize 28.
+REF_BYTES_2D = {
+    "owen": (16, 0),
+    "support": (16 + 28, 0),
+    "factors_eos": (52 + 32 + 20, 0),
+    "velocity_gradient": (76 + 32, 0),
+    "switch": (28 + 32, 0),
+    "accel_dudt": (100 + 96, 16),
+    "accel_dudt+switch": (100 + 96 + 28 + 32, 16),
+    "balance": (32, 0),
+    "compatible_energy": (28, 0),
+}
+
+
+def step_bytes(N: int, E: float, r: float) -> float:
+    """B_alg(step) = N (2*632 + 52 + 164 r) + E (2*60 + 4 r): the reference
+    dataflow of one RK2 step, 2D FP32, r = adjacency rebuilds per step."""
+    return N * (2 * 632 + 52 + 164 * r) + E * (2 * 60 + 4 * r)
+
+
+def measured_traffic(kernel_prefix: str):
+    """HBM bytes per dispatch of a kernel from the committed counter summary
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "crksph_step_shear2d_1M_hbm_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    for name, v in data.get("kernels", {}).items():
+        if name.startswith(kernel_prefix):
+            return v["traffic_bytes"]
+    return None
 
 
 def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
@@ -143,9 +153,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--cpu-nx", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tuning-file", default=None, help="load (or save after tuning) the lane-group sizes")
+    ap.add_argument("--tuning-file", default=DEFAULT_TUNING,
+                    help="load (or, with --retune or when absent, save after tuning) the lane-group sizes")
     ap.add_argument("--retune", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB buffer 4 times (calibrates the HBM traffic counters)")
@@ -178,18 +189,26 @@ def main():
     # dominant pass by total time
     dom = max(summary.items(), key=lambda kv: kv[1][1]) if summary else None
     roofline = None
+    from warpsph_amd.runtime import runtime
+    fused = bool(getattr(runtime(s.state.positions.device), "fuse_switch", False))
     if dom is not None:
         name, (calls, total_ms, mean_ms) = dom
-        pb = pass_bytes(name, 2)
+        key = name + ("+switch" if name == "accel_dudt" and fused else "")
+        pb = REF_BYTES_2D.get(key)
         if pb is not None:
-            stages = 2 if name == "compatible_energy" else 1
-            bytes_launch = pb[0] * n + pb[1] * E_mean + pb[2] * P_mean * stages
+            bytes_launch = pb[0] * n + pb[1] * P_mean
             achieved = bytes_launch / (mean_ms * 1e-3)
+            kernel = {"accel_dudt": "crk::k_accel_dudt", "support": "crk::k_support", "factors_eos": "crk::k_moments",
+                      "compatible_energy": "crk::k_compat"}.get(name, "crk::k_" + name)
             roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK, "traffic": None, "kernel": name,
+                        "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kernel), "kernel": kernel,
                         "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": mean_ms,
-                        "bytes_per_particle": pb[0], "bytes_per_edge": pb[1], "bytes_per_pair": pb[2],
-                        "edges_per_launch": E_mean, "pairs_per_launch": P_mean * stages}
+                        "bytes_per_particle": pb[0], "bytes_per_pair": pb[1], "pairs_per_launch": P_mean}
+    rebuilds = summary.get("adjacency_build", (0, 0.0, 0.0))[0] / max(args.steps, 1) if summary else 0.0
+    sb = step_bytes(n, E_mean, rebuilds)
+    step_roofline = {"bytes_per_step": sb, "edges": E_mean, "rebuilds_per_step": rebuilds,
+                     "achieved": sb / (secs / args.steps) / 1e9, "unit": "GB/s",
+                     "frac": sb / (secs / args.steps) / HBM_PEAK}
 
     cpu = None
     l1 = None
@@ -227,6 +246,7 @@ def main():
                                f"n_H=32, Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy",
                    "particles": n, "edges_per_stage": E_mean, "pairs_per_stage": P_mean, "dt": dt},
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "density_l1_vs_oracle": None if l1 is None else {"value": l1, "sample": f"{args.cpu_nx}^2, {1 + args.cpu_steps} steps"},
         "passes": passes,

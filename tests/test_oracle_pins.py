@@ -271,3 +271,22 @@ def test_sod_matches_exact_riemann_solution():
     l1 = float(np.sum(m / rho * np.abs(rho - exact)) / np.sum(m / rho * exact))
     print(f"Sod L1(rho) = {l1:.4f}")
     assert l1 < 0.01
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_owen_lut_is_invertible(dim):
+    """psi_H(n_h) strictly increasing (unique inversion) and the FP64 linear
+    interpolation returns the table nodes exactly."""
+    import ctypes as C
+    O = _oracle()
+    n = 4096
+    nh = np.empty(n)
+    ps = np.empty(n)
+    assert O.lib().orc_owen_lut(0, dim, n, 2.0, 6.0, nh.ctypes.data_as(C.c_void_p), ps.ctypes.data_as(C.c_void_p)) == 0
+    assert np.all(np.diff(nh) > 0) and np.all(np.diff(ps) > 0)
+    out = np.empty(64)
+    idx = np.linspace(0, n - 1, 64).astype(int)
+    x = np.ascontiguousarray(ps[idx])
+    assert O.lib().orc_lut_interp(n, ps.ctypes.data_as(C.c_void_p), nh.ctypes.data_as(C.c_void_p), 64,
+                                  x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)) == 0
+    assert np.allclose(out, nh[idx], rtol=1e-13)

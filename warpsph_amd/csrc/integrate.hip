@@ -105,10 +105,10 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
                                                    const float* __restrict__ m_u, float* __restrict__ du0,
                                                    const float* __restrict__ u0, float dt, float* __restrict__ u,
                                                    float* __restrict__ dudt_last) {
-  const int g = threadIdx.x % G;
-  int32_t k = (int32_t)blockIdx.x * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
-  const bool valid = k < n;
-  if (!valid) k = n - 1;
+  const Lane L = lane_of<G>(n);  // XCD-aware row order
+  const int g = L.g;
+  const int32_t k = L.k;
+  const bool valid = L.valid;
   float vi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) vi[d] = vhs[(int64_t)k * D + d];

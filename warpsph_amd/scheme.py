@@ -124,8 +124,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         with timing.scope("switch"):
             abi.call("crk_switch", rt.ctx, C.byref(cfg), C.byref(cst), float(dt), s)
     elif sw == ViscositySwitch.Learned:
-        from .learned import apply_learned_switch
-        apply_learned_switch(rt, cfg, st, float(dt), compParams)
+        raise NotImplementedError("the learned viscosity switch is not part of this build; use "
+                                  "ViscositySwitch.CullenDehnen2010 or NoneSwitch")
     # accel + dudt (crkSPH.py:241-284), divergence and dEdt (:324-325)
     with timing.scope("accel_dudt"):
         if fused:

@@ -260,6 +260,15 @@ int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* 
  * NULL, max |h_new/h - 1| to it (device float).                            */
 int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                        const float* hminmax_dev, float* max_ratio_dev, void* stream);
+/* The last Owen iteration of a stage fused with the per-row parts of
+ * crk_support: after the new h_i, a second sweep over the same row computes
+ * the CRK volume V_i = 1 / sum_j W(x_ij, h_i) and (Cullen-Dehnen switch) the
+ * signal speed with the new h_i.  Follow it, when the adjacency stays valid
+ * for the new supports, with crk_pairs_factors_eos (which compacts the pairs);
+ * otherwise rebuild, crk_stage_begin and crk_support as usual.  Same results
+ * as crk_owen_iteration + crk_support.                                      */
+int crk_owen_support(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                     const float* hminmax_dev, float* max_ratio_dev, void* stream);
 /* hminmax_dev[0] = min supports, [1] = max supports (user array). */
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_dev, void* stream);
 /* Support sweep of the bound adjacency for the current positions and supports
@@ -282,6 +291,14 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* edges, void* st
  * moment-matrix fallback.                                                   */
 int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                     int32_t* n_singular_dev, void* stream);
+/* crk_factors_eos fused with crk_support's compaction: one sweep over the
+ * full Verlet rows compacts the pairs with |x_ij| < max(h_i, h_j) into
+ * edges->count / edges->nbr (as crk_support does) while accumulating the CRK
+ * moments; V must be current (crk_owen_support or crk_support on the bound
+ * adjacency).  The context keeps edges->count and edges->nbr for the passes
+ * that follow.                                                              */
+int crk_pairs_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                          CrkEdges* edges, int32_t* n_singular_dev, void* stream);
 /* Velocity gradient (crkSPH.py:158-172): J_i = sum_j V_j (v_j - v_i) (x) gradW^R_ij,
  * drhodt = -tr(J) rho.  Writes update->drhodt. */
 int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, void* stream);

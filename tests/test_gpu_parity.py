@@ -317,3 +317,43 @@ def test_sod_t_end_density_l1():
     print(f"steps {steps}: density L1 gpu vs oracle {l1:.2e}, gpu vs exact {l1_exact:.4f}")
     assert l1 <= 1e-4
     assert l1_exact < 0.01
+
+
+@pytest.mark.parametrize("fuse", [False, True])
+def test_support_fused_or_separate(fuse):
+    """V and v_sig from the final Owen sweep plus the compaction inside the
+    moments pass (crk_owen_support + crk_pairs_factors_eos) and the separate
+    support sweep (crk_support + crk_factors_eos) both give the oracle's
+    stage (shear layer and 3D blast)."""
+    from oracle import oracle as O
+    from warpsph_amd import abi, sample
+    from warpsph_amd.runtime import runtime
+    from warpsph_amd.scheme import crkSPH_step
+    for name in ("shear2d", "blast3d"):
+        announce(f"stage {name} fuse_support={fuse}")
+        system, config, scheme = make_case(name)
+        cpu = cpu_copy(system, config, scheme)
+        dt = sample.cfl_dt(system.state, config, scheme) * (2.0 / 3.0)
+        rt = runtime(system.state.positions.device)
+        rt.fuse_support = fuse
+        try:
+            update, adj, st = crkSPH_step(system, dt, config, scheme)
+            n, D = st.positions.shape
+            ex = {}
+            for fid, key in ((0, "V"), (7, "vsig")):
+                out = torch.empty(n, dtype=torch.float32, device=st.positions.device)
+                abi.call("crk_export_field", rt.ctx, fid, abi.ptr(out), rt.stream())
+                ex[key] = out
+            torch.cuda.synchronize()
+        finally:
+            rt.fuse_support = False
+        sim = O.from_system(cpu, config, scheme, 32)
+        sim.evaluate(dt)
+        errs = {k: relerr(getattr(st, a).cpu().numpy(), sim.get(k)) for a, k in STATE_FIELDS}
+        errs["V"] = relerr(ex["V"].cpu().numpy(), sim.get("V"))
+        errs["vsig"] = relerr(ex["vsig"].cpu().numpy(), sim.get("vsig"))
+        errs["dvdt"] = relerr(update.dvdt.cpu().numpy(), sim.get("dvdt"))
+        _, _, ep = user_rows(adj)
+        errs["ap"] = relerr(st.ap_ij.cpu().numpy()[ep], sim.edge_field(0, "ap").reshape(-1, D))
+        print(name, errs)
+        assert all(v <= RTOL for v in errs.values()), (name, errs)

@@ -120,6 +120,8 @@ _PROTOS = {
     "crk_stage_rebind": [_P, C.POINTER(CrkConfig), C.POINTER(CrkAdjacency), C.POINTER(CrkState), _P],
     "crk_owen_iteration": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
     "crk_minmax": [_P, C.c_int32, _P, _P, _P],
+    "crk_owen_support": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
+    "crk_pairs_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.POINTER(CrkEdges), _P, _P],
     "crk_support": [_P, C.POINTER(CrkConfig), C.POINTER(CrkEdges), _P],
     "crk_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P],
     "crk_velocity_gradient": [_P, C.POINTER(CrkConfig), C.POINTER(CrkUpdate), _P],

@@ -72,13 +72,15 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
                 for _ in range(reps):
                     s = integ.function(_copy(base), crkSPH_step, dt, config, scheme).state
             summ = pt.summary()
-            t = summ.get(SCOPES[name], (0, 0.0, 0.0))[1]
+            if SCOPES[name] not in summ:  # the pass does not run in this configuration
+                break
+            t = summ[SCOPES[name]][1]
             report.setdefault(name, {})[g] = t
             if best_t is None or t < best_t:
                 best, best_t = g, t
         current[name] = best
         set_groups(dev, {name: best})
-        if verbose:
+        if verbose and name in report:
             print(f"autotune {name}: " + ", ".join(f"G={g}:{t:.3f}ms" for g, t in report[name].items()) + f" -> {best}")
     rt.tuning_report = report
     return current

@@ -60,6 +60,7 @@ struct CrkContext {
   int32_t* pair_cnt = nullptr;
   int32_t* pair_nbr = nullptr;
   bool have_support = false;
+  bool have_volume = false;  // V (and v_sig) current for the bound adjacency and supports
   // lanes per particle row for each pass (crk_set_group_size)
   int32_t group[CRK_NPASS] = {4, 8, 2, 8, 8, 16, 32, 32, 32};
   // bound adjacency / user state

@@ -130,13 +130,23 @@ __device__ inline double lut_interp(const double* __restrict__ xs, const double*
   return ys[il] * (1.0 - a) + ys[ir] * a;
 }
 
-template <int D, int G, int K>
+// FINAL: the last Owen iteration of a stage also computes, with the new h_i,
+// the per-row parts of the support sweep (the CRK volume V_i and, with the
+// Cullen-Dehnen switch, the signal speed) in a second sweep over the row,
+// whose neighbour data the first sweep has just brought into the caches.
+struct OwenFinal {
+  const float *v, *u;
+  float cfac;
+  float *V, *vsig;
+};
+
+template <int D, int G, int K, bool FINAL>
 __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, int32_t n,
                                                  const float* __restrict__ x, float* __restrict__ h,
                                                  const int32_t* __restrict__ perm, float* __restrict__ supports,
                                                  const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh,
                                                  int nlut, float nhT, const float* __restrict__ hminmax,
-                                                 float* __restrict__ ratio) {
+                                                 float* __restrict__ ratio, OwenFinal fin) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3];
@@ -158,17 +168,57 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
   }
   float S = group_total<G>(acc);
   float rmax = 0.f;
+  // every lane of the group evaluates the same new support (FINAL needs it)
+  float ps = powf(S, 1.f / (float)D);
+  float nh = (float)lut_interp(lut_psiH, lut_nh, nlut, (double)ps);
+  float sr = nhT / nh;
+  float a = sr >= 1.f ? 0.4f * (1.f + 1.f / (sr * sr * sr)) : 0.4f * (1.f + sr * sr);
+  float hn = (1.f - a + a * sr) * hi;
+  float lo = hminmax[0] * 0.25f, up = hminmax[1] * 4.0f;
+  hn = hn < lo ? lo : (hn > up ? up : hn);
   if (L.valid && L.g == 0) {
-    float ps = powf(S, 1.f / (float)D);
-    float nh = (float)lut_interp(lut_psiH, lut_nh, nlut, (double)ps);
-    float sr = nhT / nh;
-    float a = sr >= 1.f ? 0.4f * (1.f + 1.f / (sr * sr * sr)) : 0.4f * (1.f + sr * sr);
-    float hn = (1.f - a + a * sr) * hi;
-    float lo = hminmax[0] * 0.25f, up = hminmax[1] * 4.0f;
-    hn = hn < lo ? lo : (hn > up ? up : hn);
     h[k] = hn;
     supports[perm[k]] = hn;
     rmax = fabsf(hn / hi - 1.f);
+  }
+  if (FINAL) {
+    // crk_support's per-row work with h_i = hn (V: gather W(x_ij, h_i) over
+    // r < h_i; v_sig over every entry, wp_vsig.py:55-86)
+    float vi[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) vi[d] = fin.v[(int64_t)k * D + d];
+    const float ihn = 1.f / hn;
+    const float ci = sqrtf(fabsf(fin.u[k]) * fin.cfac);
+    const float eps = 1.0e-14f * hn;
+    float vs = 0.f;
+    Comp vol;
+    for (int32_t q = b + L.g; q < e; q += G) {
+      int32_t j = rows.nbr[q];
+      float dx[3];
+      pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+      float r = fsqrt(dot<D>(dx, dx));
+      if (fin.vsig) {
+        float mu = 0.f;
+#pragma unroll
+        for (int c = 0; c < D; ++c) mu = mu + (vi[c] - fin.v[(int64_t)j * D + c]) * dx[c];
+        mu = mu * frcp(r + eps);
+        float vv = 0.5f * (ci + fsqrt(fabsf(fin.u[j]) * fin.cfac)) - mu;
+        if (mu > 0.f) vv = 0.f;
+        vs = vv > vs ? vv : vs;
+      }
+      float qq = r * ihn;
+      if (qq < 1.f) {
+        float kk, dk;
+        kwd<K, D>(qq, kk, dk);
+        vol.add(C * kk * ipow<D>(ihn));
+      }
+    }
+    vs = group_max<G>(vs);
+    float Sv = group_total<G>(vol);
+    if (L.valid && L.g == 0) {
+      fin.V[k] = 1.f / Sv;
+      if (fin.vsig) fin.vsig[k] = vs;
+    }
   }
   if (ratio) {  // max |h_new / h - 1|, one atomic per block
     rmax = block_reduce(rmax, MaxOp());
@@ -190,33 +240,17 @@ struct MomOut {
   int32_t* nsing;
 };
 
-template <int D, int G, int K>
-__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, float C, int32_t n,
-                                                    const float* __restrict__ x, const float* __restrict__ h,
-                                                    const float* __restrict__ V, const float* __restrict__ m,
-                                                    const float* __restrict__ u, float gamma,
-                                                    const int32_t* __restrict__ perm, MomOut o) {
-  constexpr int NM2 = D * (D + 1) / 2;
-  Lane L = lane_of<G>(n);
-  const int32_t k = L.k;
-  float xi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
+template <int D>
+struct MomAcc {
+  static constexpr int NM2 = D * (D + 1) / 2;
   Comp m0, m1[D], m2[NM2], dm0[D], dm1[D * D], dm2[NM2 * D], S0, S1[D];
-  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
-    float dx[3];
-    pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float r = fsqrt(dot<D>(dx, dx));
-    float hj = h[j];
-    if (!(r < hj)) continue;
+  // one pair with r < h_j: W = W(x_ij, h_j) and its x_i-gradient
+  template <int K>
+  __device__ inline void add(float C, const float* dx, float r, float hj, float Vj, float mj) {
     WdW w = wdw<K, D>(C, r, inv_or_zero(r), frcp(hj));
     float g[3];
 #pragma unroll
     for (int d = 0; d < D; ++d) g[d] = w.fac * dx[d];
-    float Vj = V[j];
-    float mj = m[j];
     float VW = Vj * w.W;
     m0.add(VW);
 #pragma unroll
@@ -251,6 +285,65 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Geo geo, float C
 #pragma unroll
     for (int a = 0; a < D; ++a) S1[a].add(mW * dx[a]);
   }
+};
+
+// COMPACT = false: iterate the pairs compacted by the support sweep.
+// COMPACT = true: iterate the full Verlet rows and compact the pairs with
+// |x_ij| < max(h_i, h_j) into anbr / acnt on the way (the support sweep's
+// compaction, for when V and v_sig were computed by the final Owen sweep).
+template <int D, int G, int K, bool COMPACT>
+__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo geo, float C, int32_t n,
+                                                    const float* __restrict__ x, const float* __restrict__ h,
+                                                    const float* __restrict__ V, const float* __restrict__ m,
+                                                    const float* __restrict__ u, float gamma,
+                                                    const int32_t* __restrict__ perm, MomOut o,
+                                                    int32_t* __restrict__ anbr, int32_t* __restrict__ acnt) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
+  MomAcc<D> acc;
+  if (!COMPACT) {
+    const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+    for (int32_t q = b + L.g; q < e; q += G) {
+      int32_t j = rows.nbr[q];
+      float dx[3];
+      pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+      float r = fsqrt(dot<D>(dx, dx));
+      float hj = h[j];
+      if (!(r < hj)) continue;
+      acc.template add<K>(C, dx, r, hj, V[j], m[j]);
+    }
+  } else {
+    const float hi = h[k];
+    const int lane = threadIdx.x & 63;
+    const int gbase = lane & ~(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int32_t b = full.off[k], e = L.valid ? full.off[k + 1] : b;
+    int32_t run = 0;
+    for (int32_t base = b; base < e; base += G) {
+      const int32_t q = base + L.g;
+      bool act = false;
+      int32_t j = 0;
+      if (q < e) {
+        j = full.nbr[q];
+        float dx[3];
+        pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+        float r = fsqrt(dot<D>(dx, dx));
+        float hj = h[j];
+        act = r < fmaxf(hi, hj);
+        if (r < hj) acc.template add<K>(C, dx, r, hj, V[j], m[j]);
+      }
+      const unsigned long long bm = __ballot(act ? 1 : 0) & gmask;
+      if (act) anbr[b + run + __popcll(bm & lt)] = j;
+      run += __popcll(bm);
+    }
+    if (L.valid && L.g == 0) acnt[k] = run;
+  }
+  Comp &m0 = acc.m0, *m1 = acc.m1, *m2 = acc.m2, *dm0 = acc.dm0, *dm1 = acc.dm1, *dm2 = acc.dm2, &S0 = acc.S0,
+       *S1 = acc.S1;
   // combine partial sums across the group
   double M0 = group_total_d<G>(m0), M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];
 #pragma unroll
@@ -511,7 +604,7 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   ctx->adj = *adj;
   ctx->st = *st;
   ctx->bound = true;
-  ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = false;
+  ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = ctx->have_volume = false;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->perm, n,
@@ -529,8 +622,8 @@ int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* 
   return crk_stage_begin(ctx, cfg, adj, st, stream);
 }
 
-int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
-                       float* max_ratio_dev, void* stream) {
+static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
+                       float* max_ratio_dev, bool final_sweep, void* stream) {
   int r = require_bound(ctx, cfg);
   if (r) return r;
   CRK_REQUIRE(st && st->supports && hminmax_dev, "crk_owen_iteration: null argument");
@@ -541,14 +634,37 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   if (max_ratio_dev) { if ((r = fill_f32(max_ratio_dev, 1, 0.f, s))) return r; }
   ctx->have_support = false;
+  ctx->have_volume = false;
   int GS = ctx->group[CRK_PASS_OWEN];
-  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
-      hipLaunchKernelGGL((k_owen<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
-                         C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
-                         F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                         hminmax_dev, max_ratio_dev));
+  if (final_sweep) {
+    float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)
+                    ? F<float>(ctx->vsig) : nullptr;
+    OwenFinal fin{F<float>(ctx->v), F<float>(ctx->u), cfg->gamma * (cfg->gamma - 1.f), F<float>(ctx->V), vs};
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_owen<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+                           geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
+                           F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
+                           hminmax_dev, max_ratio_dev, fin));
+    ctx->have_volume = true;
+  } else {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_owen<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+                           geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
+                           F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
+                           hminmax_dev, max_ratio_dev, OwenFinal{}));
+  }
   CRK_LAUNCH_CHECK("k_owen");
   return CRK_OK;
+}
+
+int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
+                       float* max_ratio_dev, void* stream) {
+  return owen_launch(ctx, cfg, st, hminmax_dev, max_ratio_dev, false, stream);
+}
+
+int crk_owen_support(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
+                     float* max_ratio_dev, void* stream) {
+  return owen_launch(ctx, cfg, st, hminmax_dev, max_ratio_dev, true, stream);
 }
 
 int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* stream) {
@@ -571,13 +687,12 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   ctx->pair_cnt = ed->count;
   ctx->pair_nbr = ed->nbr;
   ctx->have_support = true;
+  ctx->have_volume = true;
   return CRK_OK;
 }
 
-int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, int32_t* n_singular_dev, void* stream) {
-  int r = require_bound(ctx, cfg);
-  if (r) return r;
-  if (!ctx->have_support) { set_error("crk_factors_eos before crk_support"); return CRK_ESTATE; }
+static int factors_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, CrkEdges* ed,
+                          int32_t* n_singular_dev, void* stream) {
   CRK_REQUIRE(st && st->densities && st->entropies && st->pressures && st->soundspeeds,
               "crk_factors_eos: state output buffers missing");
   hipStream_t s = (hipStream_t)stream;
@@ -588,13 +703,43 @@ int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, i
             F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->sP), st->densities, st->entropies, st->pressures,
             st->soundspeeds, n_singular_dev};
   int GM = ctx->group[CRK_PASS_MOMENTS];
-  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
-      hipLaunchKernelGGL((k_moments<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo,
-                         C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V), F<float>(ctx->m),
-                         F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo));
+  if (ed) {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
+        hipLaunchKernelGGL((k_moments<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, ARows{},
+                           rows_of(ctx), geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V),
+                           F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo, ed->nbr, ed->count));
+    ctx->pair_cnt = ed->count;
+    ctx->pair_nbr = ed->nbr;
+    ctx->have_support = true;
+  } else {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
+        hipLaunchKernelGGL((k_moments<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
+                           arows_of(ctx), Rows{}, geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V),
+                           F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo, nullptr, nullptr));
+  }
   CRK_LAUNCH_CHECK("k_moments");
   ctx->have_factors = true;
   return CRK_OK;
+}
+
+int crk_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, int32_t* n_singular_dev, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  if (!ctx->have_support) { set_error("crk_factors_eos before crk_support"); return CRK_ESTATE; }
+  return factors_launch(ctx, cfg, st, nullptr, n_singular_dev, stream);
+}
+
+int crk_pairs_factors_eos(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, CrkEdges* ed,
+                          int32_t* n_singular_dev, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(ed && ed->count && (ctx->adj.n_edges == 0 || ed->nbr),
+              "crk_pairs_factors_eos: pair buffers (count, nbr) missing");
+  if (!ctx->have_volume) {
+    set_error("crk_pairs_factors_eos needs V from crk_owen_support (or crk_support) on the bound adjacency");
+    return CRK_ESTATE;
+  }
+  return factors_launch(ctx, cfg, st, ed, n_singular_dev, stream);
 }
 
 int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, const CrkEdges* ed,

@@ -33,6 +33,11 @@ class Runtime:
         # evaluate the Cullen-Dehnen switch inside the acceleration sweep when
         # the viscosity does not read alpha (False: separate crk_switch pass)
         self.fuse_switch = True
+        # compute V and v_sig in the final Owen sweep and compact the pairs in
+        # the moments pass (True), or run the separate crk_support pass
+        # (False; measured faster on MI355X: the compaction costs more inside
+        # the moments pass than in its own sweep)
+        self.fuse_support = False
 
     def stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)

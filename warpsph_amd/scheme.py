@@ -69,13 +69,16 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         ratio = rt.scratch[2:3]
         abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
         iters = int(compParams.adaptiveSupportIterations)
+        # a single iteration also computes V and v_sig with the new supports
+        # (crk_owen_support); the support sweep then folds into the moments
+        fused_support = iters == 1 and rt.fuse_support
         for it in range(iters):
             adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
             A = adj.cabi()
             abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
             with timing.scope("owen"):
-                abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm),
-                         abi.ptr(ratio if iters > 1 else None), s)
+                abi.call("crk_owen_support" if fused_support else "crk_owen_iteration", rt.ctx, C.byref(cfg),
+                         C.byref(cst), abi.ptr(hmm), abi.ptr(ratio if iters > 1 else None), s)
             if it + 1 == iters:
                 break
             # hMin, hMax of the new supports bound the next iteration (optimalSupportOwen.py:103-104)
@@ -87,6 +90,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
 
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
     owen = compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen
+    if not owen:
+        fused_support = False
     adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports)
     system.adjacency = adj
     A = adj.cabi()
@@ -98,12 +103,18 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     # which also computes the apparent volume) and their pair data
     st.pairs = StagePairs(adj, dim, st.masses)
     ed = st.pairs.cabi()
-    with timing.scope("support"):
-        abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
-    # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
     rt.nsing.zero_()
-    with timing.scope("factors_eos"):
-        abi.call("crk_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(rt.nsing), s)
+    if fused_support and not rebuilt:
+        # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101) with the
+        # support sweep's compaction in the same pass
+        with timing.scope("factors_eos"):
+            abi.call("crk_pairs_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(ed), abi.ptr(rt.nsing), s)
+    else:
+        with timing.scope("support"):
+            abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
+        # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
+        with timing.scope("factors_eos"):
+            abi.call("crk_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(rt.nsing), s)
     update = CompressibleSystemUpdate(
         dxdt=st.velocities,  # dx/dt = v (the stage's velocity tensor, read only)
         dvdt=torch.empty_like(st.velocities),

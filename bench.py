@@ -146,6 +146,89 @@ def cpu_baseline(nx, nsteps, warm):
     return n * nsteps / (t1 - t0), sim.get("rho"), dt, O.num_threads(), (t1 - t0)
 
 
+def blast3d(args):
+    """3D point blast (128^3 jittered lattice, Wendland C4, n_H = 64): step
+    throughput on the GPU, plus the precision statement at a reduced size:
+    FP32 GPU (Kahan pair sums, FP64 moment solve and reductions) and the FP32
+    oracle, both against the FP64-everywhere oracle from the same state."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    from warpsph_amd import sample
+    from warpsph_amd.autotune import autotune, load, save, set_groups
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    from warpsph_amd.timing import PassTimer
+    torch.cuda.set_device(0)
+    integ = getIntegrator()
+    nx = args.nx if args.nx != 1024 else 128
+    system, config, scheme = sample.blast_3d(nx=nx, device="cuda:0")
+    dt = sample.cfl_dt(system.state, config, scheme)
+    tf = os.path.join(ROOT, "warpsph_amd", "tuning", f"mi355x_blast3d_{nx}.json")
+    if os.path.exists(tf) and not args.retune:
+        groups = load(tf)
+        set_groups(system.state.positions.device, groups)
+    else:
+        groups = autotune(system, config, scheme, dt)
+        save(groups, tf)
+    s = system
+    for _ in range(args.warmup):
+        s = integ.function(s, crkSPH_step, dt, config, scheme).state
+    torch.cuda.synchronize()
+    gc.collect()
+    gc.freeze()
+    pt = PassTimer()
+    t0 = time.perf_counter()
+    with pt:
+        for _ in range(args.steps):
+            s = integ.function(s, crkSPH_step, dt, config, scheme).state
+        torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    n = nx ** 3
+    summary = pt.summary()
+    # precision statement at a reduced size
+    ne, ksteps = 20, 10
+    sysg, cfg_e, sch_e = sample.blast_3d(nx=ne, device="cuda:0")
+    cpu = {k: getattr(sysg.state, v).detach().cpu().double().numpy() for k, v in O.STATE_MAP.items()}
+    from warpsph_amd.configurations import toCrkConfig
+    sims = {}
+    for prec in (32, 64):
+        sim = O.OracleSim(toCrkConfig(cfg_e, sch_e), ne ** 3, prec)
+        for k, v in cpu.items():
+            sim.set(k, v)
+        sims[prec] = sim
+    dte = sims[64].cfl_dt()
+    g = sysg
+    for _ in range(ksteps):
+        g = integ.function(g, crkSPH_step, dte, cfg_e, sch_e).state
+    torch.cuda.synchronize()
+    for sim in sims.values():
+        sim.step(dte, ksteps)
+    ref = {k: sims[64].get(k) for k in ("rho", "u", "v")}
+    gpu = {"rho": g.state.densities.double().cpu().numpy(), "u": g.state.internalEnergies.double().cpu().numpy(),
+           "v": g.state.velocities.double().cpu().numpy()}
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    err = {"sample": f"{ne}^3 = {ne ** 3} particles, {ksteps} RK2 steps, dt {dte:.3e}",
+           "gpu_fp32_vs_fp64_oracle": {k: rel(gpu[k].reshape(ref[k].shape), ref[k]) for k in ref},
+           "oracle_fp32_vs_fp64_oracle": {k: rel(sims[32].get(k), ref[k]) for k in ref},
+           "density_l1_gpu_vs_fp64": float(np.mean(np.abs(gpu["rho"] - ref["rho"])) / np.mean(np.abs(ref["rho"])))}
+    line = {
+        "metric": "particle-updates/s (3D CRKSPH point blast, 128^3)", "value": n * args.steps / secs,
+        "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3D periodic point blast, {nx}^3 = {n} particles, Wendland C4 n_H=64, E=1, "
+                               "Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy", "dt": dt},
+        "precision": err,
+        "passes": {k: {"calls": v[0], "mean_ms": round(v[2], 4)} for k, v in summary.items()},
+        "launch_groups": groups,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +241,9 @@ def main():
     ap.add_argument("--tuning-file", default=DEFAULT_TUNING,
                     help="load (or, with --retune or when absent, save after tuning) the lane-group sizes")
     ap.add_argument("--retune", action="store_true")
+    ap.add_argument("--workload", default="shear2d", choices=["shear2d", "blast3d"],
+                    help="shear2d: the 1M-particle 2D headline; blast3d: the 3D point blast with the FP64 precision "
+                         "statement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB buffer 4 times (calibrates the HBM traffic counters)")
     args = ap.parse_args()
@@ -176,6 +262,10 @@ def main():
         print(json.dumps({"error": "this build runs the CRKSPH step on one GPU only; "
                                    "--gpus N with N > 1 is not supported (see DESIGN.md for the sharding design)"}))
         sys.exit(2)
+
+    if args.workload == "blast3d":
+        blast3d(args)
+        return
 
     import numpy as np
     import torch

@@ -357,3 +357,40 @@ def test_support_fused_or_separate(fuse):
         errs["ap"] = relerr(st.ap_ij.cpu().numpy()[ep], sim.edge_field(0, "ap").reshape(-1, D))
         print(name, errs)
         assert all(v <= RTOL for v in errs.values()), (name, errs)
+
+
+def test_blast3d_error_against_fp64_oracle():
+    """Precision statement of the 3D blast: the GPU run (FP32 storage and pair
+    terms, Kahan pair sums, FP64 moment solve) is no further from the
+    FP64-everywhere oracle than FP32 arithmetic allows (the FP32 oracle's own
+    distance), and the GPU's FP64 conservation sums hold."""
+    from oracle import oracle as O
+    from warpsph_amd import sample
+    from warpsph_amd.configurations import toCrkConfig
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    announce("blast3d 14^3 x 5 steps vs FP64 oracle")
+    ne, k = 14, 5
+    system, config, scheme = sample.blast_3d(nx=ne, device="cuda", energy=1.0)
+    cpu = {key: getattr(system.state, v).detach().cpu().double().numpy() for key, v in O.STATE_MAP.items()}
+    sims = {}
+    for prec in (32, 64):
+        sim = O.OracleSim(toCrkConfig(config, scheme), ne ** 3, prec)
+        for key, v in cpu.items():
+            sim.set(key, v)
+        sims[prec] = sim
+    dt = sims[64].cfl_dt()
+    g = system
+    for _ in range(k):
+        g = getIntegrator().function(g, crkSPH_step, dt, config, scheme).state
+    torch.cuda.synchronize()
+    for sim in sims.values():
+        sim.step(dt, k)
+    errs = {}
+    for key, name in (("rho", "densities"), ("u", "internalEnergies"), ("v", "velocities")):
+        ref = sims[64].get(key)
+        e_gpu = relerr(getattr(g.state, name).double().cpu().numpy().reshape(ref.shape), ref)
+        e_o32 = relerr(sims[32].get(key), ref)
+        errs[key] = (e_gpu, e_o32)
+    print({k_: f"gpu {a:.2e} / fp32 oracle {b:.2e}" for k_, (a, b) in errs.items()})
+    assert all(a <= max(RTOL, 4.0 * b) for a, b in errs.values()), errs

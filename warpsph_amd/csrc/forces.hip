@@ -32,8 +32,10 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
   Comp acc[D * D];
   Comp dacc;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
+    const int32_t j = jn;
+    if (q + G < e) jn = rows.nbr[q + G];
     float dx[3];
     pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     float r = fsqrt(dot<D>(dx, dx));
@@ -134,8 +136,10 @@ __global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, float C,
   const float ih = 1.f / hi;
   Comp acc;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
+    const int32_t j = jn;
+    if (q + G < e) jn = rows.nbr[q + G];
     float dx[3];
     pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     float qq = fsqrt(dot<D>(dx, dx)) * ih;
@@ -229,8 +233,10 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
   const float ViOm = Vi / mi;
   Comp acc[D], accu, accs;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
-    const int32_t j = rows.nbr[q0];
+    const int32_t j = jn;
+    if (q0 + G < e) jn = rows.nbr[q0 + G];
     float dx[3];
     pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
     const float r2 = dot<D>(dx, dx);
@@ -362,8 +368,10 @@ __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const
   const float4 ri = rec[k];
   const float mdt = m[k] * dt;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q = b + L.g; q < e; q += G) {
-    const int32_t j = rows.nbr[q];
+    const int32_t j = jn;
+    if (q + G < e) jn = rows.nbr[q + G];
     float a[3];
 #pragma unroll
     for (int c = 0; c < D; ++c) a[c] = ap[(int64_t)q * D + c] + av[(int64_t)q * D + c];

@@ -86,8 +86,10 @@ __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g,
                                   const float* vi, float mdt, float& sa, float& ca) {
   const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
   const float4 ri = s.rec[k];
+  int32_t jn = b + g < e ? s.nbr[b + g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q = b + g; q < e; q += G) {
-    const int32_t j = s.nbr[q];
+    const int32_t j = jn;
+    if (q + G < e) jn = s.nbr[q + G];
     float a[3], d0 = 0.f;
 #pragma unroll
     for (int c = 0; c < D; ++c) {

@@ -25,18 +25,25 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
                          const float* __restrict__ Vel, const float* __restrict__ H, const float* __restrict__ M,
                          const float* __restrict__ U, const float* __restrict__ A0, const float* __restrict__ DIV,
                          const float* __restrict__ AL, float* x, float* v, float* h, float* m, float* u, float* a0,
-                         float* divp, float* alpha) {
+                         float* divp, float* alpha, float cfac, float4* __restrict__ geo4,
+                         float4* __restrict__ vel4) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int32_t i = perm[k];
+  float xs[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    x[(int64_t)k * D + d] = X[(int64_t)i * D + d];
-    v[(int64_t)k * D + d] = Vel[(int64_t)i * D + d];
+    xs[d] = X[(int64_t)i * D + d];
+    vs[d] = Vel[(int64_t)i * D + d];
+    x[(int64_t)k * D + d] = xs[d];
+    v[(int64_t)k * D + d] = vs[d];
   }
-  h[k] = H[i];
+  const float hk = H[i], uk = U[i];
+  h[k] = hk;
   m[k] = M[i];
-  u[k] = U[i];
+  u[k] = uk;
+  geo4[k] = make_float4(xs[0], xs[1], xs[2], hk);
+  vel4[k] = make_float4(vs[0], vs[1], vs[2], sqrtf(fabsf(uk) * cfac));
   a0[k] = A0 ? A0[i] : 1.f;
   divp[k] = DIV ? DIV[i] : 0.f;
   alpha[k] = AL ? AL[i] : 1.f;
@@ -49,13 +56,14 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
 // v_sig = max_j (0.5 (c_i + c_j) - mu_ij if mu_ij <= 0 else 0),
 // mu_ij = (v_i - v_j).x_ij / (r_ij + 1e-14 h_i), over every entry
 // (wp_vsig.py:55-86), c = sqrt(|u| gamma (gamma - 1)).
-template <int D, int G, int K>
+template <int D, int G, int K, bool VS>
 __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C, int32_t n,
                                                     const float* __restrict__ x, const float* __restrict__ h,
                                                     const float* __restrict__ v, const float* __restrict__ u,
                                                     float cfac, int32_t* __restrict__ anbr,
                                                     int32_t* __restrict__ acnt, float* __restrict__ vsig,
-                                                    float* __restrict__ V) {
+                                                    float* __restrict__ V, const float4* __restrict__ geo4,
+                                                    const float4* __restrict__ vel4) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3], vi[3];
@@ -63,7 +71,7 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   for (int d = 0; d < D; ++d) { xi[d] = x[(int64_t)k * D + d]; vi[d] = v[(int64_t)k * D + d]; }
   const float hi = h[k];
   const float ih = 1.f / hi;
-  const float ci = sqrtf(fabsf(u[k]) * cfac);
+  const float ci = vel4[k].w;
   const float eps = 1.0e-14f * hi;
   const int lane = threadIdx.x & 63;
   const int gbase = lane & ~(G - 1);
@@ -73,22 +81,28 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   int32_t run = 0;
   float vs = 0.f;
   Comp vol;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t base = b; base < e; base += G) {
     const int32_t q = base + L.g;
     const bool inside = q < e;
     bool act = false;
     int32_t j = 0;
     if (inside) {
-      j = rows.nbr[q];
+      j = jn;
+      if (q + G < e) jn = rows.nbr[q + G];
+      const float4 gj = geo4[j];
+      const float xj[3] = {gj.x, gj.y, gj.z};
       float dx[3];
-      pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+      pair_dx<D>(geo, xi, xj, dx);
       float r = fsqrt(dot<D>(dx, dx));
-      if (vsig) {
+      if (VS) {
+        const float4 wj = vel4[j];
+        const float vj[3] = {wj.x, wj.y, wj.z};
         float mu = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; ++a) mu = mu + (vi[a] - v[(int64_t)j * D + a]) * dx[a];
+        for (int a = 0; a < D; ++a) mu = mu + (vi[a] - vj[a]) * dx[a];
         mu = mu * frcp(r + eps);
-        float vv = 0.5f * (ci + fsqrt(fabsf(u[j]) * cfac)) - mu;
+        float vv = 0.5f * (ci + wj.w) - mu;
         if (mu > 0.f) vv = 0.f;
         vs = vv > vs ? vv : vs;
       }
@@ -98,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
         kwd<K, D>(qq, kk, dk);
         vol.add(C * kk * ipow<D>(ih));
       }
-      act = r < fmaxf(hi, h[j]);
+      act = r < fmaxf(hi, gj.w);
     }
     const unsigned long long m = __ballot(act ? 1 : 0) & gmask;
     if (act) {
@@ -111,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   if (!L.valid || L.g != 0) return;
   acnt[k] = run;
   V[k] = 1.f / S;
-  if (vsig) vsig[k] = vs;
+  if (VS) vsig[k] = vs;
 }
 
 // ------------------------------------------------------------ Owen
@@ -146,7 +160,8 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
                                                  const int32_t* __restrict__ perm, float* __restrict__ supports,
                                                  const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh,
                                                  int nlut, float nhT, const float* __restrict__ hminmax,
-                                                 float* __restrict__ ratio, OwenFinal fin) {
+                                                 float* __restrict__ ratio, float4* __restrict__ geo4,
+                                                 OwenFinal fin) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3];
@@ -156,8 +171,10 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
   const float ih = 1.f / hi;
   Comp acc;
   const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q = b + L.g; q < e; q += G) {
-    int32_t j = rows.nbr[q];
+    const int32_t j = jn;
+    if (q + G < e) jn = rows.nbr[q + G];
     float dx[3];
     pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
     float qq = fsqrt(dot<D>(dx, dx)) * ih;
@@ -178,6 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
   hn = hn < lo ? lo : (hn > up ? up : hn);
   if (L.valid && L.g == 0) {
     h[k] = hn;
+    geo4[k].w = hn;
     supports[perm[k]] = hn;
     rmax = fabsf(hn / hi - 1.f);
   }
@@ -192,8 +210,10 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
     const float eps = 1.0e-14f * hn;
     float vs = 0.f;
     Comp vol;
+    int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
     for (int32_t q = b + L.g; q < e; q += G) {
-      int32_t j = rows.nbr[q];
+      const int32_t j = jn;
+      if (q + G < e) jn = rows.nbr[q + G];
       float dx[3];
       pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
       float r = fsqrt(dot<D>(dx, dx));
@@ -306,8 +326,10 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
   MomAcc<D> acc;
   if (!COMPACT) {
     const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+    int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
     for (int32_t q = b + L.g; q < e; q += G) {
-      int32_t j = rows.nbr[q];
+      const int32_t j = jn;
+      if (q + G < e) jn = rows.nbr[q + G];
       float dx[3];
       pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
       float r = fsqrt(dot<D>(dx, dx));
@@ -323,12 +345,14 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
     const unsigned long long lt = (1ull << lane) - 1ull;
     const int32_t b = full.off[k], e = L.valid ? full.off[k + 1] : b;
     int32_t run = 0;
+    int32_t jn = b + L.g < e ? full.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
     for (int32_t base = b; base < e; base += G) {
       const int32_t q = base + L.g;
       bool act = false;
       int32_t j = 0;
       if (q < e) {
-        j = full.nbr[q];
+        j = jn;
+        if (q + G < e) jn = full.nbr[q + G];
         float dx[3];
         pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
         float r = fsqrt(dot<D>(dx, dx));
@@ -564,6 +588,8 @@ static int alloc_stage(CrkContext* ctx, int32_t n, int D) {
   for (Buf* b : sd) { int st = ensure(*b, f * D); if (st) return st; }
   Buf* sdd[] = {&ctx->gB, &ctx->J};
   for (Buf* b : sdd) { int st = ensure(*b, f * D * D); if (st) return st; }
+  Buf* s4[] = {&ctx->geo4, &ctx->vel4};
+  for (Buf* b : s4) { int st = ensure(*b, f * 4); if (st) return st; }
   return CRK_OK;
 }
 
@@ -613,7 +639,9 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
                                          (const float*)st->internalEnergies, (const float*)st->alpha0s,
                                          (const float*)st->divergence, (const float*)st->alphas, F<float>(ctx->x),
                                          F<float>(ctx->v), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->u),
-                                         F<float>(ctx->a0), F<float>(ctx->divp), F<float>(ctx->alpha)));
+                                         F<float>(ctx->a0), F<float>(ctx->divp), F<float>(ctx->alpha),
+                                         cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
+                                         F<float4>(ctx->vel4)));
   CRK_LAUNCH_CHECK("k_gather");
   return CRK_OK;
 }
@@ -644,14 +672,14 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
         hipLaunchKernelGGL((k_owen<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                           hminmax_dev, max_ratio_dev, fin));
+                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin));
     ctx->have_volume = true;
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_owen<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                           hminmax_dev, max_ratio_dev, OwenFinal{}));
+                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), OwenFinal{}));
   }
   CRK_LAUNCH_CHECK("k_owen");
   return CRK_OK;
@@ -679,10 +707,17 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)
                   ? F<float>(ctx->vsig) : nullptr;
   int GS = ctx->group[CRK_PASS_SUPPORT];
-  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
-      hipLaunchKernelGGL((k_support<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo,
-                         C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u),
-                         cfac, ed->nbr, ed->count, vs, F<float>(ctx->V)));
+  if (vs) {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_support<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+                           geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u), cfac,
+                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4)));
+  } else {
+    CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_support<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
+                           geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u), cfac,
+                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4)));
+  }
   CRK_LAUNCH_CHECK("k_support");
   ctx->pair_cnt = ed->count;
   ctx->pair_nbr = ed->nbr;

@@ -11,7 +11,7 @@
 //      found by scanning the 3^d neighbouring cells (k_count / k_fill).
 // All integer decisions use exactly rounded FP32 arithmetic so that the CPU
 // oracle reproduces them bit for bit.
-#include "common.hpp"
+#include "pairs.hpp"
 
 namespace crk {
 
@@ -158,21 +158,34 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
   if (k == n - 1 || keys[k + 1] != kk) ce[kk] = k + 1;
 }
 
-template <int D, bool FILL>
-__global__ void k_neighbors(GridDev g, CellScan sc, Geo geo, float s, const float* __restrict__ xb,
-                            const float* __restrict__ hb, const int32_t* __restrict__ cs,
-                            const int32_t* __restrict__ ce, int32_t n, int32_t* __restrict__ counts_or_offsets,
-                            int32_t* __restrict__ nbr) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+// Neighbour rows: for sorted particle k, every particle of the 3^D cells
+// around its cell with |x_k - x_j| < s max(h_k, h_j) (exact arithmetic),
+// cells in scan order and candidates ascending; count pass (FILL = false)
+// then fill pass.  G lanes per row: the lanes of a group test
+// consecutive candidates of a cell, and the hits are compacted with a group
+// ballot, so the row is written with coalesced stores in the same order as
+// the one-thread-per-row scan (cells in scan order, candidates ascending).
+template <int D, int G, bool FILL>
+__global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g, CellScan sc, Geo geo, float s,
+                                                        const float* __restrict__ xb, const float* __restrict__ hb,
+                                                        const int32_t* __restrict__ cs, const int32_t* __restrict__ ce,
+                                                        int32_t n, int32_t* __restrict__ counts_or_offsets,
+                                                        int32_t* __restrict__ nbr) {
+  const Lane L = lane_of<G>(n);
+  if (!L.valid) return;  // whole groups only: every lane of a group shares its row
+  const int32_t k = L.k;
   float xi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = xb[(int64_t)k * D + d];
-  float hi = hb[k];
+  const float hi = hb[k];
   int c[3];
   cell_of<D>(g, xi, c);
-  int32_t cnt = 0;
-  int64_t out = FILL ? (int64_t)counts_or_offsets[k] : 0;
+  const int lane = threadIdx.x & 63;
+  const int gbase = lane & ~(G - 1);
+  const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int32_t run = 0;
+  const int64_t out = FILL ? (int64_t)counts_or_offsets[k] : 0;
   for (int oz = 0; oz < sc.cnt[2]; ++oz)
     for (int oy = 0; oy < sc.cnt[1]; ++oy)
       for (int ox = 0; ox < sc.cnt[0]; ++ox) {
@@ -188,19 +201,26 @@ __global__ void k_neighbors(GridDev g, CellScan sc, Geo geo, float s, const floa
           cc[d] = q;
         }
         if (!ok) continue;
-        uint32_t code = morton<D>(cc);
-        int32_t b = cs[code], e = ce[code];
-        for (int32_t kk = b; kk < e; ++kk) {
-          float r2 = exact_r2<D>(geo, xi, &xb[(int64_t)kk * D]);
-          float R = __fmul_rn(s, fmaxf(hi, hb[kk]));
-          if (r2 < __fmul_rn(R, R)) {
-            if (FILL) nbr[out + cnt] = kk;
-            ++cnt;
+        const uint32_t code = morton<D>(cc);
+        const int32_t b = cs[code], e = ce[code];
+        for (int32_t base = b; base < e; base += G) {
+          const int32_t kk = base + L.g;
+          bool hit = false;
+          if (kk < e) {
+            const float r2 = exact_r2<D>(geo, xi, &xb[(int64_t)kk * D]);
+            const float R = __fmul_rn(s, fmaxf(hi, hb[kk]));
+            hit = r2 < __fmul_rn(R, R);
           }
+          const unsigned long long m = __ballot(hit ? 1 : 0) & gmask;
+          if (FILL && hit) nbr[out + run + __popcll(m & lt)] = kk;
+          run += __popcll(m);
         }
       }
-  if (!FILL) counts_or_offsets[k] = cnt;
+  if (!FILL && L.g == 0) counts_or_offsets[k] = run;
 }
+// lanes per row of the neighbour scan (candidates per row: ~9, ~180, ~830 in 1D/2D/3D)
+template <int D>
+constexpr int nbr_group() { return D == 1 ? 4 : (D == 2 ? 16 : 32); }
 
 // validity: red[0] = max |x - xb|^2, red[1] = max h/hb, red[2] = min hb
 // (grid-stride, one atomic per block)
@@ -292,8 +312,9 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   CRK_LAUNCH_CHECK("k_cells");
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
-  hipLaunchKernelGGL((k_neighbors<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale,
-                     xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr);
+  constexpr int GN = nbr_group<D>();
+  hipLaunchKernelGGL((k_neighbors_g<D, GN, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
+                     cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr);
   CRK_LAUNCH_CHECK("k_neighbors(count)");
   int64_t* total = (int64_t*)ctx->pinned.p;
   if ((st = ensure(ctx->s_red, 64))) return st;
@@ -356,12 +377,10 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
   GridDev g = make_griddev(*grid, *cfg);
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
-  switch (cfg->domain.dim) {
-    case 1: hipLaunchKernelGGL((k_neighbors<1, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, (int32_t*)row_offsets, nbr); break;
-    case 2: hipLaunchKernelGGL((k_neighbors<2, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, (int32_t*)row_offsets, nbr); break;
-    case 3: hipLaunchKernelGGL((k_neighbors<3, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, (int32_t*)row_offsets, nbr); break;
-    default: set_error("dim must be 1..3"); return CRK_EINVAL;
-  }
+  CRK_DIM_DISPATCH(cfg->domain.dim,
+      hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), true>), dim3(group_blocks(n, nbr_group<DD>())),
+                         dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
+                         (int32_t*)row_offsets, nbr));
   CRK_LAUNCH_CHECK("k_neighbors(fill)");
   return CRK_OK;
 }

@@ -55,9 +55,10 @@ struct CrkContext {
   int32_t dim = 0;
   int64_t cap_n = 0;
   crk::Buf x, v, h, m, u, a0, divp, alpha, V, A, B, gA, gB, rho, P, c, sP, J, divcd, vsig, Rcd, dvdt, vh, vhs, du0;
-  // packed neighbour records (sorted order, float4): geo = {x (3, zero
-  // padded), h}, vel = {v (3, zero padded), sqrt(|u| gamma (gamma - 1))}
-  crk::Buf geo4, vel4;
+  // packed neighbour data (sorted order): the acceleration pass's full
+  // record (Rec<D>::NF floats) and the narrow float4 records of the lighter
+  // passes, geo = {x (3, zero padded), h}, vel = {v (3, zero padded), c}
+  crk::Buf rec, geo4, vel4;
   // in-support pairs of each row, compacted in row order (caller-owned
   // buffers of the stage, bound by crk_support)
   int32_t* pair_cnt = nullptr;

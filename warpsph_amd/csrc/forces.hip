@@ -208,6 +208,41 @@ struct AdOut {
   float *ap, *av, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
 };
 
+// The acceleration pass gathers nearly every per-particle field of its
+// neighbours; one coalesced pass packs them into the neighbour record
+// (Rec<D>) first, so that a neighbour costs NF/4 16-byte loads.
+template <int D>
+__global__ void k_pack_rec(int32_t n, AdIn in, const float* __restrict__ divcd, float4* __restrict__ rec) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float r[Rec<D>::NF];
+#pragma unroll
+  for (int c = 0; c < Rec<D>::NF; ++c) r[c] = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    r[Rec<D>::X + d] = in.x[(int64_t)k * D + d];
+    r[Rec<D>::VEL + d] = in.v[(int64_t)k * D + d];
+    r[Rec<D>::B + d] = in.B[(int64_t)k * D + d];
+    r[Rec<D>::GA + d] = in.gA[(int64_t)k * D + d];
+  }
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) {
+    r[Rec<D>::J + a] = in.J[(int64_t)k * D * D + a];
+    r[Rec<D>::GB + a] = in.gB[(int64_t)k * D * D + a];
+  }
+  r[Rec<D>::H] = in.h[k];
+  r[Rec<D>::V] = in.V[k];
+  r[Rec<D>::RHO] = in.rho[k];
+  r[Rec<D>::C] = in.c[k];
+  r[Rec<D>::A] = in.A[k];
+  r[Rec<D>::P] = in.P[k];
+  r[Rec<D>::M] = in.m[k];
+  r[Rec<D>::DIV] = divcd ? divcd[k] : 0.f;
+  float4* p = rec + (int64_t)k * (Rec<D>::NF / 4);
+#pragma unroll
+  for (int c = 0; c < Rec<D>::NF / 4; ++c) p[c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+}
+
 // SW: the Cullen-Dehnen switch of the next step is evaluated in the same
 // sweep (valid when the viscosity does not read alpha, i.e. without
 // alpha-weighted Q): S_i = sum_j m_j sgn(div_j) W(x_ij, h_i) over r < h_i.
@@ -215,7 +250,7 @@ template <int D, int G, int K, bool SW>
 __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, float C, LimParams lp, float inv_kscale,
                                                        float Cl, float Cq, int alphaQ, int32_t n, AdIn in,
                                                        const int32_t* __restrict__ perm, AdOut o, CdIn cin,
-                                                       CdParams cp, CdOut cout) {
+                                                       CdParams cp, CdOut cout, const float4* __restrict__ rec) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3], vi[3], Ji[D * D];
@@ -237,18 +272,17 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
   for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
     const int32_t j = jn;
     if (q0 + G < e) jn = rows.nbr[q0 + G];
+    float rj[Rec<D>::NF];
+    rec_load<D>(rec, j, rj);
     float dx[3];
-    pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    pair_dx<D>(geo, xi, &rj[Rec<D>::X], dx);
     const float r2 = dot<D>(dx, dx);
     const float r = fsqrt(r2);
-    const float hj = in.h[j];
+    const float hj = rj[Rec<D>::H];
     const float ihj = frcp(hj);
     const float ir = inv_or_zero(r);
-    float vj[3], Jj[D * D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
-#pragma unroll
-    for (int a = 0; a < D * D; ++a) Jj[a] = in.J[(int64_t)j * D * D + a];
+    const float* vj = &rj[Rec<D>::VEL];
+    const float* Jj = &rj[Rec<D>::J];
     float phi = limiter_phi<D>(lp, dx, r, ihi, ihj, Ji, Jj);
     float phi2 = 0.5f * phi;
     float vdi[3], vdj[3], vdjU[3];
@@ -261,13 +295,18 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
       vdj[a] = vj[a] + phi2 * cJ;   // accel.py:137-141
       vdjU[a] = vj[a] - phi2 * cJ;  // dudt.py:123 (J_j (-x_ij))
     }
-    const CorrD<D> crj = load_corr<D>(in.A, in.B, in.gA, in.gB, j);
+    CorrD<D> crj;
+    crj.A = rj[Rec<D>::A];
+#pragma unroll
+    for (int a = 0; a < D; ++a) { crj.B[a] = rj[Rec<D>::B + a]; crj.gA[a] = rj[Rec<D>::GA + a]; }
+#pragma unroll
+    for (int a = 0; a < D * D; ++a) crj.gB[a] = rj[Rec<D>::GB + a];
     WdW wj = wdw<K, D>(C, r, ir, ihj);  // W(x_ij, h_j) for the i-side gradient
     WdW wi = wdw<K, D>(C, r, ir, ihi);  // W(x_ji, h_i) for the j-side gradient
     if (SW && wi.W != 0.f) {
-      float dj = cin.divcd[j];
+      float dj = rj[Rec<D>::DIV];
       float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);
-      accs.add(cin.m[j] * sg * wi.W);
+      accs.add(rj[Rec<D>::M] * sg * wi.W);
     }
     float gi[3], gjr[3], dxn[3];
 #pragma unroll
@@ -292,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
     const float fi = ismi * rdi, fj = ismj * rdj;
     float mui = fminf(wx * fi, 0.f), muj = fminf(wx * fj, 0.f);
     float muiU = fminf(wxU * fi, 0.f), mujU = fminf(wxU * fj, 0.f);
-    float rhoj = in.rho[j], cj = in.c[j];
+    float rhoj = rj[Rec<D>::RHO], cj = rj[Rec<D>::C];
     float Qi = rhoi * (-Cl * ci * mui + Cq * mui * mui);
     float Qj = rhoj * (-Cl * cj * muj + Cq * muj * muj);
     float QiU = rhoi * (-Cl * ci * muiU + Cq * muiU * muiU);
@@ -301,8 +340,8 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
       float wa = 0.5f * (ali + in.alpha[j]);
       Qi = Qi * wa; Qj = Qj * wa; QiU = QiU * wa; QjU = QjU * wa;
     }
-    const float VV = ViOm * in.V[j];
-    const float Pj = in.P[j];
+    const float VV = ViOm * rj[Rec<D>::V];
+    const float Pj = rj[Rec<D>::P];
     const float pP = -(Pi + Pj) * VV;
     const float pQ = -(Qi + Qj) * VV;
     float apv[3], avv[3];
@@ -473,17 +512,23 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
           F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
   AdOut out{ed->ap, ed->av, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
   int GS = ctx->group[CRK_PASS_ACCEL];
+  const float* divcd = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN ||
+                        cfg->switchParams.scheme == CRK_SWITCH_LEARNED) ? F<float>(ctx->divcd) : nullptr;
+  CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_pack_rec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, in, divcd,
+                                                F<float4>(ctx->rec)));
+  CRK_LAUNCH_CHECK("k_pack_rec");
   if (sw) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
                            arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
                            cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, cd_in(ctx), cd_params(cfg, dt),
-                           cd_out(ctx, st)));
+                           cd_out(ctx, st), F<float4>(ctx->rec)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
                            arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
-                           cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, CdIn{}, CdParams{}, CdOut{}));
+                           cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, CdIn{}, CdParams{}, CdOut{},
+                           F<float4>(ctx->rec)));
   }
   CRK_LAUNCH_CHECK("k_accel_dudt");
   ctx->have_accel = true;

@@ -45,6 +45,49 @@ __device__ inline void store_vec(float* p, const float* v) {
   }
 }
 
+// ------------------------------------------------------------ neighbour records
+// Every per-particle field that the acceleration pass gathers from its
+// neighbours, packed per sorted particle into NF floats (whole float4s): a
+// neighbour then costs NF/4 16-byte loads from one or two cache lines
+// instead of one load per field array.  The lighter passes gather a few
+// narrow fields and stay on the SoA arrays (a wide record would touch a
+// cache line per neighbour for a few bytes).
+template <int D>
+struct Rec;
+template <>
+struct Rec<1> {
+  static constexpr int NF = 16, X = 0, H = 1, V = 2, VEL = 3, RHO = 4, C = 5, J = 6, A = 7, B = 8, GA = 9, GB = 10,
+                       P = 11, M = 12, DIV = 13;
+};
+template <>
+struct Rec<2> {
+  static constexpr int NF = 24, X = 0, H = 2, V = 3, VEL = 4, RHO = 6, C = 7, J = 8, A = 12, B = 13, GA = 15,
+                       GB = 17, P = 21, M = 22, DIV = 23;
+};
+template <>
+struct Rec<3> {
+  static constexpr int NF = 40, X = 0, H = 3, V = 4, VEL = 5, RHO = 8, C = 9, P = 10, M = 11, DIV = 12, A = 13,
+                       B = 14, GA = 17, GB = 20, J = 29;
+};
+// the whole record of particle j (loads of fields the caller does not use
+// are removed by the compiler)
+template <int D>
+__device__ inline void rec_load(const float4* __restrict__ rec, int32_t j, float* r) {
+  const float4* p = rec + (int64_t)j * (Rec<D>::NF / 4);
+#pragma unroll
+  for (int c = 0; c < Rec<D>::NF / 4; ++c) {
+    const float4 t = p[c];
+    r[4 * c] = t.x;
+    r[4 * c + 1] = t.y;
+    r[4 * c + 2] = t.z;
+    r[4 * c + 3] = t.w;
+  }
+}
+template <int D>
+__device__ inline float* rec_ptr(float4* rec, int32_t k) {
+  return reinterpret_cast<float*>(rec + (int64_t)k * (Rec<D>::NF / 4));
+}
+
 // ------------------------------------------------------------ lane groups
 struct Comp {  // compensated value s - c
   float s = 0.f, c = 0.f;

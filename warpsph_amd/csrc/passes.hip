@@ -317,7 +317,8 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
                                                     const float* __restrict__ V, const float* __restrict__ m,
                                                     const float* __restrict__ u, float gamma,
                                                     const int32_t* __restrict__ perm, MomOut o,
-                                                    int32_t* __restrict__ anbr, int32_t* __restrict__ acnt) {
+                                                    int32_t* __restrict__ anbr, int32_t* __restrict__ acnt,
+                                                    const float4* __restrict__ geo4) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3];
@@ -331,9 +332,11 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
       const int32_t j = jn;
       if (q + G < e) jn = rows.nbr[q + G];
       float dx[3];
-      pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+      const float4 gj = geo4[j];
+      const float xj[3] = {gj.x, gj.y, gj.z};
+      pair_dx<D>(geo, xi, xj, dx);
       float r = fsqrt(dot<D>(dx, dx));
-      float hj = h[j];
+      float hj = gj.w;
       if (!(r < hj)) continue;
       acc.template add<K>(C, dx, r, hj, V[j], m[j]);
     }
@@ -354,9 +357,11 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
         j = jn;
         if (q + G < e) jn = full.nbr[q + G];
         float dx[3];
-        pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+        const float4 gj = geo4[j];
+        const float xj[3] = {gj.x, gj.y, gj.z};
+        pair_dx<D>(geo, xi, xj, dx);
         float r = fsqrt(dot<D>(dx, dx));
-        float hj = h[j];
+        float hj = gj.w;
         act = r < fmaxf(hi, hj);
         if (r < hj) acc.template add<K>(C, dx, r, hj, V[j], m[j]);
       }
@@ -588,6 +593,8 @@ static int alloc_stage(CrkContext* ctx, int32_t n, int D) {
   for (Buf* b : sd) { int st = ensure(*b, f * D); if (st) return st; }
   Buf* sdd[] = {&ctx->gB, &ctx->J};
   for (Buf* b : sdd) { int st = ensure(*b, f * D * D); if (st) return st; }
+  { int st = ensure(ctx->rec, sizeof(float) * (size_t)n * (D == 1 ? Rec<1>::NF : D == 2 ? Rec<2>::NF : Rec<3>::NF));
+    if (st) return st; }
   Buf* s4[] = {&ctx->geo4, &ctx->vel4};
   for (Buf* b : s4) { int st = ensure(*b, f * 4); if (st) return st; }
   return CRK_OK;
@@ -742,7 +749,8 @@ static int factors_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
         hipLaunchKernelGGL((k_moments<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, ARows{},
                            rows_of(ctx), geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V),
-                           F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo, ed->nbr, ed->count));
+                           F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo, ed->nbr, ed->count,
+                           F<float4>(ctx->geo4)));
     ctx->pair_cnt = ed->count;
     ctx->pair_nbr = ed->nbr;
     ctx->have_support = true;
@@ -750,7 +758,8 @@ static int factors_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
         hipLaunchKernelGGL((k_moments<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
                            arows_of(ctx), Rows{}, geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->V),
-                           F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo, nullptr, nullptr));
+                           F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo, nullptr, nullptr,
+                           F<float4>(ctx->geo4)));
   }
   CRK_LAUNCH_CHECK("k_moments");
   ctx->have_factors = true;

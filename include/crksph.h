@@ -249,6 +249,19 @@ typedef struct CrkEdges {          /* one stage's interacting pairs, caller owne
 /* Gathers the state into sorted order (x, v, h, m, u, alpha0, divergence). */
 int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj,
                     const CrkState* st, void* stream);
+/* crk_stage_begin fused with the Verlet validity test of the adjacency for the
+ * current state (the test of crk_adjacency_valid, with xb, hb the positions
+ * and supports the adjacency was built at) and with the support range
+ * (hminmax_dev[0] = min h, [1] = max h, for crk_owen_iteration).  Blocking:
+ * *valid = 1 when the adjacency holds every pair within the Verlet radius.
+ * When *valid = 0 the caller rebuilds and calls this again.  After it,
+ * crk_owen_iteration also records max h_new / h_b for crk_owen_valid.     */
+int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj,
+                            const CrkState* st, const float* xb, const float* hb,
+                            float* hminmax_dev, int32_t* valid, void* stream);
+/* Verlet validity after the Owen update of a checked stage (positions are
+ * unchanged, so only the supports moved).  Blocking.                       */
+int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* stream);
 /* Re-bind to a new adjacency mid-stage (after a rebuild), keeping h. */
 int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj,
                      const CrkState* st, void* stream);

@@ -79,6 +79,11 @@ struct CrkContext {
   // scratch
   crk::Buf s_keys0, s_keys1, s_vals0, s_vals1, s_hist, s_scan, s_red, s_counts;
   crk::Buf pinned;  // host pinned buffer for blocking reads
+  // Verlet validity of the bound stage (crk_stage_begin_checked / crk_owen_valid)
+  crk::Buf s_chk;
+  const float* check_hb = nullptr;
+  float check_d2 = 0.f, check_hbmin = 0.f;
+  bool owen_checked = false;
 };
 
 namespace crk {

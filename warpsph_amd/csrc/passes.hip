@@ -49,6 +49,81 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
   alpha[k] = AL ? AL[i] : 1.f;
 }
 
+// The gather of crk_stage_begin fused with the Verlet validity test of the
+// bound adjacency (built at xb, hb) and the support range for Owen:
+// red[0] = max |x - xb|^2 (exact), red[1] = max h / hb (exact), red[2] =
+// min hb; hmm[0] = min h, hmm[1] = max h.  One atomic per block and cell.
+template <int D>
+__global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo geo, const float* __restrict__ X,
+                               const float* __restrict__ Vel, const float* __restrict__ H,
+                               const float* __restrict__ M, const float* __restrict__ U,
+                               const float* __restrict__ A0, const float* __restrict__ DIV,
+                               const float* __restrict__ AL, float* x, float* v, float* h, float* m, float* u,
+                               float* a0, float* divp, float* alpha, float cfac, float4* __restrict__ geo4,
+                               float4* __restrict__ vel4, const float* __restrict__ xb,
+                               const float* __restrict__ hb, float* __restrict__ red, float* __restrict__ hmm) {
+  float d2 = 0.f, g = 0.f, hbm = 3.0e38f, hlo = 3.0e38f, hhi = 0.f;
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int32_t i = perm[k];
+    float xs[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      xs[d] = X[(int64_t)i * D + d];
+      vs[d] = Vel[(int64_t)i * D + d];
+      x[(int64_t)k * D + d] = xs[d];
+      v[(int64_t)k * D + d] = vs[d];
+    }
+    const float hk = H[i], uk = U[i], hbk = hb[k];
+    h[k] = hk;
+    m[k] = M[i];
+    u[k] = uk;
+    geo4[k] = make_float4(xs[0], xs[1], xs[2], hk);
+    vel4[k] = make_float4(vs[0], vs[1], vs[2], sqrtf(fabsf(uk) * cfac));
+    a0[k] = A0 ? A0[i] : 1.f;
+    divp[k] = DIV ? DIV[i] : 0.f;
+    alpha[k] = AL ? AL[i] : 1.f;
+    d2 = fmaxf(d2, exact_r2<D>(geo, xs, &xb[(int64_t)k * D]));
+    g = fmaxf(g, __fdiv_rn(hk, hbk));
+    hbm = fminf(hbm, hbk);
+    hlo = fminf(hlo, hk);
+    hhi = fmaxf(hhi, hk);
+  }
+  d2 = block_reduce(d2, MaxOp());
+  g = block_reduce(g, MaxOp());
+  hbm = block_reduce(hbm, MinOp());
+  hlo = block_reduce(hlo, MinOp());
+  hhi = block_reduce(hhi, MaxOp());
+  if (threadIdx.x == 0) {
+    atomic_max_pos(&red[0], d2);
+    atomic_max_pos(&red[1], g);
+    atomic_min_pos(&red[2], hbm);
+    atomic_min_pos(&hmm[0], hlo);
+    atomic_max_pos(&hmm[1], hhi);
+  }
+}
+// grid of the checked gather: enough blocks to fill the chip, few enough
+// that one atomic per block and cell stays cheap
+inline unsigned check_blocks(int32_t n) {
+  unsigned b = nblocks(n);
+  return b < 1024u ? b : 1024u;
+}
+// max h / hb over the sorted particles (after the Owen update)
+__global__ void k_hb_ratio(const float* __restrict__ h, const float* __restrict__ hb, int32_t n, float* red) {
+  float g = 0.f;
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    g = fmaxf(g, __fdiv_rn(h[k], hb[k]));
+  g = block_reduce(g, MaxOp());
+  if (threadIdx.x == 0) atomic_max_pos(red, g);
+}
+__global__ void k_check_init(float* red, float* hmm) {
+  red[0] = 0.f;
+  red[1] = 0.f;
+  red[2] = 3.0e38f;
+  red[3] = 0.f;  // max h / hb after Owen
+  hmm[0] = 3.0e38f;
+  hmm[1] = 0.f;
+}
+
 // ------------------------------------------------------------ support sweep
 // One pass over the full adjacency after the Owen update: the CRK apparent
 // volume V_i = 1 / sum_j W(x_ij, h_i); the in-support pairs compacted per row
@@ -638,6 +713,8 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   ctx->st = *st;
   ctx->bound = true;
   ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = ctx->have_volume = false;
+  ctx->check_hb = nullptr;
+  ctx->owen_checked = false;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->perm, n,
@@ -657,6 +734,68 @@ int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* 
   return crk_stage_begin(ctx, cfg, adj, st, stream);
 }
 
+int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj, const CrkState* st,
+                            const float* xb, const float* hb, float* hminmax_dev, int32_t* valid, void* stream) {
+  CRK_REQUIRE(ctx && cfg && xb && hb && hminmax_dev && valid, "crk_stage_begin_checked: null argument");
+  int r = validate_adj(adj, st);
+  if (r) return r;
+  int D = cfg->domain.dim;
+  CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
+  CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 2, "unknown kernel %d", cfg->kernel);
+  if ((r = alloc_stage(ctx, adj->n, D))) return r;
+  if ((r = ensure(ctx->s_chk, 16 * sizeof(float)))) return r;
+  ctx->n = adj->n;
+  ctx->dim = D;
+  ctx->adj = *adj;
+  ctx->st = *st;
+  ctx->bound = true;
+  ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = ctx->have_volume = false;
+  ctx->check_hb = hb;
+  ctx->owen_checked = false;
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = adj->n;
+  float* red = F<float>(ctx->s_chk);
+  hipLaunchKernelGGL(k_check_init, dim3(1), dim3(1), 0, s, red, hminmax_dev);
+  Geo geo = make_geo(*cfg);
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather_check<DD>, dim3(check_blocks(n)), dim3(kBlock), 0, s, adj->perm, n, geo,
+                                         (const float*)st->positions, (const float*)st->velocities,
+                                         (const float*)st->supports, (const float*)st->masses,
+                                         (const float*)st->internalEnergies, (const float*)st->alpha0s,
+                                         (const float*)st->divergence, (const float*)st->alphas, F<float>(ctx->x),
+                                         F<float>(ctx->v), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->u),
+                                         F<float>(ctx->a0), F<float>(ctx->divp), F<float>(ctx->alpha),
+                                         cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
+                                         F<float4>(ctx->vel4), xb, hb, red, hminmax_dev));
+  CRK_LAUNCH_CHECK("k_gather_check");
+  float* hostv = (float*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, red, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  ctx->check_d2 = hostv[0];
+  ctx->check_hbmin = hostv[2];
+  const double sv = cfg->verletScale, g = hostv[1];
+  *valid = g < sv && 2.0 * std::sqrt((double)hostv[0]) <= (sv - g) * (double)hostv[2];
+  return CRK_OK;
+}
+
+int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* stream) {
+  CRK_REQUIRE(ctx && cfg && valid, "crk_owen_valid: null argument");
+  if (!ctx->owen_checked) {
+    set_error("crk_owen_valid needs crk_stage_begin_checked followed by crk_owen_iteration");
+    return CRK_ESTATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  float* red = F<float>(ctx->s_chk) + 3;  // cleared by crk_stage_begin_checked
+  hipLaunchKernelGGL(k_hb_ratio, dim3(check_blocks(ctx->n)), dim3(kBlock), 0, s, F<float>(ctx->h), ctx->check_hb,
+                     ctx->n, red);
+  CRK_LAUNCH_CHECK("k_hb_ratio");
+  float* hostv = (float*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, red, sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  const double sv = cfg->verletScale, g = hostv[0];
+  *valid = g < sv && 2.0 * std::sqrt((double)ctx->check_d2) <= (sv - g) * (double)ctx->check_hbmin;
+  return CRK_OK;
+}
+
 static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
                        float* max_ratio_dev, bool final_sweep, void* stream) {
   int r = require_bound(ctx, cfg);
@@ -670,6 +809,7 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   if (max_ratio_dev) { if ((r = fill_f32(max_ratio_dev, 1, 0.f, s))) return r; }
   ctx->have_support = false;
   ctx->have_volume = false;
+  ctx->owen_checked = ctx->check_hb != nullptr;
   int GS = ctx->group[CRK_PASS_OWEN];
   if (final_sweep) {
     float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)

@@ -113,6 +113,31 @@ def adjacency_valid(cfg: abi.CrkConfig, adj, positions: torch.Tensor, supports: 
     return bool(v.value)
 
 
+def rebuild_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor):
+    with timing.scope("adjacency_build"):
+        return build_adjacency(cfg, positions, supports)
+
+
+def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: torch.Tensor):
+    """crk_stage_begin fused with the Verlet test of the prior adjacency and the
+    support range; rebuilds (and binds the new list) when the prior list no
+    longer holds every pair within the Verlet radius.  Returns the adjacency."""
+    rt = runtime(st.positions.device)
+    adj = prior if prior is not None and prior.n == st.numParticles else None
+    for _ in range(2):
+        if adj is None:
+            adj = rebuild_adjacency(cfg, st.positions, st.supports)
+        A = adj.cabi()
+        ok = C.c_int32()
+        with timing.scope("stage_begin"):
+            abi.call("crk_stage_begin_checked", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), abi.ptr(adj.xb),
+                     abi.ptr(adj.hb), abi.ptr(hminmax), C.byref(ok), rt.stream())
+        if ok.value:
+            return adj
+        adj = None
+    raise RuntimeError("an adjacency built for the current state failed its own validity test")
+
+
 def ensure_adjacency(cfg: abi.CrkConfig, prior, positions: torch.Tensor, supports: torch.Tensor):
     """buildVerletList(..., priorNeighborhood=prior): reuse the prior list
     while every pair inside the current supports is still in it."""

@@ -16,7 +16,7 @@ from . import abi, timing
 from .configurations import (CRKSPHConfig, SimulationConfig, crkSPHConfigToDict, dictToCRKSPHConfig,
                              toCrkConfig)
 from .enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, ViscositySwitch
-from .runtime import ensure_adjacency, runtime
+from .runtime import begin_checked, ensure_adjacency, rebuild_adjacency, runtime
 from .systems import CompressibleSystemUpdate, CompSPHState, CompSPHSystem, StagePairs
 
 
@@ -62,37 +62,54 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     cst = st.cabi()
     adj = system.adjacency
 
-    if compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen:
+    owen = compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen
+    fused_support = False
+    owen_checked = False
+    if owen:
         # evaluateOptimalSupportOwen, optimalSupportOwen.py:35-127
         rt.ensure_lut(cfg.kernel, dim, cfg.targetNeighbors)
         hmm = rt.scratch[0:2]
         ratio = rt.scratch[2:3]
-        abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
         iters = int(compParams.adaptiveSupportIterations)
         # a single iteration also computes V and v_sig with the new supports
         # (crk_owen_support); the support sweep then folds into the moments
         fused_support = iters == 1 and rt.fuse_support
-        for it in range(iters):
-            adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
-            A = adj.cabi()
-            abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
+        if iters == 1:
+            # the stage's gather carries the Verlet test of the prior list and
+            # the support range hMin, hMax (buildVerletList with priorNeighborhood)
+            adj = begin_checked(cfg, adj, st, cst, hmm)
             with timing.scope("owen"):
                 abi.call("crk_owen_support" if fused_support else "crk_owen_iteration", rt.ctx, C.byref(cfg),
-                         C.byref(cst), abi.ptr(hmm), abi.ptr(ratio if iters > 1 else None), s)
-            if it + 1 == iters:
-                break
-            # hMin, hMax of the new supports bound the next iteration (optimalSupportOwen.py:103-104)
+                         C.byref(cst), abi.ptr(hmm), None, s)
+            owen_checked = True
+        else:
             abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
-            if float(ratio.item()) < compParams.adaptiveSupportThreshold:
-                break
+            for it in range(iters):
+                adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
+                A = adj.cabi()
+                abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
+                with timing.scope("owen"):
+                    abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm),
+                             abi.ptr(ratio), s)
+                if it + 1 == iters:
+                    break
+                # hMin, hMax of the new supports bound the next iteration (optimalSupportOwen.py:103-104)
+                abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
+                if float(ratio.item()) < compParams.adaptiveSupportThreshold:
+                    break
         # The reference's final gather-density pass (:116-125) is overwritten by
         # computeCRKFactors (crkSPH.py:61) before any use; it is not evaluated.
 
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
-    owen = compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen
-    if not owen:
-        fused_support = False
-    adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports)
+    if owen_checked:
+        # positions are unchanged since the checked gather: only the supports moved
+        ok = C.c_int32()
+        abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), C.byref(ok), s)
+        rebuilt = not ok.value
+        if rebuilt:
+            adj = rebuild_adjacency(cfg, st.positions, st.supports)
+    else:
+        adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports)
     system.adjacency = adj
     A = adj.cabi()
     if rebuilt or not owen:

@@ -229,6 +229,63 @@ def blast3d(args):
     print(json.dumps(line))
 
 
+def ensemble(args):
+    """Batched ensemble: ``--runs`` independent 2D two-state runs (sizes
+    log-uniform in [4096, 65536], the data generator's seeds) stepped together,
+    every run with its own CFL dt.  Reports whole-ensemble particle-updates/s."""
+    import numpy as np
+    import torch
+    from warpsph_amd import ensemble as ens_mod
+    from warpsph_amd.autotune import load, set_groups
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    from warpsph_amd.timing import PassTimer
+    torch.cuda.set_device(0)
+    integ = getIntegrator()
+    t0 = time.perf_counter()
+    e, config, scheme = ens_mod.sample_ensemble(n_runs=args.runs, seed=args.seed, device="cuda:0")
+    gen_s = time.perf_counter() - t0
+    if os.path.exists(args.tuning_file):
+        groups = load(args.tuning_file)
+        set_groups(e.state.positions.device, groups)
+    else:
+        groups = None
+    n = e.runs.n
+    prev = None
+    for _ in range(args.warmup):
+        prev = ens_mod.cfl_dt(e, config, scheme, prev)
+        e = integ.function(e, crkSPH_step, prev, config, scheme).state
+    torch.cuda.synchronize()
+    gc.collect()
+    gc.freeze()
+    pt = PassTimer()
+    t0 = time.perf_counter()
+    with pt:
+        for _ in range(args.steps):
+            prev = ens_mod.cfl_dt(e, config, scheme, prev)
+            e = integ.function(e, crkSPH_step, prev, config, scheme).state
+        torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    summary = pt.summary()
+    counts = e.runs.counts
+    line = {
+        "metric": "particle-updates/s (2D CRKSPH batched ensemble)", "value": n * args.steps / secs,
+        "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.runs} independent 2D two-state runs (random line split, rho, P ~ U[0.1,1], "
+                               "v ~ U[-0.5,0.5]^2, gamma 5/3), N log-uniform in [4096, 65536], Wendland C2 "
+                               "n_H=32, Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy, per-run CFL dt",
+                   "particles": n, "runs": args.runs, "run_min": int(counts.min()), "run_max": int(counts.max()),
+                   "seed": args.seed},
+        "generate_s": gen_s,
+        "t_run_range": [float(np.min(e.t)), float(np.max(e.t))],
+        "passes": {k: {"calls": v[0], "mean_ms": round(v[2], 4)} for k, v in summary.items()},
+        "launch_groups": groups,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,9 +298,11 @@ def main():
     ap.add_argument("--tuning-file", default=DEFAULT_TUNING,
                     help="load (or, with --retune or when absent, save after tuning) the lane-group sizes")
     ap.add_argument("--retune", action="store_true")
-    ap.add_argument("--workload", default="shear2d", choices=["shear2d", "blast3d"],
+    ap.add_argument("--workload", default="shear2d", choices=["shear2d", "blast3d", "ensemble"],
                     help="shear2d: the 1M-particle 2D headline; blast3d: the 3D point blast with the FP64 precision "
-                         "statement")
+                         "statement; ensemble: the batched two-state runs (--runs, --seed)")
+    ap.add_argument("--runs", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB buffer 4 times (calibrates the HBM traffic counters)")
     args = ap.parse_args()
@@ -265,6 +324,9 @@ def main():
 
     if args.workload == "blast3d":
         blast3d(args)
+        return
+    if args.workload == "ensemble":
+        ensemble(args)
         return
 
     import numpy as np

@@ -144,6 +144,46 @@ const char* crk_version(void);
  * CrkEdges (in this order) written to out[0..n-1]. */
 int crk_struct_sizes(int64_t* out, int32_t n);
 
+/* ---------------------------------------------- independent runs (ensemble)
+ * Many independent simulations stepped together: the caller concatenates
+ * their particles (run r = user indices [offsets[r], offsets[r+1])) and
+ * binds the run table; every pass then covers all runs in one launch.  The
+ * reference steps one simulation per call (crkSPH.py:19-366); no pair ever
+ * joins two runs, and while runs are bound:
+ *  - crk_adjacency_count/fill take grid[n_runs] (each from crk_grid_plan with
+ *    the run's own h_max and extent, see crk_run_extents); cell_start/end hold
+ *    the sum of the table sizes, run r's cells after those of runs 0..r-1.
+ *    Each run keeps its index range in sorted order.
+ *  - reductions are per run: hminmax_dev holds 2 n_runs floats (run r at
+ *    [2r, 2r+1]), crk_minmax writes 2 n_runs, and the Verlet tests
+ *    (crk_adjacency_valid, crk_stage_begin_checked, crk_owen_valid) pass when
+ *    every run's list holds (all lists are rebuilt together).  The Owen
+ *    max_ratio is the maximum over the runs.
+ *  - with dt_particles (device [n], each particle's step, see crk_runs_dt and
+ *    crk_cfl_dt_runs) the scalar dt of crk_switch, crk_switch_accel_dudt,
+ *    crk_balance, crk_compatible_energy and crk_edges_export is ignored.
+ * Calls with a different particle count fail while runs are bound.
+ * n_runs = 0 unbinds (one simulation).  Blocking when the table changes;
+ * binding the same offsets again only updates dt_particles.
+ * No counterpart in the reference: it serves the batched ensemble.          */
+int crk_bind_runs(CrkContext* ctx, int32_t n_runs, const int32_t* offsets_host,
+                  const float* dt_particles, void* stream);
+/* Run id of every particle (device int32[n]) of the bound runs. */
+int crk_run_ids(CrkContext* ctx, int32_t n, int32_t* run_out, void* stream);
+/* Per-run extent for grid planning (blocking): out_host[7r..7r+6] = h_max,
+ * lo[3], hi[3] of run r (lo/hi over the dimensions in use, 0 elsewhere).     */
+int crk_run_extents(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* x,
+                    const float* h, float* out_host, void* stream);
+/* dt_particles[i] = (float) dt_runs[run of i] (dt_runs on the host). */
+int crk_runs_dt(CrkContext* ctx, int32_t n, const double* dt_runs, float* dt_particles,
+                void* stream);
+/* crk_cfl_dt of every bound run (blocking): dt_out[r] with the clamps and
+ * the growth limit of prev_dt[r] (prev_dt may be NULL); dt_particles, when
+ * not NULL, receives each particle's (float) dt.                           */
+int crk_cfl_dt_runs(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* supports,
+                    const float* internalEnergies, const double* prev_dt, double* dt_out,
+                    float* dt_particles, void* stream);
+
 /* ------------------------------------------------------ host utilities */
 /* Kernel normalisation C_d and kernel scale H/h (Dehnen & Aly 2012). */
 int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kernel_scale);
@@ -252,7 +292,8 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
 /* crk_stage_begin fused with the Verlet validity test of the adjacency for the
  * current state (the test of crk_adjacency_valid, with xb, hb the positions
  * and supports the adjacency was built at) and with the support range
- * (hminmax_dev[0] = min h, [1] = max h, for crk_owen_iteration).  Blocking:
+ * (hminmax_dev[0] = min h, [1] = max h, for crk_owen_iteration; per run
+ * when runs are bound).  Blocking:
  * *valid = 1 when the adjacency holds every pair within the Verlet radius.
  * When *valid = 0 the caller rebuilds and calls this again.  After it,
  * crk_owen_iteration also records max h_new / h_b for crk_owen_valid.     */
@@ -282,7 +323,8 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
  * as crk_owen_iteration + crk_support.                                      */
 int crk_owen_support(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                      const float* hminmax_dev, float* max_ratio_dev, void* stream);
-/* hminmax_dev[0] = min supports, [1] = max supports (user array). */
+/* hminmax_dev[0] = min supports, [1] = max supports (user array); per run
+ * ([2r], [2r+1]) when runs are bound. */
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_dev, void* stream);
 /* Support sweep of the bound adjacency for the current positions and supports
  * (after the Owen update, before crk_factors_eos).  One pass over every edge:
@@ -364,6 +406,11 @@ int crk_axpy(int64_t count, const float* y0, const float* d, float dt, float* y,
 /* y = y0 + dt * (b0*d0 + b1*d1) */
 int crk_rk_combine(int64_t count, const float* y0, const float* d0, const float* d1,
                    float b0, float b1, float dt, float* y, void* stream);
+/* The same with each particle's step dtp[n] (n particles of comps values). */
+int crk_axpy_p(int64_t n, int32_t comps, const float* y0, const float* d, const float* dtp,
+               float* y, void* stream);
+int crk_rk_combine_p(int64_t n, int32_t comps, const float* y0, const float* d0, const float* d1,
+                     float b0, float b1, const float* dtp, float* y, void* stream);
 /* Wrap positions into the periodic domain. */
 int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream);
 /* Compatible-energy finalize (compSPH/multistep.py:52-94, systems/compSPH.py:87-129):

@@ -1,0 +1,157 @@
+"""Batched ensembles on the GPU: each run of an ensemble must evolve as the
+same run stepped alone.
+
+Integer work (sort permutation, Morton keys, cell tables, neighbour rows) of
+every run must match its standalone build bit for bit (shifted by the run's
+offsets); per-run CFL dt must be identical; fields after RK2 steps within
+the per-step FP32 bar RTOL against the standalone run and against the CPU
+oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from tests.util import RTOL, cpu_copy, relerr
+
+pytestmark = pytest.mark.gpu
+
+SIDES = (24, 32, 40)  # 576, 1024, 1600 particles
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def make_ensemble(sides=SIDES, seed=7):
+    from warpsph_amd.ensemble import concatenate, two_state_run
+    parts = [two_state_run(int(nx), seed * 100003 + r, device="cuda") for r, nx in enumerate(sides)]
+    ens = concatenate([p[0] for p in parts])
+    return ens, parts, parts[0][1], parts[0][2]
+
+
+@pytest.fixture(autouse=True)
+def unbind():
+    yield
+    from warpsph_amd.runtime import runtime
+    runtime("cuda:0").bind_runs(None)
+
+
+def test_ensemble_adjacency_matches_single_runs():
+    from warpsph_amd.configurations import toCrkConfig
+    from warpsph_amd.runtime import build_adjacency, runtime
+    announce("ensemble adjacency")
+    ens, parts, config, scheme = make_ensemble()
+    cfg = toCrkConfig(config, scheme)
+    runs = ens.runs
+    rt = runtime("cuda:0")
+    rt.bind_runs(runs)
+    ea = build_adjacency(cfg, ens.state.positions, ens.state.supports, runs)
+    rt.bind_runs(None)
+    e_perm = ea.perm.cpu().numpy()
+    e_keys = ea.keys.cpu().numpy().view(np.uint32)
+    e_off = ea.row_offsets.cpu().numpy().astype(np.int64)
+    e_nbr = ea.nbr.cpu().numpy()
+    e_cs, e_ce = ea.cell_start.cpu().numpy(), ea.cell_end.cpu().numpy()
+    base = 0
+    for r, (sysr, _, _) in enumerate(parts):
+        b, e = int(runs.offsets[r]), int(runs.offsets[r + 1])
+        sa = build_adjacency(cfg, sysr.state.positions, sysr.state.supports)
+        g = ea.grid[r]
+        assert (g.bits, list(g.ncell)) == (sa.grid.bits, list(sa.grid.ncell))
+        T = int(sa.grid.table_size)
+        assert np.array_equal(e_perm[b:e] - b, sa.perm.cpu().numpy())
+        assert np.array_equal(e_keys[b:e] - base, sa.keys.cpu().numpy().view(np.uint32))
+        s_cs, s_ce = sa.cell_start.cpu().numpy(), sa.cell_end.cpu().numpy()
+        full = s_ce > 0
+        assert np.array_equal(e_cs[base:base + T][full], s_cs[full] + b)
+        assert np.array_equal(e_ce[base:base + T][full], s_ce[full] + b)
+        assert not np.any(e_ce[base:base + T][~full])
+        s_off = sa.row_offsets.cpu().numpy().astype(np.int64)
+        assert np.array_equal(e_off[b:e + 1] - e_off[b], s_off)
+        rows = e_nbr[e_off[b]:e_off[e]]
+        assert rows.size == 0 or (rows.min() >= b and rows.max() < e), "a row holds another run's particle"
+        assert np.array_equal(rows - b, sa.nbr.cpu().numpy())
+        base += T
+
+
+def _fields(st):
+    return {"x": st.positions, "v": st.velocities, "u": st.internalEnergies, "rho": st.densities,
+            "h": st.supports, "alpha": st.alphas}
+
+
+def test_ensemble_steps_match_single_runs():
+    from warpsph_amd import sample
+    from warpsph_amd.ensemble import cfl_dt, run_system
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    announce("ensemble steps vs single runs")
+    ens, parts, config, scheme = make_ensemble()
+    integ = getIntegrator()
+    singles = [run_system(ens, r) for r in range(len(parts))]
+    nsteps = 3
+    e = ens
+    for _ in range(nsteps):
+        dts = cfl_dt(e, config, scheme)
+        for r in range(len(singles)):
+            d1 = sample.cfl_dt(singles[r].state, config, scheme)
+            assert d1 == dts[r], (r, d1, dts[r])
+            singles[r] = integ.function(singles[r], crkSPH_step, d1, config, scheme).state
+        e = integ.function(e, crkSPH_step, dts, config, scheme).state
+    torch.cuda.synchronize()
+    assert np.allclose(e.t, [s.t for s in singles], rtol=0, atol=0)
+    errs = {}
+    for r, s in enumerate(singles):
+        b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
+        fe, fs = _fields(e.state), _fields(s.state)
+        for k in fe:
+            errs[(r, k)] = relerr(fe[k][b:q].cpu().numpy(), fs[k].cpu().numpy())
+    print(errs)
+    assert all(v <= RTOL for v in errs.values()), errs
+
+
+def test_ensemble_matches_oracle():
+    from oracle import oracle as O
+    from warpsph_amd.ensemble import cfl_dt, run_system
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    announce("ensemble vs oracle")
+    ens, parts, config, scheme = make_ensemble(sides=(16, 20), seed=11)
+    cpus = [cpu_copy(run_system(ens, r), config, scheme) for r in range(2)]
+    integ = getIntegrator()
+    nsteps = 2
+    dts = cfl_dt(ens, config, scheme) * (2.0 / 3.0)
+    e = ens
+    for _ in range(nsteps):
+        e = integ.function(e, crkSPH_step, dts, config, scheme).state
+    torch.cuda.synchronize()
+    for r in range(2):
+        sim = O.from_system(cpus[r], config, scheme, 32)
+        sim.step(float(dts[r]), nsteps)
+        b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
+        st = e.state
+        errs = {"v": relerr(st.velocities[b:q].cpu().numpy(), sim.get("v")),
+                "u": relerr(st.internalEnergies[b:q].cpu().numpy(), sim.get("u")),
+                "rho": relerr(st.densities[b:q].cpu().numpy(), sim.get("rho")),
+                "h": relerr(st.supports[b:q].cpu().numpy(), sim.get("h"))}
+        print(r, errs)
+        assert all(v <= nsteps * RTOL for v in errs.values()), (r, errs)
+
+
+def test_unbound_calls_after_an_ensemble_step():
+    """After an ensemble step the library is back to one simulation."""
+    from warpsph_amd import sample
+    from warpsph_amd.ensemble import cfl_dt
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    announce("ensemble then single")
+    ens, parts, config, scheme = make_ensemble(sides=(16, 20), seed=3)
+    integ = getIntegrator()
+    integ.function(ens, crkSPH_step, cfl_dt(ens, config, scheme), config, scheme)
+    s1 = parts[0][0]
+    dt = sample.cfl_dt(s1.state, config, scheme)
+    out = integ.function(s1, crkSPH_step, dt, config, scheme).state
+    torch.cuda.synchronize()
+    assert torch.isfinite(out.state.densities).all()

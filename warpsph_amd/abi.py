@@ -148,6 +148,13 @@ _PROTOS = {
     "crk_cfl_dt": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, C.c_double,
                    C.POINTER(C.c_double), _P],
     "crk_conservation": [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P],
+    "crk_bind_runs": [_P, C.c_int32, _P, _P, _P],
+    "crk_run_ids": [_P, C.c_int32, _P, _P],
+    "crk_run_extents": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P],
+    "crk_runs_dt": [_P, C.c_int32, _P, _P, _P],
+    "crk_cfl_dt_runs": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P, _P, _P],
+    "crk_axpy_p": [C.c_int64, C.c_int32, _P, _P, _P, _P, _P],
+    "crk_rk_combine_p": [C.c_int64, C.c_int32, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P],
 }
 # every symbol include/crksph.h declares (checked by tests/test_abi.py)
 EXPORTED = sorted(list(_PROTOS.keys()) + ["crk_version"])

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/crksph.h"
 
@@ -82,8 +83,16 @@ struct CrkContext {
   // Verlet validity of the bound stage (crk_stage_begin_checked / crk_owen_valid)
   crk::Buf s_chk;
   const float* check_hb = nullptr;
-  float check_d2 = 0.f, check_hbmin = 0.f;
+  std::vector<float> check_d2, check_hbmin;  // per run
   bool owen_checked = false;
+  // independent runs (crk_bind_runs): run r is particles [run_off[r], run_off[r+1])
+  // in user order and, for adjacencies built while bound, in sorted order too
+  int32_t n_runs = 0;  // 0: one run, no run table
+  int32_t n_chunks = 0;
+  std::vector<int32_t> run_off;
+  crk::Buf runs_dev;     // chunks (int2 [n_chunks]) | offsets [n_runs + 1] | run id [n]
+  crk::Buf s_gridruns;   // per-run grids of the last crk_adjacency_count
+  const float* dtp = nullptr;  // per-particle dt of the bound runs (may be null)
 };
 
 namespace crk {
@@ -281,10 +290,58 @@ inline unsigned reduce_blocks(int64_t n) {
   return b < 512u ? (b == 0 ? 1u : b) : 512u;
 }
 
+// ------------------------------------------------------------ independent runs
+// Reductions over bound runs are segmented: block b reduces the chunk
+// {first particle, run} = chunk[b], at most kChunk particles of one run, and
+// issues one atomic per result into the run's slot.  Without runs the same
+// kernels reduce the whole set grid-stride into slot 0.
+constexpr int32_t kChunk = 2048;
+constexpr int32_t kMaxRuns = 1 << 16;
+struct RunsDev {
+  const int32_t* run = nullptr;  // [n] run of each particle, null: one run
+  const int32_t* off = nullptr;  // [n_runs + 1]
+  const int2* chunk = nullptr;   // [n_chunks]
+};
+struct Span {
+  int32_t k, e, step, r;
+};
+__device__ inline Span block_span(const RunsDev& R, int32_t n) {
+  if (R.chunk) {
+    const int2 c = R.chunk[blockIdx.x];
+    const int32_t e = min(c.x + kChunk, R.off[c.y + 1]);
+    return Span{c.x + (int32_t)threadIdx.x, e, (int32_t)blockDim.x, c.y};
+  }
+  return Span{(int32_t)(blockIdx.x * blockDim.x + threadIdx.x), n, (int32_t)(gridDim.x * blockDim.x), 0};
+}
+__device__ inline int32_t run_index(const int32_t* run, int32_t k) { return run ? run[k] : 0; }
+inline RunsDev runs_of(const CrkContext* c) {
+  RunsDev R;
+  if (c->n_runs > 0) {
+    R.chunk = (const int2*)c->runs_dev.p;
+    R.off = (const int32_t*)(R.chunk + c->n_chunks);
+    R.run = R.off + (c->n_runs + 1);
+  }
+  return R;
+}
+inline const int32_t* run_ids(const CrkContext* c) { return c->n_runs > 0 ? runs_of(c).run : nullptr; }
+// grid of a segmented reduction (plain: the grid-stride grid)
+inline unsigned span_blocks(const CrkContext* c, unsigned plain) { return c->n_runs > 0 ? (unsigned)c->n_chunks : plain; }
+inline int32_t run_count(const CrkContext* c) { return c->n_runs > 0 ? c->n_runs : 1; }
+// the step of particle k: per particle when the bound runs carry one
+struct DtArg {
+  float dt;
+  const float* p;
+  __device__ inline float at(int32_t k) const { return p ? p[k] : dt; }
+};
+inline DtArg dt_arg(const CrkContext* c, float dt) { return DtArg{dt, c->n_runs > 0 ? c->dtp : nullptr}; }
+
 // ------------------------------------------------------------ small host helpers
 int scan_exclusive_i32(CrkContext* ctx, const int32_t* in, int32_t* out, int32_t n, int64_t* total_dev_or_null,
                        hipStream_t s);
 int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 int fill_f32(float* p, int64_t n, float v, hipStream_t s);
+int ensure_pinned(CrkContext* ctx, size_t bytes);
+// every run's particle count matches the bound table (n particles in all)
+int require_runs_n(const CrkContext* ctx, int32_t n, const char* who);
 
 }  // namespace crk

@@ -1,5 +1,6 @@
 // core.hip -- context, error reporting, scans and host utilities of the
 // gfx950 CRKSPH library.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -62,6 +63,33 @@ int fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
   CRK_LAUNCH_CHECK("k_fill_f32");
   return CRK_OK;
+}
+
+// host pinned staging for blocking reads (and uploads of small tables);
+// every user synchronises the stream before it returns, so it is idle here
+int ensure_pinned(CrkContext* ctx, size_t bytes) {
+  if (bytes <= ctx->pinned.bytes) return CRK_OK;
+  void* p = nullptr;
+  CRK_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  if (ctx->pinned.p) CRK_HIP(hipHostFree(ctx->pinned.p));
+  ctx->pinned.p = p;
+  ctx->pinned.bytes = bytes;
+  return CRK_OK;
+}
+
+int require_runs_n(const CrkContext* ctx, int32_t n, const char* who) {
+  if (ctx->n_runs > 0 && ctx->run_off.back() != n) {
+    set_error("%s: %d particles, but the bound runs hold %d", who, n, ctx->run_off.back());
+    return CRK_EINVAL;
+  }
+  return CRK_OK;
+}
+
+// run id of every particle, one block per chunk
+__global__ void k_run_ids(const int2* __restrict__ chunk, const int32_t* __restrict__ off, int32_t* __restrict__ run) {
+  const int2 c = chunk[blockIdx.x];
+  const int32_t e = min(c.x + kChunk, off[c.y + 1]);
+  for (int32_t k = c.x + (int32_t)threadIdx.x; k < e; k += blockDim.x) run[k] = c.y;
 }
 
 // ------------------------------------------------------------ exclusive scan
@@ -206,6 +234,59 @@ int crk_ctx_create(CrkContext** out) {
   return CRK_OK;
 }
 
+int crk_bind_runs(CrkContext* ctx, int32_t n_runs, const int32_t* offsets, const float* dt_particles,
+                  void* stream) {
+  CRK_REQUIRE(ctx, "crk_bind_runs: null context");
+  CRK_REQUIRE(n_runs >= 0 && n_runs <= kMaxRuns, "crk_bind_runs: n_runs must be in [0, %d] (got %d)", kMaxRuns,
+              n_runs);
+  if (n_runs == 0) {
+    ctx->n_runs = 0;
+    ctx->dtp = nullptr;
+    return CRK_OK;
+  }
+  CRK_REQUIRE(offsets && offsets[0] == 0, "crk_bind_runs: offsets must start at 0");
+  for (int32_t r = 0; r < n_runs; ++r)
+    CRK_REQUIRE(offsets[r + 1] > offsets[r], "crk_bind_runs: run %d is empty or offsets decrease", r);
+  const bool same = ctx->n_runs == n_runs && (int32_t)ctx->run_off.size() == n_runs + 1 &&
+                    std::equal(ctx->run_off.begin(), ctx->run_off.end(), offsets);
+  ctx->dtp = dt_particles;
+  if (same) return CRK_OK;
+  const int32_t n = offsets[n_runs];
+  std::vector<int32_t> table;  // chunks (pairs) | offsets
+  for (int32_t r = 0; r < n_runs; ++r)
+    for (int32_t k = offsets[r]; k < offsets[r + 1]; k += kChunk) {
+      table.push_back(k);
+      table.push_back(r);
+    }
+  const int32_t n_chunks = (int32_t)(table.size() / 2);
+  table.insert(table.end(), offsets, offsets + n_runs + 1);
+  int st;
+  if ((st = ensure(ctx->runs_dev, sizeof(int32_t) * (table.size() + (size_t)n)))) return st;
+  if ((st = ensure_pinned(ctx, std::max<size_t>(sizeof(int32_t) * table.size(), 16 * sizeof(float) * n_runs))))
+    return st;
+  hipStream_t s = (hipStream_t)stream;
+  std::memcpy(ctx->pinned.p, table.data(), sizeof(int32_t) * table.size());
+  CRK_HIP(hipMemcpyAsync(ctx->runs_dev.p, ctx->pinned.p, sizeof(int32_t) * table.size(), hipMemcpyHostToDevice, s));
+  ctx->n_runs = n_runs;
+  ctx->n_chunks = n_chunks;
+  ctx->run_off.assign(offsets, offsets + n_runs + 1);
+  RunsDev R = runs_of(ctx);
+  hipLaunchKernelGGL(k_run_ids, dim3(n_chunks), dim3(kBlock), 0, s, R.chunk, R.off, (int32_t*)R.run);
+  CRK_LAUNCH_CHECK("k_run_ids");
+  CRK_HIP(hipStreamSynchronize(s));
+  return CRK_OK;
+}
+
+int crk_run_ids(CrkContext* ctx, int32_t n, int32_t* run_out, void* stream) {
+  CRK_REQUIRE(ctx && run_out, "crk_run_ids: null argument");
+  CRK_REQUIRE(ctx->n_runs > 0, "crk_run_ids: no runs bound");
+  int st = require_runs_n(ctx, n, "crk_run_ids");
+  if (st) return st;
+  CRK_HIP(hipMemcpyAsync(run_out, run_ids(ctx), sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                         (hipStream_t)stream));
+  return CRK_OK;
+}
+
 int crk_ctx_destroy(CrkContext* c) {
   if (!c) return CRK_OK;
   (void)hipDeviceSynchronize();
@@ -213,7 +294,7 @@ int crk_ctx_destroy(CrkContext* c) {
                  &c->rho, &c->P, &c->c, &c->sP, &c->vhs, &c->J, &c->divcd, &c->vsig, &c->Rcd, &c->dvdt, &c->vh, &c->du0,
                  &c->rec, &c->geo4, &c->vel4,
                  &c->lut_nh, &c->lut_psiH, &c->s_keys0, &c->s_keys1, &c->s_vals0, &c->s_vals1, &c->s_hist,
-                 &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk};
+                 &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk, &c->runs_dev, &c->s_gridruns};
   for (Buf* b : bufs) release(*b);
   if (c->pinned.p) (void)hipHostFree(c->pinned.p);
   delete c;

@@ -91,7 +91,8 @@ struct CdOut {
   float *a0, *alpha, *Rcd, *alphas_u, *alpha0s_u;
 };
 struct CdParams {
-  float dt, fkern, amin, amax, beta;
+  DtArg dt;
+  float fkern, amin, amax, beta;
   int limitXi, evolve;
 };
 
@@ -106,7 +107,8 @@ __device__ inline void cullen_row(const CdIn& in, const CdParams& p, int32_t k, 
   float nom = p.beta * (t2 * t2) * dk_;
   float Xi = p.limitXi ? (nom * nom + 1e-14f * hi) / (nom * nom + 0.f + 1e-14f * hi)
                        : (nom * nom) / (nom * nom + 0.f + 1e-14f * hi);
-  float ddiv = (dk_ - in.divp[k]) / p.dt;
+  const float dt = p.dt.at(k);
+  float ddiv = (dk_ - in.divp[k]) / dt;
   float md = -ddiv;
   float Ai = Xi * (md > 0.f ? md : 0.f);
   float vsg = fabsf(in.vsig[k]);
@@ -117,7 +119,7 @@ __device__ inline void cullen_row(const CdIn& in, const CdParams& p, int32_t k, 
   float a0k = o.a0[k];
   float adot = (al - a0k) / tau;
   float na = al > a0k ? al : a0k;
-  na = al < na ? na + adot * p.dt : na;
+  na = al < na ? na + adot * dt : na;
   na = na < p.amin ? p.amin : (na > p.amax ? p.amax : na);
   o.alpha[k] = na;
   o.alphas_u[i] = na;
@@ -387,9 +389,10 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
 // where it is consumed (the compatible-energy finalize, the export).
 template <int D>
 __global__ void k_balance_rec(int32_t n, const float* __restrict__ v, const float* __restrict__ dvdt,
-                              const float* __restrict__ sP, float hdt, float4* __restrict__ rec) {
+                              const float* __restrict__ sP, DtArg dt, float4* __restrict__ rec) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  const float hdt = 0.5f * dt.at(k);
   float vh[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int d = 0; d < D; ++d) vh[d] = v[(int64_t)k * D + d] + hdt * dvdt[(int64_t)k * D + d];
@@ -399,13 +402,13 @@ __global__ void k_balance_rec(int32_t n, const float* __restrict__ v, const floa
 // f_ij over the pair slots (optional output of crk_balance)
 template <int D, int G>
 __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float4* __restrict__ rec,
-                                                    const float* __restrict__ m, float dt,
+                                                    const float* __restrict__ m, DtArg dt,
                                                     const float* __restrict__ ap, const float* __restrict__ av,
                                                     float* __restrict__ f) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   const float4 ri = rec[k];
-  const float mdt = m[k] * dt;
+  const float mdt = m[k] * dt.at(k);
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
   for (int32_t q = b + L.g; q < e; q += G) {
@@ -422,9 +425,9 @@ __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const
 
 using namespace crk;
 
-static CdParams cd_params(const CrkConfig* cfg, float dt) {
+static CdParams cd_params(const CrkContext* ctx, const CrkConfig* cfg, float dt) {
   const CrkSwitchParams& sp = cfg->switchParams;
-  return CdParams{dt, (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim)), sp.alpha_min, sp.alpha_max,
+  return CdParams{dt_arg(ctx, dt), (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim)), sp.alpha_min, sp.alpha_max,
                   sp.beta_xi, sp.limitXi, cfg->evolveAlpha0};
 }
 static CdIn cd_in(CrkContext* ctx) {
@@ -471,7 +474,7 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
   if (r) return r;
   CRK_REQUIRE(st && st->alphas, "crk_switch: state->alphas missing");
   CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch: state->alpha0s missing");
-  CRK_REQUIRE(dt > 0.f, "crk_switch: dt must be positive");
+  CRK_REQUIRE(dt > 0.f || dt_arg(ctx, dt).p, "crk_switch: dt must be positive");
   if (!ctx->have_velgrad) { set_error("crk_switch before crk_velocity_gradient"); return CRK_ESTATE; }
   if (cfg->switchParams.scheme != CRK_SWITCH_CULLEN_DEHNEN) {
     set_error("crk_switch: switch scheme %d is not the Cullen-Dehnen switch", cfg->switchParams.scheme);
@@ -484,7 +487,7 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
   int GS = ctx->group[CRK_PASS_SWITCH];
   CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
       hipLaunchKernelGGL((k_cullen<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo, C,
-                         n, cd_in(ctx), cd_params(cfg, dt), ctx->adj.perm, cd_out(ctx, st)));
+                         n, cd_in(ctx), cd_params(ctx, cfg, dt), ctx->adj.perm, cd_out(ctx, st)));
   CRK_LAUNCH_CHECK("k_cullen");
   return CRK_OK;
 }
@@ -521,7 +524,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
                            arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
-                           cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, cd_in(ctx), cd_params(cfg, dt),
+                           cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, cd_in(ctx), cd_params(ctx, cfg, dt),
                            cd_out(ctx, st), F<float4>(ctx->rec)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
@@ -550,7 +553,7 @@ int crk_switch_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
                                     "then crk_accel_dudt");
   CRK_REQUIRE(st && st->alphas, "crk_switch_accel_dudt: state->alphas missing");
   CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch_accel_dudt: state->alpha0s missing");
-  CRK_REQUIRE(dt > 0.f, "crk_switch_accel_dudt: dt must be positive");
+  CRK_REQUIRE(dt > 0.f || dt_arg(ctx, dt).p, "crk_switch_accel_dudt: dt must be positive");
   return accel_dudt(ctx, cfg, st, up, ed, true, dt, stream);
 }
 
@@ -564,17 +567,17 @@ int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, v
   if (!ctx->have_accel) { set_error("crk_balance before crk_accel_dudt"); return CRK_ESTATE; }
   hipStream_t s = (hipStream_t)stream;
   int32_t n = ctx->n;
-  float hdt = (float)(0.5 * (double)dt);
+  const DtArg dta = dt_arg(ctx, dt);
   float4* rec = (float4*)ed->rec;
   CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_balance_rec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, n,
-                                                F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->sP), hdt,
+                                                F<float>(ctx->v), F<float>(ctx->dvdt), F<float>(ctx->sP), dta,
                                                 rec));
   CRK_LAUNCH_CHECK("k_balance_rec");
   if (ed->f && ctx->adj.n_edges > 0) {
     int GS = ctx->group[CRK_PASS_BALANCE];
     CRK_DIM_G_DISPATCH(ctx->dim, GS,
         hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n, rec,
-                           F<float>(ctx->m), dt, ed->ap, ed->av, ed->f));
+                           F<float>(ctx->m), dta, ed->ap, ed->av, ed->f));
     CRK_LAUNCH_CHECK("k_balance");
   }
   return CRK_OK;

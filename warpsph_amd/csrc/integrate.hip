@@ -1,6 +1,7 @@
 // integrate.hip -- RK2 helpers, the compatible-energy finalize and the
 // reductions of the CRKSPH step (CFL dt, FP64 conservation sums, min/max).
 #include <cmath>
+#include <cstring>
 
 #include "pairs.hpp"
 
@@ -16,6 +17,19 @@ __global__ void k_combine(int64_t cnt, const float* __restrict__ y0, const float
                           const float* __restrict__ d1, float b0, float b1, float dt, float* __restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < cnt) y[i] = y0[i] + dt * (b0 * d0[i] + b1 * d1[i]);
+}
+
+// the same with a step per particle: element e belongs to particle e / comps
+__global__ void k_axpy_p(int64_t cnt, int32_t comps, const float* __restrict__ y0, const float* __restrict__ d,
+                         const float* __restrict__ dtp, float* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) y[i] = y0[i] + dtp[i / comps] * d[i];
+}
+__global__ void k_combine_p(int64_t cnt, int32_t comps, const float* __restrict__ y0, const float* __restrict__ d0,
+                            const float* __restrict__ d1, float b0, float b1, const float* __restrict__ dtp,
+                            float* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) y[i] = y0[i] + dtp[i / comps] * (b0 * d0[i] + b1 * d1[i]);
 }
 
 __global__ void k_wrap(int32_t n, int D, Geo g, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
@@ -36,10 +50,11 @@ __global__ void k_wrap(int32_t n, int D, Geo g, float lo0, float lo1, float lo2,
 template <int D>
 __global__ void k_vhalf_sorted(const int32_t* __restrict__ perm, int32_t n, const float* __restrict__ v0,
                                const float* __restrict__ a0, const float* __restrict__ a1, float b0, float b1,
-                               float dt, float* __restrict__ vhs) {
+                               DtArg dta, float* __restrict__ vhs) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int64_t i = perm[k];
+  const float dt = dta.at(k);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     float fa = 0.f + b0 * a0[i * D + d];
@@ -105,7 +120,7 @@ template <int D, int G, int NST, int MODE>
 __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, int32_t n,
                                                    const float* __restrict__ vhs, CompIn s0, CompIn s1,
                                                    const float* __restrict__ m_u, float* __restrict__ du0,
-                                                   const float* __restrict__ u0, float dt, float* __restrict__ u,
+                                                   const float* __restrict__ u0, DtArg dta, float* __restrict__ u,
                                                    float* __restrict__ dudt_last) {
   const Lane L = lane_of<G>(n);  // XCD-aware row order
   const int g = L.g;
@@ -115,6 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
 #pragma unroll
   for (int d = 0; d < D; ++d) vi[d] = vhs[(int64_t)k * D + d];
   const int32_t i = perm[k];
+  const float dt = dta.at(k);
   const float mdt = m_u[i] * dt;
   float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
   compat_sum<D, G>(s0, k, valid, g, vhs, vi, mdt, sa, ca);
@@ -133,36 +149,103 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
 }
 
 // ------------------------------------------------------------ reductions
-__global__ void k_minmax_init(float* mm) {
-  mm[0] = 3.0e38f;
-  mm[1] = -3.0e38f;
+__global__ void k_minmax_init(float* mm, int32_t nr) {
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+    mm[2 * r] = 3.0e38f;
+    mm[2 * r + 1] = -3.0e38f;
+  }
 }
-// min/max of positive values via bit-pattern atomics (grid-stride, one
-// atomic per block)
-__global__ void k_minmax(const float* __restrict__ a, int32_t n, float* mm) {
+// min/max of positive values via bit-pattern atomics (segmented grid-stride,
+// one atomic per block): mm[2r], mm[2r+1] of run r
+__global__ void k_minmax(const float* __restrict__ a, int32_t n, float* mm, RunsDev R) {
   float lo = 3.0e38f, hi = 0.f;
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const Span sp = block_span(R, n);
+  for (int32_t i = sp.k; i < sp.e; i += sp.step) {
     lo = fminf(lo, a[i]);
     hi = fmaxf(hi, a[i]);
   }
   lo = block_reduce(lo, MinOp());
   hi = block_reduce(hi, MaxOp());
   if (threadIdx.x == 0) {
-    atomic_min_pos(&mm[0], lo);
-    atomicMax((int*)&mm[1], __float_as_int(hi));
+    atomic_min_pos(&mm[2 * sp.r], lo);
+    atomicMax((int*)&mm[2 * sp.r + 1], __float_as_int(hi));
   }
 }
 
 // dt_i = cfl * hmin / (c_i xi), c_i = sqrt(|u| g (g-1))  (timestep/compressible.py:12-66)
+// (per run: mm[2r] = min h of run r, out[r])
 __global__ void k_cfl(const float* __restrict__ u, int32_t n, float gamma, float cfl, float xi,
-                      const float* __restrict__ mm, float* out) {
+                      const float* __restrict__ mm, float* out, RunsDev R) {
   float d = 3.0e38f;
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const Span sp = block_span(R, n);
+  const float hmin = mm[2 * sp.r];
+  for (int32_t i = sp.k; i < sp.e; i += sp.step) {
     float cs = sqrtf(fabsf(u[i]) * gamma * (gamma - 1.f));
-    d = fminf(d, cfl * mm[0] / (cs * xi));
+    d = fminf(d, cfl * hmin / (cs * xi));
   }
   d = block_reduce(d, MinOp());
-  if (threadIdx.x == 0) atomic_min_pos(out, d);
+  if (threadIdx.x == 0) atomic_min_pos(&out[sp.r], d);
+}
+// dtp[k] = dt of the run of particle k
+__global__ void k_dt_broadcast(const int32_t* __restrict__ run, int32_t n, const float* __restrict__ dtr,
+                               float* __restrict__ dtp) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dtp[k] = dtr[run[k]];
+}
+
+// Per-run extent: o[7r] = max h (bits of a non-negative float), o[7r+1+d]
+// = min x_d and o[7r+4+d] = max x_d as order-preserving keys of signed floats.
+__device__ inline uint32_t fkey(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ inline float fkey_inv(uint32_t k) {
+  const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  float f;
+  std::memcpy(&f, &b, sizeof f);
+  return f;
+}
+__global__ void k_extent_init(uint32_t* o, int32_t nr) {
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+    o[7 * r] = 0u;
+    for (int d = 0; d < 3; ++d) {
+      o[7 * r + 1 + d] = 0xffffffffu;
+      o[7 * r + 4 + d] = 0u;
+    }
+  }
+}
+template <int D>
+__global__ void k_extent(const float* __restrict__ x, const float* __restrict__ h, int32_t n, uint32_t* o,
+                         RunsDev R) {
+  float hm = 0.f, lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = 3.0e38f;
+    hi[d] = -3.0e38f;
+  }
+  const Span sp = block_span(R, n);
+  for (int32_t i = sp.k; i < sp.e; i += sp.step) {
+    hm = fmaxf(hm, h[i]);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = fminf(lo[d], x[(int64_t)i * D + d]);
+      hi[d] = fmaxf(hi[d], x[(int64_t)i * D + d]);
+    }
+  }
+  hm = block_reduce(hm, MaxOp());
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = block_reduce(lo[d], MinOp());
+    hi[d] = block_reduce(hi[d], MaxOp());
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(&o[7 * sp.r], __float_as_uint(hm));
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      atomicMin(&o[7 * sp.r + 1 + d], fkey(lo[d]));
+      atomicMax(&o[7 * sp.r + 4 + d], fkey(hi[d]));
+    }
+  }
 }
 
 // FP64 conservation partials: per block [m, m v (D), m u, m |v|^2 / 2]
@@ -251,6 +334,27 @@ int crk_rk_combine(int64_t count, const float* y0, const float* d0, const float*
   return CRK_OK;
 }
 
+int crk_axpy_p(int64_t n, int32_t comps, const float* y0, const float* d, const float* dtp, float* y,
+               void* stream) {
+  CRK_REQUIRE(n >= 0 && comps >= 1 && (n == 0 || (y0 && d && dtp && y)), "crk_axpy_p: invalid arguments");
+  if (n == 0) return CRK_OK;
+  const int64_t cnt = n * comps;
+  hipLaunchKernelGGL(k_axpy_p, dim3(nblocks(cnt)), dim3(kBlock), 0, (hipStream_t)stream, cnt, comps, y0, d, dtp, y);
+  CRK_LAUNCH_CHECK("k_axpy_p");
+  return CRK_OK;
+}
+
+int crk_rk_combine_p(int64_t n, int32_t comps, const float* y0, const float* d0, const float* d1, float b0, float b1,
+                     const float* dtp, float* y, void* stream) {
+  CRK_REQUIRE(n >= 0 && comps >= 1 && (n == 0 || (y0 && d0 && d1 && dtp && y)), "crk_rk_combine_p: invalid arguments");
+  if (n == 0) return CRK_OK;
+  const int64_t cnt = n * comps;
+  hipLaunchKernelGGL(k_combine_p, dim3(nblocks(cnt)), dim3(kBlock), 0, (hipStream_t)stream, cnt, comps, y0, d0, d1,
+                     b0, b1, dtp, y);
+  CRK_LAUNCH_CHECK("k_combine_p");
+  return CRK_OK;
+}
+
 int crk_wrap_positions(const CrkConfig* cfg, int32_t n, float* x, void* stream) {
   CRK_REQUIRE(cfg && x && n > 0, "crk_wrap_positions: invalid arguments");
   Geo g = make_geo(*cfg);
@@ -275,8 +379,11 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   CRK_REQUIRE(adj1->n_edges == 0 || (e1->nbr && e1->ap && e1->av), "stage-1 pairs missing");
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
+  const DtArg dta = dt_arg(ctx, dt);
+  CRK_REQUIRE(dt > 0.f || dta.p, "crk_compatible_energy: dt must be positive");
   hipStream_t s = (hipStream_t)stream;
   int st;
+  if ((st = require_runs_n(ctx, n, "crk_compatible_energy"))) return st;
   if ((st = ensure(ctx->vhs, sizeof(float) * (size_t)n * D))) return st;
   if ((st = ensure(ctx->du0, sizeof(float) * (size_t)n))) return st;
   float* vhs = (float*)ctx->vhs.p;
@@ -289,10 +396,10 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   unsigned nb = nblocks((int64_t)n * GS);
 #define VHALF(DD, A)                                                                                          \
   hipLaunchKernelGGL(k_vhalf_sorted<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, (A)->perm, n, v0, dvdt0, dvdt1, b0, \
-                     b1, dt, vhs)
+                     b1, dta, vhs)
 #define COMPAT(DD, GG, NST, MODE, A, X, Y)                                                                     \
   hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, n, vhs, X, Y, masses, \
-                     du0, u0, dt, u_out, dudt_last)
+                     du0, u0, dta, u_out, dudt_last)
 #define RUN(DD, GG)                     \
   if (shared) {                         \
     VHALF(DD, adj0);                    \
@@ -329,9 +436,12 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
 
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* mm, void* stream) {
   CRK_REQUIRE(ctx && values && mm && n > 0, "crk_minmax: invalid arguments");
+  int st = require_runs_n(ctx, n, "crk_minmax");
+  if (st) return st;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, mm);
-  hipLaunchKernelGGL(k_minmax, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, values, n, mm);
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(kBlock), 0, s, mm, run_count(ctx));
+  hipLaunchKernelGGL(k_minmax, dim3(span_blocks(ctx, reduce_blocks(n))), dim3(kBlock), 0, s, values, n, mm,
+                     runs_of(ctx));
   CRK_LAUNCH_CHECK("k_minmax");
   return CRK_OK;
 }
@@ -343,10 +453,13 @@ int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* su
   int st = ensure(ctx->s_red, 64);
   if (st) return st;
   float* red = (float*)ctx->s_red.p;  // [0,1] min/max h, [2] dt
-  if ((st = crk_minmax(ctx, n, supports, red, stream))) return st;
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(kBlock), 0, s, red, 1);
+  hipLaunchKernelGGL(k_minmax, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, supports, n, red, RunsDev{});
+  CRK_LAUNCH_CHECK("k_minmax");
   if ((st = fill_f32(red + 2, 1, 3.0e38f, s))) return st;
   float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
-  hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, red, red + 2);
+  hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, red,
+                     red + 2, RunsDev{});
   CRK_LAUNCH_CHECK("k_cfl");
   float* hostv = (float*)ctx->pinned.p;
   CRK_HIP(hipMemcpyAsync(hostv, red + 2, sizeof(float), hipMemcpyDeviceToHost, s));
@@ -355,6 +468,97 @@ int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* su
   dt = std::fmin(std::fmax(dt, (double)cfg->minDt), (double)cfg->maxDt);
   if (prev_dt > 0 && dt > prev_dt) dt = std::fmin(dt, (double)cfg->dtGrowthFactor * prev_dt);
   *dt_out = dt;
+  return CRK_OK;
+}
+
+int crk_cfl_dt_runs(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* supports, const float* u,
+                    const double* prev_dt, double* dt_out, float* dt_particles, void* stream) {
+  CRK_REQUIRE(ctx && cfg && supports && u && dt_out && n > 0, "crk_cfl_dt_runs: invalid arguments");
+  CRK_REQUIRE(ctx->n_runs > 0, "crk_cfl_dt_runs: no runs bound (crk_bind_runs)");
+  int st = require_runs_n(ctx, n, "crk_cfl_dt_runs");
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t R = ctx->n_runs;
+  if ((st = ensure(ctx->s_red, sizeof(float) * 3 * (size_t)R))) return st;
+  if ((st = ensure_pinned(ctx, sizeof(float) * (size_t)R))) return st;
+  float* mm = (float*)ctx->s_red.p;  // [2r, 2r+1] min/max h of run r; [2R + r] dt of run r
+  float* dtr = mm + 2 * R;
+  const RunsDev Rd = runs_of(ctx);
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(kBlock), 0, s, mm, R);
+  hipLaunchKernelGGL(k_minmax, dim3(ctx->n_chunks), dim3(kBlock), 0, s, supports, n, mm, Rd);
+  CRK_LAUNCH_CHECK("k_minmax");
+  if ((st = fill_f32(dtr, R, 3.0e38f, s))) return st;
+  float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
+  hipLaunchKernelGGL(k_cfl, dim3(ctx->n_chunks), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, mm, dtr,
+                     Rd);
+  CRK_LAUNCH_CHECK("k_cfl");
+  float* hostv = (float*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, dtr, sizeof(float) * (size_t)R, hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  // the clamps of crk_cfl_dt, run by run
+  for (int32_t r = 0; r < R; ++r) {
+    double dt = hostv[r];
+    dt = std::fmin(std::fmax(dt, (double)cfg->minDt), (double)cfg->maxDt);
+    if (prev_dt && prev_dt[r] > 0 && dt > prev_dt[r]) dt = std::fmin(dt, (double)cfg->dtGrowthFactor * prev_dt[r]);
+    dt_out[r] = dt;
+    hostv[r] = (float)dt;
+  }
+  if (dt_particles) {
+    CRK_HIP(hipMemcpyAsync(dtr, hostv, sizeof(float) * (size_t)R, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_dt_broadcast, dim3(nblocks(n)), dim3(kBlock), 0, s, Rd.run, n, dtr, dt_particles);
+    CRK_LAUNCH_CHECK("k_dt_broadcast");
+    CRK_HIP(hipStreamSynchronize(s));
+  }
+  return CRK_OK;
+}
+
+int crk_run_extents(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* x, const float* h,
+                    float* out, void* stream) {
+  CRK_REQUIRE(ctx && cfg && x && h && out && n > 0, "crk_run_extents: invalid arguments");
+  const int D = cfg->domain.dim;
+  CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
+  int st = require_runs_n(ctx, n, "crk_run_extents");
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t R = run_count(ctx);
+  if ((st = ensure(ctx->s_red, sizeof(uint32_t) * 7 * (size_t)R))) return st;
+  if ((st = ensure_pinned(ctx, sizeof(uint32_t) * 7 * (size_t)R))) return st;
+  uint32_t* o = (uint32_t*)ctx->s_red.p;
+  hipLaunchKernelGGL(k_extent_init, dim3(1), dim3(kBlock), 0, s, o, R);
+  const unsigned nb = span_blocks(ctx, reduce_blocks(n));
+  const RunsDev Rd = runs_of(ctx);
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_extent<DD>, dim3(nb), dim3(kBlock), 0, s, x, h, n, o, Rd));
+  CRK_LAUNCH_CHECK("k_extent");
+  uint32_t* hostv = (uint32_t*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, o, sizeof(uint32_t) * 7 * (size_t)R, hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  for (int32_t r = 0; r < R; ++r) {
+    std::memcpy(&out[7 * r], &hostv[7 * r], sizeof(float));
+    for (int d = 0; d < 3; ++d) {
+      out[7 * r + 1 + d] = d < D ? fkey_inv(hostv[7 * r + 1 + d]) : 0.f;
+      out[7 * r + 4 + d] = d < D ? fkey_inv(hostv[7 * r + 4 + d]) : 0.f;
+    }
+  }
+  return CRK_OK;
+}
+
+int crk_runs_dt(CrkContext* ctx, int32_t n, const double* dt_runs, float* dt_particles, void* stream) {
+  CRK_REQUIRE(ctx && dt_runs && dt_particles, "crk_runs_dt: null argument");
+  CRK_REQUIRE(ctx->n_runs > 0, "crk_runs_dt: no runs bound (crk_bind_runs)");
+  int st = require_runs_n(ctx, n, "crk_runs_dt");
+  if (st) return st;
+  const int32_t R = ctx->n_runs;
+  for (int32_t r = 0; r < R; ++r) CRK_REQUIRE(dt_runs[r] > 0, "crk_runs_dt: dt of run %d must be positive", r);
+  hipStream_t s = (hipStream_t)stream;
+  if ((st = ensure(ctx->s_red, sizeof(float) * 3 * (size_t)R))) return st;
+  if ((st = ensure_pinned(ctx, sizeof(float) * (size_t)R))) return st;
+  float* dtr = (float*)ctx->s_red.p + 2 * R;
+  float* hostv = (float*)ctx->pinned.p;
+  for (int32_t r = 0; r < R; ++r) hostv[r] = (float)dt_runs[r];
+  CRK_HIP(hipMemcpyAsync(dtr, hostv, sizeof(float) * (size_t)R, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_dt_broadcast, dim3(nblocks(n)), dim3(kBlock), 0, s, run_ids(ctx), n, dtr, dt_particles);
+  CRK_LAUNCH_CHECK("k_dt_broadcast");
+  CRK_HIP(hipStreamSynchronize(s));
   return CRK_OK;
 }
 

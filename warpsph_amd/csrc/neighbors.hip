@@ -55,6 +55,18 @@ static GridDev make_griddev(const CrkGrid& g, const CrkConfig& cfg) {
   return d;
 }
 
+// Independent runs in one adjacency (an ensemble): every run has its own
+// grid (its own support range), and run r's cells occupy the key range
+// [base, base + table size) after the cells of runs 0..r-1, so that the sort
+// groups particles by run and then by cell and no run sees another's cells.
+// The runs are contiguous in user order, so each keeps its index range in
+// sorted order and one run-id array serves both orders.
+struct GridRun {
+  GridDev g;
+  CellScan sc;
+  uint32_t base;
+};
+
 template <int D>
 __device__ inline void cell_of(const GridDev& g, const float* x, int* c) {
 #pragma unroll
@@ -64,16 +76,22 @@ __device__ inline void cell_of(const GridDev& g, const float* x, int* c) {
 }
 
 template <int D>
-__global__ void k_keys(const float* __restrict__ x, int32_t n, GridDev g, uint32_t* __restrict__ keys,
-                       int32_t* __restrict__ vals) {
+__global__ void k_keys(const float* __restrict__ x, int32_t n, GridDev g, const GridRun* __restrict__ runs,
+                       const int32_t* __restrict__ run_of, uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float xi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)i * D + d];
   int c[3];
-  cell_of<D>(g, xi, c);
-  keys[i] = morton<D>(c);
+  if (runs) {
+    const GridRun& gr = runs[run_of[i]];
+    cell_of<D>(gr.g, xi, c);
+    keys[i] = gr.base + morton<D>(c);
+  } else {
+    cell_of<D>(g, xi, c);
+    keys[i] = morton<D>(c);
+  }
   vals[i] = i;
 }
 
@@ -165,12 +183,13 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
 // consecutive candidates of a cell, and the hits are compacted with a group
 // ballot, so the row is written with coalesced stores in the same order as
 // the one-thread-per-row scan (cells in scan order, candidates ascending).
-template <int D, int G, bool FILL>
-__global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g, CellScan sc, Geo geo, float s,
+template <int D, int G, bool FILL, bool RUNS>
+__global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0, Geo geo, float s,
                                                         const float* __restrict__ xb, const float* __restrict__ hb,
                                                         const int32_t* __restrict__ cs, const int32_t* __restrict__ ce,
                                                         int32_t n, int32_t* __restrict__ counts_or_offsets,
-                                                        int32_t* __restrict__ nbr) {
+                                                        int32_t* __restrict__ nbr, const GridRun* __restrict__ runs,
+                                                        const int32_t* __restrict__ run_sorted) {
   const Lane L = lane_of<G>(n);
   if (!L.valid) return;  // whole groups only: every lane of a group shares its row
   const int32_t k = L.k;
@@ -178,6 +197,15 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g, CellScan sc, 
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = xb[(int64_t)k * D + d];
   const float hi = hb[k];
+  GridDev g = g0;
+  CellScan sc = sc0;
+  uint32_t base = 0;
+  if (RUNS) {  // the run's own grid and key range
+    const GridRun& gr = runs[run_sorted[k]];
+    g = gr.g;
+    sc = gr.sc;
+    base = gr.base;
+  }
   int c[3];
   cell_of<D>(g, xi, c);
   const int lane = threadIdx.x & 63;
@@ -201,10 +229,10 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g, CellScan sc, 
           cc[d] = q;
         }
         if (!ok) continue;
-        const uint32_t code = morton<D>(cc);
+        const uint32_t code = base + morton<D>(cc);
         const int32_t b = cs[code], e = ce[code];
-        for (int32_t base = b; base < e; base += G) {
-          const int32_t kk = base + L.g;
+        for (int32_t cb = b; cb < e; cb += G) {
+          const int32_t kk = cb + L.g;
           bool hit = false;
           if (kk < e) {
             const float r2 = exact_r2<D>(geo, xi, &xb[(int64_t)kk * D]);
@@ -222,14 +250,15 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g, CellScan sc, 
 template <int D>
 constexpr int nbr_group() { return D == 1 ? 4 : (D == 2 ? 16 : 32); }
 
-// validity: red[0] = max |x - xb|^2, red[1] = max h/hb, red[2] = min hb
-// (grid-stride, one atomic per block)
+// validity of each run r: red[3r] = max |x - xb|^2, red[3r+1] = max h/hb,
+// red[3r+2] = min hb (segmented grid-stride, one atomic per block)
 template <int D>
 __global__ void k_valid(Geo geo, const int32_t* __restrict__ perm, const float* __restrict__ xb,
                         const float* __restrict__ hb, const float* __restrict__ x, const float* __restrict__ h,
-                        int32_t n, float* red) {
+                        int32_t n, float* red, RunsDev R) {
   float d2 = 0.f, g = 0.f, hm = 3.0e38f;
-  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+  const Span sp = block_span(R, n);
+  for (int32_t k = sp.k; k < sp.e; k += sp.step) {
     int32_t i = perm[k];
     float xi[3];
 #pragma unroll
@@ -242,16 +271,18 @@ __global__ void k_valid(Geo geo, const int32_t* __restrict__ perm, const float* 
   g = block_reduce(g, MaxOp());
   hm = block_reduce(hm, MinOp());
   if (threadIdx.x == 0) {
-    atomic_max_pos(&red[0], d2);
-    atomic_max_pos(&red[1], g);
-    atomic_min_pos(&red[2], hm);
+    atomic_max_pos(&red[3 * sp.r], d2);
+    atomic_max_pos(&red[3 * sp.r + 1], g);
+    atomic_min_pos(&red[3 * sp.r + 2], hm);
   }
 }
 
-__global__ void k_valid_init(float* red) {
-  red[0] = 0.f;
-  red[1] = 0.f;
-  red[2] = 3.0e38f;
+__global__ void k_valid_init(float* red, int32_t nr) {
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+    red[3 * r] = 0.f;
+    red[3 * r + 1] = 0.f;
+    red[3 * r + 2] = 3.0e38f;
+  }
 }
 
 __global__ void k_export(const int32_t* __restrict__ perm, const int32_t* __restrict__ off,
@@ -269,13 +300,46 @@ __global__ void k_export(const int32_t* __restrict__ perm, const int32_t* __rest
   }
 }
 
+// per-run grids of the bound runs on the device (ctx->s_gridruns); returns
+// the total cell-table size
+static int upload_gridruns(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grids, int64_t* table,
+                           hipStream_t s) {
+  const int32_t R = ctx->n_runs;
+  int st;
+  if ((st = ensure(ctx->s_gridruns, sizeof(GridRun) * (size_t)R))) return st;
+  if ((st = ensure_pinned(ctx, sizeof(GridRun) * (size_t)R))) return st;
+  GridRun* h = (GridRun*)ctx->pinned.p;
+  int64_t base = 0;
+  for (int32_t r = 0; r < R; ++r) {
+    h[r].g = make_griddev(grids[r], *cfg);
+    h[r].sc = make_scan(grids[r], *cfg);
+    h[r].base = (uint32_t)base;
+    base += grids[r].table_size;
+  }
+  CRK_HIP(hipMemcpyAsync(ctx->s_gridruns.p, h, sizeof(GridRun) * (size_t)R, hipMemcpyHostToDevice, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  *table = base;
+  return CRK_OK;
+}
+
+static int key_bits(int64_t table) {
+  int b = 0;
+  while (((int64_t)1 << b) < table) ++b;
+  return b;
+}
+
 template <int D>
 static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n, const float* x,
                       const float* h, uint32_t* keys, int32_t* perm, int32_t* rank, int32_t* cell_start,
                       int32_t* cell_end, int32_t* row_offsets, float* xb, float* hb, int64_t* n_edges,
                       hipStream_t s) {
   GridDev g = make_griddev(*grid, *cfg);
+  const bool runs = ctx->n_runs > 0;
+  int64_t table = grid->table_size;
   int st;
+  if (runs && (st = upload_gridruns(ctx, cfg, grid, &table, s))) return st;
+  const GridRun* gr = runs ? (const GridRun*)ctx->s_gridruns.p : nullptr;
+  const int32_t* run = run_ids(ctx);
   // keys and the identity permutation
   if ((st = ensure(ctx->s_keys0, sizeof(uint32_t) * (size_t)n))) return st;
   if ((st = ensure(ctx->s_keys1, sizeof(uint32_t) * (size_t)n))) return st;
@@ -285,9 +349,9 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   uint32_t* k1 = (uint32_t*)ctx->s_keys1.p;
   int32_t* v0 = (int32_t*)ctx->s_vals0.p;
   int32_t* v1 = (int32_t*)ctx->s_vals1.p;
-  hipLaunchKernelGGL(k_keys<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, x, n, g, k0, v0);
+  hipLaunchKernelGGL(k_keys<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, x, n, g, gr, run, k0, v0);
   CRK_LAUNCH_CHECK("k_keys");
-  int keybits = D * grid->bits;
+  int keybits = runs ? key_bits(table) : D * grid->bits;
   int npass = (keybits + 7) / 8;
   int32_t ntiles = (int32_t)((n + kRsTile - 1) / kRsTile);
   if ((st = ensure(ctx->s_hist, sizeof(int32_t) * 256 * (size_t)ntiles))) return st;
@@ -306,15 +370,21 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   CRK_HIP(hipMemcpyAsync(perm, v0, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_post_sort<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, perm, x, h, n, rank, xb, hb);
   CRK_LAUNCH_CHECK("k_post_sort");
-  if ((st = fill_i32(cell_start, grid->table_size, 0, s))) return st;
-  if ((st = fill_i32(cell_end, grid->table_size, 0, s))) return st;
+  if ((st = fill_i32(cell_start, table, 0, s))) return st;
+  if ((st = fill_i32(cell_end, table, 0, s))) return st;
   hipLaunchKernelGGL(k_cells, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, cell_start, cell_end);
   CRK_LAUNCH_CHECK("k_cells");
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
   constexpr int GN = nbr_group<D>();
-  hipLaunchKernelGGL((k_neighbors_g<D, GN, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
-                     cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr);
+  if (runs)
+    hipLaunchKernelGGL((k_neighbors_g<D, GN, false, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc,
+                       geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr, gr,
+                       run);
+  else
+    hipLaunchKernelGGL((k_neighbors_g<D, GN, false, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc,
+                       geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr,
+                       gr, run);
   CRK_LAUNCH_CHECK("k_neighbors(count)");
   int64_t* total = (int64_t*)ctx->pinned.p;
   if ((st = ensure(ctx->s_red, 64))) return st;
@@ -348,8 +418,17 @@ int crk_adjacency_count(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gr
   CRK_REQUIRE(n > 0, "crk_adjacency_count: n must be positive (got %d)", n);
   CRK_REQUIRE(x && h && keys && perm && rank && cell_start && cell_end && row_offsets && xb && hb,
               "crk_adjacency_count: null buffer");
-  CRK_REQUIRE(grid->dim == cfg->domain.dim, "grid/config dimension mismatch");
-  CRK_REQUIRE(grid->table_size >= 1 && grid->table_size <= ((int64_t)1 << 30), "bad grid table size");
+  int st = require_runs_n(ctx, n, "crk_adjacency_count");
+  if (st) return st;
+  int64_t table = 0;
+  for (int32_t r = 0; r < run_count(ctx); ++r) {
+    CRK_REQUIRE(grid[r].dim == cfg->domain.dim, "grid/config dimension mismatch (grid %d)", r);
+    CRK_REQUIRE(grid[r].table_size >= 1 && grid[r].table_size <= ((int64_t)1 << 30), "bad grid table size (grid %d)",
+                r);
+    table += grid[r].table_size;
+  }
+  CRK_REQUIRE(table <= ((int64_t)1 << 30), "the runs' cell tables hold %lld cells; at most 2^30 are supported",
+              (long long)table);
   hipStream_t s = (hipStream_t)stream;
   switch (cfg->domain.dim) {
     case 1: return count_impl<1>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s);
@@ -374,13 +453,25 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
     return CRK_ECAPACITY;
   }
   if (E > 0) CRK_REQUIRE(nbr, "crk_adjacency_fill: null neighbour buffer");
+  int st = require_runs_n(ctx, n, "crk_adjacency_fill");
+  if (st) return st;
   GridDev g = make_griddev(*grid, *cfg);
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
-  CRK_DIM_DISPATCH(cfg->domain.dim,
-      hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), true>), dim3(group_blocks(n, nbr_group<DD>())),
-                         dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
-                         (int32_t*)row_offsets, nbr));
+  // with runs bound: the per-run grids uploaded by crk_adjacency_count
+  const GridRun* gr = ctx->n_runs > 0 ? (const GridRun*)ctx->s_gridruns.p : nullptr;
+  const int32_t* run = run_ids(ctx);
+  if (gr) {
+    CRK_DIM_DISPATCH(cfg->domain.dim,
+        hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), true, true>), dim3(group_blocks(n, nbr_group<DD>())),
+                           dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
+                           (int32_t*)row_offsets, nbr, gr, run));
+  } else {
+    CRK_DIM_DISPATCH(cfg->domain.dim,
+        hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), true, false>), dim3(group_blocks(n, nbr_group<DD>())),
+                           dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
+                           (int32_t*)row_offsets, nbr, gr, run));
+  }
   CRK_LAUNCH_CHECK("k_neighbors(fill)");
   return CRK_OK;
 }
@@ -390,24 +481,29 @@ int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const 
   CRK_REQUIRE(ctx && cfg && perm && xb && hb && x && h && valid, "crk_adjacency_valid: null argument");
   CRK_REQUIRE(n > 0, "n must be positive");
   hipStream_t s = (hipStream_t)stream;
-  int st = ensure(ctx->s_red, 64);
+  int st = require_runs_n(ctx, n, "crk_adjacency_valid");
   if (st) return st;
+  const int32_t nr = run_count(ctx);
+  if ((st = ensure(ctx->s_red, sizeof(float) * 3 * (size_t)nr))) return st;
+  if ((st = ensure_pinned(ctx, sizeof(float) * 3 * (size_t)nr))) return st;
   float* red = (float*)ctx->s_red.p;
-  hipLaunchKernelGGL(k_valid_init, dim3(1), dim3(1), 0, s, red);
+  hipLaunchKernelGGL(k_valid_init, dim3(1), dim3(kBlock), 0, s, red, nr);
   Geo geo = make_geo(*cfg);
-  switch (cfg->domain.dim) {
-    case 1: hipLaunchKernelGGL(k_valid<1>, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
-    case 2: hipLaunchKernelGGL(k_valid<2>, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
-    case 3: hipLaunchKernelGGL(k_valid<3>, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, geo, perm, xb, hb, x, h, n, red); break;
-    default: set_error("dim must be 1..3"); return CRK_EINVAL;
-  }
+  const RunsDev R = runs_of(ctx);
+  const unsigned nb = span_blocks(ctx, reduce_blocks(n));
+  CRK_DIM_DISPATCH(cfg->domain.dim, hipLaunchKernelGGL(k_valid<DD>, dim3(nb), dim3(kBlock), 0, s, geo, perm, xb, hb,
+                                                       x, h, n, red, R));
   CRK_LAUNCH_CHECK("k_valid");
   float* hostv = (float*)ctx->pinned.p;
-  CRK_HIP(hipMemcpyAsync(hostv, red, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipMemcpyAsync(hostv, red, 3 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
-  double sv = cfg->verletScale;
-  double g = hostv[1];
-  int ok = g < sv && 2.0 * std::sqrt((double)hostv[0]) <= (sv - g) * (double)hostv[2];
+  // every run's list must hold (the lists of all runs are rebuilt together)
+  const double sv = cfg->verletScale;
+  int ok = 1;
+  for (int32_t r = 0; r < nr; ++r) {
+    const double g = hostv[3 * r + 1];
+    ok = ok && g < sv && 2.0 * std::sqrt((double)hostv[3 * r]) <= (sv - g) * (double)hostv[3 * r + 2];
+  }
   *valid = ok;
   return CRK_OK;
 }

@@ -50,9 +50,10 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
 }
 
 // The gather of crk_stage_begin fused with the Verlet validity test of the
-// bound adjacency (built at xb, hb) and the support range for Owen:
-// red[0] = max |x - xb|^2 (exact), red[1] = max h / hb (exact), red[2] =
-// min hb; hmm[0] = min h, hmm[1] = max h.  One atomic per block and cell.
+// bound adjacency (built at xb, hb) and the support range for Owen, per run
+// r: red[4r] = max |x - xb|^2 (exact), red[4r+1] = max h / hb (exact),
+// red[4r+2] = min hb; hmm[2r] = min h, hmm[2r+1] = max h.  One atomic per
+// block and cell.
 template <int D>
 __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo geo, const float* __restrict__ X,
                                const float* __restrict__ Vel, const float* __restrict__ H,
@@ -61,9 +62,11 @@ __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo 
                                const float* __restrict__ AL, float* x, float* v, float* h, float* m, float* u,
                                float* a0, float* divp, float* alpha, float cfac, float4* __restrict__ geo4,
                                float4* __restrict__ vel4, const float* __restrict__ xb,
-                               const float* __restrict__ hb, float* __restrict__ red, float* __restrict__ hmm) {
+                               const float* __restrict__ hb, float* __restrict__ red, float* __restrict__ hmm,
+                               RunsDev R) {
   float d2 = 0.f, g = 0.f, hbm = 3.0e38f, hlo = 3.0e38f, hhi = 0.f;
-  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+  const Span sp = block_span(R, n);
+  for (int32_t k = sp.k; k < sp.e; k += sp.step) {
     const int32_t i = perm[k];
     float xs[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -94,11 +97,11 @@ __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo 
   hlo = block_reduce(hlo, MinOp());
   hhi = block_reduce(hhi, MaxOp());
   if (threadIdx.x == 0) {
-    atomic_max_pos(&red[0], d2);
-    atomic_max_pos(&red[1], g);
-    atomic_min_pos(&red[2], hbm);
-    atomic_min_pos(&hmm[0], hlo);
-    atomic_max_pos(&hmm[1], hhi);
+    atomic_max_pos(&red[4 * sp.r], d2);
+    atomic_max_pos(&red[4 * sp.r + 1], g);
+    atomic_min_pos(&red[4 * sp.r + 2], hbm);
+    atomic_min_pos(&hmm[2 * sp.r], hlo);
+    atomic_max_pos(&hmm[2 * sp.r + 1], hhi);
   }
 }
 // grid of the checked gather: enough blocks to fill the chip, few enough
@@ -107,21 +110,25 @@ inline unsigned check_blocks(int32_t n) {
   unsigned b = nblocks(n);
   return b < 1024u ? b : 1024u;
 }
-// max h / hb over the sorted particles (after the Owen update)
-__global__ void k_hb_ratio(const float* __restrict__ h, const float* __restrict__ hb, int32_t n, float* red) {
+// max h / hb of each run over the sorted particles (after the Owen update)
+// into red[4r+3]
+__global__ void k_hb_ratio(const float* __restrict__ h, const float* __restrict__ hb, int32_t n, float* red,
+                           RunsDev R) {
   float g = 0.f;
-  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    g = fmaxf(g, __fdiv_rn(h[k], hb[k]));
+  const Span sp = block_span(R, n);
+  for (int32_t k = sp.k; k < sp.e; k += sp.step) g = fmaxf(g, __fdiv_rn(h[k], hb[k]));
   g = block_reduce(g, MaxOp());
-  if (threadIdx.x == 0) atomic_max_pos(red, g);
+  if (threadIdx.x == 0) atomic_max_pos(&red[4 * sp.r + 3], g);
 }
-__global__ void k_check_init(float* red, float* hmm) {
-  red[0] = 0.f;
-  red[1] = 0.f;
-  red[2] = 3.0e38f;
-  red[3] = 0.f;  // max h / hb after Owen
-  hmm[0] = 3.0e38f;
-  hmm[1] = 0.f;
+__global__ void k_check_init(float* red, float* hmm, int32_t nr) {
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+    red[4 * r] = 0.f;
+    red[4 * r + 1] = 0.f;
+    red[4 * r + 2] = 3.0e38f;
+    red[4 * r + 3] = 0.f;  // max h / hb after Owen
+    hmm[2 * r] = 3.0e38f;
+    hmm[2 * r + 1] = 0.f;
+  }
 }
 
 // ------------------------------------------------------------ support sweep
@@ -236,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
                                                  const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh,
                                                  int nlut, float nhT, const float* __restrict__ hminmax,
                                                  float* __restrict__ ratio, float4* __restrict__ geo4,
-                                                 OwenFinal fin) {
+                                                 OwenFinal fin, const int32_t* __restrict__ run) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3];
@@ -266,7 +273,8 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
   float sr = nhT / nh;
   float a = sr >= 1.f ? 0.4f * (1.f + 1.f / (sr * sr * sr)) : 0.4f * (1.f + sr * sr);
   float hn = (1.f - a + a * sr) * hi;
-  float lo = hminmax[0] * 0.25f, up = hminmax[1] * 4.0f;
+  const int32_t rr = run_index(run, k);  // the support range of the particle's run
+  float lo = hminmax[2 * rr] * 0.25f, up = hminmax[2 * rr + 1] * 4.0f;
   hn = hn < lo ? lo : (hn > up ? up : hn);
   if (L.valid && L.g == 0) {
     h[k] = hn;
@@ -604,14 +612,14 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
 template <int D>
 __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* __restrict__ nbr,
                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ pnbr, int32_t n,
-                               const int32_t* __restrict__ perm, const float* __restrict__ m_u, float dt,
+                               const int32_t* __restrict__ perm, const float* __restrict__ m_u, DtArg dt,
                                const float* __restrict__ ap, const float* __restrict__ av,
                                const float4* __restrict__ rec, const float* __restrict__ f,
                                float* __restrict__ ap_o, float* __restrict__ av_o, float* __restrict__ f_o) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int32_t b = off[k], e = off[k + 1], c = cnt[k];
-  const float mdt = (f_o && !f) ? m_u[perm[k]] * dt : 0.f;
+  const float mdt = (f_o && !f) ? m_u[perm[k]] * dt.at(k) : 0.f;
   const float4 ri = (f_o && !f) ? rec[k] : make_float4(0.f, 0.f, 0.f, 0.f);
   int32_t p = 0;
   for (int32_t q = b; q < e; ++q) {
@@ -706,6 +714,7 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
   CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 2, "unknown kernel %d", cfg->kernel);
+  if ((r = require_runs_n(ctx, adj->n, "crk_stage_begin"))) return r;
   if ((r = alloc_stage(ctx, adj->n, D))) return r;
   ctx->n = adj->n;
   ctx->dim = D;
@@ -742,8 +751,11 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
   CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 2, "unknown kernel %d", cfg->kernel);
+  if ((r = require_runs_n(ctx, adj->n, "crk_stage_begin_checked"))) return r;
   if ((r = alloc_stage(ctx, adj->n, D))) return r;
-  if ((r = ensure(ctx->s_chk, 16 * sizeof(float)))) return r;
+  const int32_t nr = run_count(ctx);
+  if ((r = ensure(ctx->s_chk, 4 * sizeof(float) * (size_t)nr))) return r;
+  if ((r = ensure_pinned(ctx, 4 * sizeof(float) * (size_t)nr))) return r;
   ctx->n = adj->n;
   ctx->dim = D;
   ctx->adj = *adj;
@@ -755,9 +767,10 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   float* red = F<float>(ctx->s_chk);
-  hipLaunchKernelGGL(k_check_init, dim3(1), dim3(1), 0, s, red, hminmax_dev);
+  hipLaunchKernelGGL(k_check_init, dim3(1), dim3(kBlock), 0, s, red, hminmax_dev, nr);
   Geo geo = make_geo(*cfg);
-  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather_check<DD>, dim3(check_blocks(n)), dim3(kBlock), 0, s, adj->perm, n, geo,
+  const RunsDev R = runs_of(ctx);
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather_check<DD>, dim3(span_blocks(ctx, check_blocks(n))), dim3(kBlock), 0, s, adj->perm, n, geo,
                                          (const float*)st->positions, (const float*)st->velocities,
                                          (const float*)st->supports, (const float*)st->masses,
                                          (const float*)st->internalEnergies, (const float*)st->alpha0s,
@@ -765,15 +778,23 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
                                          F<float>(ctx->v), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->u),
                                          F<float>(ctx->a0), F<float>(ctx->divp), F<float>(ctx->alpha),
                                          cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
-                                         F<float4>(ctx->vel4), xb, hb, red, hminmax_dev));
+                                         F<float4>(ctx->vel4), xb, hb, red, hminmax_dev, R));
   CRK_LAUNCH_CHECK("k_gather_check");
   float* hostv = (float*)ctx->pinned.p;
-  CRK_HIP(hipMemcpyAsync(hostv, red, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipMemcpyAsync(hostv, red, 4 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
-  ctx->check_d2 = hostv[0];
-  ctx->check_hbmin = hostv[2];
-  const double sv = cfg->verletScale, g = hostv[1];
-  *valid = g < sv && 2.0 * std::sqrt((double)hostv[0]) <= (sv - g) * (double)hostv[2];
+  // every run's list must hold (the lists of all runs are rebuilt together)
+  ctx->check_d2.resize(nr);
+  ctx->check_hbmin.resize(nr);
+  const double sv = cfg->verletScale;
+  int ok = 1;
+  for (int32_t q = 0; q < nr; ++q) {
+    ctx->check_d2[q] = hostv[4 * q];
+    ctx->check_hbmin[q] = hostv[4 * q + 2];
+    const double g = hostv[4 * q + 1];
+    ok = ok && g < sv && 2.0 * std::sqrt((double)hostv[4 * q]) <= (sv - g) * (double)hostv[4 * q + 2];
+  }
+  *valid = ok;
   return CRK_OK;
 }
 
@@ -783,16 +804,26 @@ int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* 
     set_error("crk_owen_valid needs crk_stage_begin_checked followed by crk_owen_iteration");
     return CRK_ESTATE;
   }
+  const int32_t nr = run_count(ctx);
+  if ((int32_t)ctx->check_d2.size() != nr) {
+    set_error("crk_owen_valid: the runs changed since crk_stage_begin_checked");
+    return CRK_ESTATE;
+  }
   hipStream_t s = (hipStream_t)stream;
-  float* red = F<float>(ctx->s_chk) + 3;  // cleared by crk_stage_begin_checked
-  hipLaunchKernelGGL(k_hb_ratio, dim3(check_blocks(ctx->n)), dim3(kBlock), 0, s, F<float>(ctx->h), ctx->check_hb,
-                     ctx->n, red);
+  float* red = F<float>(ctx->s_chk);  // cleared by crk_stage_begin_checked
+  hipLaunchKernelGGL(k_hb_ratio, dim3(span_blocks(ctx, check_blocks(ctx->n))), dim3(kBlock), 0, s, F<float>(ctx->h),
+                     ctx->check_hb, ctx->n, red, runs_of(ctx));
   CRK_LAUNCH_CHECK("k_hb_ratio");
   float* hostv = (float*)ctx->pinned.p;
-  CRK_HIP(hipMemcpyAsync(hostv, red, sizeof(float), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipMemcpyAsync(hostv, red, 4 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
-  const double sv = cfg->verletScale, g = hostv[0];
-  *valid = g < sv && 2.0 * std::sqrt((double)ctx->check_d2) <= (sv - g) * (double)ctx->check_hbmin;
+  const double sv = cfg->verletScale;
+  int ok = 1;
+  for (int32_t q = 0; q < nr; ++q) {
+    const double g = hostv[4 * q + 3];
+    ok = ok && g < sv && 2.0 * std::sqrt((double)ctx->check_d2[q]) <= (sv - g) * (double)ctx->check_hbmin[q];
+  }
+  *valid = ok;
   return CRK_OK;
 }
 
@@ -819,14 +850,14 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
         hipLaunchKernelGGL((k_owen<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin));
+                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin, run_ids(ctx)));
     ctx->have_volume = true;
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_owen<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), OwenFinal{}));
+                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), OwenFinal{}, run_ids(ctx)));
   }
   CRK_LAUNCH_CHECK("k_owen");
   return CRK_OK;
@@ -934,12 +965,15 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
   if (adj->n_edges == 0) return CRK_OK;
   CRK_REQUIRE(adj->nbr && ed->nbr, "crk_edges_export: neighbour lists missing");
   CRK_REQUIRE((!ap_out || ed->ap) && (!av_out || ed->av), "crk_edges_export: pair data missing");
-  CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->ap && ed->av && dt > 0.f),
+  const DtArg dta = dt_arg(ctx, dt);
+  CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->ap && ed->av && (dt > 0.f || dta.p)),
               "crk_edges_export: f needs edges->f, or edges->rec with masses, ap, av and dt > 0");
+  int r = dta.p ? require_runs_n(ctx, adj->n, "crk_edges_export") : 0;
+  if (r) return r;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(dim, hipLaunchKernelGGL(k_edges_export<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->row_offsets,
-                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dt, ed->ap, ed->av,
+                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->ap, ed->av,
                                            (const float4*)ed->rec, ed->f, ap_out, av_out, f_out));
   CRK_LAUNCH_CHECK("k_edges_export");
   return CRK_OK;

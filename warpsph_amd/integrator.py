@@ -18,6 +18,8 @@ import ctypes as C
 from dataclasses import dataclass, field
 from typing import Any, List
 
+import numpy as np
+
 from . import abi
 from .enumTypes import IntegrationSchemeType
 
@@ -41,9 +43,32 @@ class RungeKutta2:
         from .runtime import runtime
         initial = state
         st0 = initial.state
-        dt = float(dt)
+        runs = getattr(initial, "runs", None)
         rt = runtime(st0.positions.device)
         s = rt.stream()
+        nD = st0.positions.numel()
+        n = st0.positions.shape[0]
+        D = nD // n
+        if runs is not None:
+            # an ensemble: every run advances by its own dt (each particle's dt)
+            dt_run = runs.set_dt(dt.detach().cpu().numpy() if hasattr(dt, "detach") else dt)
+            dtp = runs.dtp
+            dt = dt_run
+
+            def axpy(cnt, y0, d, y):
+                abi.call("crk_axpy_p", n, cnt // n, abi.ptr(y0), abi.ptr(d), abi.ptr(dtp), abi.ptr(y), s)
+
+            def combine(cnt, y0, d0, d1, b0, b1, y):
+                abi.call("crk_rk_combine_p", n, cnt // n, abi.ptr(y0), abi.ptr(d0), abi.ptr(d1), b0, b1,
+                         abi.ptr(dtp), abi.ptr(y), s)
+        else:
+            dt = float(dt)
+
+            def axpy(cnt, y0, d, y):
+                abi.call("crk_axpy", cnt, abi.ptr(y0), abi.ptr(d), dt, abi.ptr(y), s)
+
+            def combine(cnt, y0, d0, d1, b0, b1, y):
+                abi.call("crk_rk_combine", cnt, abi.ptr(y0), abi.ptr(d0), abi.ptr(d1), b0, b1, dt, abi.ptr(y), s)
         # stage 1 reads the initial positions, velocities and energies
         sys1 = initial.stageSystem("share")
         upd1, adj1, st1 = f(sys1, dt, config, compParams, verbose=verbose)
@@ -51,34 +76,32 @@ class RungeKutta2:
         sys2 = sys1.stageSystem("alloc")
         sys2.adjacency = adj1
         s2 = sys2.state
-        nD = st0.positions.numel()
-        n = st0.positions.shape[0]
-        abi.call("crk_axpy", nD, abi.ptr(st0.positions), abi.ptr(upd1.dxdt), dt, abi.ptr(s2.positions), s)
-        abi.call("crk_axpy", nD, abi.ptr(st0.velocities), abi.ptr(upd1.dvdt), dt, abi.ptr(s2.velocities), s)
-        abi.call("crk_axpy", n, abi.ptr(st0.internalEnergies), abi.ptr(upd1.dudt), dt, abi.ptr(s2.internalEnergies), s)
+        axpy(nD, st0.positions, upd1.dxdt, s2.positions)
+        axpy(nD, st0.velocities, upd1.dvdt, s2.velocities)
+        axpy(n, st0.internalEnergies, upd1.dudt, s2.internalEnergies)
         upd2, adj2, stt2 = f(sys2, dt, config, compParams, verbose=verbose)
         # final state
         b0, b1 = self.weights
         fin = sys2
         x_new = st0.positions.new_empty(st0.positions.shape)
         v_new = st0.velocities.new_empty(st0.velocities.shape)
-        abi.call("crk_rk_combine", nD, abi.ptr(st0.positions), abi.ptr(upd1.dxdt), abi.ptr(upd2.dxdt), b0, b1, dt,
-                 abi.ptr(x_new), s)
-        abi.call("crk_rk_combine", nD, abi.ptr(st0.velocities), abi.ptr(upd1.dvdt), abi.ptr(upd2.dvdt), b0, b1, dt,
-                 abi.ptr(v_new), s)
+        combine(nD, st0.positions, upd1.dxdt, upd2.dxdt, b0, b1, x_new)
+        combine(nD, st0.velocities, upd1.dvdt, upd2.dvdt, b0, b1, v_new)
         if not compParams.compatibleEnergy:
             u_new = st0.internalEnergies.new_empty(st0.internalEnergies.shape)
-            abi.call("crk_rk_combine", n, abi.ptr(st0.internalEnergies), abi.ptr(upd1.dudt), abi.ptr(upd2.dudt),
-                     b0, b1, dt, abi.ptr(u_new), s)
+            combine(n, st0.internalEnergies, upd1.dudt, upd2.dudt, b0, b1, u_new)
             fin.state.internalEnergies = u_new
         stages = [(adj1, st1), (adj2, stt2)]
-        fin.finalize(initial, dt, stages, [upd1, upd2], [b0, b1], config, compParams)
+        fin.finalize(initial, float(np.max(dt)) if runs is not None else dt, stages, [upd1, upd2], [b0, b1], config,
+                     compParams)
         fin.state.positions = x_new
         fin.state.velocities = v_new
         from .configurations import toCrkConfig
         cfg = toCrkConfig(config, compParams)
         abi.call("crk_wrap_positions", C.byref(cfg), n, abi.ptr(x_new), s)
         fin.t = initial.t + dt
+        if runs is not None:
+            rt.bind_runs(None)  # direct library calls after the step see one simulation
         return IntegrationResult(state=fin, stages=[StageResult(aux=[adj1, st1], update=upd1),
                                                     StageResult(aux=[adj2, stt2], update=upd2)])
 

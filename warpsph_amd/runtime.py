@@ -42,6 +42,13 @@ class Runtime:
     def stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
+    def bind_runs(self, runs, dtp=None):
+        """Bind an ensemble's run table and per-particle dt (None: one simulation)."""
+        if runs is None:
+            abi.call("crk_bind_runs", self.ctx, 0, None, None, self.stream())
+        else:
+            runs.bind(self, dtp)
+
     def ensure_lut(self, kernel: int, dim: int, target_neighbors: float, nlut: int = 4096,
                    nmin: float = 2.0, nmax: float = 6.0):
         """Owen table (optimalSupportOwen.py:52-54: nMin 2, nMax 6, 2^12 entries)."""
@@ -68,23 +75,46 @@ def runtime(device) -> Runtime:
     return rt
 
 
-def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor):
-    """Morton keys -> stable LSD radix sort -> cell table -> CSR (all on the GPU)."""
+def _run_grids(cfg: abi.CrkConfig, rt, runs, positions: torch.Tensor, supports: torch.Tensor):
+    """One grid per run of the bound ensemble (the run's h_max and extent)."""
+    R = runs.n_runs
+    ext = np.empty(7 * R, np.float32)
+    abi.call("crk_run_extents", rt.ctx, C.byref(cfg), runs.n, abi.ptr(positions), abi.ptr(supports),
+             C.c_void_p(ext.ctypes.data), rt.stream())
+    grids = (abi.CrkGrid * R)()
+    for r in range(R):
+        lo = np.ascontiguousarray(ext[7 * r + 1:7 * r + 4])
+        hi = np.ascontiguousarray(ext[7 * r + 4:7 * r + 7])
+        abi.call("crk_grid_plan", C.byref(cfg), float(ext[7 * r]), C.c_void_p(lo.ctypes.data),
+                 C.c_void_p(hi.ctypes.data), C.byref(grids[r]))
+    return grids
+
+
+def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor, runs=None):
+    """Morton keys -> stable LSD radix sort -> cell table -> CSR (all on the GPU).
+    With ``runs`` (an ensemble, bound to the runtime) every run has its own grid
+    and its rows hold only its own particles."""
     from .systems import AdjacencyList, capacity
     rt = runtime(positions.device)
     n = int(positions.shape[0])
     D = int(cfg.domain.dim)
     dev = positions.device
-    hmax = float(supports.max().item())
-    lo = np.zeros(3, np.float32)
-    hi = np.zeros(3, np.float32)
-    if not all(cfg.domain.periodic[d] for d in range(D)):
-        lo[:D] = positions.min(dim=0).values.cpu().numpy()
-        hi[:D] = positions.max(dim=0).values.cpu().numpy()
-    grid = abi.CrkGrid()
-    abi.call("crk_grid_plan", C.byref(cfg), hmax, C.c_void_p(lo.ctypes.data), C.c_void_p(hi.ctypes.data),
-             C.byref(grid))
-    T = int(grid.table_size)
+    if runs is not None:
+        grid = _run_grids(cfg, rt, runs, positions, supports)
+        T = int(sum(g.table_size for g in grid))
+        gref = grid
+    else:
+        hmax = float(supports.max().item())
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        if not all(cfg.domain.periodic[d] for d in range(D)):
+            lo[:D] = positions.min(dim=0).values.cpu().numpy()
+            hi[:D] = positions.max(dim=0).values.cpu().numpy()
+        grid = abi.CrkGrid()
+        abi.call("crk_grid_plan", C.byref(cfg), hmax, C.c_void_p(lo.ctypes.data), C.c_void_p(hi.ctypes.data),
+                 C.byref(grid))
+        T = int(grid.table_size)
+        gref = C.byref(grid)
     keys = torch.empty(n, dtype=torch.int32, device=dev)  # uint32 bit patterns
     perm = torch.empty(n, dtype=torch.int32, device=dev)
     rank = torch.empty(n, dtype=torch.int32, device=dev)
@@ -94,17 +124,19 @@ def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch
     xb = torch.empty(n * D, dtype=torch.float32, device=dev)
     hb = torch.empty(n, dtype=torch.float32, device=dev)
     E = C.c_int64()
-    abi.call("crk_adjacency_count", rt.ctx, C.byref(cfg), C.byref(grid), n, abi.ptr(positions), abi.ptr(supports),
+    abi.call("crk_adjacency_count", rt.ctx, C.byref(cfg), gref, n, abi.ptr(positions), abi.ptr(supports),
              abi.ptr(keys), abi.ptr(perm), abi.ptr(rank), abi.ptr(cs), abi.ptr(ce), abi.ptr(off), abi.ptr(xb),
              abi.ptr(hb), C.byref(E), rt.stream())
     nbr = torch.empty(capacity(E.value), dtype=torch.int32, device=dev)[:E.value]
-    abi.call("crk_adjacency_fill", rt.ctx, C.byref(cfg), C.byref(grid), n, abi.ptr(cs), abi.ptr(ce), abi.ptr(off),
+    abi.call("crk_adjacency_fill", rt.ctx, C.byref(cfg), gref, n, abi.ptr(cs), abi.ptr(ce), abi.ptr(off),
              abi.ptr(xb), abi.ptr(hb), abi.ptr(nbr), E.value, rt.stream())
-    return AdjacencyList(n, grid, keys, perm, rank, cs, ce, off, nbr, xb.view(n, D), hb)
+    adj = AdjacencyList(n, grid, keys, perm, rank, cs, ce, off, nbr, xb.view(n, D), hb)
+    adj.runs = runs
+    return adj
 
 
-def adjacency_valid(cfg: abi.CrkConfig, adj, positions: torch.Tensor, supports: torch.Tensor) -> bool:
-    if adj is None or adj.n != positions.shape[0]:
+def adjacency_valid(cfg: abi.CrkConfig, adj, positions: torch.Tensor, supports: torch.Tensor, runs=None) -> bool:
+    if adj is None or adj.n != positions.shape[0] or getattr(adj, "runs", None) is not runs:
         return False
     rt = runtime(positions.device)
     v = C.c_int32()
@@ -113,20 +145,21 @@ def adjacency_valid(cfg: abi.CrkConfig, adj, positions: torch.Tensor, supports: 
     return bool(v.value)
 
 
-def rebuild_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor):
+def rebuild_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor, runs=None):
     with timing.scope("adjacency_build"):
-        return build_adjacency(cfg, positions, supports)
+        return build_adjacency(cfg, positions, supports, runs)
 
 
-def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: torch.Tensor):
+def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: torch.Tensor, runs=None):
     """crk_stage_begin fused with the Verlet test of the prior adjacency and the
     support range; rebuilds (and binds the new list) when the prior list no
     longer holds every pair within the Verlet radius.  Returns the adjacency."""
     rt = runtime(st.positions.device)
-    adj = prior if prior is not None and prior.n == st.numParticles else None
+    adj = prior if prior is not None and prior.n == st.numParticles and getattr(prior, "runs", None) is runs \
+        else None
     for _ in range(2):
         if adj is None:
-            adj = rebuild_adjacency(cfg, st.positions, st.supports)
+            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
         A = adj.cabi()
         ok = C.c_int32()
         with timing.scope("stage_begin"):
@@ -138,13 +171,13 @@ def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: tor
     raise RuntimeError("an adjacency built for the current state failed its own validity test")
 
 
-def ensure_adjacency(cfg: abi.CrkConfig, prior, positions: torch.Tensor, supports: torch.Tensor):
+def ensure_adjacency(cfg: abi.CrkConfig, prior, positions: torch.Tensor, supports: torch.Tensor, runs=None):
     """buildVerletList(..., priorNeighborhood=prior): reuse the prior list
     while every pair inside the current supports is still in it."""
     with timing.scope("adjacency_check"):
-        ok = prior is not None and adjacency_valid(cfg, prior, positions, supports)
+        ok = prior is not None and adjacency_valid(cfg, prior, positions, supports, runs)
     if ok:
         return prior, False
     with timing.scope("adjacency_build"):
-        adj = build_adjacency(cfg, positions, supports)
+        adj = build_adjacency(cfg, positions, supports, runs)
     return adj, True

@@ -10,6 +10,7 @@ from __future__ import annotations
 import ctypes as C
 from typing import Any, Optional
 
+import numpy as np
 import torch
 
 from . import abi, timing
@@ -47,8 +48,17 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     pressures, soundspeeds, alphas (alpha0s when evolveAlpha0), divergence,
     pairs (read as ap_ij, av_ij, f_ij); ``system.adjacency`` is the adjacency used.
     Returns ``(update, adjacency, state)``.
+
+    For an ensemble (``system.runs`` set, warpsph_amd.ensemble) ``dt`` is the
+    per-run dt (or one value for every run); each run is evaluated as it
+    would be alone.
     """
-    dt = float(dt.detach().cpu().item()) if isinstance(dt, torch.Tensor) else float(dt)
+    runs = getattr(system, "runs", None)
+    if runs is not None:
+        runs.set_dt(dt.detach().cpu().numpy() if isinstance(dt, torch.Tensor) else dt)
+        dt = float(np.max(runs.dt))  # (the library reads each particle's dt)
+    else:
+        dt = float(dt.detach().cpu().item()) if isinstance(dt, torch.Tensor) else float(dt)
     st = system.state
     _require(st, ("positions", "velocities", "supports", "masses", "internalEnergies"))
     _fill_defaults(st)
@@ -59,6 +69,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     cfg = toCrkConfig(config, compParams)
     rt = runtime(st.positions.device)
     s = rt.stream()
+    rt.bind_runs(runs)
+    if runs is not None and runs.n != n:
+        raise ValueError(f"the ensemble's runs hold {runs.n} particles, the state {n}")
     cst = st.cabi()
     adj = system.adjacency
 
@@ -68,7 +81,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     if owen:
         # evaluateOptimalSupportOwen, optimalSupportOwen.py:35-127
         rt.ensure_lut(cfg.kernel, dim, cfg.targetNeighbors)
-        hmm = rt.scratch[0:2]
+        hmm = rt.scratch[0:2] if runs is None else runs.hmm
         ratio = rt.scratch[2:3]
         iters = int(compParams.adaptiveSupportIterations)
         # a single iteration also computes V and v_sig with the new supports
@@ -77,7 +90,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         if iters == 1:
             # the stage's gather carries the Verlet test of the prior list and
             # the support range hMin, hMax (buildVerletList with priorNeighborhood)
-            adj = begin_checked(cfg, adj, st, cst, hmm)
+            adj = begin_checked(cfg, adj, st, cst, hmm, runs)
             with timing.scope("owen"):
                 abi.call("crk_owen_support" if fused_support else "crk_owen_iteration", rt.ctx, C.byref(cfg),
                          C.byref(cst), abi.ptr(hmm), None, s)
@@ -85,7 +98,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         else:
             abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
             for it in range(iters):
-                adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports)
+                adj, _ = ensure_adjacency(cfg, adj, st.positions, st.supports, runs)
                 A = adj.cabi()
                 abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
                 with timing.scope("owen"):
@@ -107,9 +120,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), C.byref(ok), s)
         rebuilt = not ok.value
         if rebuilt:
-            adj = rebuild_adjacency(cfg, st.positions, st.supports)
+            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
     else:
-        adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports)
+        adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports, runs)
     system.adjacency = adj
     A = adj.cabi()
     if rebuilt or not owen:
@@ -119,6 +132,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     # the interacting pairs of this stage (compacted by the support sweep,
     # which also computes the apparent volume) and their pair data
     st.pairs = StagePairs(adj, dim, st.masses)
+    if runs is not None:
+        st.pairs.runs, st.pairs.dtp = runs, runs.dtp
     ed = st.pairs.cabi()
     rt.nsing.zero_()
     if fused_support and not rebuilt:

@@ -53,6 +53,9 @@ class StagePairs:
         self.dim = int(dim)
         self.masses = masses
         self.dt = 0.0  # set by the derivative evaluation (energy partition)
+        # an ensemble's run table and per-particle dt of the stage (or None)
+        self.runs = None
+        self.dtp = None
         self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)
         self.nbr = torch.empty(E, dtype=torch.int32, device=dev)
         self.ap = torch.empty((E, dim), dtype=torch.float32, device=dev)
@@ -79,6 +82,7 @@ class StagePairs:
             f = torch.empty(E, dtype=torch.float32, device=dev)
             if E > 0:
                 rt = runtime(dev)
+                rt.bind_runs(self.runs, self.dtp)
                 A = a.cabi()
                 ed = self.cabi()
                 abi.call("crk_edges_export", rt.ctx, self.dim, C.byref(A), C.byref(ed), abi.ptr(self.masses),
@@ -226,6 +230,7 @@ class AdjacencyList:
         self.xb = xb
         self.hb = hb
         self.n_edges = int(nbr.shape[0])
+        self.runs = None  # the ensemble run table it was built for (per-run grids)
         self._export = None
 
     @property
@@ -290,16 +295,18 @@ class CompSPHSystem:
     state: CompSPHState = None
     adjacency: Optional[AdjacencyList] = None
     domain: Any = None
-    t: float = 0.0
+    t: Any = 0.0  # per-run times (numpy array) for an ensemble
+    # run table of an ensemble (warpsph_amd.ensemble.Runs); None: one simulation
+    runs: Any = None
 
     def stageSystem(self, integrated: str) -> "CompSPHSystem":
         """System of one integrator stage (see CompSPHState.stageState)."""
         return CompSPHSystem(state=self.state.stageState(integrated), adjacency=self.adjacency, t=self.t,
-                             domain=self.domain)
+                             domain=self.domain, runs=self.runs)
 
     def initializeNewState(self, *args, verbose=False, **kwargs) -> "CompSPHSystem":
         return CompSPHSystem(state=self.state.initializeNewState(), adjacency=self.adjacency, t=self.t,
-                             domain=self.domain)
+                             domain=self.domain, runs=self.runs)
 
     def finalize(self, initialState, dt, returnValues, updateValues, weights, config, compParams, *args, **kwargs):
         """systems/compSPH.py:87-129 with the compatible-energy update of
@@ -317,6 +324,8 @@ class CompSPHSystem:
             rt = runtime(st0.positions.device)
             cfg = toCrkConfig(config, compParams)
             (a0, s0), (a1, s1) = returnValues
+            # an ensemble's per-particle dt (the dt argument is then ignored)
+            rt.bind_runs(s0.pairs.runs, s0.pairs.dtp)
             e0 = s0.pairs.cabi()
             e1 = s1.pairs.cabi()
             u_new = torch.empty_like(st0.internalEnergies)

@@ -105,7 +105,7 @@ typedef struct CrkConfig {
   float              cflFactor, minDt, maxDt, dtGrowthFactor;
   float              gamma;             /* CompressibleSPHConfig.gamma */
   int32_t            energyScheme;      /* CRK_ENERGY_* (only CRK is built) */
-  int32_t            adaptiveSupportScheme;   /* CRK_SUPPORT_NONE / _OWEN */
+  int32_t            adaptiveSupportScheme;   /* CRK_SUPPORT_NONE / _MONAGHAN / _OWEN */
   int32_t            adaptiveSupportIterations;
   float              adaptiveSupportThreshold;
   int32_t            compatibleEnergy;
@@ -323,6 +323,15 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
  * as crk_owen_iteration + crk_support.                                      */
 int crk_owen_support(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                      const float* hminmax_dev, float* max_ratio_dev, void* stream);
+/* Monaghan adaptive support, one Newton iteration
+ * (evaluateOptimalSupportMonaghan, optimalSupportMonaghan.py:20-108, with
+ * computeOmega, wp_omega.py:52-73, 143): rho_i = sum_j m_j W(x_ij, h_i),
+ * F = h - volumeToSupportHelper(m/rho, n_H, d), Omega = 1 + h/(rho d)
+ * sum_j m_j dW/dh, h' = h - F/(Omega + 1e-6), clamped to [hMin/4, 4 hMax].
+ * Same arguments and outputs as crk_owen_iteration (max_ratio is
+ * max |h'/(h + 1e-6) - 1|); crk_owen_valid tests the list afterwards.       */
+int crk_monaghan_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
+                           const float* hminmax_dev, float* max_ratio_dev, void* stream);
 /* hminmax_dev[0] = min supports, [1] = max supports (user array); per run
  * ([2r], [2r+1]) when runs are bound. */
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_dev, void* stream);

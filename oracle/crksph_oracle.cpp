@@ -451,6 +451,72 @@ struct Sim {
 
   const R* X(int i) const { return &x[(size_t)i * D]; }
 
+  // ---------------------------------------------------- Monaghan support
+  // evaluateOptimalSupportMonaghan, optimalSupportMonaghan.py:20-108 (called
+  // with SupportScheme.Gather, crkSPH.py:40): Newton iterations on
+  // F(h) = h - volumeToSupportHelper(m / rho, n_H, d) (computeH/F, :10-18;
+  // utils/support.py:18-44) with rho_i = sum_j m_j W(x_ij, h_i) (Density,
+  // gather) and dF/dh = Omega_i = 1 - sum_j (-h_i / (rho_i d)) m_j dW/dh
+  // (computeOmega_Func_i, wp_omega.py:52-73, and the 1 - out of :143),
+  // dW/dh = -C/h^(d+1) (d k(q) + q k'(q)).  nan_to_num of rho (-> 1, then
+  // clamped to 1e-12), of Omega and of the step (-> 0) as at :58-69;
+  // h' = h - F/(Omega + 1e-6), non-finite h' keeps h; clamp [hMin/4, 4 hMax];
+  // stop when max |h'/(h + 1e-6) - 1| < threshold (:77-90).  The final
+  // density pass (:96-107) is not performed (overwritten at crkSPH.py:61).
+  static R supportOfVolume(R V, R nT) {
+    if (D == 1) return nT * V / R(2);
+    if (D == 2) return std::sqrt(nT * V / R(3.14159265358979323846));
+    return std::pow(nT * V / R(3.14159265358979323846) * R(3) / R(4), R(1) / R(3));
+  }
+  int monaghan() {
+    R hmin = h[0], hmax = h[0];
+    for (int i = 1; i < n; ++i) { hmin = std::min(hmin, h[i]); hmax = std::max(hmax, h[i]); }
+    const R nT = R(cfg.targetNeighbors);
+    for (int it = 0; it < cfg.adaptiveSupportIterations; ++it) {
+      int st = ensureAdj();
+      if (st) return st;
+      std::vector<R> hn(n), ratio(n);
+      forRows(adj, [&](int i, int64_t b, int64_t e) {
+        Sum<R> racc, wacc;
+        const R hi = h[i];
+        for (int64_t k = b; k < e; ++k) {
+          int j = adj.nbr[k];
+          R dx[3];
+          minImage<R>(geo, X(i), X(j), dx);
+          R q = std::sqrt(norm2<R>(dx, D)) / hi;
+          if (!(q < R(1))) continue;
+          R kk = kval<R>(cfg.kernel, D, q), dk = kder<R>(cfg.kernel, D, q);
+          R cw = K.C / K.hpow(hi, D);
+          racc.add(m[j] * (cw * kk));
+          wacc.add(m[j] * (-cw / hi * (R(D) * kk + q * dk)));
+        }
+        const R rho_ = racc.get(), S = wacc.get();
+        R rs = std::isfinite(rho_) ? rho_ : R(1);
+        rs = std::max(rs, R(1e-12));
+        const R F = hi - supportOfVolume(m[i] / rs, nT);
+        const R dh_drho = -hi / (rho_ * R(D));
+        R om = R(1) - dh_drho * S;
+        if (!std::isfinite(om)) om = R(0);
+        R stp = F / (om + R(1e-6));
+        if (!std::isfinite(stp)) stp = R(0);
+        R hnew = hi - stp;
+        if (!std::isfinite(hnew)) hnew = hi;
+        const R lo = hmin * R(0.25), up = hmax * R(4.0);
+        hnew = hnew < lo ? lo : (hnew > up ? up : hnew);
+        hn[i] = hnew;
+        ratio[i] = std::fabs(hnew / (hi + R(1e-6)) - R(1));
+      });
+      R maxr = R(0), nmin = hn[0], nmax = hn[0];
+      for (int i = 0; i < n; ++i) {
+        maxr = std::max(maxr, ratio[i]);
+        nmin = std::min(nmin, hn[i]); nmax = std::max(nmax, hn[i]);
+      }
+      h = hn; hmin = nmin; hmax = nmax;
+      if (maxr < R(cfg.adaptiveSupportThreshold)) break;
+    }
+    return CRK_OK;
+  }
+
   // ---------------------------------------------------- Owen support
   // evaluateOptimalSupportOwen, optimalSupportOwen.py:35-127, with the psi0
   // pass of wp_psi0.py:72-103 (gather, h_i) and computeNewSupport (:27-33).
@@ -960,6 +1026,9 @@ struct Sim {
     int st;
     if (cfg.adaptiveSupportScheme == CRK_SUPPORT_OWEN) {
       st = owen();
+      if (st) return st;
+    } else if (cfg.adaptiveSupportScheme == CRK_SUPPORT_MONAGHAN) {
+      st = monaghan();
       if (st) return st;
     }
     st = ensureAdj();

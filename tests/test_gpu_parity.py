@@ -19,14 +19,21 @@ CASES = {
     "shear2d": dict(kind="shear", nx=64),
     "sod1d": dict(kind="sod"),
     "blast3d": dict(kind="blast", nx=16),
+    # Monaghan's Newton support (the reference's default adaptive scheme), two iterations
+    "shear2d_monaghan": dict(kind="shear", nx=48, monaghan=2),
 }
 
 
 def make_case(name, device="cuda"):
     from warpsph_amd import sample
+    from warpsph_amd.enumTypes import AdaptiveSupportScheme
     c = CASES[name]
     if c["kind"] == "shear":
-        return sample.shear_layer_2d(nx=c["nx"], device=device)
+        system, config, scheme = sample.shear_layer_2d(nx=c["nx"], device=device)
+        if c.get("monaghan"):
+            scheme.adaptiveSupportScheme = AdaptiveSupportScheme.Monaghan
+            scheme.adaptiveSupportIterations = c["monaghan"]
+        return system, config, scheme
     if c["kind"] == "sod":
         return sample.sod_1d(device=device)
     return sample.blast_3d(nx=c["nx"], device=device, energy=1.0)

@@ -124,6 +124,7 @@ _PROTOS = {
     "crk_owen_iteration": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
     "crk_minmax": [_P, C.c_int32, _P, _P, _P],
     "crk_owen_support": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
+    "crk_monaghan_iteration": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
     "crk_pairs_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.POINTER(CrkEdges), _P, _P],
     "crk_support": [_P, C.POINTER(CrkConfig), C.POINTER(CrkEdges), _P],
     "crk_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P],

@@ -294,8 +294,8 @@ def toCrkConfig(config: SimulationConfig, scheme: CRKSPHConfig) -> abi.CrkConfig
     if scheme.energyScheme != EnergyScheme.CRK:
         raise ValueError(f"energy scheme {scheme.energyScheme} is not built (only EnergyScheme.CRK)")
     ads = scheme.adaptiveSupportScheme
-    if ads not in (AdaptiveSupportScheme.Owen, AdaptiveSupportScheme.NoScheme):
-        raise ValueError(f"adaptive support scheme {ads} is not built (Owen, NoScheme)")
+    if ads not in (AdaptiveSupportScheme.Owen, AdaptiveSupportScheme.Monaghan, AdaptiveSupportScheme.NoScheme):
+        raise ValueError(f"adaptive support scheme {ads} is not built (Owen, Monaghan, NoScheme)")
     c.adaptiveSupportScheme = int(ads.value)
     c.adaptiveSupportIterations = int(scheme.adaptiveSupportIterations)
     c.adaptiveSupportThreshold = float(scheme.adaptiveSupportThreshold)

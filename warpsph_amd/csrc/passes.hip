@@ -329,6 +329,84 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
   }
 }
 
+// ------------------------------------------------------------ Monaghan support
+// volumeToSupportHelper (utils/support.py:18-44): the radius holding n_T
+// neighbours of volume V, in the reference's operation order
+template <int D>
+__device__ inline float support_of_volume(float V, float nT) {
+  if (D == 1) return nT * V / 2.f;
+  if (D == 2) return sqrtf(nT * V / 3.14159265f);
+  return powf(nT * V / 3.14159265f * 3.f / 4.f, 1.f / 3.f);
+}
+
+// evaluateOptimalSupportMonaghan (optimalSupportMonaghan.py:20-108), one
+// Newton iteration on F(h) = h - H(m / rho) (computeH, :10-18) over the
+// Verlet row: rho_i = sum_j m_j W(x_ij, h_i) (Density, Gather) and
+// Omega_i = 1 - sum_j (-h_i / (rho_i d)) m_j dW/dh(x_ij, h_i)
+// (computeOmega, wp_omega.py:52-73, 143) with
+// dW/dh = -C / h^(d+1) (d k(q) + q k'(q));  h' = h - F / (Omega + 1e-6),
+// non-finite Omega, step or h' dropped as the reference's nan_to_num
+// (:66-71), clamped to [hMin/4, 4 hMax] of the particle's run (:73).
+// ratio: max |h' / (h + 1e-6) - 1| (:77-90).
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_monaghan(Rows rows, Geo geo, float C, int32_t n,
+                                                     const float* __restrict__ x, float* __restrict__ h,
+                                                     const float* __restrict__ m, const int32_t* __restrict__ perm,
+                                                     float* __restrict__ supports, float nT,
+                                                     const float* __restrict__ hminmax, float* __restrict__ ratio,
+                                                     float4* __restrict__ geo4, const int32_t* __restrict__ run) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
+  const float hi = h[k];
+  const float ih = 1.f / hi;
+  Comp racc, wacc;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
+  for (int32_t q = b + L.g; q < e; q += G) {
+    const int32_t j = jn;
+    if (q + G < e) jn = rows.nbr[q + G];
+    float dx[3];
+    pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+    const float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float kk, dk;
+    kwd<K, D>(qq, kk, dk);
+    const float cw = C * ipow<D>(ih);
+    const float mj = m[j];
+    racc.add(mj * (cw * kk));
+    wacc.add(mj * (-cw * ih * ((float)D * kk + qq * dk)));
+  }
+  const float rho = group_total<G>(racc);
+  const float S = group_total<G>(wacc);
+  float rs = isfinite(rho) ? rho : 1.f;
+  rs = rs < 1e-12f ? 1e-12f : rs;
+  const float F = hi - support_of_volume<D>(m[k] / rs, nT);
+  const float dh_drho = -hi / (rho * (float)D);
+  float om = 1.f - dh_drho * S;
+  om = isfinite(om) ? om : 0.f;
+  float step = F / (om + 1e-6f);
+  step = isfinite(step) ? step : 0.f;
+  float hn = hi - step;
+  hn = isfinite(hn) ? hn : hi;
+  const int32_t rr = run_index(run, k);  // the support range of the particle's run
+  const float lo = hminmax[2 * rr] * 0.25f, up = hminmax[2 * rr + 1] * 4.0f;
+  hn = hn < lo ? lo : (hn > up ? up : hn);
+  float rmax = 0.f;
+  if (L.valid && L.g == 0) {
+    h[k] = hn;
+    geo4[k].w = hn;
+    supports[perm[k]] = hn;
+    rmax = fabsf(hn / (hi + 1e-6f) - 1.f);
+  }
+  if (ratio) {
+    rmax = block_reduce(rmax, MaxOp());
+    if (threadIdx.x == 0) atomic_max_pos(ratio, rmax);
+  }
+}
+
 // ------------------------------------------------------------ CRK moments
 // Linear CRK (Frontiere, Raskin & Owen 2017) with W_ij = W(x_ij, h_j):
 //   m0 = sum V_j W, m1 = sum V_j x W, m2 = sum V_j x x W and x_i-gradients,
@@ -871,6 +949,28 @@ int crk_owen_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
 int crk_owen_support(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
                      float* max_ratio_dev, void* stream) {
   return owen_launch(ctx, cfg, st, hminmax_dev, max_ratio_dev, true, stream);
+}
+
+int crk_monaghan_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
+                           float* max_ratio_dev, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(st && st->supports && hminmax_dev, "crk_monaghan_iteration: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t n = ctx->n;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  if (max_ratio_dev) { if ((r = fill_f32(max_ratio_dev, 1, 0.f, s))) return r; }
+  ctx->have_support = false;
+  ctx->have_volume = false;
+  ctx->owen_checked = ctx->check_hb != nullptr;  // crk_owen_valid tests the new supports
+  int GS = ctx->group[CRK_PASS_OWEN];
+  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+      hipLaunchKernelGGL((k_monaghan<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo, C,
+                         n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), ctx->adj.perm, st->supports,
+                         cfg->targetNeighbors, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx)));
+  CRK_LAUNCH_CHECK("k_monaghan");
+  return CRK_OK;
 }
 
 int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* stream) {

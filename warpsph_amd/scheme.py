@@ -75,24 +75,29 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     cst = st.cabi()
     adj = system.adjacency
 
-    owen = compParams.adaptiveSupportScheme == AdaptiveSupportScheme.Owen
+    scheme_h = compParams.adaptiveSupportScheme
+    owen = scheme_h in (AdaptiveSupportScheme.Owen, AdaptiveSupportScheme.Monaghan)  # an adaptive-h pass runs
     fused_support = False
     owen_checked = False
     if owen:
-        # evaluateOptimalSupportOwen, optimalSupportOwen.py:35-127
-        rt.ensure_lut(cfg.kernel, dim, cfg.targetNeighbors)
+        # evaluateOptimalSupport (optimalSupport.py:12-43): Owen,
+        # optimalSupportOwen.py:35-127, or Monaghan, optimalSupportMonaghan.py:20-108
+        monaghan = scheme_h == AdaptiveSupportScheme.Monaghan
+        if not monaghan:
+            rt.ensure_lut(cfg.kernel, dim, cfg.targetNeighbors)
+        update = "crk_monaghan_iteration" if monaghan else "crk_owen_iteration"
         hmm = rt.scratch[0:2] if runs is None else runs.hmm
         ratio = rt.scratch[2:3]
         iters = int(compParams.adaptiveSupportIterations)
-        # a single iteration also computes V and v_sig with the new supports
-        # (crk_owen_support); the support sweep then folds into the moments
-        fused_support = iters == 1 and rt.fuse_support
+        # a single Owen iteration also computes V and v_sig with the new
+        # supports (crk_owen_support); the support sweep then folds into the moments
+        fused_support = iters == 1 and rt.fuse_support and not monaghan
         if iters == 1:
             # the stage's gather carries the Verlet test of the prior list and
             # the support range hMin, hMax (buildVerletList with priorNeighborhood)
             adj = begin_checked(cfg, adj, st, cst, hmm, runs)
             with timing.scope("owen"):
-                abi.call("crk_owen_support" if fused_support else "crk_owen_iteration", rt.ctx, C.byref(cfg),
+                abi.call("crk_owen_support" if fused_support else update, rt.ctx, C.byref(cfg),
                          C.byref(cst), abi.ptr(hmm), None, s)
             owen_checked = True
         else:
@@ -102,16 +107,17 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
                 A = adj.cabi()
                 abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
                 with timing.scope("owen"):
-                    abi.call("crk_owen_iteration", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm),
-                             abi.ptr(ratio), s)
+                    abi.call(update, rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(hmm), abi.ptr(ratio), s)
                 if it + 1 == iters:
                     break
-                # hMin, hMax of the new supports bound the next iteration (optimalSupportOwen.py:103-104)
+                # hMin, hMax of the new supports bound the next iteration
+                # (optimalSupportOwen.py:103-104, optimalSupportMonaghan.py:74-75)
                 abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
                 if float(ratio.item()) < compParams.adaptiveSupportThreshold:
                     break
-        # The reference's final gather-density pass (:116-125) is overwritten by
-        # computeCRKFactors (crkSPH.py:61) before any use; it is not evaluated.
+        # The reference's final gather-density pass (optimalSupportOwen.py:116-125,
+        # optimalSupportMonaghan.py:96-107) is overwritten by computeCRKFactors
+        # (crkSPH.py:61) before any use; it is not evaluated.
 
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
     if owen_checked:

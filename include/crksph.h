@@ -141,7 +141,7 @@ int  crk_last_error(char* buf, size_t n);
 const char* crk_version(void);
 /* ABI self-check: sizeof of CrkDomain, CrkDiffusionParams, CrkViscosityParams,
  * CrkSwitchParams, CrkConfig, CrkGrid, CrkAdjacency, CrkState, CrkUpdate,
- * CrkEdges (in this order) written to out[0..n-1]. */
+ * CrkEdges, CrkMlp (in this order) written to out[0..n-1]. */
 int crk_struct_sizes(int64_t* out, int32_t n);
 
 /* ---------------------------------------------- independent runs (ensemble)
@@ -407,6 +407,47 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
                      void* stream);
 
 int crk_export_field(CrkContext* ctx, int32_t field, float* out_user, void* stream);
+
+/* ----------------------------------------- learned viscosity switch (MLP)
+ * No counterpart in the reference: a per-particle MLP 8 -> 64 -> 64 -> 1
+ * (SiLU, SiLU, sigmoid) gives the viscosity alpha in [0, 1] in place of the
+ * Cullen-Dehnen switch (CullenDehnen2010.py:218-331), which stays the
+ * fallback for particles whose features leave the training range.  BF16
+ * weights and activations on the matrix cores, FP32 accumulation.
+ * Parameters (FP32): W1[64][8] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.  */
+#define CRK_MLP_IN 8
+#define CRK_MLP_HIDDEN 64
+#define CRK_MLP_NPARAM (64 * 8 + 64 + 64 * 64 + 64 + 64 + 1)
+#define CRK_MLP_NBF16 (64 * 16 + 64 * 64 + 64)
+typedef struct CrkMlp {
+  const float*    params;  /* device [CRK_MLP_NPARAM] */
+  const uint16_t* wbf16;   /* device [CRK_MLP_NBF16], from crk_mlp_pack */
+  float mu[8], sigma[8];   /* input normalisation x' = (f - mu) / sigma */
+  float lo[8], hi[8];      /* raw feature range of the training data */
+} CrkMlp;
+/* BF16 operand copy of the parameters. */
+int crk_mlp_pack(const float* params, uint16_t* wbf16, void* stream);
+/* alpha[p] = sigmoid(MLP(x[p])), x [n][8] normalised features. */
+int crk_mlp_forward(const CrkMlp* model, int32_t n, const float* x, float* alpha, void* stream);
+/* Gradient of mean_p (alpha_p - target_p)^2 over the batch into grad
+ * [CRK_MLP_NPARAM + 1] (last entry: the loss).  Deterministic. */
+int crk_mlp_grad(CrkContext* ctx, const CrkMlp* model, int32_t n, const float* x, const float* target,
+                 float* grad, void* stream);
+/* Raw features of the bound stage (after crk_velocity_gradient), user order
+ * [n][8]: h div/c, h^2 (div - div_prev)/(dt c^2), h |curl v|/c, h (rho/m)^(1/d),
+ * c/v_sig, h |grad rho|/rho, h |grad P|/P, neighbours/n_H (div: the
+ * Cullen-Dehnen divergence).  Needs a Cullen-Dehnen or learned switch.     */
+int crk_learned_features(CrkContext* ctx, const CrkConfig* cfg, float dt, float* features_out,
+                         void* stream);
+/* The learned switch of the bound stage (in place of crk_switch): the
+ * Cullen-Dehnen switch, then alpha = MLP(features) where every feature lies
+ * in [lo, hi], the Cullen-Dehnen alpha elsewhere.  Writes st->alphas and the
+ * stage's alpha (read by crk_accel_dudt with alpha-weighted Q).            */
+int crk_learned_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt,
+                       const CrkMlp* model, void* stream);
+/* One Adam step (bias-corrected, step >= 1) on the FP32 parameters. */
+int crk_mlp_adam(int32_t n_param, float* params, float* m1, float* m2, const float* grad, float lr,
+                 float beta1, float beta2, float eps, int32_t step, void* stream);
 
 /* ------------------------------------------------ integrator helpers
  * Heun RK2 (b = 1/2, 1/2).  All user order.                                 */

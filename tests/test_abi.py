@@ -30,10 +30,10 @@ def test_library_exports_every_symbol():
 
 def test_struct_layout_matches_library():
     from warpsph_amd import abi
-    out = (C.c_int64 * 10)()
-    abi.call("crk_struct_sizes", out, 10)
+    out = (C.c_int64 * 11)()
+    abi.call("crk_struct_sizes", out, 11)
     mirrors = [abi.CrkDomain, abi.CrkDiffusionParams, abi.CrkViscosityParams, abi.CrkSwitchParams, abi.CrkConfig,
-               abi.CrkGrid, abi.CrkAdjacency, abi.CrkState, abi.CrkUpdate, abi.CrkEdges]
+               abi.CrkGrid, abi.CrkAdjacency, abi.CrkState, abi.CrkUpdate, abi.CrkEdges, abi.CrkMlp]
     assert [C.sizeof(m) for m in mirrors] == list(out)
     # pointer blocks use natural alignment: the leading int32 is padded
     assert abi.CrkState.positions.offset == 8 and abi.CrkAdjacency.n_edges.offset == 8

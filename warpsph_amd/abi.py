@@ -100,6 +100,17 @@ class CrkEdges(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "ap", "av", "rec", "f")]
 
 
+MLP_IN = 8
+MLP_HIDDEN = 64
+MLP_NPARAM = 64 * 8 + 64 + 64 * 64 + 64 + 64 + 1
+MLP_NBF16 = 64 * 16 + 64 * 64 + 64
+
+
+class CrkMlp(C.Structure):
+    _fields_ = [("params", C.c_void_p), ("wbf16", C.c_void_p), ("mu", C.c_float * 8), ("sigma", C.c_float * 8),
+                ("lo", C.c_float * 8), ("hi", C.c_float * 8)]
+
+
 _P = C.c_void_p
 _PROTOS = {
     "crk_ctx_create": [C.POINTER(C.c_void_p)],
@@ -156,6 +167,12 @@ _PROTOS = {
     "crk_cfl_dt_runs": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P, _P, _P],
     "crk_axpy_p": [C.c_int64, C.c_int32, _P, _P, _P, _P, _P],
     "crk_rk_combine_p": [C.c_int64, C.c_int32, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P],
+    "crk_mlp_pack": [_P, _P, _P],
+    "crk_mlp_forward": [C.POINTER(CrkMlp), C.c_int32, _P, _P, _P],
+    "crk_mlp_grad": [_P, C.POINTER(CrkMlp), C.c_int32, _P, _P, _P, _P],
+    "crk_mlp_adam": [C.c_int32, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, _P],
+    "crk_learned_features": [_P, C.POINTER(CrkConfig), C.c_float, _P, _P],
+    "crk_learned_switch": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.c_float, C.POINTER(CrkMlp), _P],
 }
 # every symbol include/crksph.h declares (checked by tests/test_abi.py)
 EXPORTED = sorted(list(_PROTOS.keys()) + ["crk_version"])

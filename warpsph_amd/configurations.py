@@ -168,6 +168,9 @@ class ViscositySwitchConfig:
     beta_d: float = 0.05
     beta_xi: float = 2.0
     limitXi: bool = True  # the reference declares limitXi twice; the later default (True) is effective
+    # extension (no reference counterpart): the trained network of
+    # ViscositySwitch.Learned (warpsph_amd.learned.LearnedSwitch)
+    model: Optional[Any] = None
 
 
 @dataclass

@@ -93,6 +93,9 @@ struct CrkContext {
   crk::Buf runs_dev;     // chunks (int2 [n_chunks]) | offsets [n_runs + 1] | run id [n]
   crk::Buf s_gridruns;   // per-run grids of the last crk_adjacency_count
   const float* dtp = nullptr;  // per-particle dt of the bound runs (may be null)
+  // learned switch: |grad rho|, |grad P| (float2), normalised features [n][8],
+  // out-of-range flags
+  crk::Buf gnorm, feat, fflag;
 };
 
 namespace crk {

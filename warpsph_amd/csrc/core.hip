@@ -208,7 +208,7 @@ int crk_struct_sizes(int64_t* out, int32_t n) {
   CRK_REQUIRE(out && n >= 0, "crk_struct_sizes: invalid arguments");
   const int64_t sz[] = {sizeof(CrkDomain), sizeof(CrkDiffusionParams), sizeof(CrkViscosityParams),
                         sizeof(CrkSwitchParams), sizeof(CrkConfig), sizeof(CrkGrid), sizeof(CrkAdjacency),
-                        sizeof(CrkState), sizeof(CrkUpdate), sizeof(CrkEdges)};
+                        sizeof(CrkState), sizeof(CrkUpdate), sizeof(CrkEdges), sizeof(CrkMlp)};
   for (int i = 0; i < n && i < (int)(sizeof(sz) / sizeof(sz[0])); ++i) out[i] = sz[i];
   return CRK_OK;
 }
@@ -294,7 +294,7 @@ int crk_ctx_destroy(CrkContext* c) {
                  &c->rho, &c->P, &c->c, &c->sP, &c->vhs, &c->J, &c->divcd, &c->vsig, &c->Rcd, &c->dvdt, &c->vh, &c->du0,
                  &c->rec, &c->geo4, &c->vel4,
                  &c->lut_nh, &c->lut_psiH, &c->s_keys0, &c->s_keys1, &c->s_vals0, &c->s_vals1, &c->s_hist,
-                 &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk, &c->runs_dev, &c->s_gridruns};
+                 &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk, &c->runs_dev, &c->s_gridruns, &c->gnorm, &c->feat, &c->fflag};
   for (Buf* b : bufs) release(*b);
   if (c->pinned.p) (void)hipHostFree(c->pinned.p);
   delete c;

@@ -157,6 +157,105 @@ __global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, float C,
   cullen_row(in, p, k, perm[k], S, hi, o);
 }
 
+// ------------------------------------------------------------ learned switch
+// |grad rho|, |grad P| with the CRK gradient of the velocity gradient pass:
+// grad f_i = sum_j V_j (f_j - f_i) grad W^R_ij over the in-support pairs
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_gradnorm(ARows rows, Geo geo, float C, int32_t n, VgIn in,
+                                                     const float* __restrict__ P, float2* __restrict__ out) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = in.x[(int64_t)k * D + d];
+  const float ri = in.rho[k], Pi = P[k];
+  CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
+  Comp gr[D], gp[D];
+  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    const int32_t j = rows.nbr[q];
+    float dx[3];
+    pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    const float r = fsqrt(dot<D>(dx, dx));
+    const float hj = in.h[j];
+    if (!(r < hj)) continue;
+    WdW w = wdw<K, D>(C, r, inv_or_zero(r), frcp(hj));
+    float g[3];
+    grad_wr<D>(cr, dx, w, g);
+    const float Vj = in.V[j];
+    const float dr = Vj * (in.rho[j] - ri), dp = Vj * (P[j] - Pi);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      gr[a].add(dr * g[a]);
+      gp[a].add(dp * g[a]);
+    }
+  }
+  float nr = 0.f, np = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const float u = group_total<G>(gr[a]), v = group_total<G>(gp[a]);
+    nr += u * u;
+    np += v * v;
+  }
+  if (L.valid && L.g == 0) out[k] = make_float2(sqrtf(nr), sqrtf(np));
+}
+
+// Features of the learned switch (dimensionless forms of div v, d(div v)/dt,
+// |curl v|, h, c, |grad rho|, |grad P| and the neighbour count):
+//   f0 = h div / c,  f1 = h^2 (div - div_prev) / (dt c^2),  f2 = h |curl v| / c,
+//   f3 = h (rho / m)^(1/d),  f4 = c / v_sig,  f5 = h |grad rho| / rho,
+//   f6 = h |grad P| / P,  f7 = (in-support neighbours) / n_H.
+// div is the Cullen-Dehnen divergence and div_prev the previous one (the
+// inputs of CullenDehnen2010.py:281).  raw: features in user order (perm);
+// otherwise normalised (f - mu) / sigma in sorted order with flag = 1 where a
+// raw feature leaves [lo, hi] or is not finite.
+struct FeatIn {
+  const float *h, *c, *rho, *m, *P, *divcd, *divp, *J, *vsig;
+  const float2* gn;
+  const int32_t* cnt;
+};
+template <int D>
+__global__ void k_features(int32_t n, FeatIn in, DtArg dt, float nT, CrkMlp model, const int32_t* __restrict__ perm,
+                           float* __restrict__ raw, float* __restrict__ xn, uint8_t* __restrict__ flag) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float h = in.h[k], c = in.c[k], rho = in.rho[k], P = in.P[k];
+  const float ic = 1.f / c;
+  const float* J = &in.J[(int64_t)k * D * D];
+  float curl = 0.f;
+  if (D == 2) curl = fabsf(J[2] - J[1]);
+  if (D == 3) {
+    const float a = J[7] - J[5], b = J[2] - J[6], d = J[3] - J[1];
+    curl = sqrtf(a * a + b * b + d * d);
+  }
+  const float div = in.divcd[k];
+  const float2 gn = in.gn[k];
+  const float vs = in.vsig[k];
+  float f[8];
+  f[0] = h * div * ic;
+  f[1] = h * h * (div - in.divp[k]) / dt.at(k) * ic * ic;
+  f[2] = h * curl * ic;
+  f[3] = h * powf(rho / in.m[k], 1.f / (float)D);
+  f[4] = vs > 0.f ? c / vs : 1.f;
+  f[5] = h * gn.x / rho;
+  f[6] = h * gn.y / P;
+  f[7] = (float)in.cnt[k] / nT;
+  if (raw) {
+    float* o = &raw[(int64_t)perm[k] * 8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q] = f[q];
+    return;
+  }
+  uint8_t out = 0;
+  float* o = &xn[(int64_t)k * 8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (!(f[q] >= model.lo[q] && f[q] <= model.hi[q])) out = 1;  // also catches NaN
+    o[q] = (f[q] - model.mu[q]) / model.sigma[q];
+  }
+  flag[k] = out;
+}
+
 // ------------------------------------------------------------ accel + dudt
 struct LimParams {
   float eta_fold, inv_eta_fold, eta_crit;
@@ -476,7 +575,7 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
   CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch: state->alpha0s missing");
   CRK_REQUIRE(dt > 0.f || dt_arg(ctx, dt).p, "crk_switch: dt must be positive");
   if (!ctx->have_velgrad) { set_error("crk_switch before crk_velocity_gradient"); return CRK_ESTATE; }
-  if (cfg->switchParams.scheme != CRK_SWITCH_CULLEN_DEHNEN) {
+  if (cfg->switchParams.scheme != CRK_SWITCH_CULLEN_DEHNEN && cfg->switchParams.scheme != CRK_SWITCH_LEARNED) {
     set_error("crk_switch: switch scheme %d is not the Cullen-Dehnen switch", cfg->switchParams.scheme);
     return CRK_EINVAL;
   }
@@ -555,6 +654,77 @@ int crk_switch_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
   CRK_REQUIRE(!cfg->evolveAlpha0 || st->alpha0s, "crk_switch_accel_dudt: state->alpha0s missing");
   CRK_REQUIRE(dt > 0.f || dt_arg(ctx, dt).p, "crk_switch_accel_dudt: dt must be positive");
   return accel_dudt(ctx, cfg, st, up, ed, true, dt, stream);
+}
+
+static int features_launch(CrkContext* ctx, const CrkConfig* cfg, float dt, const CrkMlp* model, float* raw,
+                           hipStream_t s) {
+  if (!ctx->have_velgrad || !ctx->have_support) {
+    set_error("learned switch features need crk_support, crk_factors_eos and crk_velocity_gradient");
+    return CRK_ESTATE;
+  }
+  const int32_t n = ctx->n;
+  const int D = ctx->dim;
+  int st;
+  if ((st = ensure(ctx->gnorm, sizeof(float2) * (size_t)n))) return st;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, D);
+  VgIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
+          F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
+          F<float>(ctx->gB)};
+  constexpr int GG = 8;
+  switch (cfg->kernel) {
+#define GN_K(KV)                                                                                                  \
+  case KV:                                                                                                        \
+    CRK_DIM_DISPATCH(D, hipLaunchKernelGGL((k_gradnorm<DD, GG, KV>), dim3(group_blocks(n, GG)), dim3(kBlock), 0, s, \
+                                           arows_of(ctx), geo, C, n, in, F<float>(ctx->P), F<float2>(ctx->gnorm))); \
+    break;
+    GN_K(0) GN_K(1) GN_K(2)
+#undef GN_K
+    default: set_error("unknown kernel %d", cfg->kernel); return CRK_EINVAL;
+  }
+  CRK_LAUNCH_CHECK("k_gradnorm");
+  FeatIn fi{F<float>(ctx->h), F<float>(ctx->c), F<float>(ctx->rho), F<float>(ctx->m), F<float>(ctx->P),
+            F<float>(ctx->divcd), F<float>(ctx->divp), F<float>(ctx->J), F<float>(ctx->vsig), F<float2>(ctx->gnorm),
+            ctx->pair_cnt};
+  CrkMlp mm = model ? *model : CrkMlp{};
+  float* xn = nullptr;
+  uint8_t* fl = nullptr;
+  if (!raw) {
+    if ((st = ensure(ctx->feat, sizeof(float) * 8 * (size_t)n))) return st;
+    if ((st = ensure(ctx->fflag, (size_t)n))) return st;
+    xn = F<float>(ctx->feat);
+    fl = F<uint8_t>(ctx->fflag);
+  }
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_features<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, fi,
+                                         dt_arg(ctx, dt), cfg->targetNeighbors, mm, ctx->adj.perm, raw, xn, fl));
+  CRK_LAUNCH_CHECK("k_features");
+  return CRK_OK;
+}
+
+int crk_learned_features(CrkContext* ctx, const CrkConfig* cfg, float dt, float* features_out, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(features_out, "crk_learned_features: null output");
+  CRK_REQUIRE(dt > 0.f || dt_arg(ctx, dt).p, "crk_learned_features: dt must be positive");
+  CRK_REQUIRE(cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED,
+              "crk_learned_features: the features need the Cullen-Dehnen divergence and signal speed");
+  return features_launch(ctx, cfg, dt, nullptr, features_out, (hipStream_t)stream);
+}
+
+int crk_learned_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt, const CrkMlp* model,
+                       void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(model && model->params && model->wbf16, "crk_learned_switch: model missing");
+  for (int q = 0; q < 8; ++q)
+    CRK_REQUIRE(model->sigma[q] > 0.f && model->lo[q] <= model->hi[q], "crk_learned_switch: bad normalisation");
+  CRK_REQUIRE(cfg->switchParams.scheme == CRK_SWITCH_LEARNED, "crk_learned_switch: switch scheme is not learned");
+  // the Cullen-Dehnen switch first: the fallback alpha (and alpha0 evolution)
+  if ((r = crk_switch(ctx, cfg, st, dt, stream))) return r;
+  hipStream_t s = (hipStream_t)stream;
+  if ((r = features_launch(ctx, cfg, dt, model, nullptr, s))) return r;
+  return mlp_infer_launch(model, ctx->n, F<float>(ctx->feat), F<uint8_t>(ctx->fflag), F<float>(ctx->alpha),
+                          F<float>(ctx->alpha), ctx->adj.perm, st->alphas, s);
 }
 
 int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, void* stream) {

@@ -173,8 +173,13 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         with timing.scope("switch"):
             abi.call("crk_switch", rt.ctx, C.byref(cfg), C.byref(cst), float(dt), s)
     elif sw == ViscositySwitch.Learned:
-        raise NotImplementedError("the learned viscosity switch is not part of this build; use "
-                                  "ViscositySwitch.CullenDehnen2010 or NoneSwitch")
+        # the MLP closure with the Cullen-Dehnen switch as its fallback
+        # (warpsph_amd/learned.py); alpha enters the viscosity with alpha-weighted Q
+        model = compParams.viscositySwitchParams.model
+        if model is None:
+            raise ValueError("ViscositySwitch.Learned needs viscositySwitchParams.model (a trained LearnedSwitch)")
+        with timing.scope("switch"):
+            abi.call("crk_learned_switch", rt.ctx, C.byref(cfg), C.byref(cst), float(dt), C.byref(model.cabi()), s)
     # accel + dudt (crkSPH.py:241-284), divergence and dEdt (:324-325)
     with timing.scope("accel_dudt"):
         if fused:

@@ -286,6 +286,86 @@ def ensemble(args):
     print(json.dumps(line))
 
 
+def learned(args):
+    """Learned viscosity closure: training data from coarse / 4x-resolution
+    pairs of the two-state generator, training on the GPU (BF16 MFMA kernels,
+    Adam), then a config-2-shape shear layer (--nx^2, alpha-weighted Q): the
+    coarse run with the learned switch and with the Cullen-Dehnen switch
+    against the Cullen-Dehnen run at 4x the particle count (density L1 on a
+    256^2 grid at --t-end), with wall times."""
+    import numpy as np
+    import torch
+    from warpsph_amd import learned as Lm
+    from warpsph_amd import sample
+    from warpsph_amd.enumTypes import ViscositySwitch
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    feats, targs = [], []
+    for k in range(args.pairs):
+        f, t = Lm.pair_samples(1000 + k, nx=args.train_nx, device="cuda:0")
+        feats.append(f)
+        targs.append(t)
+        print(f"[learned] pair {k}: {f.shape[0]} samples, target mean {float(t.mean()):.3f}", flush=True)
+    X, T = torch.cat(feats), torch.cat(targs)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    model, hist = Lm.fit(X, T, epochs=args.epochs, batch=16384, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    model.save(args.model_out or os.path.join(ROOT, "warpsph_amd", "tuning", "learned_switch.npz"))
+    print(f"[learned] trained: loss {hist[0]:.5f} -> {hist[-1]:.5f} ({train_s:.1f} s)", flush=True)
+
+    def run(nx, switch, mdl=None):
+        system, config, scheme = sample.shear_layer_2d(nx=nx, device="cuda:0")
+        scheme.alphaWeightedViscosity = True
+        scheme.viscositySwitchParams.scheme = switch
+        scheme.viscositySwitchParams.model = mdl
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        s = Lm.advance(system, config, scheme, args.t_end)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        rho, _ = Lm.grid_density(s.state.positions, s.state.masses, 256)
+        fb = None
+        if mdl is not None:  # fraction of particles on the Cullen-Dehnen fallback in the final state
+            f = Lm.stage_features(s, config, scheme)
+            lo = torch.as_tensor(mdl.lo, device=f.device)
+            hi = torch.as_tensor(mdl.hi, device=f.device)
+            fb = float((~((f >= lo) & (f <= hi)).all(dim=1)).float().mean())
+        return rho, wall, fb
+
+    rho_ref, wall_ref, _ = run(2 * args.nx, ViscositySwitch.CullenDehnen2010)
+    print(f"[learned] 4x run: {wall_ref:.1f} s", flush=True)
+    rho_cd, wall_cd, _ = run(args.nx, ViscositySwitch.CullenDehnen2010)
+    rho_ml, wall_ml, fallback = run(args.nx, ViscositySwitch.Learned, model)
+
+    def l1(a, b):
+        return float((a - b).abs().mean() / b.abs().mean())
+
+    hi_ref = None
+    if args.hi_ref:  # a 16x-resolution Cullen-Dehnen run as the reference of all three
+        rho_hi, wall_hi, _ = run(4 * args.nx, ViscositySwitch.CullenDehnen2010)
+        print(f"[learned] 16x run: {wall_hi:.1f} s", flush=True)
+        hi_ref = {"l1_learned_vs_16x": l1(rho_ml, rho_hi), "l1_cullen_dehnen_vs_16x": l1(rho_cd, rho_hi),
+                  "l1_cullen_dehnen_4x_vs_16x": l1(rho_ref, rho_hi), "wall_s_16x": wall_hi}
+
+    # the fraction of particles that fell back to Cullen-Dehnen in the evaluation's last state
+    line = {
+        "metric": "density L1 vs 4x-resolution run, learned vs Cullen-Dehnen switch (2D shear layer)",
+        "value": l1(rho_ml, rho_ref), "unit": "relative L1", "higher_is_better": False, "n_gpus": 1,
+        "dtype": "bf16 MLP (fp32 accumulate), f32 solver", "data": "synthetic",
+        "config": {"workload": f"shear layer {args.nx}^2 (alpha-weighted Q) to t = {args.t_end}, reference "
+                               f"{2 * args.nx}^2; training: {args.pairs} two-state pairs {args.train_nx}^2 / "
+                               f"{2 * args.train_nx}^2, {int(X.shape[0])} samples, {args.epochs} epochs"},
+        "l1_learned_vs_4x": l1(rho_ml, rho_ref), "l1_cullen_dehnen_vs_4x": l1(rho_cd, rho_ref),
+        "wall_s": {"learned": wall_ml, "cullen_dehnen": wall_cd, "cullen_dehnen_4x": wall_ref},
+        "speedup_learned_vs_4x": wall_ref / wall_ml, "fallback_fraction_final": fallback,
+        "against_16x": hi_ref,
+        "training": {"generate_s": gen_s, "train_s": train_s, "loss_first": hist[0], "loss_last": hist[-1]},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,10 +378,16 @@ def main():
     ap.add_argument("--tuning-file", default=DEFAULT_TUNING,
                     help="load (or, with --retune or when absent, save after tuning) the lane-group sizes")
     ap.add_argument("--retune", action="store_true")
-    ap.add_argument("--workload", default="shear2d", choices=["shear2d", "blast3d", "ensemble"],
+    ap.add_argument("--workload", default="shear2d", choices=["shear2d", "blast3d", "ensemble", "learned"],
                     help="shear2d: the 1M-particle 2D headline; blast3d: the 3D point blast with the FP64 precision "
                          "statement; ensemble: the batched two-state runs (--runs, --seed)")
     ap.add_argument("--runs", type=int, default=256)
+    ap.add_argument("--pairs", type=int, default=6, help="learned: training pairs")
+    ap.add_argument("--train-nx", type=int, default=256, help="learned: coarse side of a training pair")
+    ap.add_argument("--epochs", type=int, default=30, help="learned: training epochs")
+    ap.add_argument("--t-end", type=float, default=0.4, help="learned: evaluation end time")
+    ap.add_argument("--hi-ref", action="store_true", help="learned: also run 16x the particles as the reference")
+    ap.add_argument("--model-out", default=None, help="learned: where to save the trained network (.npz)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB buffer 4 times (calibrates the HBM traffic counters)")
@@ -327,6 +413,9 @@ def main():
         return
     if args.workload == "ensemble":
         ensemble(args)
+        return
+    if args.workload == "learned":
+        learned(args)
         return
 
     import numpy as np

@@ -66,3 +66,12 @@ def test_sample_ensemble_small():
     assert len(counts) == 4 and counts.min() >= 256 and counts.max() <= 1024
     assert ens.state.positions.shape[0] == counts.sum()
     assert np.all(ens.t == 0)
+
+
+def test_same_seed_same_problem_at_every_resolution():
+    a, _, _ = two_state_run(16, seed=9, device="cpu")
+    b, _, _ = two_state_run(32, seed=9, device="cpu")
+    for f in ("densities", "pressures", "velocities"):
+        va = np.unique(getattr(a.state, f).numpy(), axis=0)
+        vb = np.unique(getattr(b.state, f).numpy(), axis=0)
+        assert np.array_equal(va, vb), f

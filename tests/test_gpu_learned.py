@@ -184,3 +184,47 @@ def test_feature_gradients_against_fp64_assembly():
     assert np.max(f5) > 0.1  # the shear layer's density jumps are seen
     # the neighbour-count feature: in-support pairs / n_H
     assert np.all(f[:, 7] > 0.3) and np.all(f[:, 7] < 3.0)
+
+
+def test_learned_switch_gain_on_a_held_out_pair():
+    """What the trained network gains: train on solver-produced pairs of the
+    two-state generator (64^2 / 128^2), then run a held-out pair's coarse
+    state with the learned and with the Cullen-Dehnen switch and compare the
+    density L1 against its 4x-resolution run.  The printed figures are the
+    test's result; it fails only if the network is much worse than the plain
+    switch or barely used."""
+    from warpsph_amd import learned as Lm
+    from warpsph_amd.ensemble import two_state_run
+    from warpsph_amd.enumTypes import ViscositySwitch
+    announce("learned gain")
+    feats, targs = [], []
+    for k in range(3):
+        f, t = Lm.pair_samples(2000 + k, nx=64, snapshots=(0.03, 0.06, 0.09), grid=16)
+        feats.append(f)
+        targs.append(t)
+    model, hist = Lm.fit(torch.cat(feats), torch.cat(targs), epochs=40, batch=4096, lr=3e-3)
+    coarse, config, scheme = two_state_run(64, 2999, device="cuda")
+    fine, _, _ = two_state_run(128, 2999, device="cuda")
+    res = {}
+    for name, sw, mdl, sysm in (("fine", ViscositySwitch.CullenDehnen2010, None, fine),
+                                ("cd", ViscositySwitch.CullenDehnen2010, None, coarse),
+                                ("learned", ViscositySwitch.Learned, model, coarse)):
+        sc = Lm._with_switch(scheme, sw)
+        sc.viscositySwitchParams.model = mdl
+        s = Lm.advance(sysm.initializeNewState(), config, sc, 0.09)
+        res[name] = Lm.grid_density(s.state.positions, s.state.masses, 16)[0]
+        if mdl is not None:
+            f = Lm.stage_features(s, config, sc)
+            lo = torch.as_tensor(mdl.lo, device=f.device)
+            hi = torch.as_tensor(mdl.hi, device=f.device)
+            used = float(((f >= lo) & (f <= hi)).all(dim=1).float().mean())
+
+    def l1(a, b):
+        return float((a - b).abs().mean() / b.abs().mean())
+
+    l1_cd, l1_ml = l1(res["cd"], res["fine"]), l1(res["learned"], res["fine"])
+    print(f"loss {hist[0]:.4f} -> {hist[-1]:.4f}; density L1 vs 4x: Cullen-Dehnen {l1_cd:.4f}, learned {l1_ml:.4f}, "
+          f"gain {(l1_cd - l1_ml) / l1_cd:+.1%}; network used on {used:.1%} of the particles")
+    assert hist[-1] < hist[0]
+    assert used > 0.9
+    assert l1_ml <= 1.10 * l1_cd

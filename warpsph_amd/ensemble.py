@@ -147,11 +147,14 @@ def two_state_run(nx: int, seed: int, device='cuda', jitter: float = 0.1, target
     """One run of the generator: nx^2 particles on a jittered lattice of the
     periodic unit square, two states split at a random line (a point uniform in
     the square, a direction uniform in [0, pi)): rho, P ~ U[0.1, 1],
-    v ~ U[-0.5, 0.5]^2 on each side.  Returns (system, config, scheme)."""
+    v ~ U[-0.5, 0.5]^2 on each side.  The states depend on the seed only, so
+    runs of one seed at different nx solve the same problem.  Returns
+    (system, config, scheme)."""
     from .sample import _domain, _state_from, lattice
     g = torch.Generator().manual_seed(int(seed))
-    pos, dx = lattice((nx, nx), (0.0, 0.0), (1.0, 1.0), jitter=jitter, generator=g)
+    # the states first: the same seed gives the same problem at every resolution
     u = torch.rand(11, generator=g, dtype=torch.float64)
+    pos, dx = lattice((nx, nx), (0.0, 0.0), (1.0, 1.0), jitter=jitter, generator=g)
     p0 = u[0:2]
     th = math.pi * float(u[2])
     normal = torch.tensor([math.cos(th), math.sin(th)], dtype=torch.float64)

@@ -168,6 +168,24 @@ int crk_struct_sizes(int64_t* out, int32_t n);
  * No counterpart in the reference: it serves the batched ensemble.          */
 int crk_bind_runs(CrkContext* ctx, int32_t n_runs, const int32_t* offsets_host,
                   const float* dt_particles, void* stream);
+/* Steps replayed without host synchronisation (graph capture): the Verlet
+ * tests of crk_adjacency_valid, crk_stage_begin_checked and crk_owen_valid
+ * with valid == NULL OR their result into a device flag of the context (1:
+ * some list failed, the step must be redone after a rebuild) instead of
+ * returning it; crk_flag_clear / crk_flag_read reset and read it (read is
+ * blocking), crk_copy_unless_flag copies bytes (a multiple of 4) only while
+ * the flag is clear, crk_runs_advance_time adds each run's dt to its time
+ * only while the flag is clear.  All but crk_flag_read are graph-capturable. */
+int crk_flag_clear(CrkContext* ctx, void* stream);
+int crk_flag_read(CrkContext* ctx, int32_t* flag, void* stream);
+int crk_copy_unless_flag(CrkContext* ctx, const void* src, void* dst, int64_t bytes, void* stream);
+int crk_runs_advance_time(CrkContext* ctx, const float* dt_runs, double* t_runs, void* stream);
+/* crk_cfl_dt_runs on the device (no host synchronisation): dt_out[r] (float)
+ * with the clamps and the growth limit of prev_dt[r] (device, may be NULL),
+ * and each particle's dt. */
+int crk_cfl_dt_device(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* supports,
+                      const float* internalEnergies, const float* prev_dt, float* dt_out,
+                      float* dt_particles, void* stream);
 /* Run id of every particle (device int32[n]) of the bound runs. */
 int crk_run_ids(CrkContext* ctx, int32_t n, int32_t* run_out, void* stream);
 /* Per-run extent for grid planning (blocking): out_host[7r..7r+6] = h_max,

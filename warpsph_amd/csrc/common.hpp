@@ -83,13 +83,16 @@ struct CrkContext {
   // Verlet validity of the bound stage (crk_stage_begin_checked / crk_owen_valid)
   crk::Buf s_chk;
   const float* check_hb = nullptr;
-  std::vector<float> check_d2, check_hbmin;  // per run
+  int32_t check_runs = 0;  // run count of the last checked stage
   bool owen_checked = false;
+  // device flag of deferred Verlet tests (1: some list failed), for steps
+  // replayed without host synchronisation
+  int32_t* d_flag = nullptr;
   // independent runs (crk_bind_runs): run r is particles [run_off[r], run_off[r+1])
   // in user order and, for adjacencies built while bound, in sorted order too
   int32_t n_runs = 0;  // 0: one run, no run table
   int32_t n_chunks = 0;
-  std::vector<int32_t> run_off;
+  std::vector<int32_t> run_off;  // the table on the device (kept while unbound)
   crk::Buf runs_dev;     // chunks (int2 [n_chunks]) | offsets [n_runs + 1] | run id [n]
   crk::Buf s_gridruns;   // per-run grids of the last crk_adjacency_count
   const float* dtp = nullptr;  // per-particle dt of the bound runs (may be null)
@@ -344,6 +347,10 @@ int scan_exclusive_i32(CrkContext* ctx, const int32_t* in, int32_t* out, int32_t
 int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 int fill_f32(float* p, int64_t n, float v, hipStream_t s);
 int ensure_pinned(CrkContext* ctx, size_t bytes);
+// OR into ctx->d_flag when some run's list fails the Verlet test:
+// red[stride r + i_d2] = max |x - xb|^2, [.. + i_g] = max h/hb, [.. + i_hbm] = min hb
+int verlet_flag_launch(CrkContext* ctx, const float* red, int stride, int i_d2, int i_g, int i_hbm, int32_t nr,
+                       double sv, hipStream_t s);
 // every run's particle count matches the bound table (n particles in all)
 int require_runs_n(const CrkContext* ctx, int32_t n, const char* who);
 

@@ -85,6 +85,28 @@ int require_runs_n(const CrkContext* ctx, int32_t n, const char* who) {
   return CRK_OK;
 }
 
+__global__ void k_verlet_flag(const float* __restrict__ red, int stride, int i_d2, int i_g, int i_hbm, int32_t nr,
+                              double sv, int32_t* flag) {
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+    const double g = red[stride * r + i_g];
+    const bool ok = g < sv && 2.0 * sqrt((double)red[stride * r + i_d2]) <= (sv - g) * (double)red[stride * r + i_hbm];
+    if (!ok) atomicOr(flag, 1);
+  }
+}
+int verlet_flag_launch(CrkContext* ctx, const float* red, int stride, int i_d2, int i_g, int i_hbm, int32_t nr,
+                       double sv, hipStream_t s) {
+  hipLaunchKernelGGL(k_verlet_flag, dim3(1), dim3(kBlock), 0, s, red, stride, i_d2, i_g, i_hbm, nr, sv, ctx->d_flag);
+  CRK_LAUNCH_CHECK("k_verlet_flag");
+  return CRK_OK;
+}
+__global__ void k_flag_set(int32_t* flag, int32_t v) { *flag = v; }
+__global__ void k_copy_unless(const int32_t* __restrict__ flag, const uint32_t* __restrict__ src,
+                              uint32_t* __restrict__ dst, int64_t n) {
+  if (*flag) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // run id of every particle, one block per chunk
 __global__ void k_run_ids(const int2* __restrict__ chunk, const int32_t* __restrict__ off, int32_t* __restrict__ run) {
   const int2 c = chunk[blockIdx.x];
@@ -230,6 +252,10 @@ int crk_ctx_create(CrkContext** out) {
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); }
   c->pinned.p = p;
   c->pinned.bytes = 4096;
+  e = hipMalloc((void**)&c->d_flag, 64);
+  if (e != hipSuccess) { (void)hipHostFree(p); delete c; return hip_fail(e, "hipMalloc", __FILE__, __LINE__); }
+  e = hipMemset(c->d_flag, 0, 64);
+  if (e != hipSuccess) { (void)hipFree(c->d_flag); (void)hipHostFree(p); delete c; return hip_fail(e, "hipMemset", __FILE__, __LINE__); }
   *out = c;
   return CRK_OK;
 }
@@ -240,17 +266,20 @@ int crk_bind_runs(CrkContext* ctx, int32_t n_runs, const int32_t* offsets, const
   CRK_REQUIRE(n_runs >= 0 && n_runs <= kMaxRuns, "crk_bind_runs: n_runs must be in [0, %d] (got %d)", kMaxRuns,
               n_runs);
   if (n_runs == 0) {
-    ctx->n_runs = 0;
+    ctx->n_runs = 0;  // the device table stays for a later bind of the same runs
     ctx->dtp = nullptr;
     return CRK_OK;
   }
   CRK_REQUIRE(offsets && offsets[0] == 0, "crk_bind_runs: offsets must start at 0");
   for (int32_t r = 0; r < n_runs; ++r)
     CRK_REQUIRE(offsets[r + 1] > offsets[r], "crk_bind_runs: run %d is empty or offsets decrease", r);
-  const bool same = ctx->n_runs == n_runs && (int32_t)ctx->run_off.size() == n_runs + 1 &&
+  const bool same = (int32_t)ctx->run_off.size() == n_runs + 1 &&
                     std::equal(ctx->run_off.begin(), ctx->run_off.end(), offsets);
   ctx->dtp = dt_particles;
-  if (same) return CRK_OK;
+  if (same) {  // no upload, no synchronisation (capturable)
+    ctx->n_runs = n_runs;
+    return CRK_OK;
+  }
   const int32_t n = offsets[n_runs];
   std::vector<int32_t> table;  // chunks (pairs) | offsets
   for (int32_t r = 0; r < n_runs; ++r)
@@ -277,6 +306,35 @@ int crk_bind_runs(CrkContext* ctx, int32_t n_runs, const int32_t* offsets, const
   return CRK_OK;
 }
 
+int crk_flag_clear(CrkContext* ctx, void* stream) {
+  CRK_REQUIRE(ctx, "crk_flag_clear: null context");
+  hipLaunchKernelGGL(k_flag_set, dim3(1), dim3(1), 0, (hipStream_t)stream, ctx->d_flag, 0);
+  CRK_LAUNCH_CHECK("k_flag_set");
+  return CRK_OK;
+}
+
+int crk_flag_read(CrkContext* ctx, int32_t* flag, void* stream) {
+  CRK_REQUIRE(ctx && flag, "crk_flag_read: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t* h = (int32_t*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(h, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  *flag = *h;
+  return CRK_OK;
+}
+
+int crk_copy_unless_flag(CrkContext* ctx, const void* src, void* dst, int64_t bytes, void* stream) {
+  CRK_REQUIRE(ctx && src && dst && bytes >= 0 && bytes % 4 == 0, "crk_copy_unless_flag: invalid arguments");
+  if (bytes == 0) return CRK_OK;
+  const int64_t n = bytes / 4;
+  unsigned nb = nblocks(n);
+  nb = nb > 2048u ? 2048u : nb;
+  hipLaunchKernelGGL(k_copy_unless, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, ctx->d_flag,
+                     (const uint32_t*)src, (uint32_t*)dst, n);
+  CRK_LAUNCH_CHECK("k_copy_unless");
+  return CRK_OK;
+}
+
 int crk_run_ids(CrkContext* ctx, int32_t n, int32_t* run_out, void* stream) {
   CRK_REQUIRE(ctx && run_out, "crk_run_ids: null argument");
   CRK_REQUIRE(ctx->n_runs > 0, "crk_run_ids: no runs bound");
@@ -297,6 +355,7 @@ int crk_ctx_destroy(CrkContext* c) {
                  &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk, &c->runs_dev, &c->s_gridruns, &c->gnorm, &c->feat, &c->fflag};
   for (Buf* b : bufs) release(*b);
   if (c->pinned.p) (void)hipHostFree(c->pinned.p);
+  if (c->d_flag) (void)hipFree(c->d_flag);
   delete c;
   return CRK_OK;
 }

@@ -186,6 +186,23 @@ __global__ void k_cfl(const float* __restrict__ u, int32_t n, float gamma, float
   d = block_reduce(d, MinOp());
   if (threadIdx.x == 0) atomic_min_pos(&out[sp.r], d);
 }
+// the clamps of crk_cfl_dt on the device (FP64, as on the host):
+// dt = min(max(raw, minDt), maxDt), limited to growth * prev when prev > 0
+__global__ void k_dt_finish(int32_t nr, const float* __restrict__ raw, const float* __restrict__ prev, double mind,
+                            double maxd, double growth, float* __restrict__ out) {
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) {
+    double dt = raw[r];
+    dt = fmin(fmax(dt, mind), maxd);
+    const double p = prev ? (double)prev[r] : -1.0;
+    if (p > 0 && dt > p) dt = fmin(dt, growth * p);
+    out[r] = (float)dt;
+  }
+}
+__global__ void k_time_advance(const int32_t* __restrict__ flag, int32_t nr, const float* __restrict__ dt,
+                               double* __restrict__ t) {
+  if (*flag) return;
+  for (int32_t r = threadIdx.x; r < nr; r += blockDim.x) t[r] += (double)dt[r];
+}
 // dtp[k] = dt of the run of particle k
 __global__ void k_dt_broadcast(const int32_t* __restrict__ run, int32_t n, const float* __restrict__ dtr,
                                float* __restrict__ dtp) {
@@ -539,6 +556,43 @@ int crk_run_extents(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const floa
       out[7 * r + 4 + d] = d < D ? fkey_inv(hostv[7 * r + 4 + d]) : 0.f;
     }
   }
+  return CRK_OK;
+}
+
+int crk_cfl_dt_device(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* supports, const float* u,
+                      const float* prev_dt, float* dt_out, float* dt_particles, void* stream) {
+  CRK_REQUIRE(ctx && cfg && supports && u && dt_out && dt_particles && n > 0, "crk_cfl_dt_device: invalid arguments");
+  CRK_REQUIRE(ctx->n_runs > 0, "crk_cfl_dt_device: no runs bound (crk_bind_runs)");
+  int st = require_runs_n(ctx, n, "crk_cfl_dt_device");
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t R = ctx->n_runs;
+  if ((st = ensure(ctx->s_red, sizeof(float) * 3 * (size_t)R))) return st;
+  float* mm = (float*)ctx->s_red.p;
+  float* raw = mm + 2 * R;
+  const RunsDev Rd = runs_of(ctx);
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(kBlock), 0, s, mm, R);
+  hipLaunchKernelGGL(k_minmax, dim3(ctx->n_chunks), dim3(kBlock), 0, s, supports, n, mm, Rd);
+  CRK_LAUNCH_CHECK("k_minmax");
+  if ((st = fill_f32(raw, R, 3.0e38f, s))) return st;
+  float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
+  hipLaunchKernelGGL(k_cfl, dim3(ctx->n_chunks), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, mm, raw,
+                     Rd);
+  CRK_LAUNCH_CHECK("k_cfl");
+  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(kBlock), 0, s, R, raw, prev_dt, (double)cfg->minDt,
+                     (double)cfg->maxDt, (double)cfg->dtGrowthFactor, dt_out);
+  CRK_LAUNCH_CHECK("k_dt_finish");
+  hipLaunchKernelGGL(k_dt_broadcast, dim3(nblocks(n)), dim3(kBlock), 0, s, Rd.run, n, dt_out, dt_particles);
+  CRK_LAUNCH_CHECK("k_dt_broadcast");
+  return CRK_OK;
+}
+
+int crk_runs_advance_time(CrkContext* ctx, const float* dt_runs, double* t_runs, void* stream) {
+  CRK_REQUIRE(ctx && dt_runs && t_runs, "crk_runs_advance_time: null argument");
+  CRK_REQUIRE(ctx->n_runs > 0, "crk_runs_advance_time: no runs bound (crk_bind_runs)");
+  hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, ctx->d_flag, ctx->n_runs,
+                     dt_runs, t_runs);
+  CRK_LAUNCH_CHECK("k_time_advance");
   return CRK_OK;
 }
 

@@ -478,7 +478,7 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
 
 int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const int32_t* perm, const float* xb,
                         const float* hb, const float* x, const float* h, int32_t* valid, void* stream) {
-  CRK_REQUIRE(ctx && cfg && perm && xb && hb && x && h && valid, "crk_adjacency_valid: null argument");
+  CRK_REQUIRE(ctx && cfg && perm && xb && hb && x && h, "crk_adjacency_valid: null argument");
   CRK_REQUIRE(n > 0, "n must be positive");
   hipStream_t s = (hipStream_t)stream;
   int st = require_runs_n(ctx, n, "crk_adjacency_valid");
@@ -494,6 +494,7 @@ int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const 
   CRK_DIM_DISPATCH(cfg->domain.dim, hipLaunchKernelGGL(k_valid<DD>, dim3(nb), dim3(kBlock), 0, s, geo, perm, xb, hb,
                                                        x, h, n, red, R));
   CRK_LAUNCH_CHECK("k_valid");
+  if (!valid) return verlet_flag_launch(ctx, red, 3, 0, 1, 2, nr, (double)cfg->verletScale, s);  // deferred
   float* hostv = (float*)ctx->pinned.p;
   CRK_HIP(hipMemcpyAsync(hostv, red, 3 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));

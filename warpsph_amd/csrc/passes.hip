@@ -823,7 +823,7 @@ int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* 
 
 int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj, const CrkState* st,
                             const float* xb, const float* hb, float* hminmax_dev, int32_t* valid, void* stream) {
-  CRK_REQUIRE(ctx && cfg && xb && hb && hminmax_dev && valid, "crk_stage_begin_checked: null argument");
+  CRK_REQUIRE(ctx && cfg && xb && hb && hminmax_dev, "crk_stage_begin_checked: null argument");
   int r = validate_adj(adj, st);
   if (r) return r;
   int D = cfg->domain.dim;
@@ -858,17 +858,15 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
                                          cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
                                          F<float4>(ctx->vel4), xb, hb, red, hminmax_dev, R));
   CRK_LAUNCH_CHECK("k_gather_check");
+  ctx->check_runs = nr;
+  const double sv = cfg->verletScale;
+  if (!valid) return verlet_flag_launch(ctx, red, 4, 0, 1, 2, nr, sv, s);  // deferred: into the device flag
   float* hostv = (float*)ctx->pinned.p;
   CRK_HIP(hipMemcpyAsync(hostv, red, 4 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
   // every run's list must hold (the lists of all runs are rebuilt together)
-  ctx->check_d2.resize(nr);
-  ctx->check_hbmin.resize(nr);
-  const double sv = cfg->verletScale;
   int ok = 1;
   for (int32_t q = 0; q < nr; ++q) {
-    ctx->check_d2[q] = hostv[4 * q];
-    ctx->check_hbmin[q] = hostv[4 * q + 2];
     const double g = hostv[4 * q + 1];
     ok = ok && g < sv && 2.0 * std::sqrt((double)hostv[4 * q]) <= (sv - g) * (double)hostv[4 * q + 2];
   }
@@ -877,13 +875,13 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
 }
 
 int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* stream) {
-  CRK_REQUIRE(ctx && cfg && valid, "crk_owen_valid: null argument");
+  CRK_REQUIRE(ctx && cfg, "crk_owen_valid: null argument");
   if (!ctx->owen_checked) {
     set_error("crk_owen_valid needs crk_stage_begin_checked followed by crk_owen_iteration");
     return CRK_ESTATE;
   }
   const int32_t nr = run_count(ctx);
-  if ((int32_t)ctx->check_d2.size() != nr) {
+  if (ctx->check_runs != nr) {
     set_error("crk_owen_valid: the runs changed since crk_stage_begin_checked");
     return CRK_ESTATE;
   }
@@ -892,14 +890,16 @@ int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* 
   hipLaunchKernelGGL(k_hb_ratio, dim3(span_blocks(ctx, check_blocks(ctx->n))), dim3(kBlock), 0, s, F<float>(ctx->h),
                      ctx->check_hb, ctx->n, red, runs_of(ctx));
   CRK_LAUNCH_CHECK("k_hb_ratio");
+  const double sv = cfg->verletScale;
+  // the positions' part of the test from the checked gather, h/hb after Owen
+  if (!valid) return verlet_flag_launch(ctx, red, 4, 0, 3, 2, nr, sv, s);  // deferred: into the device flag
   float* hostv = (float*)ctx->pinned.p;
   CRK_HIP(hipMemcpyAsync(hostv, red, 4 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
-  const double sv = cfg->verletScale;
   int ok = 1;
   for (int32_t q = 0; q < nr; ++q) {
     const double g = hostv[4 * q + 3];
-    ok = ok && g < sv && 2.0 * std::sqrt((double)ctx->check_d2[q]) <= (sv - g) * (double)ctx->check_hbmin[q];
+    ok = ok && g < sv && 2.0 * std::sqrt((double)hostv[4 * q]) <= (sv - g) * (double)hostv[4 * q + 2];
   }
   *valid = ok;
   return CRK_OK;

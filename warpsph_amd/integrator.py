@@ -49,11 +49,12 @@ class RungeKutta2:
         nD = st0.positions.numel()
         n = st0.positions.shape[0]
         D = nD // n
+        device_dt = isinstance(dt, str) and dt == "device"
         if runs is not None:
             # an ensemble: every run advances by its own dt (each particle's dt)
-            dt_run = runs.set_dt(dt.detach().cpu().numpy() if hasattr(dt, "detach") else dt)
+            if not device_dt:
+                dt = runs.set_dt(dt.detach().cpu().numpy() if hasattr(dt, "detach") else dt)
             dtp = runs.dtp
-            dt = dt_run
 
             def axpy(cnt, y0, d, y):
                 abi.call("crk_axpy_p", n, cnt // n, abi.ptr(y0), abi.ptr(d), abi.ptr(dtp), abi.ptr(y), s)
@@ -92,14 +93,14 @@ class RungeKutta2:
             combine(n, st0.internalEnergies, upd1.dudt, upd2.dudt, b0, b1, u_new)
             fin.state.internalEnergies = u_new
         stages = [(adj1, st1), (adj2, stt2)]
-        fin.finalize(initial, float(np.max(dt)) if runs is not None else dt, stages, [upd1, upd2], [b0, b1], config,
-                     compParams)
+        fin.finalize(initial, 1.0 if device_dt else (float(np.max(dt)) if runs is not None else dt), stages,
+                     [upd1, upd2], [b0, b1], config, compParams)
         fin.state.positions = x_new
         fin.state.velocities = v_new
         from .configurations import toCrkConfig
         cfg = toCrkConfig(config, compParams)
         abi.call("crk_wrap_positions", C.byref(cfg), n, abi.ptr(x_new), s)
-        fin.t = initial.t + dt
+        fin.t = initial.t if device_dt else initial.t + dt  # (device dt: the caller keeps the times)
         if runs is not None:
             rt.bind_runs(None)  # direct library calls after the step see one simulation
         return IntegrationResult(state=fin, stages=[StageResult(aux=[adj1, st1], update=upd1),

@@ -38,6 +38,9 @@ class Runtime:
         # (False; measured faster on MI355X: the compaction costs more inside
         # the moments pass than in its own sweep)
         self.fuse_support = False
+        # deferred Verlet tests (graph replay): the tests set the context's
+        # device flag instead of returning, and nothing is rebuilt
+        self.deferred = False
 
     def stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -157,6 +160,13 @@ def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: tor
     rt = runtime(st.positions.device)
     adj = prior if prior is not None and prior.n == st.numParticles and getattr(prior, "runs", None) is runs \
         else None
+    if rt.deferred:
+        if adj is None:
+            raise RuntimeError("a deferred step needs the adjacency of the state")
+        A = adj.cabi()
+        abi.call("crk_stage_begin_checked", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), abi.ptr(adj.xb),
+                 abi.ptr(adj.hb), abi.ptr(hminmax), None, rt.stream())
+        return adj
     for _ in range(2):
         if adj is None:
             adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
@@ -174,6 +184,13 @@ def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: tor
 def ensure_adjacency(cfg: abi.CrkConfig, prior, positions: torch.Tensor, supports: torch.Tensor, runs=None):
     """buildVerletList(..., priorNeighborhood=prior): reuse the prior list
     while every pair inside the current supports is still in it."""
+    rt = runtime(positions.device)
+    if rt.deferred:
+        if prior is None or prior.n != positions.shape[0] or getattr(prior, "runs", None) is not runs:
+            raise RuntimeError("a deferred step needs the adjacency of the state")
+        abi.call("crk_adjacency_valid", rt.ctx, C.byref(cfg), prior.n, abi.ptr(prior.perm), abi.ptr(prior.xb),
+                 abi.ptr(prior.hb), abi.ptr(positions), abi.ptr(supports), None, rt.stream())
+        return prior, False
     with timing.scope("adjacency_check"):
         ok = prior is not None and adjacency_valid(cfg, prior, positions, supports, runs)
     if ok:

@@ -20,6 +20,10 @@ from .enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, ViscositySw
 from .runtime import begin_checked, ensure_adjacency, rebuild_adjacency, runtime
 from .systems import CompressibleSystemUpdate, CompSPHState, CompSPHSystem, StagePairs
 
+# dt argument of a step whose per-particle dt is already on the device
+# (bound with the run table; graph replay)
+DEVICE_DT = "device"
+
 
 def _require(st: CompSPHState, names):
     for k in names:
@@ -55,8 +59,11 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     """
     runs = getattr(system, "runs", None)
     if runs is not None:
-        runs.set_dt(dt.detach().cpu().numpy() if isinstance(dt, torch.Tensor) else dt)
-        dt = float(np.max(runs.dt))  # (the library reads each particle's dt)
+        if isinstance(dt, str) and dt == DEVICE_DT:
+            dt = 1.0  # each particle's dt is already on the device (the library reads it)
+        else:
+            runs.set_dt(dt.detach().cpu().numpy() if isinstance(dt, torch.Tensor) else dt)
+            dt = float(np.max(runs.dt))  # (the library reads each particle's dt)
     else:
         dt = float(dt.detach().cpu().item()) if isinstance(dt, torch.Tensor) else float(dt)
     st = system.state
@@ -92,6 +99,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # a single Owen iteration also computes V and v_sig with the new
         # supports (crk_owen_support); the support sweep then folds into the moments
         fused_support = iters == 1 and rt.fuse_support and not monaghan
+        if iters > 1 and rt.deferred:
+            raise ValueError("deferred (graph) steps support one adaptive-support iteration")
         if iters == 1:
             # the stage's gather carries the Verlet test of the prior list and
             # the support range hMin, hMax (buildVerletList with priorNeighborhood)
@@ -122,9 +131,13 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
     if owen_checked:
         # positions are unchanged since the checked gather: only the supports moved
-        ok = C.c_int32()
-        abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), C.byref(ok), s)
-        rebuilt = not ok.value
+        if rt.deferred:
+            abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), None, s)  # into the device flag
+            rebuilt = False
+        else:
+            ok = C.c_int32()
+            abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), C.byref(ok), s)
+            rebuilt = not ok.value
         if rebuilt:
             adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
     else:

@@ -243,7 +243,8 @@ def ensemble(args):
     torch.cuda.set_device(0)
     integ = getIntegrator()
     t0 = time.perf_counter()
-    e, config, scheme = ens_mod.sample_ensemble(n_runs=args.runs, seed=args.seed, device="cuda:0")
+    e, config, scheme = ens_mod.sample_ensemble(n_runs=args.runs, nmin=args.run_min, nmax=args.run_max,
+                                                seed=args.seed, device="cuda:0")
     gen_s = time.perf_counter() - t0
     if os.path.exists(args.tuning_file):
         groups = load(args.tuning_file)
@@ -251,22 +252,40 @@ def ensemble(args):
     else:
         groups = None
     n = e.runs.n
-    prev = None
-    for _ in range(args.warmup):
-        prev = ens_mod.cfl_dt(e, config, scheme, prev)
-        e = integ.function(e, crkSPH_step, prev, config, scheme).state
-    torch.cuda.synchronize()
-    gc.collect()
-    gc.freeze()
-    pt = PassTimer()
-    t0 = time.perf_counter()
-    with pt:
+    summary, graph_stats = {}, None
+    if args.graph:
+        from warpsph_amd.graph import GraphStepper
+        gs = GraphStepper(e, config, scheme, integ)
+        for _ in range(args.warmup):
+            gs.step()
+        torch.cuda.synchronize()
+        gc.collect()
+        gc.freeze()
+        r0, c0, d0 = gs.replays, gs.captures, gs.redos
+        t0 = time.perf_counter()
         for _ in range(args.steps):
+            gs.step()
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        graph_stats = {"replays": gs.replays - r0, "captures": gs.captures - c0, "eager_redos": gs.redos - d0}
+        e.t = gs.times()
+    else:
+        prev = None
+        for _ in range(args.warmup):
             prev = ens_mod.cfl_dt(e, config, scheme, prev)
             e = integ.function(e, crkSPH_step, prev, config, scheme).state
         torch.cuda.synchronize()
-    secs = time.perf_counter() - t0
-    summary = pt.summary()
+        gc.collect()
+        gc.freeze()
+        pt = PassTimer()
+        t0 = time.perf_counter()
+        with pt:
+            for _ in range(args.steps):
+                prev = ens_mod.cfl_dt(e, config, scheme, prev)
+                e = integ.function(e, crkSPH_step, prev, config, scheme).state
+            torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        summary = pt.summary()
     counts = e.runs.counts
     line = {
         "metric": "particle-updates/s (2D CRKSPH batched ensemble)", "value": n * args.steps / secs,
@@ -274,10 +293,12 @@ def ensemble(args):
         "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.runs} independent 2D two-state runs (random line split, rho, P ~ U[0.1,1], "
-                               "v ~ U[-0.5,0.5]^2, gamma 5/3), N log-uniform in [4096, 65536], Wendland C2 "
-                               "n_H=32, Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy, per-run CFL dt",
+                               f"v ~ U[-0.5,0.5]^2, gamma 5/3), N log-uniform in [{args.run_min}, {args.run_max}], "
+                               "Wendland C2 n_H=32, Owen h, Cullen-Dehnen switch, Heun RK2 + compatible energy, "
+                               "per-run CFL dt" + (", steps replayed as HIP graphs" if args.graph else ""),
                    "particles": n, "runs": args.runs, "run_min": int(counts.min()), "run_max": int(counts.max()),
                    "seed": args.seed},
+        "graph": graph_stats,
         "generate_s": gen_s,
         "t_run_range": [float(np.min(e.t)), float(np.max(e.t))],
         "passes": {k: {"calls": v[0], "mean_ms": round(v[2], 4)} for k, v in summary.items()},
@@ -382,6 +403,9 @@ def main():
                     help="shear2d: the 1M-particle 2D headline; blast3d: the 3D point blast with the FP64 precision "
                          "statement; ensemble: the batched two-state runs (--runs, --seed)")
     ap.add_argument("--runs", type=int, default=256)
+    ap.add_argument("--run-min", type=int, default=4096, help="ensemble: smallest run")
+    ap.add_argument("--run-max", type=int, default=65536, help="ensemble: largest run")
+    ap.add_argument("--graph", action="store_true", help="ensemble: replay the steps as HIP graphs")
     ap.add_argument("--pairs", type=int, default=6, help="learned: training pairs")
     ap.add_argument("--train-nx", type=int, default=256, help="learned: coarse side of a training pair")
     ap.add_argument("--epochs", type=int, default=30, help="learned: training epochs")

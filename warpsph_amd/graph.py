@@ -55,6 +55,7 @@ class GraphStepper:
         self.t = torch.tensor(np.asarray(system.t, dtype=np.float64).reshape(R), device=self.dev)
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self._graph_adj = None
+        self._pool = torch.cuda.graph_pool_handle()  # one memory pool for every capture
         self.replays = self.captures = self.redos = 0
 
     # ---------------------------------------------------------------- body
@@ -115,7 +116,7 @@ class GraphStepper:
     def _capture(self):
         """The step as a graph for the current adjacency (no work runs)."""
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with torch.cuda.graph(g, pool=self._pool):
             self._body()
         self.graph, self._graph_adj = g, self.system.adjacency
         self.captures += 1
@@ -131,7 +132,7 @@ class GraphStepper:
             # a new adjacency: this step runs the deferred body eagerly (which
             # also allocates every buffer at its size), then the graph is
             # captured for the following steps
-            self.graph = None
+            self.graph = None  # (its memory returns to the pool)
             self._body()
             if self._flag():
                 self._redo_eager()

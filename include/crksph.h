@@ -294,8 +294,10 @@ typedef struct CrkEdges {          /* one stage's interacting pairs, caller owne
   int32_t* nbr;   /* [E]     sorted neighbour j of each pair: row k's pairs fill the
                              slots row_offsets[k] .. row_offsets[k] + count[k] - 1
                              (adjacency row order kept); the other slots are unused */
-  float*   ap;    /* [E*dim] pairwise pressure acceleration, same slots      */
-  float*   av;    /* [E*dim] pairwise viscous acceleration, same slots       */
+  float*   pf;    /* [E*2*dim] pair forces of each slot: ap_ij (dim) then
+                             av_ij (dim), the pairwise pressure and viscous
+                             accelerations; one record per pair so the
+                             acceleration pass writes whole cache lines       */
   float*   rec;   /* [n*4]   energy-partition record of sorted particle k:
                              v_half = v + dt/2 dvdt (dim components, zero padded
                              to 3) and s = P/rho^gamma (crk_balance)           */
@@ -358,7 +360,7 @@ int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_de
  *  - the CRK apparent volume V_i = 1 / sum_j W(x_ij, h_i) (computeCRKFactors);
  *  - the pairs with |x_ij| < max(h_i, h_j) are compacted per row (row order
  *    kept) into edges->count / edges->nbr; every later pass of the stage, and
- *    the pair data ap, av, f, use these slots.  The other (Verlet skin) edges
+ *    the pair data (pf: ap, av; f) use these slots.  The other (Verlet skin) edges
  *    have ap = av = 0 (both CRK gradients vanish there) and are not stored;
  *    crk_edges_export expands to the full adjacency order;
  *  - with the Cullen-Dehnen switch, the signal speed
@@ -389,7 +391,8 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
 int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt, void* stream);
 /* computeCrkSPHAccelWarp + computeCrkSPHdudtWarp fused (crk/accel.py:15-250,
  * crk/dudt.py:15-229).  Writes update->dvdt, update->dudt, update->dEdt,
- * edges->ap, edges->av, and st->divergence = -drhodt/rho (crkSPH.py:324). */
+ * edges->pf (ap_ij, av_ij per pair slot), and st->divergence = -drhodt/rho
+ * (crkSPH.py:324). */
 int crk_accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                    CrkUpdate* up, CrkEdges* edges, void* stream);
 /* crk_switch followed by crk_accel_dudt in one sweep over the pairs: the

@@ -97,7 +97,7 @@ class CrkUpdate(C.Structure):
 
 
 class CrkEdges(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "ap", "av", "rec", "f")]
+    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "pf", "rec", "f")]
 
 
 MLP_IN = 8

@@ -263,9 +263,10 @@ struct LimParams {
 };
 
 // limiter.py:11-16 (van Leer), :28-86 (computeVanLeer), :90-127 (crkLimiter)
+// cI = J_i x_ij, cJ = J_j x_ij (shared with the velocity reconstruction)
 template <int D>
 __device__ inline float limiter_phi(const LimParams& lp, const float* dx, float r, float ihi, float ihj,
-                                    const float* Ji, const float* Jj) {
+                                    const float* cI, const float* cJ) {
   float factor = 1.f;
   if (lp.crk) {
     float eta = r * fminf(ihi, ihj);
@@ -277,11 +278,8 @@ __device__ inline float limiter_phi(const LimParams& lp, const float* dx, float 
     float gi = 0.f, gj = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float ci = 0.f, cj = 0.f;
-#pragma unroll
-      for (int gm = 0; gm < D; ++gm) { ci = ci + Ji[a * D + gm] * dx[gm]; cj = cj + Jj[a * D + gm] * dx[gm]; }
-      gi = gi + ci * dx[a];
-      gj = gj + cj * dx[a];
+      gi = gi + cI[a] * dx[a];
+      gj = gj + cJ[a] * dx[a];
     }
     gi = 0.25f * gi;
     gj = 0.25f * gj;
@@ -306,8 +304,23 @@ struct AdIn {
   const float* drhodt_u;
 };
 struct AdOut {
-  float *ap, *av, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
+  float *pf, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
 };
+
+// pair-force record of a slot: ap (D) then av (D); 2D: one 16-byte store
+template <int D>
+__device__ inline void store_pf(float* p, const float* ap, const float* av) {
+  if (D == 2) {
+    *reinterpret_cast<float4*>(p) = make_float4(ap[0], ap[1], av[0], av[1]);
+  } else if (D == 1) {
+    *reinterpret_cast<float2*>(p) = make_float2(ap[0], av[0]);
+  } else {
+    float2* q = reinterpret_cast<float2*>(p);
+    q[0] = make_float2(ap[0], ap[1]);
+    q[1] = make_float2(ap[2], av[0]);
+    q[2] = make_float2(av[1], av[2]);
+  }
+}
 
 // The acceleration pass gathers nearly every per-particle field of its
 // neighbours; one coalesced pass packs them into the neighbour record
@@ -384,17 +397,22 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
     const float ir = inv_or_zero(r);
     const float* vj = &rj[Rec<D>::VEL];
     const float* Jj = &rj[Rec<D>::J];
-    float phi = limiter_phi<D>(lp, dx, r, ihi, ihj, Ji, Jj);
+    float cI[3], cJ[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      cI[a] = 0.f;
+      cJ[a] = 0.f;
+#pragma unroll
+      for (int gm = 0; gm < D; ++gm) { cI[a] = cI[a] + Ji[a * D + gm] * dx[gm]; cJ[a] = cJ[a] + Jj[a * D + gm] * dx[gm]; }
+    }
+    float phi = limiter_phi<D>(lp, dx, r, ihi, ihj, cI, cJ);
     float phi2 = 0.5f * phi;
     float vdi[3], vdj[3], vdjU[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float cI = 0.f, cJ = 0.f;
-#pragma unroll
-      for (int gm = 0; gm < D; ++gm) { cI = cI + Ji[a * D + gm] * dx[gm]; cJ = cJ + Jj[a * D + gm] * dx[gm]; }
-      vdi[a] = vi[a] - phi2 * cI;
-      vdj[a] = vj[a] + phi2 * cJ;   // accel.py:137-141
-      vdjU[a] = vj[a] - phi2 * cJ;  // dudt.py:123 (J_j (-x_ij))
+      vdi[a] = vi[a] - phi2 * cI[a];
+      vdj[a] = vj[a] + phi2 * cJ[a];   // accel.py:137-141
+      vdjU[a] = vj[a] - phi2 * cJ[a];  // dudt.py:123 (J_j (-x_ij))
     }
     CorrD<D> crj;
     crj.A = rj[Rec<D>::A];
@@ -452,8 +470,7 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
       avv[a] = pQ * gw[a];
       acc[a].add(apv[a] + avv[a]);
     }
-    store_vec<D>(&o.ap[(int64_t)q0 * D], apv);
-    store_vec<D>(&o.av[(int64_t)q0 * D], avv);
+    store_pf<D>(&o.pf[(int64_t)q0 * 2 * D], apv, avv);
     float dt_ = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];
@@ -502,8 +519,7 @@ __global__ void k_balance_rec(int32_t n, const float* __restrict__ v, const floa
 template <int D, int G>
 __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const float4* __restrict__ rec,
                                                     const float* __restrict__ m, DtArg dt,
-                                                    const float* __restrict__ ap, const float* __restrict__ av,
-                                                    float* __restrict__ f) {
+                                                    const float* __restrict__ pf, float* __restrict__ f) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   const float4 ri = rec[k];
@@ -515,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const
     if (q + G < e) jn = rows.nbr[q + G];
     float a[3];
 #pragma unroll
-    for (int c = 0; c < D; ++c) a[c] = ap[(int64_t)q * D + c] + av[(int64_t)q * D + c];
+    for (int c = 0; c < D; ++c) a[c] = pf[(int64_t)q * 2 * D + c] + pf[(int64_t)q * 2 * D + D + c];
     f[q] = pair_fraction<D>(ri, rec[j], mdt, a);
   }
 }
@@ -597,7 +613,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   if (r) return r;
   CRK_REQUIRE(st && st->divergence, "crk_accel_dudt: state->divergence missing");
   CRK_REQUIRE(up && up->dvdt && up->dudt && up->dEdt && up->drhodt, "crk_accel_dudt: update buffers missing");
-  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->ap && ed->av)), "crk_accel_dudt: edge buffers missing");
+  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || ed->pf), "crk_accel_dudt: pair-force buffer missing");
   CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
               "crk_accel_dudt: edges differ from the pairs of crk_support");
   if (!ctx->have_velgrad) { set_error("crk_accel_dudt before crk_velocity_gradient"); return CRK_ESTATE; }
@@ -612,7 +628,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   AdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V),
           F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A),
           F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
-  AdOut out{ed->ap, ed->av, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
+  AdOut out{ed->pf, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
   int GS = ctx->group[CRK_PASS_ACCEL];
   const float* divcd = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN ||
                         cfg->switchParams.scheme == CRK_SWITCH_LEARNED) ? F<float>(ctx->divcd) : nullptr;
@@ -731,7 +747,7 @@ int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, v
   int r = require_bound(ctx, cfg);
   if (r) return r;
   CRK_REQUIRE(ed && ed->rec, "crk_balance: edges->rec missing");
-  CRK_REQUIRE(!ed->f || ctx->adj.n_edges == 0 || (ed->ap && ed->av), "crk_balance: edge buffers missing");
+  CRK_REQUIRE(!ed->f || ctx->adj.n_edges == 0 || ed->pf, "crk_balance: pair-force buffer missing");
   CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
               "crk_balance: edges differ from the pairs of crk_support");
   if (!ctx->have_accel) { set_error("crk_balance before crk_accel_dudt"); return CRK_ESTATE; }
@@ -747,7 +763,7 @@ int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, v
     int GS = ctx->group[CRK_PASS_BALANCE];
     CRK_DIM_G_DISPATCH(ctx->dim, GS,
         hipLaunchKernelGGL((k_balance<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), n, rec,
-                           F<float>(ctx->m), dta, ed->ap, ed->av, ed->f));
+                           F<float>(ctx->m), dta, ed->pf, ed->f));
     CRK_LAUNCH_CHECK("k_balance");
   }
   return CRK_OK;

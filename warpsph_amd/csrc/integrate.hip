@@ -67,7 +67,7 @@ __global__ void k_vhalf_sorted(const int32_t* __restrict__ perm, int32_t n, cons
 // and energy-partition records
 struct CompIn {
   const int32_t *off, *cnt, *nbr;
-  const float *ap, *av;
+  const float* pf;  // per slot: ap (D) then av (D)
   const float4* rec;
   float b;
 };
@@ -106,11 +106,16 @@ __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g,
     const int32_t j = jn;
     if (q + G < e) jn = s.nbr[q + G];
     float a[3], d0 = 0.f;
+    if (D == 2) {
+      const float4 p = *reinterpret_cast<const float4*>(&s.pf[(int64_t)q * 4]);
+      a[0] = p.x + p.z;
+      a[1] = p.y + p.w;
+    } else {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      a[c] = s.ap[(int64_t)q * D + c] + s.av[(int64_t)q * D + c];
-      d0 = d0 + (vhs[(int64_t)j * D + c] - vi[c]) * a[c];
+      for (int c = 0; c < D; ++c) a[c] = s.pf[(int64_t)q * 2 * D + c] + s.pf[(int64_t)q * 2 * D + D + c];
     }
+#pragma unroll
+    for (int c = 0; c < D; ++c) d0 = d0 + (vhs[(int64_t)j * D + c] - vi[c]) * a[c];
     const float f = pair_fraction<D>(ri, s.rec[j], mdt, a);
     kahan_add(sa, ca, s.b * f * d0);
   }
@@ -392,8 +397,8 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   CRK_REQUIRE(n > 0 && adj0->n == n && adj1->n == n, "crk_compatible_energy: particle count mismatch");
   CRK_REQUIRE(adj0->perm && adj1->perm && adj0->row_offsets && adj1->row_offsets && e0->count && e1->count,
               "crk_compatible_energy: adjacency/pair counts missing");
-  CRK_REQUIRE(adj0->n_edges == 0 || (e0->nbr && e0->ap && e0->av), "stage-0 pairs missing");
-  CRK_REQUIRE(adj1->n_edges == 0 || (e1->nbr && e1->ap && e1->av), "stage-1 pairs missing");
+  CRK_REQUIRE(adj0->n_edges == 0 || (e0->nbr && e0->pf), "stage-0 pairs missing");
+  CRK_REQUIRE(adj1->n_edges == 0 || (e1->nbr && e1->pf), "stage-1 pairs missing");
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
   const DtArg dta = dt_arg(ctx, dt);
@@ -407,8 +412,8 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   float* du0 = (float*)ctx->du0.p;
   // both stages in one sweep when they share the sorted order
   const bool shared = adj0->perm == adj1->perm;
-  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->ap, e0->av, (const float4*)e0->rec, b0};
-  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->ap, e1->av, (const float4*)e1->rec, b1};
+  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->pf, (const float4*)e0->rec, b0};
+  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->pf, (const float4*)e1->rec, b1};
   const int GS = ctx->group[CRK_PASS_COMPAT];
   unsigned nb = nblocks((int64_t)n * GS);
 #define VHALF(DD, A)                                                                                          \

@@ -691,7 +691,7 @@ template <int D>
 __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* __restrict__ nbr,
                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ pnbr, int32_t n,
                                const int32_t* __restrict__ perm, const float* __restrict__ m_u, DtArg dt,
-                               const float* __restrict__ ap, const float* __restrict__ av,
+                               const float* __restrict__ pf,
                                const float4* __restrict__ rec, const float* __restrict__ f,
                                float* __restrict__ ap_o, float* __restrict__ av_o, float* __restrict__ f_o) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -705,8 +705,8 @@ __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* _
     const int64_t sl = b + p;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      if (ap_o) ap_o[(int64_t)q * D + a] = hit ? ap[sl * D + a] : 0.f;
-      if (av_o) av_o[(int64_t)q * D + a] = hit ? av[sl * D + a] : 0.f;
+      if (ap_o) ap_o[(int64_t)q * D + a] = hit ? pf[sl * 2 * D + a] : 0.f;
+      if (av_o) av_o[(int64_t)q * D + a] = hit ? pf[sl * 2 * D + D + a] : 0.f;
     }
     if (f_o) {
       float fv = 0.f;
@@ -716,7 +716,7 @@ __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* _
         } else {
           float a[3];
 #pragma unroll
-          for (int d = 0; d < D; ++d) a[d] = ap[sl * D + d] + av[sl * D + d];
+          for (int d = 0; d < D; ++d) a[d] = pf[sl * 2 * D + d] + pf[sl * 2 * D + D + d];
           fv = pair_fraction<D>(ri, rec[nbr[q]], mdt, a);
         }
       }
@@ -1064,16 +1064,16 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
   CRK_REQUIRE(adj->n > 0 && adj->row_offsets && adj->perm && ed->count, "crk_edges_export: adjacency/pairs missing");
   if (adj->n_edges == 0) return CRK_OK;
   CRK_REQUIRE(adj->nbr && ed->nbr, "crk_edges_export: neighbour lists missing");
-  CRK_REQUIRE((!ap_out || ed->ap) && (!av_out || ed->av), "crk_edges_export: pair data missing");
+  CRK_REQUIRE((!ap_out && !av_out) || ed->pf, "crk_edges_export: pair data missing");
   const DtArg dta = dt_arg(ctx, dt);
-  CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->ap && ed->av && (dt > 0.f || dta.p)),
-              "crk_edges_export: f needs edges->f, or edges->rec with masses, ap, av and dt > 0");
+  CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->pf && (dt > 0.f || dta.p)),
+              "crk_edges_export: f needs edges->f, or edges->rec with masses, pair forces and dt > 0");
   int r = dta.p ? require_runs_n(ctx, adj->n, "crk_edges_export") : 0;
   if (r) return r;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(dim, hipLaunchKernelGGL(k_edges_export<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->row_offsets,
-                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->ap, ed->av,
+                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->pf,
                                            (const float4*)ed->rec, ed->f, ap_out, av_out, f_out));
   CRK_LAUNCH_CHECK("k_edges_export");
   return CRK_OK;

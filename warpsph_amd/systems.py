@@ -58,14 +58,14 @@ class StagePairs:
         self.dtp = None
         self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)
         self.nbr = torch.empty(E, dtype=torch.int32, device=dev)
-        self.ap = torch.empty((E, dim), dtype=torch.float32, device=dev)
-        self.av = torch.empty((E, dim), dtype=torch.float32, device=dev)
+        # pair forces per slot: ap_ij (dim) then av_ij (dim)
+        self.pf = torch.empty((E, 2, dim), dtype=torch.float32, device=dev)
         self.rec = torch.empty((adjacency.n, 4), dtype=torch.float32, device=dev)
         self._full = None
 
     def cabi(self) -> abi.CrkEdges:
         """Pointers for the C-ABI; f_ij stays implicit (evaluated from rec)."""
-        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.ap, self.av, self.rec)), None)
+        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.pf, self.rec)), None)
 
     @property
     def n_pairs(self) -> int:

@@ -211,6 +211,12 @@ int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kerne
  * (only the psiH column is used by the step). Host function, blocking.    */
 int crk_owen_lut(int32_t kernel, int32_t dim, int32_t nlut, double n_min, double n_max,
                  double* n_h_out, double* psiH_out);
+/* The same table computed on the GPU directly into the context (one thread
+ * per entry, FP64; within an ulp of crk_owen_lut), and a blocking read of the
+ * context's table. */
+int crk_owen_lut_device(CrkContext* ctx, int32_t kernel, int32_t dim, int32_t nlut, double n_min,
+                        double n_max, double nh_target, void* stream);
+int crk_owen_lut_read(CrkContext* ctx, double* n_h_out, double* psiH_out, void* stream);
 /* Upload an Owen table into the context (device copy). */
 int crk_set_owen_lut(CrkContext* ctx, int32_t nlut, const double* n_h, const double* psiH,
                      double nh_target, void* stream);
@@ -352,6 +358,11 @@ int crk_owen_support(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
  * max |h'/(h + 1e-6) - 1|); crk_owen_valid tests the list afterwards.       */
 int crk_monaghan_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
                            const float* hminmax_dev, float* max_ratio_dev, void* stream);
+/* Density by gather over the bound adjacency, rho_i = sum_j m_j W(x_ij, h_i)
+ * (warpOperation Density, SupportScheme.Gather; the final pass of
+ * optimalSupportOwen.py:116-125, overwritten in the step and therefore not
+ * run there; used by the initial-condition generators).  User order. */
+int crk_density_gather(CrkContext* ctx, const CrkConfig* cfg, float* rho_user, void* stream);
 /* hminmax_dev[0] = min supports, [1] = max supports (user array); per run
  * ([2r], [2r+1]) when runs are bound. */
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* minmax_dev, void* stream);

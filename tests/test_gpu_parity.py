@@ -401,3 +401,24 @@ def test_blast3d_error_against_fp64_oracle():
         errs[key] = (e_gpu, e_o32)
     print({k_: f"gpu {a:.2e} / fp32 oracle {b:.2e}" for k_, (a, b) in errs.items()})
     assert all(a <= max(RTOL, 4.0 * b) for a, b in errs.values()), errs
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_owen_table_on_gpu_matches_host(kernel, dim):
+    """The Owen psi_H table computed on the GPU equals the host table (and the
+    oracle's) to FP64 rounding."""
+    import ctypes as C
+    from oracle import oracle as O
+    from warpsph_amd import abi
+    from warpsph_amd.runtime import runtime
+    announce(f"owen table kernel={kernel} dim={dim}")
+    rt = runtime("cuda:0")
+    abi.call("crk_owen_lut_device", rt.ctx, kernel, dim, 4096, 2.0, 6.0, 3.0, rt.stream())
+    nh = np.empty(4096)
+    ps = np.empty(4096)
+    abi.call("crk_owen_lut_read", rt.ctx, C.c_void_p(nh.ctypes.data), C.c_void_p(ps.ctypes.data), rt.stream())
+    onh, ops = O.owen_lut(kernel, dim)
+    assert np.array_equal(nh, onh)
+    assert np.max(np.abs(ps - ops) / ops) < 4e-16
+    rt._lut_key = None  # the next step uploads its own table

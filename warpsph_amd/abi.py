@@ -119,6 +119,8 @@ _PROTOS = {
     "crk_kernel_constants": [C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     "crk_owen_lut": [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, _P, _P],
     "crk_set_owen_lut": [_P, C.c_int32, _P, _P, C.c_double, _P],
+    "crk_owen_lut_device": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _P],
+    "crk_owen_lut_read": [_P, _P, _P, _P],
     "crk_grid_plan": [C.POINTER(CrkConfig), C.c_float, _P, _P, C.POINTER(CrkGrid)],
     "crk_adjacency_count": [_P, C.POINTER(CrkConfig), C.POINTER(CrkGrid), C.c_int32, _P, _P,
                             _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64), _P],
@@ -138,6 +140,7 @@ _PROTOS = {
     "crk_monaghan_iteration": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P, _P],
     "crk_pairs_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.POINTER(CrkEdges), _P, _P],
     "crk_support": [_P, C.POINTER(CrkConfig), C.POINTER(CrkEdges), _P],
+    "crk_density_gather": [_P, C.POINTER(CrkConfig), _P, _P],
     "crk_factors_eos": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P, _P],
     "crk_velocity_gradient": [_P, C.POINTER(CrkConfig), C.POINTER(CrkUpdate), _P],
     "crk_switch": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), C.c_float, _P],

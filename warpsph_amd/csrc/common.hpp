@@ -137,6 +137,7 @@ __host__ __device__ inline T kval(int kernel, int dim, T q) {
 }
 template <class T>
 __host__ __device__ inline T kder(int kernel, int dim, T q) {
+#pragma clang fp contract(off)  // the same rounding on host and device (Owen table)
   T t = T(1) - q;
   if (kernel == CRK_KERNEL_WENDLAND2) {
     if (dim == 1) return T(-12) * q * t * t;

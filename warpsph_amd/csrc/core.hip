@@ -200,22 +200,32 @@ int scan_exclusive_i32(CrkContext* ctx, const int32_t* in, int32_t* out, int32_t
 // computePsi (src/warpSPH/modules/adaptiveSupport/wp_psi.py:8-99), psiH
 // column: psiH(n_h) = (sum_{eta = 1/n_h, 2/n_h, ... < 1} |k'(eta)| C_d * ring(eta))^(1/d)
 // with ring = 2 (1D), 2 pi eta n_h (2D), 4 pi eta^2 n_h^2 (3D); FP64.
-static void owen_table(int kernel, int dim, int nlut, double nmin, double nmax, double* nh, double* psiH) {
-  double C = kernel_C(kernel, dim);
-  for (int i = 0; i < nlut; ++i) {
-    double dn = (nmax - nmin) / ((double)nlut - 1.0);
-    double n_h = nmin + (double)i * dn;
-    double deta = 1.0 / n_h;
-    double etar = deta, result = 0.0;
-    while (etar < 1.0) {
-      double corr = dim == 1 ? 2.0 : (dim == 2 ? 2.0 * M_PI * etar / deta : 4.0 * M_PI * (etar * etar) / (deta * deta));
-      double val = std::fabs(kder<double>(kernel, dim, etar) * C / 1.0);
-      result += val * corr;
-      etar += deta;
-    }
-    nh[i] = n_h;
-    psiH[i] = std::pow(result, 1.0 / (double)dim);
+__host__ __device__ inline void owen_entry(int kernel, int dim, int nlut, double nmin, double nmax, int i, double* nh,
+                                            double* psiH) {
+#pragma clang fp contract(off)  // host and device round every operation alike
+  const double C = kernel_C(kernel, dim);
+  const double dn = (nmax - nmin) / ((double)nlut - 1.0);
+  const double n_h = nmin + (double)i * dn;
+  const double deta = 1.0 / n_h;
+  double etar = deta, result = 0.0;
+  while (etar < 1.0) {
+    const double corr =
+        dim == 1 ? 2.0 : (dim == 2 ? 2.0 * M_PI * etar / deta : 4.0 * M_PI * (etar * etar) / (deta * deta));
+    const double val = fabs(kder<double>(kernel, dim, etar) * C / 1.0);
+    result += val * corr;
+    etar += deta;
   }
+  nh[i] = n_h;
+  psiH[i] = pow(result, 1.0 / (double)dim);
+}
+static void owen_table(int kernel, int dim, int nlut, double nmin, double nmax, double* nh, double* psiH) {
+  for (int i = 0; i < nlut; ++i) owen_entry(kernel, dim, nlut, nmin, nmax, i, nh, psiH);
+}
+// the same table computed on the device, one entry per thread (FP64; pow may
+// differ from the host's in the last bit)
+__global__ void k_owen_table(int kernel, int dim, int nlut, double nmin, double nmax, double* nh, double* psiH) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nlut) owen_entry(kernel, dim, nlut, nmin, nmax, i, nh, psiH);
 }
 
 }  // namespace crk
@@ -376,6 +386,32 @@ int crk_owen_lut(int32_t kernel, int32_t dim, int32_t nlut, double n_min, double
   CRK_REQUIRE(nlut >= 2 && n_max > n_min && n_min > 0, "invalid table range");
   CRK_REQUIRE(n_h_out && psiH_out, "null output");
   owen_table(kernel, dim, nlut, n_min, n_max, n_h_out, psiH_out);
+  return CRK_OK;
+}
+
+int crk_owen_lut_device(CrkContext* ctx, int32_t kernel, int32_t dim, int32_t nlut, double n_min, double n_max,
+                        double nh_target, void* stream) {
+  CRK_REQUIRE(ctx, "crk_owen_lut_device: null context");
+  CRK_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel %d", kernel);
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3 (got %d)", dim);
+  CRK_REQUIRE(nlut >= 2 && n_max > n_min && n_min > 0, "invalid table range");
+  int st = ensure(ctx->lut_nh, sizeof(double) * nlut);
+  if (st) return st;
+  if ((st = ensure(ctx->lut_psiH, sizeof(double) * nlut))) return st;
+  hipLaunchKernelGGL(k_owen_table, dim3(nblocks(nlut)), dim3(kBlock), 0, (hipStream_t)stream, kernel, dim, nlut,
+                     n_min, n_max, (double*)ctx->lut_nh.p, (double*)ctx->lut_psiH.p);
+  CRK_LAUNCH_CHECK("k_owen_table");
+  ctx->nlut = nlut;
+  ctx->nh_target = nh_target;
+  return CRK_OK;
+}
+
+int crk_owen_lut_read(CrkContext* ctx, double* n_h_out, double* psiH_out, void* stream) {
+  CRK_REQUIRE(ctx && n_h_out && psiH_out && ctx->nlut >= 2, "crk_owen_lut_read: no table");
+  hipStream_t s = (hipStream_t)stream;
+  CRK_HIP(hipMemcpyAsync(n_h_out, ctx->lut_nh.p, sizeof(double) * ctx->nlut, hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipMemcpyAsync(psiH_out, ctx->lut_psiH.p, sizeof(double) * ctx->nlut, hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
   return CRK_OK;
 }
 

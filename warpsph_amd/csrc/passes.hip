@@ -407,6 +407,37 @@ __global__ void __launch_bounds__(kBlock) k_monaghan(Rows rows, Geo geo, float C
   }
 }
 
+// Density by gather over the Verlet row, rho_i = sum_j m_j W(x_ij, h_i)
+// (warpOperation Density with SupportScheme.Gather: the final pass of
+// optimalSupportOwen.py:116-125 and the initial density of
+// sedov/initial.py:83-93), written in user order.
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_density(Rows rows, Geo geo, float C, int32_t n,
+                                                    const float* __restrict__ x, const float* __restrict__ h,
+                                                    const float* __restrict__ m, const int32_t* __restrict__ perm,
+                                                    float* __restrict__ rho_u) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  float xi[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
+  const float ih = 1.f / h[k];
+  Comp acc;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    const int32_t j = rows.nbr[q];
+    float dx[3];
+    pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
+    const float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float kk, dk;
+    kwd<K, D>(qq, kk, dk);
+    acc.add(m[j] * (C * kk * ipow<D>(ih)));
+  }
+  const float rho = group_total<G>(acc);
+  if (L.valid && L.g == 0) rho_u[perm[k]] = rho;
+}
+
 // ------------------------------------------------------------ CRK moments
 // Linear CRK (Frontiere, Raskin & Owen 2017) with W_ij = W(x_ij, h_j):
 //   m0 = sum V_j W, m1 = sum V_j x W, m2 = sum V_j x x W and x_i-gradients,
@@ -970,6 +1001,22 @@ int crk_monaghan_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState
                          n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), ctx->adj.perm, st->supports,
                          cfg->targetNeighbors, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx)));
   CRK_LAUNCH_CHECK("k_monaghan");
+  return CRK_OK;
+}
+
+int crk_density_gather(CrkContext* ctx, const CrkConfig* cfg, float* rho_user, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(rho_user, "crk_density_gather: null output");
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t n = ctx->n;
+  Geo geo = make_geo(*cfg);
+  float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
+  int GS = ctx->group[CRK_PASS_OWEN];
+  CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
+      hipLaunchKernelGGL((k_density<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo, C, n,
+                         F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), ctx->adj.perm, rho_user));
+  CRK_LAUNCH_CHECK("k_density");
   return CRK_OK;
 }
 

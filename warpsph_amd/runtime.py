@@ -58,12 +58,9 @@ class Runtime:
         key = (kernel, dim, float(target_neighbors), nlut, nmin, nmax)
         if self._lut_key == key:
             return
-        nh = np.empty(nlut, np.float64)
-        ps = np.empty(nlut, np.float64)
-        abi.call("crk_owen_lut", kernel, dim, nlut, nmin, nmax, C.c_void_p(nh.ctypes.data), C.c_void_p(ps.ctypes.data))
+        # computed on the GPU (crk_owen_lut_device; within an ulp of crk_owen_lut)
         target = float(nH_to_n_h(float(np.float32(target_neighbors)), dim))
-        abi.call("crk_set_owen_lut", self.ctx, nlut, C.c_void_p(nh.ctypes.data), C.c_void_p(ps.ctypes.data), target,
-                 self.stream())
+        abi.call("crk_owen_lut_device", self.ctx, kernel, dim, nlut, nmin, nmax, target, self.stream())
         self._lut_key = key
 
 

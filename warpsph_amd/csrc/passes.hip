@@ -163,22 +163,30 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   int32_t run = 0;
   float vs = 0.f;
   Comp vol;
-  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
+  // software pipeline: the neighbour's records one iteration ahead, its id two ahead
+  int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
+  int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
+  float4 gn = geo4[jc], wn = VS ? vel4[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
   for (int32_t base = b; base < e; base += G) {
     const int32_t q = base + L.g;
     const bool inside = q < e;
     bool act = false;
     int32_t j = 0;
     if (inside) {
-      j = jn;
-      if (q + G < e) jn = rows.nbr[q + G];
-      const float4 gj = geo4[j];
+      j = jc;
+      const float4 gj = gn;
+      const float4 wj = wn;
+      if (q + G < e) {
+        jc = jn;
+        gn = geo4[jc];
+        if (VS) wn = vel4[jc];
+        if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+      }
       const float xj[3] = {gj.x, gj.y, gj.z};
       float dx[3];
       pair_dx<D>(geo, xi, xj, dx);
       float r = fsqrt(dot<D>(dx, dx));
       if (VS) {
-        const float4 wj = vel4[j];
         const float vj[3] = {wj.x, wj.y, wj.z};
         float mu = 0.f;
 #pragma unroll
@@ -253,17 +261,32 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
   const float ih = 1.f / hi;
   Comp acc;
   const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
-  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
-  for (int32_t q = b + L.g; q < e; q += G) {
-    const int32_t j = jn;
-    if (q + G < e) jn = rows.nbr[q + G];
-    float dx[3];
-    pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
-    float qq = fsqrt(dot<D>(dx, dx)) * ih;
-    if (!(qq < 1.f)) continue;
-    float kk, dk;
-    kwd<K, D>(qq, kk, dk);
-    acc.add(C * fabsf(dk));
+  {
+    // software pipeline: the neighbour's position one iteration ahead, its id two ahead
+    int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
+    float xn[3];
+    {
+      const int32_t j0 = b + L.g < e ? rows.nbr[b + L.g] : 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) xn[d] = x[(int64_t)j0 * D + d];
+    }
+    for (int32_t q = b + L.g; q < e; q += G) {
+      float xj[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) xj[d] = xn[d];
+      if (q + G < e) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) xn[d] = x[(int64_t)jn * D + d];
+        if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+      }
+      float dx[3];
+      pair_dx<D>(geo, xi, xj, dx);
+      float qq = fsqrt(dot<D>(dx, dx)) * ih;
+      if (!(qq < 1.f)) continue;
+      float kk, dk;
+      kwd<K, D>(qq, kk, dk);
+      acc.add(C * fabsf(dk));
+    }
   }
   float S = group_total<G>(acc);
   float rmax = 0.f;

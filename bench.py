@@ -53,18 +53,23 @@ def step_bytes(N: int, E: float, r: float) -> float:
     return N * (2 * 632 + 52 + 164 * r) + E * (2 * 60 + 4 * r)
 
 
-def measured_traffic(kernel_prefix: str):
-    """HBM bytes per dispatch of a kernel from the committed counter summary
-    (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration), or None."""
+def measured_traffic(*kernel_prefixes: str):
+    """HBM bytes per launch of an entry point (the sum over its kernels) from
+    the committed counter summary (FETCH_SIZE / WRITE_SIZE scaled by the copy
+    calibration), or None."""
     path = os.path.join(ROOT, "profiles", "crksph_step_shear2d_1M_hbm_counters.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         data = json.load(f)
-    for name, v in data.get("kernels", {}).items():
-        if name.startswith(kernel_prefix):
-            return v["traffic_bytes"]
-    return None
+    total, found = 0.0, 0
+    for prefix in kernel_prefixes:
+        for name, v in data.get("kernels", {}).items():
+            if name.startswith(prefix):
+                total += v["traffic_bytes"]
+                found += 1
+                break
+    return total if found == len(kernel_prefixes) else None
 
 
 def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
@@ -465,8 +470,11 @@ def main():
             achieved = bytes_launch / (mean_ms * 1e-3)
             kernel = {"accel_dudt": "crk::k_accel_dudt", "support": "crk::k_support", "factors_eos": "crk::k_moments",
                       "compatible_energy": "crk::k_compat"}.get(name, "crk::k_" + name)
+            # the acceleration entry point launches the record pack and the sweep
+            kernels = (kernel, "crk::k_pack_rec") if name == "accel_dudt" else (kernel,)
             roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kernel), "kernel": kernel,
+                        "frac": achieved / HBM_PEAK, "traffic": measured_traffic(*kernels),
+                        "kernel": " + ".join(kernels),
                         "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": mean_ms,
                         "bytes_per_particle": pb[0], "bytes_per_pair": pb[1], "pairs_per_launch": P_mean}
     rebuilds = summary.get("adjacency_build", (0, 0.0, 0.0))[0] / max(args.steps, 1) if summary else 0.0

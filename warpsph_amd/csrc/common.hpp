@@ -58,8 +58,12 @@ struct CrkContext {
   crk::Buf x, v, h, m, u, a0, divp, alpha, V, A, B, gA, gB, rho, P, c, sP, J, divcd, vsig, Rcd, dvdt, vh, vhs, du0;
   // packed neighbour data (sorted order): the acceleration pass's full
   // record (Rec<D>::NF floats) and the narrow float4 records of the lighter
-  // passes, geo = {x (3, zero padded), h}, vel = {v (3, zero padded), c}
-  crk::Buf rec, geo4, vel4;
+  // passes, geo = {x (3, zero padded), h}, vel = {v (3, zero padded), c};
+  // vm2 = {V, m} (written with V, read by the moments pass); vg4 and mr,
+  // the velocity-gradient pass's record (written by the moments pass):
+  // d < 3: vg4 = {v (2, zero padded), V, m / rho}; d = 3: vg4 = {v, V} and
+  // mr = m / rho
+  crk::Buf rec, geo4, vel4, vm2, vg4, mr;
   // in-support pairs of each row, compacted in row order (caller-owned
   // buffers of the stage, bound by crk_support)
   int32_t* pair_cnt = nullptr;

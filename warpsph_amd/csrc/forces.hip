@@ -14,6 +14,10 @@ namespace crk {
 // volumes).
 struct VgIn {
   const float *x, *h, *v, *V, *m, *rho, *c, *A, *B, *gA, *gB;
+  // neighbour records: geo = {x, h}; vg = {v, V, m / rho} (d < 3) or
+  // {v, V} with mr = m / rho (d = 3)
+  const float4 *geo4, *vg4;
+  const float* mr;
 };
 struct VgOut {
   float *J, *divcd, *vsig, *drhodt_u;
@@ -32,18 +36,28 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
   Comp acc[D * D];
   Comp dacc;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
+  // software pipeline: the neighbour's records one iteration ahead, its id two ahead
+  int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
+  int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
+  float4 gn = in.geo4[jc], wn = in.vg4[jc];
+  float mn = (D == 3 && CULLEN) ? in.mr[jc] : 0.f;
   for (int32_t q = b + L.g; q < e; q += G) {
-    const int32_t j = jn;
-    if (q + G < e) jn = rows.nbr[q + G];
+    const float4 gj = gn, wj = wn;
+    const float mrj = D < 3 ? wj.w : mn;
+    if (q + G < e) {
+      jc = jn;
+      gn = in.geo4[jc];
+      wn = in.vg4[jc];
+      if (D == 3 && CULLEN) mn = in.mr[jc];
+      if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+    }
     float dx[3];
-    pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
+    const float xj[3] = {gj.x, gj.y, gj.z};
+    pair_dx<D>(geo, xi, xj, dx);
     float r = fsqrt(dot<D>(dx, dx));
     float ir = inv_or_zero(r);
-    float vj[3];
-#pragma unroll
-    for (int d = 0; d < D; ++d) vj[d] = in.v[(int64_t)j * D + d];
-    float hj = in.h[j];
+    const float vj[3] = {wj.x, wj.y, wj.z};
+    float hj = gj.w;
     if (CULLEN) {
       float H = fmaxf(hi, hj);
       if (r < H) {
@@ -51,14 +65,14 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
         float dt_ = 0.f;
 #pragma unroll
         for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * dx[a];
-        dacc.add(in.m[j] * frcp(in.rho[j]) * (w.fac * dt_));
+        dacc.add(mrj * (w.fac * dt_));
       }
     }
     if (!(r < hj)) continue;
     WdW w = wdw<K, D>(C, r, ir, frcp(hj));
     float g[3];
     grad_wr<D>(cr, dx, w, g);
-    float Vj = in.V[j];
+    float Vj = D < 3 ? wj.z : wj.w;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       float dv = Vj * (vj[a] - vi[a]);
@@ -567,7 +581,7 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
   bool cullen = cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED;
   VgIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->V), F<float>(ctx->m),
           F<float>(ctx->rho), F<float>(ctx->c), F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA),
-          F<float>(ctx->gB)};
+          F<float>(ctx->gB), F<float4>(ctx->geo4), F<float4>(ctx->vg4), F<float>(ctx->mr)};
   VgOut out{F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig), up->drhodt};
   int GS = ctx->group[CRK_PASS_VELGRAD];
   if (cullen) {

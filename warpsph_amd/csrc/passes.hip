@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
                                                     float cfac, int32_t* __restrict__ anbr,
                                                     int32_t* __restrict__ acnt, float* __restrict__ vsig,
                                                     float* __restrict__ V, const float4* __restrict__ geo4,
-                                                    const float4* __restrict__ vel4) {
+                                                    const float4* __restrict__ vel4, const float* __restrict__ m,
+                                                    float2* __restrict__ vm2) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3], vi[3];
@@ -214,7 +215,9 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   float S = group_total<G>(vol);
   if (!L.valid || L.g != 0) return;
   acnt[k] = run;
-  V[k] = 1.f / S;
+  const float Vk = 1.f / S;
+  V[k] = Vk;
+  vm2[k] = make_float2(Vk, m[k]);
   if (VS) vsig[k] = vs;
 }
 
@@ -239,9 +242,10 @@ __device__ inline double lut_interp(const double* __restrict__ xs, const double*
 // Cullen-Dehnen switch, the signal speed) in a second sweep over the row,
 // whose neighbour data the first sweep has just brought into the caches.
 struct OwenFinal {
-  const float *v, *u;
+  const float *v, *u, *m;
   float cfac;
   float *V, *vsig;
+  float2* vm2;
 };
 
 template <int D, int G, int K, bool FINAL>
@@ -342,7 +346,9 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
     vs = group_max<G>(vs);
     float Sv = group_total<G>(vol);
     if (L.valid && L.g == 0) {
-      fin.V[k] = 1.f / Sv;
+      const float Vk = 1.f / Sv;
+      fin.V[k] = Vk;
+      fin.vm2[k] = make_float2(Vk, fin.m[k]);
       if (fin.vsig) fin.vsig[k] = vs;
     }
   }
@@ -473,6 +479,11 @@ struct MomOut {
   float *A, *B, *gA, *gB, *rho, *P, *c, *sP;
   float *dens_u, *ent_u, *pres_u, *cs_u;
   int32_t* nsing;
+  // neighbour records: {V, m} read; the velocity-gradient record written
+  const float2* vm2;
+  const float* v;
+  float4* vg4;
+  float* mr;
 };
 
 template <int D>
@@ -553,7 +564,8 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
       float r = fsqrt(dot<D>(dx, dx));
       float hj = gj.w;
       if (!(r < hj)) continue;
-      acc.template add<K>(C, dx, r, hj, V[j], m[j]);
+      const float2 vmj = o.vm2[j];
+      acc.template add<K>(C, dx, r, hj, vmj.x, vmj.y);
     }
   } else {
     const float hi = h[k];
@@ -578,7 +590,10 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
         float r = fsqrt(dot<D>(dx, dx));
         float hj = gj.w;
         act = r < fmaxf(hi, hj);
-        if (r < hj) acc.template add<K>(C, dx, r, hj, V[j], m[j]);
+        if (r < hj) {
+          const float2 vmj = o.vm2[j];
+          acc.template add<K>(C, dx, r, hj, vmj.x, vmj.y);
+        }
       }
       const unsigned long long bm = __ballot(act ? 1 : 0) & gmask;
       if (act) anbr[b + run + __popcll(bm & lt)] = j;
@@ -722,6 +737,21 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
   for (int a = 0; a < D; ++a) bs = bs + Bf[a] * s1[a];
   float rh = Af * bs;
   o.rho[k] = rh;
+  {
+    // the velocity-gradient pass's neighbour record (m / rho as that pass
+    // evaluates it)
+    const float2 vmk = o.vm2[k];
+    const float mrk = vmk.y * frcp(rh);
+    float vk[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; ++d) vk[d] = o.v[(int64_t)k * D + d];
+    if (D < 3) {
+      o.vg4[k] = make_float4(vk[0], vk[1], vmk.x, mrk);
+    } else {
+      o.vg4[k] = make_float4(vk[0], vk[1], vk[2], vmk.x);
+      o.mr[k] = mrk;
+    }
+  }
   float ui = u[k];
   float cc = sqrtf(fabsf(ui) * gamma * (gamma - 1.f));
   float P = (gamma - 1.f) * rh * ui;
@@ -810,8 +840,10 @@ static int alloc_stage(CrkContext* ctx, int32_t n, int D) {
   for (Buf* b : sdd) { int st = ensure(*b, f * D * D); if (st) return st; }
   { int st = ensure(ctx->rec, sizeof(float) * (size_t)n * (D == 1 ? Rec<1>::NF : D == 2 ? Rec<2>::NF : Rec<3>::NF));
     if (st) return st; }
-  Buf* s4[] = {&ctx->geo4, &ctx->vel4};
+  Buf* s4[] = {&ctx->geo4, &ctx->vel4, &ctx->vg4};
   for (Buf* b : s4) { int st = ensure(*b, f * 4); if (st) return st; }
+  { int st = ensure(ctx->vm2, f * 2); if (st) return st; }
+  { int st = ensure(ctx->mr, D == 3 ? f : 4); if (st) return st; }
   return CRK_OK;
 }
 
@@ -977,7 +1009,8 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   if (final_sweep) {
     float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)
                     ? F<float>(ctx->vsig) : nullptr;
-    OwenFinal fin{F<float>(ctx->v), F<float>(ctx->u), cfg->gamma * (cfg->gamma - 1.f), F<float>(ctx->V), vs};
+    OwenFinal fin{F<float>(ctx->v), F<float>(ctx->u), F<float>(ctx->m), cfg->gamma * (cfg->gamma - 1.f),
+                  F<float>(ctx->V), vs, F<float2>(ctx->vm2)};
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_owen<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
@@ -1059,12 +1092,14 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_support<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u), cfac,
-                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4)));
+                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4),
+                           F<float>(ctx->m), F<float2>(ctx->vm2)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_support<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u), cfac,
-                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4)));
+                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4),
+                           F<float>(ctx->m), F<float2>(ctx->vm2)));
   }
   CRK_LAUNCH_CHECK("k_support");
   ctx->pair_cnt = ed->count;
@@ -1084,7 +1119,8 @@ static int factors_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
   float C = (float)kernel_C(cfg->kernel, cfg->domain.dim);
   MomOut mo{F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->rho),
             F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->sP), st->densities, st->entropies, st->pressures,
-            st->soundspeeds, n_singular_dev};
+            st->soundspeeds, n_singular_dev, F<float2>(ctx->vm2), F<float>(ctx->v), F<float4>(ctx->vg4),
+            F<float>(ctx->mr)};
   int GM = ctx->group[CRK_PASS_MOMENTS];
   if (ed) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,

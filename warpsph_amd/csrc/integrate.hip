@@ -101,10 +101,25 @@ __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g,
                                   const float* vi, float mdt, float& sa, float& ca) {
   const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
   const float4 ri = s.rec[k];
-  int32_t jn = b + g < e ? s.nbr[b + g] : 0;  // neighbour ids loaded one iteration ahead
+  // software pipeline: the neighbour's records one iteration ahead, its id two ahead
+  int32_t jc = b + g < e ? s.nbr[b + g] : 0;
+  int32_t jn = b + g + G < e ? s.nbr[b + g + G] : 0;
+  float4 rn = s.rec[jc];
+  float vn[3];
+#pragma unroll
+  for (int c = 0; c < D; ++c) vn[c] = vhs[(int64_t)jc * D + c];
   for (int32_t q = b + g; q < e; q += G) {
-    const int32_t j = jn;
-    if (q + G < e) jn = s.nbr[q + G];
+    const float4 rj = rn;
+    float vj[3];
+#pragma unroll
+    for (int c = 0; c < D; ++c) vj[c] = vn[c];
+    if (q + G < e) {
+      jc = jn;
+      rn = s.rec[jc];
+#pragma unroll
+      for (int c = 0; c < D; ++c) vn[c] = vhs[(int64_t)jc * D + c];
+      if (q + 2 * G < e) jn = s.nbr[q + 2 * G];
+    }
     float a[3], d0 = 0.f;
     if (D == 2) {
       const float4 p = *reinterpret_cast<const float4*>(&s.pf[(int64_t)q * 4]);
@@ -115,8 +130,8 @@ __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g,
       for (int c = 0; c < D; ++c) a[c] = s.pf[(int64_t)q * 2 * D + c] + s.pf[(int64_t)q * 2 * D + D + c];
     }
 #pragma unroll
-    for (int c = 0; c < D; ++c) d0 = d0 + (vhs[(int64_t)j * D + c] - vi[c]) * a[c];
-    const float f = pair_fraction<D>(ri, s.rec[j], mdt, a);
+    for (int c = 0; c < D; ++c) d0 = d0 + (vj[c] - vi[c]) * a[c];
+    const float f = pair_fraction<D>(ri, rj, mdt, a);
     kahan_add(sa, ca, s.b * f * d0);
   }
 }

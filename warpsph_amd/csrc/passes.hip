@@ -553,18 +553,26 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
   MomAcc<D> acc;
   if (!COMPACT) {
     const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-    int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
+    // software pipeline: the neighbour's records one iteration ahead, its id two ahead
+    int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
+    int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
+    float4 gn = geo4[jc];
+    float2 vn = o.vm2[jc];
     for (int32_t q = b + L.g; q < e; q += G) {
-      const int32_t j = jn;
-      if (q + G < e) jn = rows.nbr[q + G];
+      const float4 gj = gn;
+      const float2 vmj = vn;
+      if (q + G < e) {
+        jc = jn;
+        gn = geo4[jc];
+        vn = o.vm2[jc];
+        if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+      }
       float dx[3];
-      const float4 gj = geo4[j];
       const float xj[3] = {gj.x, gj.y, gj.z};
       pair_dx<D>(geo, xi, xj, dx);
       float r = fsqrt(dot<D>(dx, dx));
       float hj = gj.w;
       if (!(r < hj)) continue;
-      const float2 vmj = o.vm2[j];
       acc.template add<K>(C, dx, r, hj, vmj.x, vmj.y);
     }
   } else {

@@ -396,19 +396,17 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
   const float ViOm = Vi / mi;
   Comp acc[D], accu, accs;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
-  // software pipeline: the neighbour's record one iteration ahead, its id two ahead
+  // neighbour ids loaded two iterations ahead (the record itself is loaded
+  // at the start of its iteration: prefetching it too costs more in
+  // occupancy than it hides)
   int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
   int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
-  float rn[Rec<D>::NF];
-  rec_load<D>(rec, jc, rn);
   for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
     const int32_t j = jc;
     float rj[Rec<D>::NF];
-#pragma unroll
-    for (int c = 0; c < Rec<D>::NF; ++c) rj[c] = rn[c];
+    rec_load<D>(rec, j, rj);
     if (q0 + G < e) {
       jc = jn;
-      rec_load<D>(rec, jc, rn);
       if (q0 + 2 * G < e) jn = rows.nbr[q0 + 2 * G];
     }
     float dx[3];

@@ -438,6 +438,10 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
                      const float* masses, float dt, float* ap_out, float* av_out, float* f_out,
                      void* stream);
 
+/* An intermediate of the bound stage in user order (for tests and
+ * diagnostics): 0 V, 1 A, 2 B, 3 grad A, 4 grad B, 5 grad v, 6 Cullen-Dehnen
+ * div, 7 v_sig, 8 R.  CRK_ESTATE for a field the stage has not computed
+ * (6-8 exist only with the Cullen-Dehnen or learned switch).              */
 int crk_export_field(CrkContext* ctx, int32_t field, float* out_user, void* stream);
 
 /* ----------------------------------------- learned viscosity switch (MLP)

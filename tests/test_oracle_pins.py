@@ -135,6 +135,53 @@ def _lattice_system(ns, lo, hi, periodic, vel_fn, switch=None, jitter=0.1, seed=
     return CompSPHSystem(state=st, adjacency=None, domain=dom, t=0.0), config, scheme
 
 
+def test_singular_moments_fall_back_to_zeroth_order():
+    """Rows whose second moment is singular: an isolated particle (itself
+    only) and three collinear particles.  The fallback A = 1/m0, B = 0 keeps
+    the zeroth-order condition sum_j V_j W^R_ij = 1; for the isolated
+    particle everything reduces to closed forms (V = 1/W(0), A = 1,
+    rho = m W(0), no force)."""
+    from warpsph_amd import sample
+    from warpsph_amd.configurations import CRKSPHConfig, buildConfig, toCrkConfig
+    from warpsph_amd.enumTypes import AdaptiveSupportScheme
+    from warpsph_amd.systems import CompSPHSystem
+    O = _oracle()
+    g = torch.Generator().manual_seed(11)
+    pos, dx = sample.lattice((12, 12), (0.0, 0.0), (1.0, 1.0), jitter=0.2, generator=g)
+    extra = torch.tensor([[3.0, 3.0], [3.0, 1.0], [3.02, 1.0], [3.04, 1.0]], dtype=torch.float32)
+    pos = torch.cat([pos, extra])
+    n = pos.shape[0]
+    vel = 0.1 * torch.sin(3.0 * pos)
+    rho = torch.ones(n, dtype=torch.float32)
+    dom = sample._domain([-1.0, -1.0], [4.0, 4.0], [False, False], "cpu")
+    config, _ = buildConfig(device=torch.device("cpu"), domain=dom, dim=2, targetNeighbors=32.0, cflFactor=0.3)
+    scheme = CRKSPHConfig(gamma=5.0 / 3.0, adaptiveSupportScheme=AdaptiveSupportScheme.NoScheme)
+    st = sample._state_from(pos, rho, vel, torch.ones(n), 5.0 / 3.0, dx, 2, 32.0, "cpu")
+    system = CompSPHSystem(state=st, adjacency=None, domain=dom, t=0.0)
+    sim, _ = _evaluate(system, config, scheme)
+    cfg = toCrkConfig(config, scheme)
+    A, B, gA, gB = sim.get("A"), sim.get("B").reshape(n, 2), sim.get("gA").reshape(n, 2), sim.get("gB").reshape(n, 2, 2)
+    h, m, V = sim.get("h"), sim.get("m"), sim.get("V")
+    W0, _ = _kernel(cfg.kernel, 2, np.zeros(1), h[n - 4])
+    iso = n - 4
+    assert np.diff(sim.adjacency(0)["offsets"])[iso] == 1  # itself only
+    assert V[iso] == pytest.approx(1.0 / W0[0], rel=1e-12)
+    assert A[iso] == pytest.approx(1.0, rel=1e-12)
+    assert np.all(B[iso] == 0.0) and np.all(gA[iso] == 0.0) and np.all(gB[iso] == 0.0)
+    assert sim.get("rho")[iso] == pytest.approx(m[iso] * W0[0], rel=1e-12)
+    assert np.all(sim.get("dvdt").reshape(n, 2)[iso] == 0.0) and sim.get("dudt")[iso] == 0.0
+    # the collinear triple: singular, zeroth order only
+    i, j, dxp, WR, gWR = _crk_gradients(sim, cfg.kernel, 2, [5.0, 5.0], [False, False])
+    s0 = np.bincount(i, weights=V[j] * WR, minlength=n)
+    for k in (n - 3, n - 2, n - 1):
+        assert np.diff(sim.adjacency(0)["offsets"])[k] == 3
+        assert np.all(B[k] == 0.0)
+        assert s0[k] == pytest.approx(1.0, abs=1e-12)
+    # the lattice rows stay first order
+    s1 = np.stack([np.bincount(i, weights=V[j] * dxp[:, a] * WR, minlength=n) for a in range(2)], 1)
+    assert np.max(np.abs(s1[:iso]) / h[:iso, None]) < 1e-12
+
+
 def test_velocity_gradient_of_linear_field_is_exact():
     M = np.array([[0.3, -1.2], [0.7, 0.25]])
     system, config, scheme = _lattice_system((16, 16), (0.0, 0.0), (1.0, 1.0), (False, False),

@@ -77,6 +77,8 @@ struct CrkContext {
   CrkState st{};
   bool bound = false;
   bool have_factors = false, have_velgrad = false, have_accel = false;
+  // Cullen-Dehnen intermediates computed in the bound stage (v_sig, div, R)
+  bool have_vsig = false, have_divcd = false, have_rcd = false;
   // Owen table
   crk::Buf lut_nh, lut_psiH;
   int32_t nlut = 0;

@@ -602,6 +602,7 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
   }
   CRK_LAUNCH_CHECK("k_velgrad");
   ctx->have_velgrad = true;
+  ctx->have_divcd = cullen;
   return CRK_OK;
 }
 
@@ -625,6 +626,7 @@ int crk_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float 
       hipLaunchKernelGGL((k_cullen<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, arows_of(ctx), geo, C,
                          n, cd_in(ctx), cd_params(ctx, cfg, dt), ctx->adj.perm, cd_out(ctx, st)));
   CRK_LAUNCH_CHECK("k_cullen");
+  ctx->have_rcd = true;
   return CRK_OK;
 }
 
@@ -671,6 +673,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   }
   CRK_LAUNCH_CHECK("k_accel_dudt");
   ctx->have_accel = true;
+  if (sw) ctx->have_rcd = true;
   return CRK_OK;
 }
 

@@ -894,6 +894,7 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   ctx->st = *st;
   ctx->bound = true;
   ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = ctx->have_volume = false;
+  ctx->have_vsig = ctx->have_divcd = ctx->have_rcd = false;
   ctx->check_hb = nullptr;
   ctx->owen_checked = false;
   hipStream_t s = (hipStream_t)stream;
@@ -934,6 +935,7 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
   ctx->st = *st;
   ctx->bound = true;
   ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = ctx->have_volume = false;
+  ctx->have_vsig = ctx->have_divcd = ctx->have_rcd = false;
   ctx->check_hb = hb;
   ctx->owen_checked = false;
   hipStream_t s = (hipStream_t)stream;
@@ -1025,6 +1027,7 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
                            hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin, run_ids(ctx)));
     ctx->have_volume = true;
+    ctx->have_vsig = vs != nullptr;
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_owen<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
@@ -1114,6 +1117,7 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   ctx->pair_nbr = ed->nbr;
   ctx->have_support = true;
   ctx->have_volume = true;
+  ctx->have_vsig = vs != nullptr;
   return CRK_OK;
 }
 
@@ -1201,6 +1205,13 @@ int crk_export_field(CrkContext* ctx, int32_t field, float* out_user, void* stre
   int D = ctx->dim;
   const float* src = nullptr;
   int comps = 1;
+  const bool have[9] = {ctx->have_volume, ctx->have_factors, ctx->have_factors, ctx->have_factors, ctx->have_factors,
+                        ctx->have_velgrad, ctx->have_divcd, ctx->have_vsig, ctx->have_rcd};
+  if (field >= 0 && field < 9 && !have[field]) {
+    set_error("crk_export_field: field %d was not computed in the bound stage (v_sig, div and R only with the "
+              "Cullen-Dehnen switch)", field);
+    return CRK_ESTATE;
+  }
   switch (field) {
     case 0: src = F<float>(ctx->V); break;
     case 1: src = F<float>(ctx->A); break;

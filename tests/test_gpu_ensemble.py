@@ -82,13 +82,15 @@ def _fields(st):
             "h": st.supports, "alpha": st.alphas}
 
 
-def test_ensemble_steps_match_single_runs():
+# the second set is ragged: a 144-particle run between runs 16x larger
+@pytest.mark.parametrize("sides", [SIDES, (12, 48, 16)])
+def test_ensemble_steps_match_single_runs(sides):
     from warpsph_amd import sample
     from warpsph_amd.ensemble import cfl_dt, run_system
     from warpsph_amd.integrator import getIntegrator
     from warpsph_amd.scheme import crkSPH_step
-    announce("ensemble steps vs single runs")
-    ens, parts, config, scheme = make_ensemble()
+    announce(f"ensemble steps vs single runs, sides {sides}")
+    ens, parts, config, scheme = make_ensemble(sides)
     integ = getIntegrator()
     singles = [run_system(ens, r) for r in range(len(parts))]
     nsteps = 3

@@ -179,10 +179,13 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
 // Neighbour rows: for sorted particle k, every particle of the 3^D cells
 // around its cell with |x_k - x_j| < s max(h_k, h_j) (exact arithmetic),
 // cells in scan order and candidates ascending; count pass (FILL = false)
-// then fill pass.  G lanes per row: the lanes of a group test
-// consecutive candidates of a cell, and the hits are compacted with a group
-// ballot, so the row is written with coalesced stores in the same order as
-// the one-thread-per-row scan (cells in scan order, candidates ascending).
+// then fill pass.  G lanes per row (G >= 3^D): lane t of the group first
+// looks up cell t's particle range (all cells of the row in one round of
+// loads instead of one dependent load per cell); then the lanes test
+// consecutive candidates of each cell in turn, and the hits are compacted
+// with a group ballot, so the row is written with coalesced stores in the
+// same order as the one-thread-per-row scan (cells in scan order, candidates
+// ascending).
 template <int D, int G, bool FILL, bool RUNS>
 __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0, Geo geo, float s,
                                                         const float* __restrict__ xb, const float* __restrict__ hb,
@@ -190,6 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
                                                         int32_t n, int32_t* __restrict__ counts_or_offsets,
                                                         int32_t* __restrict__ nbr, const GridRun* __restrict__ runs,
                                                         const int32_t* __restrict__ run_sorted) {
+  static_assert(G >= (D == 1 ? 3 : (D == 2 ? 9 : 27)), "a group needs a lane per cell of the scan");
   const Lane L = lane_of<G>(n);
   if (!L.valid) return;  // whole groups only: every lane of a group shares its row
   const int32_t k = L.k;
@@ -212,38 +216,72 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
   const int gbase = lane & ~(G - 1);
   const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
   const unsigned long long lt = (1ull << lane) - 1ull;
+  // this lane's cell of the scan: t = ox + cnt0 (oy + cnt1 oz)
+  const int ncell = sc.cnt[0] * sc.cnt[1] * sc.cnt[2];
+  int32_t cb0 = 0, ce0 = 0;
+  if (L.g < ncell) {
+    const int t = L.g;
+    const int ox = t % sc.cnt[0], oy = (t / sc.cnt[0]) % sc.cnt[1], oz = t / (sc.cnt[0] * sc.cnt[1]);
+    const int o[3] = {sc.off[0][ox], sc.off[1][oy], sc.off[2][oz]};
+    int cc[3] = {0, 0, 0};
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      int q = c[d] + o[d];
+      int nn = g.ncell[d];
+      if (g.per[d]) { q %= nn; if (q < 0) q += nn; }
+      else if (q < 0 || q >= nn) ok = false;
+      cc[d] = q;
+    }
+    if (ok) {
+      const uint32_t code = base + morton<D>(cc);
+      cb0 = cs[code];
+      ce0 = ce[code];
+    }
+  }
   int32_t run = 0;
   const int64_t out = FILL ? (int64_t)counts_or_offsets[k] : 0;
-  for (int oz = 0; oz < sc.cnt[2]; ++oz)
-    for (int oy = 0; oy < sc.cnt[1]; ++oy)
-      for (int ox = 0; ox < sc.cnt[0]; ++ox) {
-        int o[3] = {sc.off[0][ox], sc.off[1][oy], sc.off[2][oz]};
-        int cc[3] = {0, 0, 0};
-        bool ok = true;
+  // the steps (cell t, candidates cb .. cb + G - 1) in scan order; each
+  // step's candidate data is loaded one step ahead (t, cb and e are uniform
+  // over the group, so are the loop trip counts)
+  int t = 0;
+  int32_t cb = __shfl(cb0, gbase), e = __shfl(ce0, gbase);
+  while (cb >= e && ++t < ncell) { cb = __shfl(cb0, gbase + t); e = __shfl(ce0, gbase + t); }
+  int32_t nk = cb + L.g;
+  bool nval = t < ncell && nk < e;
+  float nx[3] = {0.f, 0.f, 0.f}, nh = 0.f;
+  if (nval) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          int q = c[d] + o[d];
-          int nn = g.ncell[d];
-          if (g.per[d]) { q %= nn; if (q < 0) q += nn; }
-          else if (q < 0 || q >= nn) ok = false;
-          cc[d] = q;
-        }
-        if (!ok) continue;
-        const uint32_t code = base + morton<D>(cc);
-        const int32_t b = cs[code], e = ce[code];
-        for (int32_t cb = b; cb < e; cb += G) {
-          const int32_t kk = cb + L.g;
-          bool hit = false;
-          if (kk < e) {
-            const float r2 = exact_r2<D>(geo, xi, &xb[(int64_t)kk * D]);
-            const float R = __fmul_rn(s, fmaxf(hi, hb[kk]));
-            hit = r2 < __fmul_rn(R, R);
-          }
-          const unsigned long long m = __ballot(hit ? 1 : 0) & gmask;
-          if (FILL && hit) nbr[out + run + __popcll(m & lt)] = kk;
-          run += __popcll(m);
-        }
-      }
+    for (int d = 0; d < D; ++d) nx[d] = xb[(int64_t)nk * D + d];
+    nh = hb[nk];
+  }
+  while (t < ncell) {
+    const int32_t kk = nk;
+    const bool val = nval;
+    float xj[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xj[d] = nx[d];
+    const float hj = nh;
+    // the next step
+    cb += G;
+    while (cb >= e && ++t < ncell) { cb = __shfl(cb0, gbase + t); e = __shfl(ce0, gbase + t); }
+    nk = cb + L.g;
+    nval = t < ncell && nk < e;
+    if (nval) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) nx[d] = xb[(int64_t)nk * D + d];
+      nh = hb[nk];
+    }
+    bool hit = false;
+    if (val) {
+      const float r2 = exact_r2<D>(geo, xi, xj);
+      const float R = __fmul_rn(s, fmaxf(hi, hj));
+      hit = r2 < __fmul_rn(R, R);
+    }
+    const unsigned long long m = __ballot(hit ? 1 : 0) & gmask;
+    if (FILL && hit) nbr[out + run + __popcll(m & lt)] = kk;
+    run += __popcll(m);
+  }
   if (!FILL && L.g == 0) counts_or_offsets[k] = run;
 }
 // lanes per row of the neighbour scan (candidates per row: ~9, ~180, ~830 in 1D/2D/3D)

@@ -237,14 +237,19 @@ int crk_grid_plan(const CrkConfig* cfg, float h_max, const float* lo, const floa
  * Step 1 (blocking: returns the edge count):
  *   keys[n] (sorted), perm[n] (sorted->user), rank[n] (user->sorted),
  *   cell_start/cell_end[grid.table_size], row_offsets[n+1], xb[n*dim], hb[n]
- *   (positions and supports at build time, sorted order).                  */
+ *   (positions and supports at build time, sorted order).  The candidates are
+ *   tested once: the rows are also collected into context scratch (a fixed
+ *   number of slots per row, about 2.5 n_H verletScale^dim), which step 2
+ *   copies; a build with a longer row tests them again in step 2 (and the
+ *   next build gets more slots).                                            */
 int crk_adjacency_count(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid,
                         int32_t n, const float* x, const float* h,
                         uint32_t* keys, int32_t* perm, int32_t* rank,
                         int32_t* cell_start, int32_t* cell_end,
                         int32_t* row_offsets, float* xb, float* hb,
                         int64_t* n_edges, void* stream);
-/* Step 2: fills nbr[n_edges] (sorted neighbour indices). */
+/* Step 2: fills nbr[n_edges] (sorted neighbour indices).  Non-blocking
+ * after step 1 of the same row_offsets on this context.                   */
 int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid,
                        int32_t n, const int32_t* cell_start, const int32_t* cell_end,
                        const int32_t* row_offsets, const float* xb, const float* hb,

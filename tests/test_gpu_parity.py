@@ -116,6 +116,31 @@ def test_adjacency_bitexact(name):
     assert np.array_equal(j, ref["nbr"])
 
 
+def test_adjacency_rows_longer_than_the_collection_slots():
+    """Rows longer than the single-pass scratch slots (supports 2.5x larger
+    than n_H implies): the build falls back to testing the candidates again
+    and stays bit-exact; later builds get more slots and stay exact too."""
+    from oracle import oracle as O
+    from warpsph_amd.configurations import toCrkConfig
+    from warpsph_amd.runtime import build_adjacency
+    announce("adjacency with overlong rows")
+    system, config, scheme = make_case("shear2d")
+    st = system.state
+    st.supports.mul_(2.5)
+    cfg = toCrkConfig(config, scheme)
+    sim = O.from_system(cpu_copy(system, config, scheme), config, scheme, 32)
+    sim.build_adjacency()
+    ref = sim.adjacency(-1)
+    for _ in range(3):
+        adj = build_adjacency(cfg, st.positions, st.supports)
+        torch.cuda.synchronize()
+        off, j, _ = user_rows(adj)
+        assert adj.n_edges == ref["n_edges"]
+        assert np.array_equal(off, ref["offsets"])
+        assert np.array_equal(j, ref["nbr"])
+    assert int(np.diff(ref["offsets"]).max()) > 2.5 * 32 * 2  # longer than the first slots
+
+
 STATE_FIELDS = [("supports", "h"), ("densities", "rho"), ("pressures", "P"), ("soundspeeds", "c"),
                 ("entropies", "s"), ("alphas", "alpha"), ("divergence", "div")]
 EXPORT_FIELDS = [(0, "V"), (1, "A"), (2, "B"), (3, "gA"), (4, "gB"), (5, "J"), (6, "divcd"), (7, "vsig")]

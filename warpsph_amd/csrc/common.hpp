@@ -64,6 +64,15 @@ struct CrkContext {
   // d < 3: vg4 = {v (2, zero padded), V, m / rho}; d = 3: vg4 = {v, V} and
   // mr = m / rho
   crk::Buf rec, geo4, vel4, vm2, vg4, mr;
+  // single-pass neighbour search: rows collected by crk_adjacency_count
+  // into nbr_tmp (nbr_cap slots per row) for crk_adjacency_fill to copy
+  crk::Buf nbr_tmp;
+  int32_t nbr_cap = 0;       // slots per row for the next build
+  int32_t nbr_cap_used = 0;  // slots per row of the collected build
+  const int32_t* collect_rows = nullptr;  // row_offsets of the collected build
+  int32_t collect_n = 0;
+  int64_t collect_E = -1;
+  bool collect_ok = false;  // no row exceeded nbr_cap
   // in-support pairs of each row, compacted in row order (caller-owned
   // buffers of the stage, bound by crk_support)
   int32_t* pair_cnt = nullptr;

@@ -178,22 +178,28 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
 
 // Neighbour rows: for sorted particle k, every particle of the 3^D cells
 // around its cell with |x_k - x_j| < s max(h_k, h_j) (exact arithmetic),
-// cells in scan order and candidates ascending; count pass (FILL = false)
-// then fill pass.  G lanes per row (G >= 3^D): lane t of the group first
+// cells in scan order and candidates ascending.  MODE 2 (crk_adjacency_count)
+// counts each row and writes it to its own cap slots of a scratch array (a
+// row longer than cap sets *overflow); MODE 1 (crk_adjacency_fill when some
+// row overflowed, or for rows counted elsewhere) fills the rows at their
+// offsets by testing the candidates again.  G lanes per row (G >= 3^D): lane t of the group first
 // looks up cell t's particle range (all cells of the row in one round of
 // loads instead of one dependent load per cell); then the lanes test
 // consecutive candidates of each cell in turn, and the hits are compacted
 // with a group ballot, so the row is written with coalesced stores in the
 // same order as the one-thread-per-row scan (cells in scan order, candidates
 // ascending).
-template <int D, int G, bool FILL, bool RUNS>
+template <int D, int G, int MODE, bool RUNS>
 __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0, Geo geo, float s,
                                                         const float* __restrict__ xb, const float* __restrict__ hb,
                                                         const int32_t* __restrict__ cs, const int32_t* __restrict__ ce,
                                                         int32_t n, int32_t* __restrict__ counts_or_offsets,
                                                         int32_t* __restrict__ nbr, const GridRun* __restrict__ runs,
-                                                        const int32_t* __restrict__ run_sorted) {
+                                                        const int32_t* __restrict__ run_sorted, int32_t cap,
+                                                        int32_t* __restrict__ overflow) {
+  constexpr bool FILL = MODE == 1;
   static_assert(G >= (D == 1 ? 3 : (D == 2 ? 9 : 27)), "a group needs a lane per cell of the scan");
+  static_assert(MODE == 1 || MODE == 2, "fill (1) or single pass (2)");
   const Lane L = lane_of<G>(n);
   if (!L.valid) return;  // whole groups only: every lane of a group shares its row
   const int32_t k = L.k;
@@ -240,7 +246,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
     }
   }
   int32_t run = 0;
-  const int64_t out = FILL ? (int64_t)counts_or_offsets[k] : 0;
+  const int64_t out = FILL ? (int64_t)counts_or_offsets[k] : (MODE == 2 ? (int64_t)k * cap : 0);
   // the steps (cell t, candidates cb .. cb + G - 1) in scan order; each
   // step's candidate data is loaded one step ahead (t, cb and e are uniform
   // over the group, so are the loop trip counts)
@@ -280,10 +286,33 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
     }
     const unsigned long long m = __ballot(hit ? 1 : 0) & gmask;
     if (FILL && hit) nbr[out + run + __popcll(m & lt)] = kk;
+    if (MODE == 2 && hit) {
+      const int32_t q = run + __popcll(m & lt);
+      if (q < cap) nbr[out + q] = kk;
+    }
     run += __popcll(m);
   }
-  if (!FILL && L.g == 0) counts_or_offsets[k] = run;
+  if (MODE == 2 && L.g == 0) {
+    counts_or_offsets[k] = run;
+    if (run > cap) atomicOr(overflow, 1);
+  }
 }
+
+// crk_adjacency_fill after a single-pass count: row k's collected entries
+// (cap slots per row) to its CSR offsets; G lanes per row
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_rows_copy(const int32_t* __restrict__ tmp, int32_t cap,
+                                                      const int32_t* __restrict__ off, int32_t n,
+                                                      int32_t* __restrict__ nbr) {
+  const Lane L = lane_of<G>(n);
+  if (!L.valid) return;
+  const int32_t k = L.k;
+  const int32_t b = off[k], e = off[k + 1];
+  const int32_t* src = tmp + (int64_t)k * cap - b;
+  for (int32_t q = b + L.g; q < e; q += G) nbr[q] = src[q];
+}
+
+__global__ void k_set_total(const int64_t* __restrict__ total, int32_t* __restrict__ dst) { *dst = (int32_t)*total; }
 // lanes per row of the neighbour scan (candidates per row: ~9, ~180, ~830 in 1D/2D/3D)
 template <int D>
 constexpr int nbr_group() { return D == 1 ? 4 : (D == 2 ? 16 : 32); }
@@ -415,29 +444,50 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
   constexpr int GN = nbr_group<D>();
-  if (runs)
-    hipLaunchKernelGGL((k_neighbors_g<D, GN, false, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc,
-                       geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr, gr,
-                       run);
-  else
-    hipLaunchKernelGGL((k_neighbors_g<D, GN, false, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc,
-                       geo, cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, (int32_t*)nullptr,
-                       gr, run);
-  CRK_LAUNCH_CHECK("k_neighbors(count)");
-  int64_t* total = (int64_t*)ctx->pinned.p;
+  // single pass: rows collected into cap slots each (cap from the expected
+  // row length, grown after an overflow; the overflowing build falls back to
+  // the fill pass)
+  if (ctx->nbr_cap <= 0) {
+    double sd = 1.0;
+    for (int d = 0; d < D; ++d) sd *= (double)cfg->verletScale;
+    double expect = 2.5 * (double)cfg->targetNeighbors * sd;
+    ctx->nbr_cap = (int32_t)std::min(1024.0, std::max(32.0, std::ceil(expect / 32.0) * 32.0));
+  }
+  const int32_t cap = ctx->nbr_cap;
+  if ((st = ensure(ctx->nbr_tmp, sizeof(int32_t) * (size_t)n * (size_t)cap))) return st;
   if ((st = ensure(ctx->s_red, 64))) return st;
   int64_t* total_dev = (int64_t*)ctx->s_red.p;
+  int32_t* overflow_dev = (int32_t*)((char*)ctx->s_red.p + 8);
+  if ((st = fill_i32(overflow_dev, 1, 0, s))) return st;
+  int32_t* tmp = (int32_t*)ctx->nbr_tmp.p;
+  if (runs)
+    hipLaunchKernelGGL((k_neighbors_g<D, GN, 2, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
+                       cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, cap,
+                       overflow_dev);
+  else
+    hipLaunchKernelGGL((k_neighbors_g<D, GN, 2, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
+                       cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, cap,
+                       overflow_dev);
+  CRK_LAUNCH_CHECK("k_neighbors(count)");
   if ((st = scan_exclusive_i32(ctx, row_offsets, row_offsets, n, total_dev, s))) return st;
-  CRK_HIP(hipMemcpyAsync(total, total_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, s, total_dev, row_offsets + n);
+  CRK_LAUNCH_CHECK("k_set_total");
+  if ((st = ensure_pinned(ctx, 16))) return st;
+  int64_t* host = (int64_t*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(host, total_dev, 12, hipMemcpyDeviceToHost, s));  // total, overflow flag
   CRK_HIP(hipStreamSynchronize(s));
-  int64_t E = *total;
+  const int64_t E = host[0];
+  const int32_t overflow = ((const int32_t*)host)[2];
   if (E > (int64_t)INT32_MAX) {
     set_error("adjacency has %lld edges; more than 2^31-1 is not supported", (long long)E);
     return CRK_ECAPACITY;
   }
-  int32_t e32 = (int32_t)E;
-  CRK_HIP(hipMemcpyAsync(row_offsets + n, &e32, sizeof(int32_t), hipMemcpyHostToDevice, s));
-  CRK_HIP(hipStreamSynchronize(s));
+  ctx->collect_rows = row_offsets;
+  ctx->collect_n = n;
+  ctx->collect_E = E;
+  ctx->collect_ok = overflow == 0;
+  ctx->nbr_cap_used = cap;
+  if (overflow) ctx->nbr_cap = std::min<int32_t>(4096, cap + cap / 2 + 32);
   *n_edges = E;
   return CRK_OK;
 }
@@ -483,9 +533,14 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
   CRK_REQUIRE(ctx && cfg && grid && cell_start && cell_end && row_offsets && xb && hb, "crk_adjacency_fill: null argument");
   CRK_REQUIRE(n > 0, "n must be positive");
   hipStream_t s = (hipStream_t)stream;
+  const bool collected = ctx->collect_rows == row_offsets && ctx->collect_n == n && ctx->collect_E >= 0;
   int32_t E = 0;
-  CRK_HIP(hipMemcpyAsync(&E, row_offsets + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  CRK_HIP(hipStreamSynchronize(s));
+  if (collected) {
+    E = (int32_t)ctx->collect_E;
+  } else {
+    CRK_HIP(hipMemcpyAsync(&E, row_offsets + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CRK_HIP(hipStreamSynchronize(s));
+  }
   if ((int64_t)E > capacity) {
     set_error("crk_adjacency_fill: %d edges exceed the capacity %lld", E, (long long)capacity);
     return CRK_ECAPACITY;
@@ -496,19 +551,31 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
   GridDev g = make_griddev(*grid, *cfg);
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
+  if (collected) {
+    ctx->collect_rows = nullptr;  // one fill per count
+    ctx->collect_E = -1;
+    if (ctx->collect_ok) {
+      if (E > 0) {
+        hipLaunchKernelGGL(k_rows_copy<16>, dim3(group_blocks(n, 16)), dim3(kBlock), 0, s,
+                           (const int32_t*)ctx->nbr_tmp.p, ctx->nbr_cap_used, row_offsets, n, nbr);
+        CRK_LAUNCH_CHECK("k_rows_copy");
+      }
+      return CRK_OK;
+    }
+  }
   // with runs bound: the per-run grids uploaded by crk_adjacency_count
   const GridRun* gr = ctx->n_runs > 0 ? (const GridRun*)ctx->s_gridruns.p : nullptr;
   const int32_t* run = run_ids(ctx);
   if (gr) {
     CRK_DIM_DISPATCH(cfg->domain.dim,
-        hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), true, true>), dim3(group_blocks(n, nbr_group<DD>())),
+        hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), 1, true>), dim3(group_blocks(n, nbr_group<DD>())),
                            dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
-                           (int32_t*)row_offsets, nbr, gr, run));
+                           (int32_t*)row_offsets, nbr, gr, run, 0, (int32_t*)nullptr));
   } else {
     CRK_DIM_DISPATCH(cfg->domain.dim,
-        hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), true, false>), dim3(group_blocks(n, nbr_group<DD>())),
+        hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), 1, false>), dim3(group_blocks(n, nbr_group<DD>())),
                            dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
-                           (int32_t*)row_offsets, nbr, gr, run));
+                           (int32_t*)row_offsets, nbr, gr, run, 0, (int32_t*)nullptr));
   }
   CRK_LAUNCH_CHECK("k_neighbors(fill)");
   return CRK_OK;

@@ -239,9 +239,10 @@ int crk_grid_plan(const CrkConfig* cfg, float h_max, const float* lo, const floa
  *   cell_start/cell_end[grid.table_size], row_offsets[n+1], xb[n*dim], hb[n]
  *   (positions and supports at build time, sorted order).  The candidates are
  *   tested once: the rows are also collected into context scratch (a fixed
- *   number of slots per row, about 2.5 n_H verletScale^dim), which step 2
- *   copies; a build with a longer row tests them again in step 2 (and the
- *   next build gets more slots).                                            */
+ *   number of slots per row, about 2.5 n_H verletScale^dim; at most 16 GiB),
+ *   which step 2 copies; a build with a longer row, or one too large for the
+ *   scratch, tests them again in step 2 (after an overflow the next build
+ *   gets more slots).                                                       */
 int crk_adjacency_count(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid,
                         int32_t n, const float* x, const float* h,
                         uint32_t* keys, int32_t* perm, int32_t* rank,

@@ -180,8 +180,9 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
 // around its cell with |x_k - x_j| < s max(h_k, h_j) (exact arithmetic),
 // cells in scan order and candidates ascending.  MODE 2 (crk_adjacency_count)
 // counts each row and writes it to its own cap slots of a scratch array (a
-// row longer than cap sets *overflow); MODE 1 (crk_adjacency_fill when some
-// row overflowed, or for rows counted elsewhere) fills the rows at their
+// row longer than cap sets *overflow); MODE 0 only counts (when the scratch
+// would be too large); MODE 1 (crk_adjacency_fill after an overflow or a
+// count-only pass, or for rows counted elsewhere) fills the rows at their
 // offsets by testing the candidates again.  G lanes per row (G >= 3^D): lane t of the group first
 // looks up cell t's particle range (all cells of the row in one round of
 // loads instead of one dependent load per cell); then the lanes test
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
                                                         int32_t* __restrict__ overflow) {
   constexpr bool FILL = MODE == 1;
   static_assert(G >= (D == 1 ? 3 : (D == 2 ? 9 : 27)), "a group needs a lane per cell of the scan");
-  static_assert(MODE == 1 || MODE == 2, "fill (1) or single pass (2)");
+  static_assert(MODE >= 0 && MODE <= 2, "count (0), fill (1) or single pass (2)");
   const Lane L = lane_of<G>(n);
   if (!L.valid) return;  // whole groups only: every lane of a group shares its row
   const int32_t k = L.k;
@@ -292,11 +293,14 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
     }
     run += __popcll(m);
   }
-  if (MODE == 2 && L.g == 0) {
+  if (MODE != 1 && L.g == 0) {
     counts_or_offsets[k] = run;
-    if (run > cap) atomicOr(overflow, 1);
+    if (MODE == 2 && run > cap) atomicOr(overflow, 1);
   }
 }
+
+// largest scratch of the single-pass search (rows x slots x 4 bytes)
+constexpr double kCollectMaxBytes = 16.0 * (1ull << 30);
 
 // crk_adjacency_fill after a single-pass count: row k's collected entries
 // (cap slots per row) to its CSR offsets; G lanes per row
@@ -454,19 +458,30 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
     ctx->nbr_cap = (int32_t)std::min(1024.0, std::max(32.0, std::ceil(expect / 32.0) * 32.0));
   }
   const int32_t cap = ctx->nbr_cap;
-  if ((st = ensure(ctx->nbr_tmp, sizeof(int32_t) * (size_t)n * (size_t)cap))) return st;
+  // the scratch is bounded (kCollectMaxBytes): larger builds count only and
+  // test the candidates again in the fill
+  const bool collect = (double)n * (double)cap * sizeof(int32_t) <= kCollectMaxBytes;
+  if (collect && (st = ensure(ctx->nbr_tmp, sizeof(int32_t) * (size_t)n * (size_t)cap))) return st;
   if ((st = ensure(ctx->s_red, 64))) return st;
   int64_t* total_dev = (int64_t*)ctx->s_red.p;
   int32_t* overflow_dev = (int32_t*)((char*)ctx->s_red.p + 8);
-  if ((st = fill_i32(overflow_dev, 1, 0, s))) return st;
-  int32_t* tmp = (int32_t*)ctx->nbr_tmp.p;
-  if (runs)
+  if ((st = fill_i32(overflow_dev, 1, collect ? 0 : 1, s))) return st;
+  int32_t* tmp = collect ? (int32_t*)ctx->nbr_tmp.p : nullptr;
+  if (collect && runs)
     hipLaunchKernelGGL((k_neighbors_g<D, GN, 2, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
                        cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, cap,
                        overflow_dev);
-  else
+  else if (collect)
     hipLaunchKernelGGL((k_neighbors_g<D, GN, 2, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
                        cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, cap,
+                       overflow_dev);
+  else if (runs)
+    hipLaunchKernelGGL((k_neighbors_g<D, GN, 0, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
+                       cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, 0,
+                       overflow_dev);
+  else
+    hipLaunchKernelGGL((k_neighbors_g<D, GN, 0, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
+                       cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, 0,
                        overflow_dev);
   CRK_LAUNCH_CHECK("k_neighbors(count)");
   if ((st = scan_exclusive_i32(ctx, row_offsets, row_offsets, n, total_dev, s))) return st;
@@ -487,7 +502,7 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   ctx->collect_E = E;
   ctx->collect_ok = overflow == 0;
   ctx->nbr_cap_used = cap;
-  if (overflow) ctx->nbr_cap = std::min<int32_t>(4096, cap + cap / 2 + 32);
+  if (overflow && collect) ctx->nbr_cap = std::min<int32_t>(4096, cap + cap / 2 + 32);
   *n_edges = E;
   return CRK_OK;
 }

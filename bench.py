@@ -433,8 +433,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs the CRKSPH step on one GPU only; "
-                                   "--gpus N with N > 1 is not supported (see DESIGN.md for the sharding design)"}))
+        print(json.dumps({"error": "the CRKSPH step runs on one GPU only; --gpus N with N > 1 is not "
+                                   "supported (domain decomposition over GPUs is not built)"}))
         sys.exit(2)
 
     if args.workload == "blast3d":

@@ -44,6 +44,21 @@ def _fill_defaults(st: CompSPHState):
         st.alphas = torch.ones_like(z)
 
 
+_FALSE = {}
+
+
+def _no_passive(n: int, device) -> torch.Tensor:
+    """The update's passive mask (no passive particles): an n-element view of
+    one cached False per device (no fill per stage; writes to it raise)."""
+    key = str(device)
+    z = _FALSE.get(key)
+    if z is None:
+        z = torch.zeros(1, dtype=torch.bool, device=device)
+        if not torch.cuda.is_current_stream_capturing():  # (graph pool memory is not kept)
+            _FALSE[key] = z
+    return z.expand(n)
+
+
 def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, compParams: CRKSPHConfig,
                 verbose: bool = False):
     """One CRKSPH derivative evaluation (crkSPH.py:19-366).
@@ -172,7 +187,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         dudt=torch.empty_like(st.internalEnergies),
         dEdt=torch.empty_like(st.internalEnergies),
         drhodt=torch.empty_like(st.internalEnergies),
-        passive=torch.zeros(n, dtype=torch.bool, device=st.positions.device))
+        passive=_no_passive(n, st.positions.device))
     cu = update.cabi()
     # velocity gradient (crkSPH.py:158-172)
     with timing.scope("velocity_gradient"):

@@ -88,6 +88,13 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     dim = int(config.domain.dim)
     if st.positions.dim() != 2 or st.positions.shape[1] != dim:
         raise ValueError(f"positions must be [N, {dim}]")
+    if getattr(compParams, "boundaryConditions", None):
+        # enforceDirichlet / computeForcing / enforceUpdates (bcs.py:16-92,
+        # called at crkSPH.py:94, 327-343) act between the fused passes of
+        # this step; with an empty list (every configuration here) they are
+        # no-ops, and a non-empty list is refused rather than ignored
+        raise NotImplementedError("boundary conditions (compParams.boundaryConditions) are not supported by the "
+                                  "HIP CRKSPH step")
     cfg = toCrkConfig(config, compParams)
     rt = runtime(st.positions.device)
     s = rt.stream()

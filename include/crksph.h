@@ -338,6 +338,20 @@ int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* 
 /* Re-bind to a new adjacency mid-stage (after a rebuild), keeping h. */
 int crk_stage_rebind(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* adj,
                      const CrkState* st, void* stream);
+/* Boundary-condition hooks (modules/boundaryConditions/bcs.py:16-92) run on
+ * the caller's user-order tensors between passes; these two calls hand their
+ * changes back to the stage.
+ * crk_state_refresh: after crk_factors_eos and enforceDirichlet
+ * (crkSPH.py:94-101): re-reads the state (positions, velocities, supports,
+ * masses, energies, alphas, divergence, densities), evaluates the
+ * equation of state (idealGas.py:6-38) from the densities and energies,
+ * writing st->entropies, pressures and soundspeeds, and (with the switch)
+ * the signal speed from the new velocities and sound speeds.
+ * crk_update_refresh: after crk_accel_dudt and computeForcing /
+ * enforceUpdates (crkSPH.py:327-339): re-reads update->dvdt for the energy
+ * partition (crk_balance).                                                 */
+int crk_state_refresh(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, void* stream);
+int crk_update_refresh(CrkContext* ctx, const CrkUpdate* up, void* stream);
 /* Owen adaptive support, one iteration (optimalSupportOwen.py:70-114):
  * psi0 pass over the bound adjacency, LUT inversion, computeNewSupport,
  * clamp to [hMin/4, 4 hMax].  hmin/hmax are the bounds of the supports at

@@ -1021,24 +1021,45 @@ struct Sim {
     });
   }
 
-  // one derivative evaluation: crkSPH_step, crkSPH.py:19-366
-  int evaluate(R dt) {
+  // one derivative evaluation: crkSPH_step, crkSPH.py:19-366, in three
+  // phases with the reference's boundary-condition hooks between them:
+  // 0 support, adjacency, CRK factors (:40-61); [enforceDirichlet, :94]
+  // 1 EOS, velocity gradient, switch, accel and dudt (:95-325);
+  //   [computeForcing, enforceUpdates, :327-339]
+  // 2 energy partition (:341-364)
+  int evaluatePhase(int phase, R dt) {
     int st;
-    if (cfg.adaptiveSupportScheme == CRK_SUPPORT_OWEN) {
-      st = owen();
+    if (phase == 0) {
+      if (cfg.adaptiveSupportScheme == CRK_SUPPORT_OWEN) {
+        st = owen();
+        if (st) return st;
+      } else if (cfg.adaptiveSupportScheme == CRK_SUPPORT_MONAGHAN) {
+        st = monaghan();
+        if (st) return st;
+      }
+      st = ensureAdj();
       if (st) return st;
-    } else if (cfg.adaptiveSupportScheme == CRK_SUPPORT_MONAGHAN) {
-      st = monaghan();
+      crkFactors();
+      return CRK_OK;
+    }
+    if (phase == 1) {
+      eos();
+      velGrad();
+      if (cfg.switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN) cullen(dt);
+      accelDudt();
+      return CRK_OK;
+    }
+    if (phase == 2) {
+      balance(dt);
+      return CRK_OK;
+    }
+    return CRK_EINVAL;
+  }
+  int evaluate(R dt) {
+    for (int p = 0; p < 3; ++p) {
+      int st = evaluatePhase(p, dt);
       if (st) return st;
     }
-    st = ensureAdj();
-    if (st) return st;
-    crkFactors();
-    eos();
-    velGrad();
-    if (cfg.switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN) cullen(dt);
-    accelDudt();
-    balance(dt);
     return CRK_OK;
   }
 
@@ -1141,6 +1162,7 @@ struct SimBase {
   virtual int set(int field, const double* in) = 0;
   virtual int get(int field, double* out) = 0;
   virtual int evaluate(double dt) = 0;
+  virtual int evaluatePhase(int phase, double dt) = 0;
   virtual int step(double dt, int nsteps) = 0;
   virtual int64_t edges(int stage) = 0;
   virtual int adjacency(int stage, CrkGrid* g, uint32_t* keys, int32_t* perm, int32_t* cstart,
@@ -1209,6 +1231,15 @@ struct SimImpl : SimBase {
     auto* a = field(f, comps);
     if (!a) return CRK_EINVAL;
     for (size_t k = 0; k < a->size(); ++k) out[k] = (double)(*a)[k];
+    return CRK_OK;
+  }
+  int evaluatePhase(int phase, double dt) override {
+    int st = s.evaluatePhase(phase, R(dt));
+    if (st || phase != 2) return st;
+    s.stages.clear();
+    typename Sim<R, D>::Stage sr;
+    sr.adj = s.adj; sr.ap = s.ap; sr.av = s.av; sr.f = s.f;
+    s.stages.push_back(std::move(sr));
     return CRK_OK;
   }
   int evaluate(double dt) override {
@@ -1325,6 +1356,7 @@ void orc_sim_destroy(void* s) { delete (SimBase*)s; }
 int orc_sim_set(void* s, int field, const double* in) { return ((SimBase*)s)->set(field, in); }
 int orc_sim_get(void* s, int field, double* out) { return ((SimBase*)s)->get(field, out); }
 int orc_sim_evaluate(void* s, double dt) { return ((SimBase*)s)->evaluate(dt); }
+int orc_sim_evaluate_phase(void* s, int phase, double dt) { return ((SimBase*)s)->evaluatePhase(phase, dt); }
 int orc_sim_step(void* s, double dt, int nsteps) { return ((SimBase*)s)->step(dt, nsteps); }
 int64_t orc_sim_edges(void* s, int stage) { return ((SimBase*)s)->edges(stage); }
 int orc_sim_adjacency(void* s, int stage, CrkGrid* g, uint32_t* keys, int32_t* perm, int32_t* cstart,

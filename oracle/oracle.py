@@ -51,6 +51,7 @@ def lib():
         L.orc_sim_set.argtypes = [P, C.c_int, P]
         L.orc_sim_get.argtypes = [P, C.c_int, P]
         L.orc_sim_evaluate.argtypes = [P, C.c_double]
+        L.orc_sim_evaluate_phase.argtypes = [P, C.c_int, C.c_double]
         L.orc_sim_step.argtypes = [P, C.c_double, C.c_int]
         L.orc_sim_edges.argtypes = [P, C.c_int]
         L.orc_sim_edges.restype = C.c_int64
@@ -168,6 +169,13 @@ class OracleSim:
         st = lib().orc_sim_evaluate(self._h, float(dt))
         if st != 0:
             raise ValueError(f"oracle evaluate failed ({st})")
+
+    def evaluate_phase(self, phase: int, dt: float):
+        """One phase of evaluate (0: support, adjacency, CRK factors; 1: EOS
+        to accel/dudt; 2: energy partition), for the boundary hooks between."""
+        st = lib().orc_sim_evaluate_phase(self._h, int(phase), float(dt))
+        if st != 0:
+            raise ValueError(f"oracle evaluate phase {phase} failed ({st})")
 
     def step(self, dt: float, nsteps: int = 1):
         st = lib().orc_sim_step(self._h, float(dt), int(nsteps))

@@ -116,6 +116,105 @@ def test_adjacency_bitexact(name):
     assert np.array_equal(j, ref["nbr"])
 
 
+def _circle_bcs():
+    """A disc (centre (0.5, 0.5), radius 0.2) holding v = (0.1, 0), u = 1.5 and
+    no heating, and a body force (0, -0.5) on every particle."""
+    from types import SimpleNamespace
+
+    def disc(p):
+        r = torch.sqrt(((p - 0.5) ** 2).sum(-1))
+        return r - 0.2, torch.zeros_like(p)
+
+    def everywhere(p):
+        return -torch.ones(p.shape[0], dtype=p.dtype, device=p.device), torch.zeros_like(p)
+
+    def hold_v(state, cfg, comp, p, d, nrm, t, dt):
+        v = torch.zeros_like(state.velocities)
+        v[:, 0] = 0.1
+        return v
+
+    def hold_u(state, cfg, comp, p, d, nrm, t, dt):
+        return torch.full_like(state.internalEnergies, 1.5)
+
+    def no_heat(state, cfg, comp, p, d, nrm, t, dt):
+        return torch.zeros_like(state.internalEnergies)
+
+    def gravity(state, cfg, comp, p, d, nrm, t, dt):
+        g = torch.zeros_like(state.positions)
+        g[:, 1] = -0.5
+        return g * state.masses.view(-1, 1)
+
+    return [SimpleNamespace(sdf=disc, dirichletFunctions={"velocities": hold_v, "internalEnergies": hold_u},
+                            updateFunctions={"dudt": no_heat}, forcingFunctions=None),
+            SimpleNamespace(sdf=everywhere, dirichletFunctions=None, updateFunctions=None,
+                            forcingFunctions=[gravity])]
+
+
+def test_boundary_hooks_stage_parity():
+    """The reference's boundary hooks (Dirichlet between the CRK factors and
+    the EOS, forcing and update functions before the energy partition) on the
+    GPU step against the oracle run in phases with the same hooks between."""
+    from types import SimpleNamespace
+    from oracle import oracle as O
+    from warpsph_amd import boundary, sample
+    from warpsph_amd.scheme import crkSPH_step
+    announce("stage shear2d with boundary hooks")
+    system, config, scheme = make_case("shear2d")
+    cpu = cpu_copy(system, config, scheme)
+    dt = sample.cfl_dt(system.state, config, scheme) * (2.0 / 3.0)
+    scheme.boundaryConditions = _circle_bcs()
+    update, adj, st = crkSPH_step(system, dt, config, scheme)
+    torch.cuda.synchronize()
+    n, D = st.positions.shape
+    sim = O.from_system(cpu, config, scheme, 32)
+
+    def state_view():
+        return SimpleNamespace(positions=torch.from_numpy(sim.get("x").reshape(n, D)),
+                               velocities=torch.from_numpy(sim.get("v").reshape(n, D)),
+                               internalEnergies=torch.from_numpy(sim.get("u")),
+                               densities=torch.from_numpy(sim.get("rho")),
+                               masses=torch.from_numpy(sim.get("m")))
+    sim.evaluate_phase(0, dt)
+    sv = state_view()
+    boundary.enforceDirichlet(SimpleNamespace(state=sv), 0.0, dt, config, scheme)
+    sim.set("v", sv.velocities.numpy())
+    sim.set("u", sv.internalEnergies.numpy())
+    sim.evaluate_phase(1, dt)
+    sv = state_view()
+    upd = SimpleNamespace(dvdt=torch.from_numpy(sim.get("dvdt").reshape(n, D)), dudt=torch.from_numpy(sim.get("dudt")))
+    upd.dvdt += boundary.computeForcing(SimpleNamespace(state=sv), 0.0, dt, config, scheme) / sv.masses.view(-1, 1)
+    boundary.enforceUpdates(upd, SimpleNamespace(state=sv), dt, 0.0, config, scheme)
+    sim.set("dvdt", upd.dvdt.numpy())
+    sim.set("dudt", upd.dudt.numpy())
+    sim.evaluate_phase(2, dt)
+    inside = (torch.sqrt(((cpu.state.positions - 0.5) ** 2).sum(-1)) < 0.2).numpy()
+    assert inside.sum() > 20
+    errs = {}
+    for attr, key in (("velocities", "v"), ("internalEnergies", "u")) + tuple(STATE_FIELDS):
+        errs[key] = relerr(getattr(st, attr).cpu().numpy(), sim.get(key))
+    errs["dvdt"] = relerr(update.dvdt.cpu().numpy(), sim.get("dvdt"))
+    errs["dudt"] = relerr(update.dudt.cpu().numpy(), sim.get("dudt"))
+    errs["dEdt"] = relerr(update.dEdt.cpu().numpy(), sim.get("dEdt"))
+    off, j, ep = user_rows(adj)
+    ref_adj = sim.adjacency(0)
+    assert np.array_equal(j, ref_adj["nbr"])
+    ap_o = sim.edge_field(0, "ap").reshape(-1, D)
+    av_o = sim.edge_field(0, "av").reshape(-1, D)
+    errs["ap"] = relerr(st.ap_ij.cpu().numpy()[ep].reshape(-1, D), ap_o)
+    errs["av"] = relerr(st.av_ij.cpu().numpy()[ep].reshape(-1, D), av_o)
+    rows_i = np.repeat(np.arange(n), np.diff(ref_adj["offsets"]))
+    vh = sim.get("v").reshape(n, D) + 0.5 * dt * sim.get("dvdt").reshape(n, D)
+    w = np.sum((vh[ref_adj["nbr"]] - vh[rows_i]) * (ap_o + av_o), axis=1)
+    errs["f*w"] = relerr(st.f_ij.cpu().numpy()[ep] * w, sim.edge_field(0, "f") * w)
+    print({k: f"{v:.2e}" for k, v in errs.items()})
+    # the hooks took effect: held velocity, energy and no heating in the disc, gravity everywhere
+    v = st.velocities.cpu().numpy()
+    assert np.allclose(v[inside], [0.1, 0.0]) and np.allclose(st.internalEnergies.cpu().numpy()[inside], 1.5)
+    assert np.all(update.dudt.cpu().numpy()[inside] == 0.0)
+    bad = {k: e for k, e in errs.items() if not e <= RTOL}
+    assert not bad, f"fields beyond {RTOL}: {bad}"
+
+
 def test_adjacency_rows_longer_than_the_collection_slots():
     """Rows longer than the single-pass scratch slots (supports 2.5x larger
     than n_H implies): the build falls back to testing the candidates again

@@ -15,11 +15,56 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in json.loads(line)["error"]
 
 
-def test_boundary_conditions_are_refused_not_ignored():
+def test_boundary_hooks_follow_the_reference_semantics():
+    """bcs.py:16-92 on CPU tensors: Dirichlet values replace a state field
+    where sdf < 0 (positions wrapped into the periodic domain first), forcing
+    sums over conditions, update functions replace update fields."""
+    import torch
+    from types import SimpleNamespace
+    from warpsph_amd import boundary, sample
+    system, config, scheme = sample.shear_layer_2d(nx=8, device="cpu")
+    st = system.state
+    st.positions[0] = torch.tensor([1.25, 0.5])  # outside [0, 1): wraps to x = 0.25
+    calls = []
+
+    def sdf(p):  # left half of the box
+        return p[:, 0] - 0.5, torch.zeros_like(p)
+
+    def zero_v(state, cfg, comp, p, d, nrm, t, dt):
+        calls.append((t, dt))
+        return torch.zeros_like(state.velocities)
+
+    def push(state, cfg, comp, p, d, nrm, t, dt):
+        return torch.ones_like(state.positions) * state.masses.view(-1, 1)
+
+    def no_heat(state, cfg, comp, p, d, nrm, t, dt):
+        return torch.full_like(state.internalEnergies, -1.0)
+
+    bc = SimpleNamespace(sdf=sdf, dirichletFunctions={"velocities": zero_v}, forcingFunctions=[push, push],
+                         updateFunctions={"dudt": no_heat})
+    scheme.boundaryConditions = [bc]
+    left = boundary.getPeriodicPositions(st.positions, config.domain)[:, 0] < 0.5
+    assert bool(left[0])
+    v_before = st.velocities.clone()
+    assert boundary.enforceDirichlet(system, 0.3, 0.01, config, scheme)
+    assert calls[-1] == (0.3, 0.01)
+    assert torch.all(st.velocities[left] == 0) and torch.equal(st.velocities[~left], v_before[~left])
+    f = boundary.computeForcing(system, 0.01, 0.3, config, scheme)
+    assert torch.allclose(f[left], 2 * st.masses[left].view(-1, 1).expand(-1, 2)) and torch.all(f[~left] == 0)
+    upd = SimpleNamespace(dudt=torch.zeros(st.numParticles))
+    assert boundary.enforceUpdates(upd, system, 0.01, 0.3, config, scheme)
+    assert torch.all(upd.dudt[left] == -1) and torch.all(upd.dudt[~left] == 0)
+
+
+def test_boundary_conditions_with_an_ensemble_are_refused():
+    import numpy as np
     import pytest
+    from types import SimpleNamespace
     from warpsph_amd import sample
+    from warpsph_amd.ensemble import Runs
     from warpsph_amd.scheme import crkSPH_step
     system, config, scheme = sample.shear_layer_2d(nx=8, device="cpu")
-    scheme.boundaryConditions = [object()]
+    system.runs = Runs([0, 32, 64], "cpu")
+    scheme.boundaryConditions = [SimpleNamespace(sdf=None)]
     with pytest.raises(NotImplementedError):
-        crkSPH_step(system, 1e-4, config, scheme)
+        crkSPH_step(system, np.array([1e-4, 1e-4]), config, scheme)

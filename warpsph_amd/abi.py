@@ -131,6 +131,8 @@ _PROTOS = {
     "crk_adjacency_export": [_P, C.c_int32, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P],
     "crk_stage_begin": [_P, C.POINTER(CrkConfig), C.POINTER(CrkAdjacency), C.POINTER(CrkState), _P],
     "crk_stage_rebind": [_P, C.POINTER(CrkConfig), C.POINTER(CrkAdjacency), C.POINTER(CrkState), _P],
+    "crk_state_refresh": [_P, C.POINTER(CrkConfig), C.POINTER(CrkState), _P],
+    "crk_update_refresh": [_P, C.POINTER(CrkUpdate), _P],
     "crk_stage_begin_checked": [_P, C.POINTER(CrkConfig), C.POINTER(CrkAdjacency), C.POINTER(CrkState), _P, _P, _P,
                                 C.POINTER(C.c_int32), _P],
     "crk_owen_valid": [_P, C.POINTER(CrkConfig), C.POINTER(C.c_int32), _P],

@@ -221,6 +221,39 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   if (VS) vsig[k] = vs;
 }
 
+// v_sig alone over the full rows (wp_vsig.py:55-86), for a state changed
+// after the support sweep (crk_state_refresh)
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_vsig(Rows rows, Geo geo, int32_t n, const float4* __restrict__ geo4,
+                                                 const float4* __restrict__ vel4, float* __restrict__ vsig) {
+  Lane L = lane_of<G>(n);
+  const int32_t k = L.k;
+  const float4 gi = geo4[k], wi = vel4[k];
+  const float xi[3] = {gi.x, gi.y, gi.z};
+  const float vi[3] = {wi.x, wi.y, wi.z};
+  const float eps = 1.0e-14f * gi.w;
+  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  float vs = 0.f;
+  for (int32_t q = b + L.g; q < e; q += G) {
+    const int32_t j = rows.nbr[q];
+    const float4 gj = geo4[j], wj = vel4[j];
+    const float xj[3] = {gj.x, gj.y, gj.z};
+    float dx[3];
+    pair_dx<D>(geo, xi, xj, dx);
+    const float r = fsqrt(dot<D>(dx, dx));
+    const float vj[3] = {wj.x, wj.y, wj.z};
+    float mu = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) mu = mu + (vi[a] - vj[a]) * dx[a];
+    mu = mu * frcp(r + eps);
+    float vv = 0.5f * (wi.w + wj.w) - mu;
+    if (mu > 0.f) vv = 0.f;
+    vs = vv > vs ? vv : vs;
+  }
+  vs = group_max<G>(vs);
+  if (L.valid && L.g == 0) vsig[k] = vs;
+}
+
 // ------------------------------------------------------------ Owen
 __device__ inline double lut_interp(const double* __restrict__ xs, const double* __restrict__ ys, int n, double x) {
   int lo = 0, hi = n;  // upper_bound
@@ -774,6 +807,53 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
   o.ent_u[i] = s;
 }
 
+// ------------------------------------------------------------ state refresh
+// After the caller changed the state between the CRK factors and the rest of
+// the stage (boundary conditions, crkSPH.py:94): the equation of state from
+// the (possibly changed) densities and energies, as at the end of the moments
+// pass, and the velocity-gradient record {v, V, m/rho}.
+template <int D>
+__global__ void k_refresh_eos(const int32_t* __restrict__ perm, int32_t n, const float* __restrict__ rho_user,
+                              const float* __restrict__ m, const float* __restrict__ V, const float* __restrict__ u,
+                              float gamma, MomOut o) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t i = perm[k];
+  const float rh = rho_user[i];
+  o.rho[k] = rh;
+  const float mrk = m[k] * frcp(rh);
+  float vk[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < D; ++d) vk[d] = o.v[(int64_t)k * D + d];
+  if (D < 3) {
+    o.vg4[k] = make_float4(vk[0], vk[1], V[k], mrk);
+  } else {
+    o.vg4[k] = make_float4(vk[0], vk[1], vk[2], V[k]);
+    o.mr[k] = mrk;
+  }
+  const float ui = u[k];
+  const float cc = sqrtf(fabsf(ui) * gamma * (gamma - 1.f));
+  const float P = (gamma - 1.f) * rh * ui;
+  const float s = ui * (gamma - 1.f) * powf(rh, 1.f - gamma);
+  o.P[k] = P;
+  o.c[k] = cc;
+  o.sP[k] = P / powf(rh, gamma);
+  o.pres_u[i] = P;
+  o.cs_u[i] = cc;
+  o.ent_u[i] = s;
+}
+
+// one user-order field of C components into sorted order
+template <int C>
+__global__ void k_gather_field(const float* __restrict__ src, const int32_t* __restrict__ perm, int32_t n,
+                               float* __restrict__ dst) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t i = perm[k];
+#pragma unroll
+  for (int c = 0; c < C; ++c) dst[(int64_t)k * C + c] = src[(int64_t)i * C + c];
+}
+
 // ------------------------------------------------------------ export
 // Slot data of a row to adjacency order: the pairs are the row's edges with
 // r < max(h_i, h_j) in row order, so one merge walk over the row matches
@@ -909,6 +989,62 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
                                          cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
                                          F<float4>(ctx->vel4)));
   CRK_LAUNCH_CHECK("k_gather");
+  return CRK_OK;
+}
+
+int crk_state_refresh(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, void* stream) {
+  int r = require_bound(ctx, cfg);
+  if (r) return r;
+  CRK_REQUIRE(st && st->n == ctx->n && st->positions && st->velocities && st->supports && st->masses &&
+                  st->internalEnergies && st->densities && st->entropies && st->pressures && st->soundspeeds,
+              "crk_state_refresh: state buffers missing or of another size");
+  if (!ctx->have_factors) { set_error("crk_state_refresh before crk_factors_eos"); return CRK_ESTATE; }
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t n = ctx->n;
+  const int D = ctx->dim;
+  ctx->st = *st;
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, ctx->adj.perm, n,
+                                         (const float*)st->positions, (const float*)st->velocities,
+                                         (const float*)st->supports, (const float*)st->masses,
+                                         (const float*)st->internalEnergies, (const float*)st->alpha0s,
+                                         (const float*)st->divergence, (const float*)st->alphas, F<float>(ctx->x),
+                                         F<float>(ctx->v), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->u),
+                                         F<float>(ctx->a0), F<float>(ctx->divp), F<float>(ctx->alpha),
+                                         cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
+                                         F<float4>(ctx->vel4)));
+  CRK_LAUNCH_CHECK("k_gather");
+  MomOut mo{F<float>(ctx->A), F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->rho),
+            F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->sP), st->densities, st->entropies, st->pressures,
+            st->soundspeeds, nullptr, F<float2>(ctx->vm2), F<float>(ctx->v), F<float4>(ctx->vg4),
+            F<float>(ctx->mr)};
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_refresh_eos<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, ctx->adj.perm, n,
+                                         (const float*)st->densities, F<float>(ctx->m), F<float>(ctx->V),
+                                         F<float>(ctx->u), cfg->gamma, mo));
+  CRK_LAUNCH_CHECK("k_refresh_eos");
+  if (ctx->have_vsig) {
+    // the switch's signal speed, from the changed velocities and sound speeds
+    // (the reference evaluates it inside the switch, CullenDehnen2010.py:291)
+    Geo geo = make_geo(*cfg);
+    const int GS = ctx->group[CRK_PASS_SUPPORT];
+    CRK_DIM_G_DISPATCH(D, GS, hipLaunchKernelGGL((k_vsig<DD, G>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
+                                                 rows_of(ctx), geo, n, F<float4>(ctx->geo4), F<float4>(ctx->vel4),
+                                                 F<float>(ctx->vsig)));
+    CRK_LAUNCH_CHECK("k_vsig");
+  }
+  return CRK_OK;
+}
+
+int crk_update_refresh(CrkContext* ctx, const CrkUpdate* up, void* stream) {
+  CRK_REQUIRE(ctx && up && up->dvdt, "crk_update_refresh: null argument");
+  if (!ctx->bound || !ctx->have_accel) { set_error("crk_update_refresh before crk_accel_dudt"); return CRK_ESTATE; }
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t n = ctx->n;
+  switch (ctx->dim) {
+    case 1: hipLaunchKernelGGL(k_gather_field<1>, dim3(nblocks(n)), dim3(kBlock), 0, s, up->dvdt, ctx->adj.perm, n, F<float>(ctx->dvdt)); break;
+    case 2: hipLaunchKernelGGL(k_gather_field<2>, dim3(nblocks(n)), dim3(kBlock), 0, s, up->dvdt, ctx->adj.perm, n, F<float>(ctx->dvdt)); break;
+    default: hipLaunchKernelGGL(k_gather_field<3>, dim3(nblocks(n)), dim3(kBlock), 0, s, up->dvdt, ctx->adj.perm, n, F<float>(ctx->dvdt)); break;
+  }
+  CRK_LAUNCH_CHECK("k_gather_field");
   return CRK_OK;
 }
 

@@ -37,6 +37,7 @@ class IntegrationResult:
 
 
 class RungeKutta2:
+    nodes = (0.0, 1.0)
     weights = (0.5, 0.5)
 
     def function(self, state, f, dt, config, compParams, verbose=False, **kwargs) -> IntegrationResult:
@@ -76,6 +77,8 @@ class RungeKutta2:
         # stage 2 starts from stage 1's state with y0 + dt k1
         sys2 = sys1.stageSystem("alloc")
         sys2.adjacency = adj1
+        if not device_dt:
+            sys2.t = initial.t + self.nodes[1] * dt  # the stage's time t + c_2 dt (read by boundary callbacks)
         s2 = sys2.state
         axpy(nD, st0.positions, upd1.dxdt, s2.positions)
         axpy(nD, st0.velocities, upd1.dvdt, s2.velocities)

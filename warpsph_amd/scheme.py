@@ -13,7 +13,7 @@ from typing import Any, Optional
 import numpy as np
 import torch
 
-from . import abi, timing
+from . import abi, boundary, timing
 from .configurations import (CRKSPHConfig, SimulationConfig, crkSPHConfigToDict, dictToCRKSPHConfig,
                              toCrkConfig)
 from .enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, ViscositySwitch
@@ -73,6 +73,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     would be alone.
     """
     runs = getattr(system, "runs", None)
+    bcs = list(getattr(compParams, "boundaryConditions", None) or [])  # (bcs.py hooks, below)
+    if bcs and runs is not None:
+        raise NotImplementedError("boundary conditions are supported for single runs (not for ensembles)")
     if runs is not None:
         if isinstance(dt, str) and dt == DEVICE_DT:
             dt = 1.0  # each particle's dt is already on the device (the library reads it)
@@ -88,15 +91,10 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     dim = int(config.domain.dim)
     if st.positions.dim() != 2 or st.positions.shape[1] != dim:
         raise ValueError(f"positions must be [N, {dim}]")
-    if getattr(compParams, "boundaryConditions", None):
-        # enforceDirichlet / computeForcing / enforceUpdates (bcs.py:16-92,
-        # called at crkSPH.py:94, 327-343) act between the fused passes of
-        # this step; with an empty list (every configuration here) they are
-        # no-ops, and a non-empty list is refused rather than ignored
-        raise NotImplementedError("boundary conditions (compParams.boundaryConditions) are not supported by the "
-                                  "HIP CRKSPH step")
     cfg = toCrkConfig(config, compParams)
     rt = runtime(st.positions.device)
+    if bcs and rt.deferred:
+        raise NotImplementedError("boundary conditions are supported for steps run eagerly (not graph-replayed)")
     s = rt.stream()
     rt.bind_runs(runs)
     if runs is not None and runs.n != n:
@@ -188,6 +186,12 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
         with timing.scope("factors_eos"):
             abi.call("crk_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(rt.nsing), s)
+    t_now = system.t
+    if bcs:
+        # enforceDirichlet between the CRK factors and the equation of state
+        # (crkSPH.py:94); the library re-reads the state and redoes the EOS
+        if boundary.enforceDirichlet(system, t_now, dt, config, compParams):
+            abi.call("crk_state_refresh", rt.ctx, C.byref(cfg), C.byref(cst), s)
     update = CompressibleSystemUpdate(
         dxdt=st.velocities,  # dx/dt = v (the stage's velocity tensor, read only)
         dvdt=torch.empty_like(st.velocities),
@@ -222,6 +226,14 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
                      float(dt), s)
         else:
             abi.call("crk_accel_dudt", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(cu), C.byref(ed), s)
+    if bcs:
+        # forcing (crkSPH.py:327-328, with the reference's argument order: the
+        # callbacks get t and dt exchanged) and enforceUpdates (:339); the
+        # energy partition re-reads dv/dt
+        forcing = boundary.computeForcing(system, t_now, dt, config, compParams)
+        update.dvdt += forcing / st.masses.view(-1, 1)
+        boundary.enforceUpdates(update, system, dt, t_now, config, compParams)
+        abi.call("crk_update_refresh", rt.ctx, C.byref(update.cabi()), s)
     # energy partition with v_half = v + dt/2 dvdt (crkSPH.py:341-364): the
     # per-particle records; f_ij is evaluated where it is used
     st.pairs.dt = float(dt)

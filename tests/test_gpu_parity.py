@@ -119,7 +119,6 @@ def test_adjacency_bitexact(name):
 def _circle_bcs():
     """A disc (centre (0.5, 0.5), radius 0.2) holding v = (0.1, 0), u = 1.5 and
     no heating, and a body force (0, -0.5) on every particle."""
-    from types import SimpleNamespace
 
     def disc(p):
         r = torch.sqrt(((p - 0.5) ** 2).sum(-1))
@@ -144,10 +143,11 @@ def _circle_bcs():
         g[:, 1] = -0.5
         return g * state.masses.view(-1, 1)
 
-    return [SimpleNamespace(sdf=disc, dirichletFunctions={"velocities": hold_v, "internalEnergies": hold_u},
-                            updateFunctions={"dudt": no_heat}, forcingFunctions=None),
-            SimpleNamespace(sdf=everywhere, dirichletFunctions=None, updateFunctions=None,
-                            forcingFunctions=[gravity])]
+    from warpsph_amd import BoundaryCondition, BoundaryConditionType
+    return [BoundaryCondition(BoundaryConditionType.dynamic, disc,
+                              dirichletFunctions={"velocities": hold_v, "internalEnergies": hold_u},
+                              updateFunctions={"dudt": no_heat}),
+            BoundaryCondition(BoundaryConditionType.forcing, everywhere, forcingFunctions=[gravity])]
 
 
 def test_boundary_hooks_stage_parity():

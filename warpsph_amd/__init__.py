@@ -5,7 +5,8 @@ The step runs in hand-written gfx950 HIP kernels (``warpsph_amd/csrc``) behind
 the C-ABI of ``include/crksph.h``; this package is the host side that keeps
 warpSPH's SimulationState / SimulationConfig / SimulationUpdate API.
 """
-from .configurations import (CRKSPHConfig, CRKViscosity, CompressibleSPHConfig, DiffusionParameters,
+from .configurations import (BoundaryCondition, BoundaryConditionType, CRKSPHConfig, CRKViscosity,
+                             CompressibleSPHConfig, DiffusionParameters,
                              DomainDescription, SimulationConfig, ViscositySwitchConfig, buildConfig,
                              buildDomainDescription, crkSPHConfigToDict, dictToCRKSPHConfig, sphKernel_xi,
                              sphKernelScale, toCrkConfig)
@@ -17,7 +18,7 @@ from .systems import AdjacencyList, CompressibleSystemUpdate, CompSPHState, Comp
 from .utils import n_h_to_nH, nH_to_n_h, volumeToSupportHelper
 
 __all__ = [
-    "AdaptiveSupportScheme", "AdjacencyList", "CRKSPHConfig", "CRKViscosity", "CompSPHState", "CompSPHSystem",
+    "AdaptiveSupportScheme", "AdjacencyList", "BoundaryCondition", "BoundaryConditionType", "CRKSPHConfig", "CRKViscosity", "CompSPHState", "CompSPHSystem",
     "CompressibleSPHConfig", "CompressibleSPHScheme", "CompressibleSystemUpdate", "DiffusionParameters",
     "DomainDescription", "EnergyScheme", "IntegrationSchemeType", "KernelFunctions", "SimulationConfig",
     "SupportScheme", "ViscositySwitch", "ViscositySwitchConfig", "buildConfig", "buildDomainDescription",

@@ -7,6 +7,7 @@ Mirrors the reference's dataclasses so that callers keep working:
   CRKViscosity                       src/warpSPH/configurations/crkSPH.py:17-36
   ViscositySwitchConfig              .../moduleConfigurations/viscositySwitchParameters.py:12-24
   CompressibleSPHConfig              src/warpSPH/configurations/compressibleConfig.py:23-40
+  BoundaryCondition & enums          .../moduleConfigurations/boundaryConditions.py:8-41
   CRKSPHConfig, crkSPHConfigToDict, dictToCRKSPHConfig   crkSPH.py:52-111
 The reference's ``wp.struct`` parameter blocks become plain dataclasses here;
 ``toCrkConfig`` packs them into the C-ABI ``CrkConfig``.
@@ -171,6 +172,43 @@ class ViscositySwitchConfig:
     # extension (no reference counterpart): the trained network of
     # ViscositySwitch.Learned (warpsph_amd.learned.LearnedSwitch)
     model: Optional[Any] = None
+
+
+# ------------------------------------------------------------------ boundary conditions
+# .../moduleConfigurations/boundaryConditions.py:8-41 (the fields the step's
+# hooks read: sdf, dirichletFunctions, updateFunctions, forcingFunctions;
+# warpsph_amd.boundary applies them)
+class BCType(Enum):
+    constant = 1
+    noSlip = 2
+    freeSlip = 3
+    extended = 4
+    zeros = 5
+
+
+class BoundaryConditionType(Enum):
+    constant = "constant"
+    dynamic = "dynamic"
+    reflective = "reflective"
+    extending = "extending"
+    forcing = "forcing"
+    dynamic_forcing = "dynamic_forcing"
+
+
+class VectorProjectionType(Enum):
+    none = "none"
+    normal = "normal"
+    tangential = "tangential"
+
+
+@dataclass
+class BoundaryCondition:
+    type: BoundaryConditionType
+    sdf: Any  # positions [N, d] -> (signed distance [N], normals [N, d]); inside where < 0
+    dirichletFunctions: Optional[Dict[str, Any]] = None  # state field -> fn(state, config, params, x, d, n, t, dt)
+    updateFunctions: Optional[Dict[str, Any]] = None     # update field -> fn(...)
+    forcingFunctions: Optional[List[Any]] = None         # [fn(...) -> force [N, d]]
+    vectorProjectionType: VectorProjectionType = VectorProjectionType.none
 
 
 @dataclass

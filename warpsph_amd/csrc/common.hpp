@@ -68,6 +68,7 @@ struct CrkContext {
   // into nbr_tmp (nbr_cap slots per row) for crk_adjacency_fill to copy
   crk::Buf nbr_tmp;
   int32_t nbr_cap = 0;       // slots per row for the next build
+  double nbr_cap_key = 0.0;  // n_H verletScale^dim the slots were sized for
   int32_t nbr_cap_used = 0;  // slots per row of the collected build
   const int32_t* collect_rows = nullptr;  // row_offsets of the collected build
   int32_t collect_n = 0;

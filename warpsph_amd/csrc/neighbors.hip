@@ -451,11 +451,14 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   // single pass: rows collected into cap slots each (cap from the expected
   // row length, grown after an overflow; the overflowing build falls back to
   // the fill pass)
-  if (ctx->nbr_cap <= 0) {
+  {
     double sd = 1.0;
     for (int d = 0; d < D; ++d) sd *= (double)cfg->verletScale;
-    double expect = 2.5 * (double)cfg->targetNeighbors * sd;
-    ctx->nbr_cap = (int32_t)std::min(1024.0, std::max(32.0, std::ceil(expect / 32.0) * 32.0));
+    const double key = (double)cfg->targetNeighbors * sd;  // expected entries per row
+    if (ctx->nbr_cap <= 0 || key != ctx->nbr_cap_key) {
+      ctx->nbr_cap = (int32_t)std::min(1024.0, std::max(32.0, std::ceil(2.5 * key / 32.0) * 32.0));
+      ctx->nbr_cap_key = key;
+    }
   }
   const int32_t cap = ctx->nbr_cap;
   // the scratch is bounded (kCollectMaxBytes): larger builds count only and

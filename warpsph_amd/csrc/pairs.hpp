@@ -83,10 +83,6 @@ __device__ inline void rec_load(const float4* __restrict__ rec, int32_t j, float
     r[4 * c + 3] = t.w;
   }
 }
-template <int D>
-__device__ inline float* rec_ptr(float4* rec, int32_t k) {
-  return reinterpret_cast<float*>(rec + (int64_t)k * (Rec<D>::NF / 4));
-}
 
 // ------------------------------------------------------------ lane groups
 struct Comp {  // compensated value s - c

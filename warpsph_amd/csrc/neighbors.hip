@@ -312,8 +312,8 @@ __global__ void __launch_bounds__(kBlock) k_rows_copy(const int32_t* __restrict_
   if (!L.valid) return;
   const int32_t k = L.k;
   const int32_t b = off[k], e = off[k + 1];
-  const int32_t* src = tmp + (int64_t)k * cap - b;
-  for (int32_t q = b + L.g; q < e; q += G) nbr[q] = src[q];
+  const int32_t* src = tmp + (int64_t)k * cap;
+  for (int32_t q = b + L.g; q < e; q += G) nbr[q] = src[q - b];
 }
 
 __global__ void k_set_total(const int64_t* __restrict__ total, int32_t* __restrict__ dst) { *dst = (int32_t)*total; }

@@ -385,6 +385,13 @@ def test_step_conserves_energy_and_momentum():
     E1, P1 = totals(s.state)
     assert abs(E1 - E0) / abs(E0) < 1e-6
     assert np.max(np.abs(P1 - P0)) < 1e-6 * float(system.state.masses.double().sum())
+    # the library's FP64 conservation sums against the same sums in torch FP64
+    c = sample.conservation(s.state)
+    assert c["total"] == pytest.approx(E1, rel=1e-12)
+    assert np.allclose(c["momentum"], P1, rtol=0, atol=1e-12 * float(s.state.masses.double().sum()))
+    assert c["mass"] == pytest.approx(float(s.state.masses.double().sum()), rel=1e-12)
+    ke = float((0.5 * s.state.masses.double() * (s.state.velocities.double() ** 2).sum(1)).sum())
+    assert c["kinetic"] == pytest.approx(ke, rel=1e-12)
 
 
 @pytest.mark.parametrize("group", [1, 2, 16, 64])

@@ -87,6 +87,23 @@ def cfl_dt(state: CompSPHState, config: SimulationConfig, scheme: CRKSPHConfig, 
     return out.value
 
 
+def conservation(state: CompSPHState) -> dict:
+    """The driver's conservation sums (examples/compressible/01-sod-shock-tube-1d.py:158-160,
+    212-214) in FP64 on the GPU: total mass, momentum, internal, kinetic and
+    total energy."""
+    import ctypes as C
+    from . import abi
+    from .runtime import runtime
+    n, dim = state.positions.shape
+    rt = runtime(state.positions.device)
+    out = (C.c_double * (dim + 3))()
+    abi.call("crk_conservation", rt.ctx, dim, n, abi.ptr(state.masses), abi.ptr(state.velocities),
+             abi.ptr(state.internalEnergies), out, rt.stream())
+    vals = list(out)
+    return {"mass": vals[0], "momentum": np.array(vals[1:dim + 1]), "internal": vals[dim + 1],
+            "kinetic": vals[dim + 2], "total": vals[dim + 1] + vals[dim + 2]}
+
+
 def shear_layer_2d(nx: int = 1024, seed: int = 2, device='cuda', jitter: float = 0.1, targetNeighbors: float = 32.0,
                    kernel=KernelFunctions.Wendland2, switch=ViscositySwitch.CullenDehnen2010, gamma: float = 5.0 / 3.0):
     """2D periodic shear layer on [0,1]^2, nx^2 particles: band |y-0.5| < 0.25

@@ -145,7 +145,12 @@ def cpu_baseline(nx, nsteps, warm):
     dt = sim.cfl_dt() * (2.0 / 3.0)
     sim.step(dt, warm)
     t0 = time.perf_counter()
-    sim.step(dt, nsteps)
+    done = 0
+    while done < nsteps:  # in chunks, with a progress line for long samples
+        k = min(10, nsteps - done)
+        sim.step(dt, k)
+        done += k
+        print(f"cpu baseline: {done}/{nsteps} steps, {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     n = system.state.positions.shape[0]
     return n * nsteps / (t1 - t0), sim.get("rho"), dt, O.num_threads(), (t1 - t0)

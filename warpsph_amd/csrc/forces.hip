@@ -381,15 +381,24 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
                                                        CdParams cp, CdOut cout, const float4* __restrict__ rec) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
+  // the row's own fields from its record (NF/4 16-byte loads)
+  float ri[Rec<D>::NF];
+  rec_load<D>(rec, k, ri);
   float xi[3], vi[3], Ji[D * D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
+  for (int d = 0; d < D; ++d) { xi[d] = ri[Rec<D>::X + d]; vi[d] = ri[Rec<D>::VEL + d]; }
 #pragma unroll
-  for (int a = 0; a < D * D; ++a) Ji[a] = in.J[(int64_t)k * D * D + a];
-  const float hi = in.h[k], rhoi = in.rho[k], Pi = in.P[k], ci = in.c[k], Vi = in.V[k], mi = in.m[k];
+  for (int a = 0; a < D * D; ++a) Ji[a] = ri[Rec<D>::J + a];
+  const float hi = ri[Rec<D>::H], rhoi = ri[Rec<D>::RHO], Pi = ri[Rec<D>::P], ci = ri[Rec<D>::C],
+              Vi = ri[Rec<D>::V], mi = ri[Rec<D>::M];
   const float ihi = 1.f / hi;
   const float ali = alphaQ ? in.alpha[k] : 1.f;
-  const CorrD<D> cri = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
+  CorrD<D> cri;
+  cri.A = ri[Rec<D>::A];
+#pragma unroll
+  for (int a = 0; a < D; ++a) { cri.B[a] = ri[Rec<D>::B + a]; cri.gA[a] = ri[Rec<D>::GA + a]; }
+#pragma unroll
+  for (int a = 0; a < D * D; ++a) cri.gB[a] = ri[Rec<D>::GB + a];
   const float smi = hi * inv_kscale;
   const float ismi = 1.f / smi;
   const float epsi = 1.0e-7f * smi * smi;

@@ -28,10 +28,10 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
                                                     const int32_t* __restrict__ perm, VgOut o) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
-  float xi[3], vi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = in.x[(int64_t)k * D + d]; vi[d] = in.v[(int64_t)k * D + d]; }
-  const float hi = in.h[k];
+  // the row's own {x, h} and {v, ...} from its records
+  const float4 gk = in.geo4[k], wk = in.vg4[k];
+  const float xi[3] = {gk.x, gk.y, gk.z}, vi[3] = {wk.x, wk.y, wk.z};
+  const float hi = gk.w;
   CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
   Comp acc[D * D];
   Comp dacc;

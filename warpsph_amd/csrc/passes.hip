@@ -149,12 +149,12 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
                                                     float2* __restrict__ vm2) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
-  float xi[3], vi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) { xi[d] = x[(int64_t)k * D + d]; vi[d] = v[(int64_t)k * D + d]; }
-  const float hi = h[k];
+  // the row's own fields from its records ({x, h} after the Owen update, {v, c})
+  const float4 gk = geo4[k], wk = vel4[k];
+  const float xi[3] = {gk.x, gk.y, gk.z}, vi[3] = {wk.x, wk.y, wk.z};
+  const float hi = gk.w;
   const float ih = 1.f / hi;
-  const float ci = vel4[k].w;
+  const float ci = wk.w;
   const float eps = 1.0e-14f * hi;
   const int lane = threadIdx.x & 63;
   const int gbase = lane & ~(G - 1);
@@ -291,10 +291,9 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
                                                  OwenFinal fin, const int32_t* __restrict__ run) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
-  float xi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
-  const float hi = h[k];
+  const float4 gk = geo4[k];  // the row's {x, h} (h before this update)
+  const float xi[3] = {gk.x, gk.y, gk.z};
+  const float hi = gk.w;
   const float ih = 1.f / hi;
   Comp acc;
   const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
@@ -580,9 +579,8 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
                                                     const float4* __restrict__ geo4) {
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
-  float xi[3];
-#pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)k * D + d];
+  const float4 gk = geo4[k];
+  const float xi[3] = {gk.x, gk.y, gk.z};
   MomAcc<D> acc;
   if (!COMPACT) {
     const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;

@@ -178,18 +178,20 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
 
 // Neighbour rows: for sorted particle k, every particle of the 3^D cells
 // around its cell with |x_k - x_j| < s max(h_k, h_j) (exact arithmetic),
-// cells in scan order and candidates ascending.  MODE 2 (crk_adjacency_count)
-// counts each row and writes it to its own cap slots of a scratch array (a
-// row longer than cap sets *overflow); MODE 0 only counts (when the scratch
-// would be too large); MODE 1 (crk_adjacency_fill after an overflow or a
-// count-only pass, or for rows counted elsewhere) fills the rows at their
-// offsets by testing the candidates again.  G lanes per row (G >= 3^D): lane t of the group first
-// looks up cell t's particle range (all cells of the row in one round of
-// loads instead of one dependent load per cell); then the lanes test
-// consecutive candidates of each cell in turn, and the hits are compacted
-// with a group ballot, so the row is written with coalesced stores in the
-// same order as the one-thread-per-row scan (cells in scan order, candidates
-// ascending).
+// cells in scan order and candidates ascending.
+//   MODE 2 (crk_adjacency_count): counts each row and writes it to its own
+//          cap slots of a scratch array (a row longer than cap sets
+//          *overflow);
+//   MODE 0: counts only (when the scratch would be too large);
+//   MODE 1 (crk_adjacency_fill after an overflow or a count-only pass, or
+//          for rows counted elsewhere): fills the rows at their offsets by
+//          testing the candidates again.
+// G lanes per row (G >= 3^D): lane t of the group first looks up cell t's
+// particle range (all cells of the row in one round of loads instead of one
+// dependent load per cell); then the lanes test consecutive candidates of
+// each cell in turn, and the hits are compacted with a group ballot, so the
+// row is written with coalesced stores in the same order as the
+// one-thread-per-row scan (cells in scan order, candidates ascending).
 template <int D, int G, int MODE, bool RUNS>
 __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0, Geo geo, float s,
                                                         const float* __restrict__ xb, const float* __restrict__ hb,

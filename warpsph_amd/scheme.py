@@ -148,6 +148,17 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # optimalSupportMonaghan.py:96-107) is overwritten by computeCRKFactors
         # (crkSPH.py:61) before any use; it is not evaluated.
 
+    # the interacting pairs of this stage (compacted by the support sweep,
+    # which also computes the apparent volume) and their pair data; set up
+    # for the list in hand before the host waits on its test (and set up
+    # again if the list is rebuilt)
+    def stage_pairs(a):
+        p = StagePairs(a, dim, st.masses)
+        if runs is not None:
+            p.runs, p.dtp = runs, runs.dtp
+        return p
+    pairs = stage_pairs(adj) if adj is not None else None
+    rt.nsing.zero_()
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
     if owen_checked:
         # positions are unchanged since the checked gather: only the supports moved
@@ -168,13 +179,10 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # (with Owen and a reused adjacency the sorted copies are current:
         # the Owen pass updated the sorted supports in place)
         abi.call("crk_stage_begin", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), s)
-    # the interacting pairs of this stage (compacted by the support sweep,
-    # which also computes the apparent volume) and their pair data
-    st.pairs = StagePairs(adj, dim, st.masses)
-    if runs is not None:
-        st.pairs.runs, st.pairs.dtp = runs, runs.dtp
+    if pairs is None or rebuilt or pairs.adjacency is not adj:
+        pairs = stage_pairs(adj)
+    st.pairs = pairs
     ed = st.pairs.cabi()
-    rt.nsing.zero_()
     if fused_support and not rebuilt:
         # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101) with the
         # support sweep's compaction in the same pass

@@ -275,13 +275,14 @@ def ensemble(args):
         torch.cuda.synchronize()
         gc.collect()
         gc.freeze()
-        r0, c0, d0 = gs.replays, gs.captures, gs.redos
+        r0, c0, d0, q0 = gs.replays, gs.captures, gs.redos, gs.eager_steps
         t0 = time.perf_counter()
         for _ in range(args.steps):
             gs.step()
         torch.cuda.synchronize()
         secs = time.perf_counter() - t0
-        graph_stats = {"replays": gs.replays - r0, "captures": gs.captures - c0, "eager_redos": gs.redos - d0}
+        graph_stats = {"replays": gs.replays - r0, "captures": gs.captures - c0, "eager_redos": gs.redos - d0,
+                       "eager_steps": gs.eager_steps - q0}
         e.t = gs.times()
     else:
         prev = None

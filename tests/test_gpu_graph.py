@@ -45,10 +45,13 @@ def _eager(system, config, scheme, nsteps, ensemble):
     return s
 
 
-def test_graph_steps_equal_eager_steps_ensemble():
+@pytest.mark.parametrize("min_life", [0.0, 4.0, 1e9])
+def test_graph_steps_equal_eager_steps_ensemble(min_life):
+    """min_life 0: every list is captured; 1e9: eager steps once two list
+    lifetimes are known; 4: a switch between the two within the run."""
     from warpsph_amd.ensemble import concatenate, two_state_run
     from warpsph_amd.graph import GraphStepper
-    announce("graph ensemble")
+    announce(f"graph ensemble min_life={min_life}")
     sides = (24, 32, 40)
     mk = lambda: concatenate([two_state_run(nx, 77 + r, device="cuda")[0] for r, nx in enumerate(sides)])
     _, config, scheme = two_state_run(24, 77, device="cuda")
@@ -56,11 +59,15 @@ def test_graph_steps_equal_eager_steps_ensemble():
     ref = _eager(mk(), config, scheme, nsteps, True)
     sysg = mk()
     gs = GraphStepper(sysg, config, scheme)
+    gs.min_life = min_life
     for _ in range(nsteps):
         gs.step()
     torch.cuda.synchronize()
-    print(f"replays {gs.replays}, captures {gs.captures}, eager redos {gs.redos}")
-    assert gs.replays >= 4 and gs.redos >= 1
+    print(f"replays {gs.replays}, captures {gs.captures}, eager redos {gs.redos}, eager steps {gs.eager_steps}")
+    if min_life == 0.0:
+        assert gs.replays >= 4 and gs.redos >= 1
+    if min_life == 1e9:
+        assert gs.eager_steps >= nsteps // 2
     for f in FIELDS:
         assert torch.equal(getattr(sysg.state, f), getattr(ref.state, f)), f
     assert np.allclose(gs.times(), ref.t, rtol=1e-6, atol=0)
@@ -75,6 +82,7 @@ def test_graph_steps_equal_eager_steps_single_run():
     ref = _eager(s0, config, scheme, nsteps, False)
     s1, _, _ = sample.shear_layer_2d(nx=48, device="cuda")
     gs = GraphStepper(s1, config, scheme)
+    gs.min_life = 0.0
     for _ in range(nsteps):
         gs.step()
     torch.cuda.synchronize()

@@ -10,7 +10,9 @@ input state (and advances each run's time) only while the flag is clear
 never needs the host; afterwards the host reads the flag once.  When some
 run's Verlet list failed, the input state is untouched: the step is redone
 eagerly (which rebuilds the lists) and the graph is captured again for the
-new adjacency.  Replayed and eager steps give the same bits.
+new adjacency.  Replayed and eager steps give the same bits.  While the
+recent lists live fewer than ``min_life`` steps on average, steps run
+eagerly instead: a capture used once or twice costs more than it saves.
 
 No counterpart in the reference (it launches every pass from Python).
 """
@@ -57,6 +59,15 @@ class GraphStepper:
         self._graph_adj = None
         self._pool = torch.cuda.graph_pool_handle()  # one memory pool for every capture
         self.replays = self.captures = self.redos = 0
+        # lists that live fewer steps than this (recent mean) are stepped
+        # eagerly.  A list of L steps costs one capture and one wasted replay
+        # and saves L-1 times (eager step - replay): on MI355X a 128^2 run
+        # captures in 2.5 ms and replays a step in 0.28 ms against 0.53 ms
+        # eager, which breaks even near L = 12 (larger runs later still)
+        self.min_life = 12.0
+        self._lives = []  # steps each recent adjacency lasted
+        self._adj_seen, self._adj_steps = None, 0
+        self.eager_steps = 0
 
     # ---------------------------------------------------------------- body
     def _body(self):
@@ -111,7 +122,6 @@ class GraphStepper:
         d32 = torch.tensor(dts.astype(np.float32), device=self.dev)
         self.dt_prev.copy_(d32)
         self.t += d32.double()
-        self.redos += 1
 
     def _capture(self):
         """The step as a graph for the current adjacency (no work runs)."""
@@ -125,8 +135,16 @@ class GraphStepper:
     def step(self):
         """One RK2 step of every run."""
         S = self.system
-        if S.adjacency is None:
-            self._redo_eager()  # the first step builds the lists
+        if S.adjacency is not self._adj_seen:  # (a rebuild since the last step)
+            if self._adj_seen is not None:
+                self._lives = (self._lives + [self._adj_steps])[-4:]
+            self._adj_seen, self._adj_steps = S.adjacency, 0
+        self._adj_steps += 1
+        if S.adjacency is None or (len(self._lives) >= 2 and sum(self._lives) / len(self._lives) < self.min_life):
+            # the first step builds the lists; short-lived lists: eager steps
+            self.graph = None
+            self._redo_eager()
+            self.eager_steps += 1
             return
         if self.graph is None or self._graph_adj is not S.adjacency:
             # a new adjacency: this step runs the deferred body eagerly (which
@@ -136,6 +154,7 @@ class GraphStepper:
             self._body()
             if self._flag():
                 self._redo_eager()
+                self.redos += 1
             else:
                 self._capture()
             return
@@ -143,6 +162,7 @@ class GraphStepper:
         self.replays += 1
         if self._flag():
             self._redo_eager()
+            self.redos += 1
 
     def times(self) -> np.ndarray:
         return self.t.cpu().numpy()

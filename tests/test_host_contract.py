@@ -68,3 +68,20 @@ def test_boundary_conditions_with_an_ensemble_are_refused():
     scheme.boundaryConditions = [SimpleNamespace(sdf=None)]
     with pytest.raises(NotImplementedError):
         crkSPH_step(system, np.array([1e-4, 1e-4]), config, scheme)
+
+
+def test_a_state_without_particles_is_refused():
+    """An empty state raises before any device work."""
+    import pytest
+    import torch
+    from warpsph_amd import sample
+    from warpsph_amd.scheme import crkSPH_step
+    system, config, scheme = sample.shear_layer_2d(nx=8, device="cpu")
+    st = system.state
+    for f in ("positions", "velocities", "supports", "masses", "internalEnergies", "densities"):
+        v = getattr(st, f)
+        if v is not None:
+            setattr(st, f, v[:0].clone())
+    assert st.numParticles == 0
+    with pytest.raises(ValueError, match="no particles"):
+        crkSPH_step(system, 1e-4, config, scheme)

@@ -91,6 +91,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     dim = int(config.domain.dim)
     if st.positions.dim() != 2 or st.positions.shape[1] != dim:
         raise ValueError(f"positions must be [N, {dim}]")
+    if n == 0:
+        raise ValueError("the state holds no particles")
     cfg = toCrkConfig(config, compParams)
     rt = runtime(st.positions.device)
     if bcs and rt.deferred:

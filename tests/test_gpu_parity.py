@@ -598,6 +598,41 @@ def test_blast3d_error_against_fp64_oracle():
     assert all(a <= max(RTOL, 4.0 * b) for a, b in errs.values()), errs
 
 
+def test_blast3d_t_end_density_l1():
+    """The 3D point blast to its end time t = 0.05 (20^3 particles, about 20
+    RK2 steps with the CFL dt sequence of the FP32 oracle): density L1 of the
+    GPU against the FP32 oracle within 1e-4, and the GPU no further from the
+    FP64 oracle than the FP32 oracle is (to a factor 4)."""
+    from oracle import oracle as O
+    from warpsph_amd import sample
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    announce("blast3d 20^3 to t = 0.05")
+    system, config, scheme = sample.blast_3d(nx=20, device="cuda", energy=1.0)
+    cpu = cpu_copy(system, config, scheme)
+    sims = {p: O.from_system(cpu, config, scheme, p) for p in (32, 64)}
+    integ = getIntegrator()
+    s, t, dt, steps = system, 0.0, -1.0, 0
+    while t < 0.05 - 1e-9:
+        dt = min(sims[32].cfl_dt(dt), 0.05 - t)
+        for sim in sims.values():
+            sim.step(dt, 1)
+        s = integ.function(s, crkSPH_step, dt, config, scheme).state
+        t += dt
+        steps += 1
+    torch.cuda.synchronize()
+    rho_g = s.state.densities.double().cpu().numpy()
+    rho_32, rho_64 = sims[32].get("rho"), sims[64].get("rho")
+    l1 = float(np.mean(np.abs(rho_g - rho_32)) / np.mean(np.abs(rho_32)))
+    l1_64 = float(np.mean(np.abs(rho_g - rho_64)) / np.mean(np.abs(rho_64)))
+    l1_32_64 = float(np.mean(np.abs(rho_32 - rho_64)) / np.mean(np.abs(rho_64)))
+    print(f"steps {steps}: density L1 gpu vs fp32 oracle {l1:.2e}, vs fp64 oracle {l1_64:.2e} "
+          f"(fp32 oracle vs fp64 {l1_32_64:.2e})")
+    assert 5 <= steps < 200
+    assert l1 <= 1e-4
+    assert l1_64 <= max(1e-6, 4.0 * l1_32_64)
+
+
 @pytest.mark.parametrize("kernel", [0, 1, 2])
 @pytest.mark.parametrize("dim", [1, 2, 3])
 def test_owen_table_on_gpu_matches_host(kernel, dim):

@@ -142,6 +142,36 @@ def test_ensemble_matches_oracle():
         assert all(v <= nsteps * RTOL for v in errs.values()), (r, errs)
 
 
+def test_ensemble_200_steps_density_l1():
+    """Three runs of a batched ensemble over 200 RK2 steps, each with its own
+    CFL dt sequence (as the ensemble workload runs): every run's density L1
+    against the FP32 oracle stepped with the same dt within 1e-4."""
+    from oracle import oracle as O
+    from warpsph_amd.ensemble import cfl_dt, run_system
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    announce("ensemble 3 runs x 200 steps vs oracle")
+    ens, parts, config, scheme = make_ensemble(sides=(16, 20, 24), seed=5)
+    sims = [O.from_system(cpu_copy(run_system(ens, r), config, scheme), config, scheme, 32) for r in range(3)]
+    integ = getIntegrator()
+    e, prev = ens, None
+    for _ in range(200):
+        dts = cfl_dt(e, config, scheme, prev)
+        prev = dts
+        e = integ.function(e, crkSPH_step, dts, config, scheme).state
+        for r, sim in enumerate(sims):
+            sim.step(float(np.float32(dts[r])), 1)
+    torch.cuda.synchronize()
+    t = np.broadcast_to(np.asarray(e.t, dtype=np.float64), (3,))
+    for r, sim in enumerate(sims):
+        b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
+        rho_g = e.state.densities[b:q].double().cpu().numpy()
+        rho_o = sim.get("rho")
+        l1 = float(np.mean(np.abs(rho_g - rho_o)) / np.mean(np.abs(rho_o)))
+        print(f"run {r} ({q - b} particles, t = {t[r]:.4f}): density L1 {l1:.2e}")
+        assert l1 <= 1e-4, (r, l1)
+
+
 def test_unbound_calls_after_an_ensemble_step():
     """After an ensemble step the library is back to one simulation."""
     from warpsph_amd import sample

@@ -97,7 +97,7 @@ struct CrkContext {
   crk::Buf s_keys0, s_keys1, s_vals0, s_vals1, s_hist, s_scan, s_red, s_counts;
   crk::Buf pinned;  // host pinned buffer for blocking reads
   // Verlet validity of the bound stage (crk_stage_begin_checked / crk_owen_valid)
-  crk::Buf s_chk;
+  crk::Buf s_chk, s_chk_part;
   const float* check_hb = nullptr;
   int32_t check_runs = 0;  // run count of the last checked stage
   bool owen_checked = false;

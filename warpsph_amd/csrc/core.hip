@@ -362,7 +362,7 @@ int crk_ctx_destroy(CrkContext* c) {
                  &c->rho, &c->P, &c->c, &c->sP, &c->vhs, &c->J, &c->divcd, &c->vsig, &c->Rcd, &c->dvdt, &c->vh, &c->du0,
                  &c->rec, &c->geo4, &c->vel4, &c->vm2, &c->vg4, &c->mr,
                  &c->lut_nh, &c->lut_psiH, &c->s_keys0, &c->s_keys1, &c->s_vals0, &c->s_vals1, &c->s_hist,
-                 &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk, &c->runs_dev, &c->s_gridruns, &c->gnorm, &c->feat, &c->fflag};
+                 &c->s_scan, &c->s_red, &c->s_counts, &c->s_chk, &c->s_chk_part, &c->runs_dev, &c->s_gridruns, &c->gnorm, &c->feat, &c->fflag};
   for (Buf* b : bufs) release(*b);
   if (c->pinned.p) (void)hipHostFree(c->pinned.p);
   if (c->d_flag) (void)hipFree(c->d_flag);

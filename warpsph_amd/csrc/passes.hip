@@ -52,8 +52,10 @@ __global__ void k_gather(const int32_t* __restrict__ perm, int32_t n, const floa
 // The gather of crk_stage_begin fused with the Verlet validity test of the
 // bound adjacency (built at xb, hb) and the support range for Owen, per run
 // r: red[4r] = max |x - xb|^2 (exact), red[4r+1] = max h / hb (exact),
-// red[4r+2] = min hb; hmm[2r] = min h, hmm[2r+1] = max h.  One atomic per
-// block and cell.
+// red[4r+2] = min hb; hmm[2r] = min h, hmm[2r+1] = max h.  Ensembles: one
+// atomic per block and cell.  One run (part != null): each block writes its
+// five values to part[8 b ..] and k_check_finish reduces them (no contended
+// atomics, so the grid can be large).
 template <int D>
 __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo geo, const float* __restrict__ X,
                                const float* __restrict__ Vel, const float* __restrict__ H,
@@ -63,7 +65,7 @@ __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo 
                                float* a0, float* divp, float* alpha, float cfac, float4* __restrict__ geo4,
                                float4* __restrict__ vel4, const float* __restrict__ xb,
                                const float* __restrict__ hb, float* __restrict__ red, float* __restrict__ hmm,
-                               RunsDev R) {
+                               RunsDev R, float* __restrict__ part) {
   float d2 = 0.f, g = 0.f, hbm = 3.0e38f, hlo = 3.0e38f, hhi = 0.f;
   const Span sp = block_span(R, n);
   for (int32_t k = sp.k; k < sp.e; k += sp.step) {
@@ -96,6 +98,17 @@ __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo 
   hbm = block_reduce(hbm, MinOp());
   hlo = block_reduce(hlo, MinOp());
   hhi = block_reduce(hhi, MaxOp());
+  if (part) {
+    if (threadIdx.x == 0) {
+      float* o = part + 8 * (int64_t)blockIdx.x;
+      o[0] = d2;
+      o[1] = g;
+      o[2] = hbm;
+      o[3] = hlo;
+      o[4] = hhi;
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
     atomic_max_pos(&red[4 * sp.r], d2);
     atomic_max_pos(&red[4 * sp.r + 1], g);
@@ -104,6 +117,34 @@ __global__ void k_gather_check(const int32_t* __restrict__ perm, int32_t n, Geo 
     atomic_max_pos(&hmm[2 * sp.r + 1], hhi);
   }
 }
+// the block values of a single-run checked gather into red[0..2], hmm[0..1]
+__global__ void k_check_finish(const float* __restrict__ part, int32_t nb, float* red, float* hmm) {
+  float d2 = 0.f, g = 0.f, hbm = 3.0e38f, hlo = 3.0e38f, hhi = 0.f;
+  for (int32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    const float* o = part + 8 * (int64_t)b;
+    d2 = fmaxf(d2, o[0]);
+    g = fmaxf(g, o[1]);
+    hbm = fminf(hbm, o[2]);
+    hlo = fminf(hlo, o[3]);
+    hhi = fmaxf(hhi, o[4]);
+  }
+  d2 = block_reduce(d2, MaxOp());
+  g = block_reduce(g, MaxOp());
+  hbm = block_reduce(hbm, MinOp());
+  hlo = block_reduce(hlo, MinOp());
+  hhi = block_reduce(hhi, MaxOp());
+  if (threadIdx.x == 0) {
+    red[0] = d2;
+    red[1] = g;
+    red[2] = hbm;
+    hmm[0] = hlo;
+    hmm[1] = hhi;
+  }
+}
+// grid of the single-run checked gather (block values reduced by
+// k_check_finish): 1M particles in 2D, per stage, 66 us at 1024 blocks,
+// 67 at 2048 or 4096 (81 us with 512 blocks of atomics)
+constexpr unsigned kCheckPartBlocks = 1024u;
 // grid of the checked gather: two blocks per CU.  The blocks' atomics on the
 // same few cells serialise in L2, so fewer blocks with longer grid-stride
 // loops are faster (1M particles in 2D, per call: 1024 blocks 90 us, 512
@@ -1080,7 +1121,15 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
   hipLaunchKernelGGL(k_check_init, dim3(1), dim3(kBlock), 0, s, red, hminmax_dev, nr);
   Geo geo = make_geo(*cfg);
   const RunsDev R = runs_of(ctx);
-  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather_check<DD>, dim3(span_blocks(ctx, check_blocks(n))), dim3(kBlock), 0, s, adj->perm, n, geo,
+  // one run: block values reduced by a second kernel; ensembles: atomics
+  unsigned nb = span_blocks(ctx, check_blocks(n));
+  float* part = nullptr;
+  if (ctx->n_runs == 0) {
+    nb = nblocks(n) < kCheckPartBlocks ? (nblocks(n) > 0 ? nblocks(n) : 1u) : kCheckPartBlocks;
+    if ((r = ensure(ctx->s_chk_part, 8 * sizeof(float) * (size_t)nb))) return r;
+    part = F<float>(ctx->s_chk_part);
+  }
+  CRK_DIM_DISPATCH(D, hipLaunchKernelGGL(k_gather_check<DD>, dim3(nb), dim3(kBlock), 0, s, adj->perm, n, geo,
                                          (const float*)st->positions, (const float*)st->velocities,
                                          (const float*)st->supports, (const float*)st->masses,
                                          (const float*)st->internalEnergies, (const float*)st->alpha0s,
@@ -1088,8 +1137,12 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
                                          F<float>(ctx->v), F<float>(ctx->h), F<float>(ctx->m), F<float>(ctx->u),
                                          F<float>(ctx->a0), F<float>(ctx->divp), F<float>(ctx->alpha),
                                          cfg->gamma * (cfg->gamma - 1.f), F<float4>(ctx->geo4),
-                                         F<float4>(ctx->vel4), xb, hb, red, hminmax_dev, R));
+                                         F<float4>(ctx->vel4), xb, hb, red, hminmax_dev, R, part));
   CRK_LAUNCH_CHECK("k_gather_check");
+  if (part) {
+    hipLaunchKernelGGL(k_check_finish, dim3(1), dim3(kBlock), 0, s, part, (int32_t)nb, red, hminmax_dev);
+    CRK_LAUNCH_CHECK("k_check_finish");
+  }
   ctx->check_runs = nr;
   const double sv = cfg->verletScale;
   if (!valid) return verlet_flag_launch(ctx, red, 4, 0, 1, 2, nr, sv, s);  // deferred: into the device flag

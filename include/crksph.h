@@ -542,6 +542,17 @@ int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* su
 int crk_conservation(CrkContext* ctx, int32_t dim, int32_t n, const float* masses,
                      const float* velocities, const float* internalEnergies,
                      double* out, void* stream);
+/* crk_conservation without any allocation and without host synchronisation:
+ * the same sums (bit-identical) written to out_dev, a device array of
+ * dim + 3 doubles.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Its scratch is one fixed file-scope device array per GPU
+ * (n <= 2^23 = 8,388,608 particles, 1.5 MiB, held until the process ends);
+ * larger n returns CRK_EINVAL.  Asynchronous on `stream`; calls that overlap
+ * on one GPU share that array and race, so serialise them. */
+int crk_conservation_noalloc(CrkContext* ctx, int32_t dim, int32_t n, const float* masses,
+                             const float* velocities, const float* internalEnergies,
+                             double* out_dev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -165,6 +165,7 @@ _PROTOS = {
     "crk_cfl_dt": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, C.c_double,
                    C.POINTER(C.c_double), _P],
     "crk_conservation": [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P],
+    "crk_conservation_noalloc": [_P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P],
     "crk_bind_runs": [_P, C.c_int32, _P, _P, _P],
     "crk_run_ids": [_P, C.c_int32, _P, _P],
     "crk_run_extents": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P],

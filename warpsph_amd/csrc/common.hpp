@@ -104,6 +104,9 @@ struct CrkContext {
   // device flag of deferred Verlet tests (1: some list failed), for steps
   // replayed without host synchronisation
   int32_t* d_flag = nullptr;
+  // device address of this device's copy of the allocation-free entry
+  // point's file-scope scratch array (hipGetSymbolAddress at context creation)
+  double* noalloc_scratch = nullptr;
   // independent runs (crk_bind_runs): run r is particles [run_off[r], run_off[r+1])
   // in user order and, for adjacencies built while bound, in sorted order too
   int32_t n_runs = 0;  // 0: one run, no run table
@@ -370,5 +373,8 @@ int verlet_flag_launch(CrkContext* ctx, const float* red, int stride, int i_d2, 
                        double sv, hipStream_t s);
 // every run's particle count matches the bound table (n particles in all)
 int require_runs_n(const CrkContext* ctx, int32_t n, const char* who);
+// resolves the device address of the allocation-free scratch array of
+// integrate.hip for the current device into ctx->noalloc_scratch
+int noalloc_setup(CrkContext* ctx);
 
 }  // namespace crk

@@ -266,6 +266,12 @@ int crk_ctx_create(CrkContext** out) {
   if (e != hipSuccess) { (void)hipHostFree(p); delete c; return hip_fail(e, "hipMalloc", __FILE__, __LINE__); }
   e = hipMemset(c->d_flag, 0, 64);
   if (e != hipSuccess) { (void)hipFree(c->d_flag); (void)hipHostFree(p); delete c; return hip_fail(e, "hipMemset", __FILE__, __LINE__); }
+  if (noalloc_setup(c) != CRK_OK) {
+    (void)hipFree(c->d_flag);
+    (void)hipHostFree(p);
+    delete c;
+    return CRK_EHIP;
+  }
   *out = c;
   return CRK_OK;
 }

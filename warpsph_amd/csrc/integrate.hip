@@ -347,6 +347,26 @@ __global__ void k_conserve_final(const double* __restrict__ partial, int32_t nb,
     for (int c = 0; c < K; ++c) out[c] = sh[c][0];
 }
 
+// ------------------------------------------------------------ allocation-free conservation sums
+// crk_conservation_noalloc keeps its per-block partial sums and its K
+// results in this file-scope array instead of context memory: one copy per
+// GPU, held until the process ends.  Every call writes each slot it reads
+// (partials first, then the final reduction), so nothing needs clearing, but
+// two calls that overlap on one GPU share it and race.
+constexpr int32_t kNoAllocMaxN = 1 << 23;  // particles per call
+constexpr size_t kNoAllocSlots = (size_t)(kNoAllocMaxN / kBlock) * 6 + 8;  // blocks x (dim + 3) + results
+static __device__ __align__(256) double scratch[kNoAllocSlots];
+
+// Resolve this device's address of `scratch` (the host-side name is only a
+// stand-in): called once per context, i.e. once per device, with that device
+// current.  Kernels receive ctx->noalloc_scratch, never the array's name.
+int noalloc_setup(CrkContext* ctx) {
+  void* p = nullptr;
+  CRK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  ctx->noalloc_scratch = (double*)p;
+  return CRK_OK;
+}
+
 }  // namespace crk
 
 using namespace crk;
@@ -666,6 +686,40 @@ int crk_conservation(CrkContext* ctx, int32_t dim, int32_t n, const float* masse
   CRK_HIP(hipMemcpyAsync(hostv, dout, sizeof(double) * K, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
   for (int c = 0; c < K; ++c) out[c] = hostv[c];
+  return CRK_OK;
+}
+
+
+int crk_conservation_noalloc(CrkContext* ctx, int32_t dim, int32_t n, const float* masses, const float* velocities,
+                             const float* u, double* out_dev, void* stream) {
+  CRK_REQUIRE(ctx && masses && velocities && u && out_dev && n > 0, "crk_conservation_noalloc: invalid arguments");
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3");
+  CRK_REQUIRE(n <= kNoAllocMaxN, "crk_conservation_noalloc: n = %d exceeds the fixed scratch capacity of %d particles",
+              n, kNoAllocMaxN);
+  CRK_REQUIRE(ctx->noalloc_scratch, "crk_conservation_noalloc: scratch address not resolved for this context");
+  int cur = -1;
+  CRK_HIP(hipGetDevice(&cur));
+  CRK_REQUIRE(cur == ctx->device, "crk_conservation_noalloc: the current device (%d) is not the context's (%d)", cur,
+              ctx->device);
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t nb = (int32_t)nblocks(n);
+  double* partial = ctx->noalloc_scratch;  // [nb][dim + 3]
+  // the same kernels and block count as crk_conservation: bit-identical sums
+  switch (dim) {
+    case 1:
+      hipLaunchKernelGGL(k_conserve<1>, dim3(nb), dim3(kBlock), 0, s, masses, velocities, u, n, partial);
+      hipLaunchKernelGGL(k_conserve_final<4>, dim3(1), dim3(kBlock), 0, s, partial, nb, out_dev);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_conserve<2>, dim3(nb), dim3(kBlock), 0, s, masses, velocities, u, n, partial);
+      hipLaunchKernelGGL(k_conserve_final<5>, dim3(1), dim3(kBlock), 0, s, partial, nb, out_dev);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_conserve<3>, dim3(nb), dim3(kBlock), 0, s, masses, velocities, u, n, partial);
+      hipLaunchKernelGGL(k_conserve_final<6>, dim3(1), dim3(kBlock), 0, s, partial, nb, out_dev);
+      break;
+  }
+  CRK_LAUNCH_CHECK("k_conserve (noalloc)");
   return CRK_OK;
 }
 

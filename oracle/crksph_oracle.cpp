@@ -280,6 +280,32 @@ double nH_to_n_h(double nH, int D) {
   return std::pow(1.0 / v, 1.0 / (double)D);
 }
 
+// computeNewSupport, src/warpSPH/modules/adaptiveSupport/optimalSupportOwen.py:27-33:
+// s = n_target / n_h; a = 0.4 (1 + s^-3) for s >= 1, 0.4 (1 + s^2) otherwise;
+// h' = (1 - a + a s) h
+template <class R>
+R owenNewSupport(R nhT, R nh, R h) {
+  R sr = nhT / nh;
+  R a_ = sr >= R(1) ? R(0.4) * (R(1) + R(1) / (sr * sr * sr)) : R(0.4) * (R(1) + sr * sr);
+  return (R(1) - a_ + a_ * sr) * h;
+}
+
+// Energy partition of the CRK scheme, compSPH/balance.py:126-154: the share
+// f_ij of the pair work dE given to particle i, from s = P / rho^gamma
+template <class R>
+R energyFraction(R dE, R si, R sj) {
+  R smin = std::min(std::fabs(si), std::fabs(sj));
+  R smax = std::max(std::fabs(si), std::fabs(sj));
+  bool mA = std::fabs(si - sj) == R(0);
+  bool mB = (dE >= R(0) && si >= sj) || (dE < R(0) && si < sj);
+  bool mC = (dE >= R(0) && si < sj) || (dE < R(0) && si >= sj);
+  R ff = R(0.5);
+  if (mA) ff = R(0.5);
+  else if (mB) ff = smin / (smin + smax + R(1.0e-14));
+  else if (mC) ff = smax / (smin + smax + R(1.0e-14));
+  return std::max(R(0), std::min(R(1), ff));
+}
+
 // ------------------------------------------------------------ adjacency
 struct Adj {
   int n = 0;
@@ -545,9 +571,7 @@ struct Sim {
         R ps = std::pow(acc.get(), R(1) / R(D));
         psiH[i] = ps;
         R nh = R(lutInterp(lut_psiH, lut_nh, (double)ps));
-        R sr = nhT / nh;
-        R a_ = sr >= R(1) ? R(0.4) * (R(1) + R(1) / (sr * sr * sr)) : R(0.4) * (R(1) + sr * sr);
-        R hnew = (R(1) - a_ + a_ * sr) * hi;
+        R hnew = owenNewSupport<R>(nhT, nh, hi);
         R lo = hmin * R(0.25), up = hmax * R(4.0);
         hnew = hnew < lo ? lo : (hnew > up ? up : hnew);
         hn[i] = hnew;
@@ -1006,17 +1030,7 @@ struct Sim {
           dot = dot + (vh[(size_t)j * D + a] - vh[(size_t)i * D + a]) * (ap[(size_t)k * D + a] + av[(size_t)k * D + a]);
         R dE = m[i] * dot * dt;
         R sj = P[j] / std::pow(rho[j], g);
-        R smin = std::min(std::fabs(si), std::fabs(sj));
-        R smax = std::max(std::fabs(si), std::fabs(sj));
-        bool mA = std::fabs(si - sj) == R(0);
-        bool mB = (dE >= R(0) && si >= sj) || (dE < R(0) && si < sj);
-        bool mC = (dE >= R(0) && si < sj) || (dE < R(0) && si >= sj);
-        R ff = R(0.5);
-        if (mA) ff = R(0.5);
-        else if (mB) ff = smin / (smin + smax + R(1.0e-14));
-        else if (mC) ff = smax / (smin + smax + R(1.0e-14));
-        ff = std::max(R(0), std::min(R(1), ff));
-        f[k] = ff;
+        f[k] = energyFraction<R>(dE, si, sj);
       }
     });
   }
@@ -1367,6 +1381,19 @@ int orc_sim_edge_field(void* s, int stage, int field, double* out) { return ((Si
 double orc_sim_cfl_dt(void* s, double prev) { return ((SimBase*)s)->cflDt(prev); }
 int orc_sim_stats(void* s, int64_t* out) { return ((SimBase*)s)->stats(out); }
 int orc_sim_build_adjacency(void* s) { return ((SimBase*)s)->buildAdjacency(); }
+// pair-level functions of the step, for hand-computed pins (FP64)
+int orc_limiter_phi(const CrkConfig* cfg, const double* dx, double hi, double hj, const double* Ji,
+                    const double* Jj, double* phi) {
+  if (!cfg || !dx || !Ji || !Jj || !phi) return CRK_EINVAL;
+  switch (cfg->domain.dim) {
+    case 1: { Sim<double, 1> s(*cfg, 1); *phi = s.limiterPhi(dx, hi, hj, Ji, Jj); return CRK_OK; }
+    case 2: { Sim<double, 2> s(*cfg, 1); *phi = s.limiterPhi(dx, hi, hj, Ji, Jj); return CRK_OK; }
+    case 3: { Sim<double, 3> s(*cfg, 1); *phi = s.limiterPhi(dx, hi, hj, Ji, Jj); return CRK_OK; }
+  }
+  return CRK_EINVAL;
+}
+double orc_energy_fraction(double dE, double si, double sj) { return energyFraction<double>(dE, si, sj); }
+double orc_owen_new_support(double nh_target, double nh, double h) { return owenNewSupport<double>(nh_target, nh, h); }
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -62,6 +62,11 @@ def lib():
         L.orc_sim_stats.argtypes = [P, P]
         L.orc_num_threads.restype = C.c_int
         L.orc_sim_build_adjacency.argtypes = [P]
+        L.orc_limiter_phi.argtypes = [C.POINTER(CrkConfig), P, C.c_double, C.c_double, P, P, P]
+        L.orc_energy_fraction.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_energy_fraction.restype = C.c_double
+        L.orc_owen_new_support.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_owen_new_support.restype = C.c_double
         _lib = L
     return _lib
 
@@ -112,6 +117,23 @@ def grid_plan(cfg: CrkConfig, hmax: float, lo=(0, 0, 0), hi=(0, 0, 0)):
 
 def morton(dim, cx, cy=0, cz=0):
     return int(lib().orc_morton(dim, cx, cy, cz))
+
+
+def limiter_phi(cfg: CrkConfig, dx, hi, hj, Ji, Jj) -> float:
+    """phi_ij of the CRK limiter (limiter.py:11-127) for one pair (FP64)."""
+    a = [np.ascontiguousarray(np.asarray(t, dtype=np.float64).ravel()) for t in (dx, Ji, Jj)]
+    out = C.c_double()
+    assert lib().orc_limiter_phi(C.byref(cfg), _p(a[0]), float(hi), float(hj), _p(a[1]), _p(a[2]),
+                                 C.byref(out)) == 0
+    return out.value
+
+
+def energy_fraction(dE, si, sj) -> float:
+    return float(lib().orc_energy_fraction(float(dE), float(si), float(sj)))
+
+
+def owen_new_support(nh_target, nh, h) -> float:
+    return float(lib().orc_owen_new_support(float(nh_target), float(nh), float(h)))
 
 
 def num_threads():

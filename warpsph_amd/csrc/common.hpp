@@ -215,15 +215,21 @@ inline Geo make_geo(const CrkConfig& c) {
 // an integer decision depends on the result.
 template <int D>
 __device__ inline float exact_r2(const Geo& g, const float* xi, const float* xj) {
+  // plain operators in a contract(off) scope: HIP's __fmul_rn / __fadd_rn are
+  // inlined header functions that carry the file's contraction flag, and
+  // hipcc fuses them into v_fmac_f32 (one rounding instead of two), which
+  // flips pairs at |x_ij| = R
+#pragma clang fp contract(off)
   float r2 = 0.f;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    float dx = __fsub_rn(xi[d], xj[d]);
+    float dx = xi[d] - xj[d];
     if (g.per[d]) {
-      if (dx > g.half[d]) dx = __fsub_rn(dx, g.L[d]);
-      else if (dx < -g.half[d]) dx = __fadd_rn(dx, g.L[d]);
+      if (dx > g.half[d]) dx = dx - g.L[d];
+      else if (dx < -g.half[d]) dx = dx + g.L[d];
     }
-    r2 = d == 0 ? __fmul_rn(dx, dx) : __fadd_rn(r2, __fmul_rn(dx, dx));
+    const float sq = dx * dx;
+    r2 = d == 0 ? sq : r2 + sq;
   }
   return r2;
 }
@@ -282,7 +288,8 @@ __host__ __device__ inline uint32_t morton(const int* c) {
   return part1by2((uint32_t)c[0]) | (part1by2((uint32_t)c[1]) << 1) | (part1by2((uint32_t)c[2]) << 2);
 }
 __device__ inline int cell_coord(float x, float origin, float inv, int n, int periodic) {
-  float t = __fmul_rn(__fsub_rn(x, origin), inv);
+#pragma clang fp contract(off)
+  float t = (x - origin) * inv;
   int c = (int)floorf(t);
   if (periodic) { c %= n; if (c < 0) c += n; }
   else { c = c < 0 ? 0 : (c >= n ? n - 1 : c); }

@@ -255,6 +255,7 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
                        int32_t n, const int32_t* cell_start, const int32_t* cell_end,
                        const int32_t* row_offsets, const float* xb, const float* hb,
                        int32_t* nbr, int64_t capacity, void* stream);
+
 /* Blocking validity test of an adjacency built at (xb, hb) for the current
  * (x, h): valid iff max h/hb < s and 2*max|x - xb| <= (s - max h/hb)*min hb. */
 int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n,
@@ -276,7 +277,29 @@ typedef struct CrkAdjacency {
   const int32_t* rank;
   const int32_t* row_offsets;
   const int32_t* nbr;
+  /* LDS tiles (crk_adjacency_tiles), both NULL when the adjacency has none:
+   * tiles [ceil(n / CRK_TILE_ROWS) * CRK_TILE_WORDS], nbr_loc [E] the
+   * tile-local index of each entry's neighbour */
+  const int32_t*  tiles;
+  const uint16_t* nbr_loc;
 } CrkAdjacency;
+
+/* LDS tiles: CRK_TILE_ROWS consecutive sorted rows share one LDS copy of the
+ * records of every particle their entries reach, at most CRK_TILE_CAP
+ * particles in at most CRK_TILE_MAXRUNS contiguous ranges of sorted indices
+ * (larger tiles are "wide": their passes gather from global memory).        */
+#define CRK_TILE_ROWS 256
+#define CRK_TILE_CAP 1152
+#define CRK_TILE_MAXRUNS 40
+#define CRK_TILE_WORDS (4 + 2 * (CRK_TILE_MAXRUNS + 1))
+/* LDS tiles of a filled adjacency (no counterpart in the reference; it serves
+ * the tiled pair passes).  keys [n] (sorted order), cell_start/end as built
+ * by crk_adjacency_count; cap <= CRK_TILE_CAP particles per tile (smaller
+ * values make more tiles wide).  Writes tiles and nbr_loc (see CrkAdjacency);
+ * asynchronous. */
+int crk_adjacency_tiles(CrkContext* ctx, const CrkAdjacency* adj, const uint32_t* keys,
+                        const int32_t* cell_start, const int32_t* cell_end, int32_t cap,
+                        int32_t* tiles, uint16_t* nbr_loc, void* stream);
 
 typedef struct CrkState {          /* user order, caller owned */
   int32_t n;
@@ -306,16 +329,19 @@ typedef struct CrkEdges {          /* one stage's interacting pairs, caller owne
   int32_t* nbr;   /* [E]     sorted neighbour j of each pair: row k's pairs fill the
                              slots row_offsets[k] .. row_offsets[k] + count[k] - 1
                              (adjacency row order kept); the other slots are unused */
-  float*   pf;    /* [E*2*dim] pair forces of each slot: ap_ij (dim) then
-                             av_ij (dim), the pairwise pressure and viscous
-                             accelerations; one record per pair so the
-                             acceleration pass writes whole cache lines       */
+  float*   pf;    /* [E*dim] a_ij = ap_ij + av_ij of each slot, the pairwise
+                             acceleration (what the compatible-energy finalize
+                             and the energy partition read)                    */
   float*   rec;   /* [n*4]   energy-partition record of sorted particle k:
                              v_half = v + dt/2 dvdt (dim components, zero padded
                              to 3) and s = P/rho^gamma (crk_balance)           */
   float*   f;     /* [E]     energy partition f_ij in the pair slots; optional:
                              NULL keeps it implicit (evaluated from rec where
                              it is used)                                       */
+  uint16_t* loc;  /* [E]     tile-local index of each pair's neighbour (written
+                             by crk_support when the adjacency has tiles)      */
+  float*   pv;    /* [E*dim] av_ij of each slot, the viscous part of a_ij
+                             (ap_ij = a_ij - av_ij)                            */
 } CrkEdges;
 
 /* Gathers the state into sorted order (x, v, h, m, u, alpha0, divergence). */

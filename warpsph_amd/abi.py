@@ -83,7 +83,14 @@ class CrkGrid(C.Structure):
 
 class CrkAdjacency(C.Structure):
     _fields_ = [("n", C.c_int32), ("n_edges", C.c_int64), ("perm", C.c_void_p), ("rank", C.c_void_p),
-                ("row_offsets", C.c_void_p), ("nbr", C.c_void_p)]
+                ("row_offsets", C.c_void_p), ("nbr", C.c_void_p), ("tiles", C.c_void_p), ("nbr_loc", C.c_void_p)]
+
+
+# LDS tiles (include/crksph.h)
+CRK_TILE_ROWS = 256
+CRK_TILE_CAP = 1152
+CRK_TILE_MAXRUNS = 40
+CRK_TILE_WORDS = 4 + 2 * (CRK_TILE_MAXRUNS + 1)
 
 
 class CrkState(C.Structure):
@@ -97,7 +104,7 @@ class CrkUpdate(C.Structure):
 
 
 class CrkEdges(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "pf", "rec", "f")]
+    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "pf", "rec", "f", "loc", "pv")]
 
 
 MLP_IN = 8
@@ -126,6 +133,7 @@ _PROTOS = {
                             _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64), _P],
     "crk_adjacency_fill": [_P, C.POINTER(CrkConfig), C.POINTER(CrkGrid), C.c_int32, _P, _P, _P,
                            _P, _P, _P, C.c_int64, _P],
+    "crk_adjacency_tiles": [_P, C.POINTER(CrkAdjacency), _P, _P, _P, C.c_int32, _P, _P, _P],
     "crk_adjacency_valid": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P, _P,
                             C.POINTER(C.c_int32), _P],
     "crk_adjacency_export": [_P, C.c_int32, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P],

@@ -78,6 +78,7 @@ struct CrkContext {
   // buffers of the stage, bound by crk_support)
   int32_t* pair_cnt = nullptr;
   int32_t* pair_nbr = nullptr;
+  uint16_t* pair_loc = nullptr;  // their tile-local indices (tiled adjacencies; else null)
   bool have_support = false;
   bool have_volume = false;  // V (and v_sig) current for the bound adjacency and supports
   // lanes per particle row for each pass (crk_set_group_size)

@@ -4,7 +4,7 @@
 // sweep) and the energy partition (src/warpSPH/schemes/crkSPH.py:158-364).
 #include <cmath>
 
-#include "pairs.hpp"
+#include "tiles.hpp"
 
 namespace crk {
 
@@ -23,10 +23,11 @@ struct VgOut {
   float *J, *divcd, *vsig, *drhodt_u;
 };
 
-template <int D, int G, int K, bool CULLEN>
-__global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C, int32_t n, VgIn in,
-                                                    const int32_t* __restrict__ perm, VgOut o) {
-  Lane L = lane_of<G>(n);
+// One row of the velocity-gradient pass from a neighbour source (geo = {x, h},
+// aux = the {v, V, m / rho} record; d = 3 reads m / rho through the ids).
+template <int D, int G, int K, bool CULLEN, class Src>
+__device__ inline void velgrad_row(const Lane& L, const Src& src, ARows rows, Geo geo, float C, int32_t n,
+                                   const VgIn& in, const int32_t* __restrict__ perm, const VgOut& o) {
   const int32_t k = L.k;
   // the row's own {x, h} and {v, ...} from its records
   const float4 gk = in.geo4[k], wk = in.vg4[k];
@@ -37,19 +38,19 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
   Comp dacc;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   // software pipeline: the neighbour's records one iteration ahead, its id two ahead
-  int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
-  int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
-  float4 gn = in.geo4[jc], wn = in.vg4[jc];
+  int32_t jc = b + L.g < e ? src.id(b + L.g) : 0;
+  int32_t jn = b + L.g + G < e ? src.id(b + L.g + G) : 0;
+  float4 gn = src.geo(jc), wn = src.aux(jc);
   float mn = (D == 3 && CULLEN) ? in.mr[jc] : 0.f;
   for (int32_t q = b + L.g; q < e; q += G) {
     const float4 gj = gn, wj = wn;
     const float mrj = D < 3 ? wj.w : mn;
     if (q + G < e) {
       jc = jn;
-      gn = in.geo4[jc];
-      wn = in.vg4[jc];
+      gn = src.geo(jc);
+      wn = src.aux(jc);
       if (D == 3 && CULLEN) mn = in.mr[jc];
-      if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+      if (q + 2 * G < e) jn = src.id(q + 2 * G);
     }
     float dx[3];
     const float xj[3] = {gj.x, gj.y, gj.z};
@@ -93,6 +94,36 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
   for (int a = 0; a < D; ++a) tr = tr + Jt[a * D + a];
   o.drhodt_u[perm[k]] = -tr * in.rho[k];
   if (CULLEN) o.divcd[k] = dv;
+}
+
+template <int D, int G, int K, bool CULLEN>
+__global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C, int32_t n, VgIn in,
+                                                    const int32_t* __restrict__ perm, VgOut o) {
+  const Lane L = lane_of<G>(n);
+  velgrad_row<D, G, K, CULLEN>(L, GlobalSrc<float4>{rows.nbr, in.geo4, in.vg4}, rows, geo, C, n, in, perm, o);
+}
+
+// One block per tile, the pairs' {x, h} and {v, V, m / rho} records staged
+// in LDS (d <= 2).
+template <int D, int G, int K, bool CULLEN>
+__global__ void __launch_bounds__(kBlock) k_velgrad_t(ARows rows, const uint16_t* __restrict__ ploc,
+                                                      const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
+                                                      VgIn in, const int32_t* __restrict__ perm, VgOut o) {
+  static_assert(D <= 2, "tiled velocity gradient: d <= 2");
+  __shared__ int32_t tab[kTileWords];
+  __shared__ float4 sg[kTileCap];
+  __shared__ float4 sw[kTileCap];
+  const int32_t t = xcd_block();
+  tile_table(tiles, t, tab);
+  const bool wide = tab[0] < 0;
+  if (!wide) tile_fill(sg, in.geo4, tab, sw, in.vg4);
+  for (int round = 0; round < G; ++round) {
+    const Lane L = tile_lane<G>(t, round, n);
+    if (wide)
+      velgrad_row<D, G, K, CULLEN>(L, GlobalSrc<float4>{rows.nbr, in.geo4, in.vg4}, rows, geo, C, n, in, perm, o);
+    else
+      velgrad_row<D, G, K, CULLEN>(L, TileSrc<float4>{ploc, rows.nbr, sg, sw}, rows, geo, C, n, in, perm, o);
+  }
 }
 
 // ------------------------------------------------------------ Cullen-Dehnen
@@ -318,21 +349,17 @@ struct AdIn {
   const float* drhodt_u;
 };
 struct AdOut {
-  float *pf, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
+  float *pf, *pv, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
 };
 
-// pair-force record of a slot: ap (D) then av (D); 2D: one 16-byte store
+// a D-vector of a slot (2D: one 8-byte store)
 template <int D>
-__device__ inline void store_pf(float* p, const float* ap, const float* av) {
+__device__ inline void store_slot(float* p, int64_t q, const float* v) {
   if (D == 2) {
-    *reinterpret_cast<float4*>(p) = make_float4(ap[0], ap[1], av[0], av[1]);
-  } else if (D == 1) {
-    *reinterpret_cast<float2*>(p) = make_float2(ap[0], av[0]);
+    reinterpret_cast<float2*>(p)[q] = make_float2(v[0], v[1]);
   } else {
-    float2* q = reinterpret_cast<float2*>(p);
-    q[0] = make_float2(ap[0], ap[1]);
-    q[1] = make_float2(ap[2], av[0]);
-    q[2] = make_float2(av[1], av[2]);
+#pragma unroll
+    for (int c = 0; c < D; ++c) p[q * D + c] = v[c];
   }
 }
 
@@ -374,12 +401,30 @@ __global__ void k_pack_rec(int32_t n, AdIn in, const float* __restrict__ divcd, 
 // SW: the Cullen-Dehnen switch of the next step is evaluated in the same
 // sweep (valid when the viscosity does not read alpha, i.e. without
 // alpha-weighted Q): S_i = sum_j m_j sgn(div_j) W(x_ij, h_i) over r < h_i.
-template <int D, int G, int K, bool SW>
-__global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, float C, LimParams lp, float inv_kscale,
-                                                       float Cl, float Cq, int alphaQ, int32_t n, AdIn in,
-                                                       const int32_t* __restrict__ perm, AdOut o, CdIn cin,
-                                                       CdParams cp, CdOut cout, const float4* __restrict__ rec) {
-  Lane L = lane_of<G>(n);
+// Record sources of the acceleration pass: the packed records in global
+// memory through the sorted ids, or a tile's LDS copy through the local ids.
+struct RecSrcG {
+  const int32_t* __restrict__ nbr;
+  const float4* __restrict__ rec;
+  __device__ inline int32_t id(int64_t q) const { return nbr[q]; }
+  template <int D>
+  __device__ inline void load(int32_t j, float* r) const { rec_load<D>(rec, j, r); }
+};
+struct RecSrcT {
+  const uint16_t* __restrict__ loc;
+  const float4* rec;  // LDS
+  __device__ inline int32_t id(int64_t q) const { return loc[q]; }
+  template <int D>
+  __device__ inline void load(int32_t j, float* r) const { rec_load<D>(rec, j, r); }
+};
+
+// One row of the acceleration pass (alpha-weighted Q reads alpha_j through
+// the sorted ids: global source only).
+template <int D, int G, int K, bool SW, class Src>
+__device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo geo, float C, const LimParams& lp,
+                                 float inv_kscale, float Cl, float Cq, int alphaQ, int32_t n, const AdIn& in,
+                                 const int32_t* __restrict__ perm, const AdOut& o, const CdIn& cin,
+                                 const CdParams& cp, const CdOut& cout, const float4* __restrict__ rec) {
   const int32_t k = L.k;
   // the row's own fields from its record (NF/4 16-byte loads)
   float ri[Rec<D>::NF];
@@ -408,15 +453,15 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
   // neighbour ids loaded two iterations ahead (the record itself is loaded
   // at the start of its iteration: prefetching it too costs more in
   // occupancy than it hides)
-  int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
-  int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
+  int32_t jc = b + L.g < e ? src.id(b + L.g) : 0;
+  int32_t jn = b + L.g + G < e ? src.id(b + L.g + G) : 0;
   for (int32_t q0 = b + L.g; q0 < e; q0 += G) {
     const int32_t j = jc;
     float rj[Rec<D>::NF];
-    rec_load<D>(rec, j, rj);
+    src.template load<D>(j, rj);
     if (q0 + G < e) {
       jc = jn;
-      if (q0 + 2 * G < e) jn = rows.nbr[q0 + 2 * G];
+      if (q0 + 2 * G < e) jn = src.id(q0 + 2 * G);
     }
     float dx[3];
     pair_dx<D>(geo, xi, &rj[Rec<D>::X], dx);
@@ -493,14 +538,16 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
     const float Pj = rj[Rec<D>::P];
     const float pP = -(Pi + Pj) * VV;
     const float pQ = -(Qi + Qj) * VV;
-    float apv[3], avv[3];
+    float av[3], aa[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      apv[a] = pP * gw[a];
-      avv[a] = pQ * gw[a];
-      acc[a].add(apv[a] + avv[a]);
+      const float apv = pP * gw[a];
+      av[a] = pQ * gw[a];
+      aa[a] = apv + av[a];
+      acc[a].add(aa[a]);
     }
-    store_pf<D>(&o.pf[(int64_t)q0 * 2 * D], apv, avv);
+    store_slot<D>(o.pf, q0, aa);
+    store_slot<D>(o.pv, q0, av);
     float dt_ = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];
@@ -524,6 +571,60 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
   o.dEdt_u[i] = mi * ke + mi * du;
   o.div_u[i] = -in.drhodt_u[i] / rhoi;
   if (SW) cullen_row(cin, cp, k, i, S, hi, cout);
+}
+
+template <int D, int G, int K, bool SW>
+__global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, float C, LimParams lp, float inv_kscale,
+                                                       float Cl, float Cq, int alphaQ, int32_t n, AdIn in,
+                                                       const int32_t* __restrict__ perm, AdOut o, CdIn cin,
+                                                       CdParams cp, CdOut cout, const float4* __restrict__ rec) {
+  const Lane L = lane_of<G>(n);
+  accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, Cl, Cq, alphaQ, n, in, perm, o,
+                         cin, cp, cout, rec);
+}
+
+// The acceleration pass on an LDS tile: 1024 threads share one tile's staged
+// records (NF floats per particle, d <= 2), G lanes per row, the tile's rows
+// in rounds of 1024 / G.
+constexpr int kAccelThreads = 1024;
+template <int D, int G, int K, bool SW>
+__global__ void __launch_bounds__(kAccelThreads) k_accel_dudt_t(ARows rows, const uint16_t* __restrict__ ploc,
+                                                                const int32_t* __restrict__ tiles, Geo geo, float C,
+                                                                LimParams lp, float inv_kscale, float Cl, float Cq,
+                                                                int32_t n, AdIn in, const int32_t* __restrict__ perm,
+                                                                AdOut o, CdIn cin, CdParams cp, CdOut cout,
+                                                                const float4* __restrict__ rec) {
+  static_assert(D <= 2, "tiled acceleration pass: d <= 2");
+  constexpr int C4 = Rec<D>::NF / 4;
+  __shared__ int32_t tab[kTileWords];
+  __shared__ float4 srec[kTileCap * C4];
+  const int32_t t = xcd_block();
+  tile_table(tiles, t, tab);
+  const bool wide = tab[0] < 0;
+  if (!wide) {
+    const int32_t U = tab[1];
+    for (int32_t e = threadIdx.x; e < U * C4; e += blockDim.x) {
+      const int32_t u = e / C4;
+      srec[e] = rec[(int64_t)tile_global(tab, u) * C4 + (e - u * C4)];
+    }
+    __syncthreads();
+  }
+  constexpr int RPR = kAccelThreads / G;  // rows per round
+  constexpr int NR = (kTileRows + RPR - 1) / RPR;
+  for (int round = 0; round < NR; ++round) {
+    const int32_t lr = round * RPR + (int32_t)threadIdx.x / G;
+    Lane L;
+    L.g = threadIdx.x % G;
+    L.k = t * kTileRows + lr;
+    L.valid = lr < kTileRows && L.k < n;
+    if (!L.valid) L.k = min(L.k, n - 1);
+    if (wide)
+      accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, Cl, Cq, 0, n, in, perm, o, cin,
+                             cp, cout, rec);
+    else
+      accel_row<D, G, K, SW>(L, RecSrcT{ploc, srec}, rows, geo, C, lp, inv_kscale, Cl, Cq, 0, n, in, perm, o, cin,
+                             cp, cout, rec);
+  }
 }
 
 // ------------------------------------------------------------ balance
@@ -561,7 +662,7 @@ __global__ void __launch_bounds__(kBlock) k_balance(ARows rows, int32_t n, const
     if (q + G < e) jn = rows.nbr[q + G];
     float a[3];
 #pragma unroll
-    for (int c = 0; c < D; ++c) a[c] = pf[(int64_t)q * 2 * D + c] + pf[(int64_t)q * 2 * D + D + c];
+    for (int c = 0; c < D; ++c) a[c] = pf[(int64_t)q * D + c];
     f[q] = pair_fraction<D>(ri, rec[j], mdt, a);
   }
 }
@@ -600,7 +701,17 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
           F<float>(ctx->gB), F<float4>(ctx->geo4), F<float4>(ctx->vg4), F<float>(ctx->mr)};
   VgOut out{F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig), up->drhodt};
   int GS = ctx->group[CRK_PASS_VELGRAD];
-  if (cullen) {
+  if (use_tiles(ctx) && ctx->pair_loc) {
+    if (cullen) {
+      CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,
+          hipLaunchKernelGGL((k_velgrad_t<DD, G, KK, true>), dim3(tile_count(n)), dim3(kBlock), 0, s,
+                             arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, n, in, ctx->adj.perm, out));
+    } else {
+      CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,
+          hipLaunchKernelGGL((k_velgrad_t<DD, G, KK, false>), dim3(tile_count(n)), dim3(kBlock), 0, s,
+                             arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, n, in, ctx->adj.perm, out));
+    }
+  } else if (cullen) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_velgrad<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
                            arows_of(ctx), geo, C, n, in, ctx->adj.perm, out));
@@ -645,7 +756,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   if (r) return r;
   CRK_REQUIRE(st && st->divergence, "crk_accel_dudt: state->divergence missing");
   CRK_REQUIRE(up && up->dvdt && up->dudt && up->dEdt && up->drhodt, "crk_accel_dudt: update buffers missing");
-  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || ed->pf), "crk_accel_dudt: pair-force buffer missing");
+  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->pf && ed->pv)), "crk_accel_dudt: pair-force buffers missing");
   CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
               "crk_accel_dudt: edges differ from the pairs of crk_support");
   if (!ctx->have_velgrad) { set_error("crk_accel_dudt before crk_velocity_gradient"); return CRK_ESTATE; }
@@ -660,14 +771,27 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   AdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V),
           F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A),
           F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
-  AdOut out{ed->pf, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
+  AdOut out{ed->pf, ed->pv, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
+  // staged on tiles unless alpha-weighted Q reads alpha_j through the sorted ids
+  // (or a group of fewer than 4 lanes would leave a round beyond the tile)
+  const bool tiled = use_tiles(ctx) && ctx->pair_loc && !cfg->alphaWeightedQ && ctx->group[CRK_PASS_ACCEL] >= 4;
   int GS = ctx->group[CRK_PASS_ACCEL];
   const float* divcd = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN ||
                         cfg->switchParams.scheme == CRK_SWITCH_LEARNED) ? F<float>(ctx->divcd) : nullptr;
   CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_pack_rec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, in, divcd,
                                                 F<float4>(ctx->rec)));
   CRK_LAUNCH_CHECK("k_pack_rec");
-  if (sw) {
+  if (tiled) {
+#define ACCEL_T(SWV, CI, CP, CO)                                                                                 \
+    CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,                                                              \
+        hipLaunchKernelGGL((k_accel_dudt_t<DD, (G < 4 ? 4 : G), KK, SWV>), dim3(tile_count(n)),                 \
+                           dim3(kAccelThreads), 0, s, arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, lp,  \
+                           inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q, n, in, ctx->adj.perm, out, CI, CP, \
+                           CO, F<float4>(ctx->rec)))
+    if (sw) { ACCEL_T(true, cd_in(ctx), cd_params(ctx, cfg, dt), cd_out(ctx, st)); }
+    else { ACCEL_T(false, CdIn{}, CdParams{}, CdOut{}); }
+#undef ACCEL_T
+  } else if (sw) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
                            arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,

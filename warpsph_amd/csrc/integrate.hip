@@ -67,7 +67,7 @@ __global__ void k_vhalf_sorted(const int32_t* __restrict__ perm, int32_t n, cons
 // and energy-partition records
 struct CompIn {
   const int32_t *off, *cnt, *nbr;
-  const float* pf;  // per slot: ap (D) then av (D)
+  const float* pf;  // per slot: a = ap + av (D)
   const float4* rec;
   float b;
 };
@@ -80,13 +80,6 @@ __device__ inline float group_total2(float s, float c) {
   return group_total<G>(a);
 }
 
-// Row sums of b_k f (v_half_j - v_half_i).(ap + av) over the in-support
-// pairs of each row (multistep.py:70-94; the skin pairs of the reference's
-// Verlet list carry ap = av = 0), G lanes per row.  NST = 2: both stages use
-// the same sorted order and are summed in one sweep (separate sums, added in
-// stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last sweep
-// (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep of
-// both stages.
 __device__ inline void kahan_add(float& s, float& c, float x) {
   float y = x - c;
   float t = s + y;
@@ -94,20 +87,32 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
   s = t;
 }
 
+// Neighbour source of the finalize: a stage's energy-partition records and
+// the step's v_half through the sorted ids.
+template <int D>
+struct CompSrcG {
+  const int32_t* __restrict__ nbr;
+  const float4* __restrict__ rec;
+  const float* __restrict__ vhs;
+  __device__ inline int32_t id(int64_t q) const { return nbr[q]; }
+  __device__ inline float4 r(int32_t j) const { return rec[j]; }
+  __device__ inline float vh(int32_t j, int c) const { return vhs[(int64_t)j * D + c]; }
+};
+
 // The sum over one stage's pairs of row k: b f_ij (v_half_j - v_half_i).a_ij
 // with f_ij from the stage's records (the energy partition of crk_balance).
-template <int D, int G>
-__device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g, const float* __restrict__ vhs,
-                                  const float* vi, float mdt, float& sa, float& ca) {
+template <int D, int G, class Src>
+__device__ inline void compat_sum(const CompIn& s, const Src& src, int32_t k, bool valid, int g, const float* vi,
+                                  float mdt, float& sa, float& ca) {
   const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
   const float4 ri = s.rec[k];
   // software pipeline: the neighbour's records one iteration ahead, its id two ahead
-  int32_t jc = b + g < e ? s.nbr[b + g] : 0;
-  int32_t jn = b + g + G < e ? s.nbr[b + g + G] : 0;
-  float4 rn = s.rec[jc];
+  int32_t jc = b + g < e ? src.id(b + g) : 0;
+  int32_t jn = b + g + G < e ? src.id(b + g + G) : 0;
+  float4 rn = src.r(jc);
   float vn[3];
 #pragma unroll
-  for (int c = 0; c < D; ++c) vn[c] = vhs[(int64_t)jc * D + c];
+  for (int c = 0; c < D; ++c) vn[c] = src.vh(jc, c);
   for (int32_t q = b + g; q < e; q += G) {
     const float4 rj = rn;
     float vj[3];
@@ -115,19 +120,19 @@ __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g,
     for (int c = 0; c < D; ++c) vj[c] = vn[c];
     if (q + G < e) {
       jc = jn;
-      rn = s.rec[jc];
+      rn = src.r(jc);
 #pragma unroll
-      for (int c = 0; c < D; ++c) vn[c] = vhs[(int64_t)jc * D + c];
-      if (q + 2 * G < e) jn = s.nbr[q + 2 * G];
+      for (int c = 0; c < D; ++c) vn[c] = src.vh(jc, c);
+      if (q + 2 * G < e) jn = src.id(q + 2 * G);
     }
     float a[3], d0 = 0.f;
     if (D == 2) {
-      const float4 p = *reinterpret_cast<const float4*>(&s.pf[(int64_t)q * 4]);
-      a[0] = p.x + p.z;
-      a[1] = p.y + p.w;
+      const float2 p = reinterpret_cast<const float2*>(s.pf)[q];
+      a[0] = p.x;
+      a[1] = p.y;
     } else {
 #pragma unroll
-      for (int c = 0; c < D; ++c) a[c] = s.pf[(int64_t)q * 2 * D + c] + s.pf[(int64_t)q * 2 * D + D + c];
+      for (int c = 0; c < D; ++c) a[c] = s.pf[(int64_t)q * D + c];
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) d0 = d0 + (vj[c] - vi[c]) * a[c];
@@ -136,13 +141,11 @@ __device__ inline void compat_sum(const CompIn& s, int32_t k, bool valid, int g,
   }
 }
 
-template <int D, int G, int NST, int MODE>
-__global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, int32_t n,
-                                                   const float* __restrict__ vhs, CompIn s0, CompIn s1,
-                                                   const float* __restrict__ m_u, float* __restrict__ du0,
-                                                   const float* __restrict__ u0, DtArg dta, float* __restrict__ u,
-                                                   float* __restrict__ dudt_last) {
-  const Lane L = lane_of<G>(n);  // XCD-aware row order
+template <int D, int G, int NST, int MODE, class S0, class S1>
+__device__ inline void compat_row(const Lane& L, const S0& src0, const S1& src1, const int32_t* __restrict__ perm,
+                                  const float* __restrict__ vhs, const CompIn& s0, const CompIn& s1,
+                                  const float* __restrict__ m_u, float* __restrict__ du0, const float* __restrict__ u0,
+                                  DtArg dta, float* __restrict__ u, float* __restrict__ dudt_last) {
   const int g = L.g;
   const int32_t k = L.k;
   const bool valid = L.valid;
@@ -153,8 +156,8 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
   const float dt = dta.at(k);
   const float mdt = m_u[i] * dt;
   float sa = 0.f, ca = 0.f, sb = 0.f, cb = 0.f;
-  compat_sum<D, G>(s0, k, valid, g, vhs, vi, mdt, sa, ca);
-  if (NST == 2) compat_sum<D, G>(s1, k, valid, g, vhs, vi, mdt, sb, cb);
+  compat_sum<D, G>(s0, src0, k, valid, g, vi, mdt, sa, ca);
+  if (NST == 2) compat_sum<D, G>(s1, src1, k, valid, g, vi, mdt, sb, cb);
   float ta = group_total2<G>(sa, ca);
   float tb = NST == 2 ? group_total2<G>(sb, cb) : 0.f;
   if (!valid || g != 0) return;
@@ -166,6 +169,24 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
     u[i] = un;
     dudt_last[i] = (un - u0[i]) / dt;
   }
+}
+
+// Row sums of b_k f (v_half_j - v_half_i).(ap + av) over the in-support pairs
+// of each row (multistep.py:70-94; the skin pairs of the reference's Verlet
+// list carry ap = av = 0), G lanes per row.  NST = 2: both stages use the
+// same sorted order and are summed in one sweep (separate sums, added in
+// stage order).  MODE 0: first of two sweeps (writes du0); MODE 1: last
+// sweep (u = u0 + (du0 + sum) dt, dudt = (u - u0)/dt); MODE 2: single sweep
+// of both stages.
+template <int D, int G, int NST, int MODE>
+__global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ perm, int32_t n,
+                                                   const float* __restrict__ vhs, CompIn s0, CompIn s1,
+                                                   const float* __restrict__ m_u, float* __restrict__ du0,
+                                                   const float* __restrict__ u0, DtArg dta, float* __restrict__ u,
+                                                   float* __restrict__ dudt_last) {
+  const Lane L = lane_of<G>(n);  // XCD-aware row order
+  compat_row<D, G, NST, MODE>(L, CompSrcG<D>{s0.nbr, s0.rec, vhs}, CompSrcG<D>{s1.nbr, s1.rec, vhs}, perm, vhs, s0,
+                              s1, m_u, du0, u0, dta, u, dudt_last);
 }
 
 // ------------------------------------------------------------ reductions
@@ -367,6 +388,40 @@ int noalloc_setup(CrkContext* ctx) {
   return CRK_OK;
 }
 
+// ------------------------------------------------------------ finalize launch
+struct CompArgs {
+  int32_t n;
+  const float *v0, *u0, *dvdt0, *dvdt1;
+  float b0, b1;
+  const float* masses;
+  DtArg dta;
+  float *vhs, *du0, *u_out, *dudt_last;
+};
+
+// One finalize sweep over the stages X (and Y when NST = 2) of adjacency A.
+// (Staging the neighbours' records on LDS tiles was measured slower here:
+// the L1 already serves these narrow gathers, and the staging costs
+// occupancy.)
+template <int D, int G, int NST, int MODE>
+static int compat_sweep(const CompArgs& a, const CrkAdjacency* A, const CompIn& X, const CompIn& Y, hipStream_t s) {
+  hipLaunchKernelGGL(k_vhalf_sorted<D>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, A->perm, a.n, a.v0, a.dvdt0, a.dvdt1,
+                     a.b0, a.b1, a.dta, a.vhs);
+  CRK_LAUNCH_CHECK("k_vhalf_sorted");
+  hipLaunchKernelGGL((k_compat<D, G, NST, MODE>), dim3(nblocks((int64_t)a.n * G)), dim3(kBlock), 0, s, A->perm, a.n,
+                     a.vhs, X, Y, a.masses, a.du0, a.u0, a.dta, a.u_out, a.dudt_last);
+  CRK_LAUNCH_CHECK("k_compat");
+  return CRK_OK;
+}
+
+template <int D, int G>
+static int compat_launch(const CompArgs& a, const CrkAdjacency* adj0, const CompIn& c0, const CrkAdjacency* adj1,
+                         const CompIn& c1, bool shared, hipStream_t s) {
+  if (shared) return compat_sweep<D, G, 2, 2>(a, adj0, c0, c1, s);
+  int st = compat_sweep<D, G, 1, 0>(a, adj0, c0, c0, s);
+  if (st) return st;
+  return compat_sweep<D, G, 1, 1>(a, adj1, c1, c1, s);
+}
+
 }  // namespace crk
 
 using namespace crk;
@@ -450,33 +505,17 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->pf, (const float4*)e0->rec, b0};
   CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->pf, (const float4*)e1->rec, b1};
   const int GS = ctx->group[CRK_PASS_COMPAT];
-  unsigned nb = nblocks((int64_t)n * GS);
-#define VHALF(DD, A)                                                                                          \
-  hipLaunchKernelGGL(k_vhalf_sorted<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, (A)->perm, n, v0, dvdt0, dvdt1, b0, \
-                     b1, dta, vhs)
-#define COMPAT(DD, GG, NST, MODE, A, X, Y)                                                                     \
-  hipLaunchKernelGGL((k_compat<DD, GG, NST, MODE>), dim3(nb), dim3(kBlock), 0, s, (A)->perm, n, vhs, X, Y, masses, \
-                     du0, u0, dta, u_out, dudt_last)
-#define RUN(DD, GG)                     \
-  if (shared) {                         \
-    VHALF(DD, adj0);                    \
-    COMPAT(DD, GG, 2, 2, adj0, c0, c1); \
-  } else {                              \
-    VHALF(DD, adj0);                    \
-    COMPAT(DD, GG, 1, 0, adj0, c0, c0); \
-    VHALF(DD, adj1);                    \
-    COMPAT(DD, GG, 1, 1, adj1, c1, c1); \
-  }
-#define RUNG(DD)                                  \
-  switch (GS) {                                   \
-    case 1: RUN(DD, 1) break;                     \
-    case 2: RUN(DD, 2) break;                     \
-    case 4: RUN(DD, 4) break;                     \
-    case 8: RUN(DD, 8) break;                     \
-    case 16: RUN(DD, 16) break;                   \
-    case 32: RUN(DD, 32) break;                   \
-    case 64: RUN(DD, 64) break;                   \
-    default: set_error("bad group size %d", GS); return CRK_EINVAL; \
+  const CompArgs ca{n, v0, u0, dvdt0, dvdt1, b0, b1, masses, dta, vhs, du0, u_out, dudt_last};
+#define RUNG(DD)                                                                                  \
+  switch (GS) {                                                                                   \
+    case 1: st = compat_launch<DD, 1>(ca, adj0, c0, adj1, c1, shared, s); break;                  \
+    case 2: st = compat_launch<DD, 2>(ca, adj0, c0, adj1, c1, shared, s); break;                  \
+    case 4: st = compat_launch<DD, 4>(ca, adj0, c0, adj1, c1, shared, s); break;                  \
+    case 8: st = compat_launch<DD, 8>(ca, adj0, c0, adj1, c1, shared, s); break;                  \
+    case 16: st = compat_launch<DD, 16>(ca, adj0, c0, adj1, c1, shared, s); break;                \
+    case 32: st = compat_launch<DD, 32>(ca, adj0, c0, adj1, c1, shared, s); break;                \
+    case 64: st = compat_launch<DD, 64>(ca, adj0, c0, adj1, c1, shared, s); break;                \
+    default: set_error("bad group size %d", GS); return CRK_EINVAL;                               \
   }
   switch (D) {
     case 1: RUNG(1) break;
@@ -484,11 +523,7 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
     case 3: RUNG(3) break;
   }
 #undef RUNG
-#undef RUN
-#undef COMPAT
-#undef VHALF
-  CRK_LAUNCH_CHECK("k_compat");
-  return CRK_OK;
+  return st;
 }
 
 int crk_minmax(CrkContext* ctx, int32_t n, const float* values, float* mm, void* stream) {

@@ -11,7 +11,7 @@
 //      found by scanning the 3^d neighbouring cells (k_count / k_fill).
 // All integer decisions use exactly rounded FP32 arithmetic so that the CPU
 // oracle reproduces them bit for bit.
-#include "pairs.hpp"
+#include "tiles.hpp"
 
 namespace crk {
 
@@ -512,6 +512,137 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   return CRK_OK;
 }
 
+// ------------------------------------------------------------ LDS tiles
+// One block per tile: the distinct cells reached by the tile's entries (an
+// LDS hash of their keys, inserted where the key changes along a thread's
+// contiguous chunk of entries), sorted by their first sorted index and
+// merged into runs; then each entry's local index j - start(cell) +
+// offset(cell).
+constexpr int kTileHash = 1024;   // hash slots (a power of two)
+constexpr int kTileCells = 512;   // distinct cells a tile may reach
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+__device__ inline int tile_hash(uint32_t key) { return (int)((key * 2654435761u) >> 22); }  // 10 bits
+
+__global__ void __launch_bounds__(kBlock) k_tiles(int32_t n, const int32_t* __restrict__ off,
+                                                  const int32_t* __restrict__ nbr, const uint32_t* __restrict__ keys,
+                                                  const int32_t* __restrict__ cs, const int32_t* __restrict__ ce,
+                                                  int32_t cap, int32_t* __restrict__ tiles,
+                                                  uint16_t* __restrict__ loc) {
+  __shared__ uint32_t hkey[kTileHash];
+  __shared__ int32_t hadd[kTileHash];  // local index = j + hadd[slot of j's cell]
+  __shared__ int32_t cslot[kTileCells], cstart[kTileCells], clen[kTileCells];
+  __shared__ int32_t sslot[kTileCells], sstart[kTileCells], slen[kTileCells];
+  __shared__ int32_t tab[kTileWords];
+  __shared__ int32_t ncell, overflow;
+  const int32_t t = blockIdx.x;
+  const int32_t r0 = t * kTileRows, r1 = min(n, r0 + kTileRows);
+  const int64_t e0 = off[r0], e1 = off[r1];
+  for (int w = threadIdx.x; w < kTileHash; w += blockDim.x) hkey[w] = kNoKey;
+  for (int w = threadIdx.x; w < kTileWords; w += blockDim.x) tab[w] = 0;
+  if (threadIdx.x == 0) { ncell = 0; overflow = 0; }
+  __syncthreads();
+  // entries in coalesced strides, four per thread in flight; a lane inserts
+  // its key only where it differs from the previous lane's (the entries of a
+  // row run through a few cells in order)
+  constexpr int kBatch = 4;
+  for (int64_t q0 = e0; q0 < e1; q0 += kBatch * (int64_t)blockDim.x) {
+    int32_t jj[kBatch];
+    uint32_t kk[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int64_t q = q0 + (int64_t)u * blockDim.x + threadIdx.x;
+      jj[u] = q < e1 ? nbr[q] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) kk[u] = jj[u] >= 0 ? keys[jj[u]] : kNoKey;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const uint32_t key = kk[u];
+      const uint32_t up = __shfl_up(key, 1);
+      if (key == kNoKey || ((threadIdx.x & 63) != 0 && up == key)) continue;
+      int h = tile_hash(key);
+      bool placed = false;
+      for (int probe = 0; probe < kTileHash && !placed; ++probe) {
+        const uint32_t old = atomicCAS(&hkey[h], kNoKey, key);
+        if (old == kNoKey) {
+          const int c = atomicAdd(&ncell, 1);
+          if (c < kTileCells) cslot[c] = h;
+          else overflow = 1;
+          placed = true;
+        } else if (old == key) {
+          placed = true;
+        } else {
+          h = (h + 1) & (kTileHash - 1);
+        }
+      }
+      if (!placed) overflow = 1;
+    }
+  }
+  __syncthreads();
+  const int32_t nc = min(ncell, kTileCells);
+  if (!overflow) {
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const uint32_t key = hkey[cslot[c]];
+      cstart[c] = cs[key];
+      clen[c] = ce[key] - cs[key];
+    }
+  }
+  __syncthreads();
+  if (!overflow) {  // rank sort by first index (distinct: distinct non-empty cells)
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      int rk = 0;
+      for (int d = 0; d < nc; ++d) rk += cstart[d] < cstart[c] ? 1 : 0;
+      sslot[rk] = cslot[c];
+      sstart[rk] = cstart[c];
+      slen[rk] = clen[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t nr = 0, base = 0;
+    bool wide = overflow != 0;
+    for (int c = 0; c < nc && !wide; ++c) {
+      if (c == 0 || sstart[c] != sstart[c - 1] + slen[c - 1]) {
+        if (nr == kTileRuns) { wide = true; break; }
+        tab[4 + 2 * nr] = sstart[c];
+        tab[5 + 2 * nr] = base;
+        ++nr;
+      }
+      hadd[sslot[c]] = base - sstart[c];
+      base += slen[c];
+    }
+    wide = wide || base > cap;
+    tab[0] = wide ? -1 : nr;
+    tab[1] = wide ? 0 : base;
+    if (!wide) {
+      tab[4 + 2 * nr] = 0;
+      tab[5 + 2 * nr] = base;  // sentinel
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < kTileWords; w += blockDim.x) tiles[(int64_t)t * kTileWords + w] = tab[w];
+  if (tab[0] < 0) return;  // wide: the passes use the 32-bit ids
+  for (int64_t q0 = e0; q0 < e1; q0 += kBatch * (int64_t)blockDim.x) {
+    int32_t jj[kBatch];
+    uint32_t kk[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int64_t q = q0 + (int64_t)u * blockDim.x + threadIdx.x;
+      jj[u] = q < e1 ? nbr[q] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) kk[u] = jj[u] >= 0 ? keys[jj[u]] : kNoKey;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (jj[u] < 0) continue;
+      int h = tile_hash(kk[u]);
+      while (hkey[h] != kk[u]) h = (h + 1) & (kTileHash - 1);  // present: inserted above
+      loc[q0 + (int64_t)u * blockDim.x + threadIdx.x] = (uint16_t)(jj[u] + hadd[h]);
+    }
+  }
+}
+
 }  // namespace crk
 
 using namespace crk;
@@ -644,6 +775,20 @@ int crk_adjacency_export(CrkContext* ctx, int32_t n, const int32_t* perm, const 
   hipLaunchKernelGGL(k_export, dim3(nblocks(n)), dim3(kBlock), 0, s, perm, row_offsets, nbr, n, i_out, j_out,
                      num_neighbors, neighbor_offsets);
   CRK_LAUNCH_CHECK("k_export");
+  return CRK_OK;
+}
+
+int crk_adjacency_tiles(CrkContext* ctx, const CrkAdjacency* adj, const uint32_t* keys, const int32_t* cell_start,
+                        const int32_t* cell_end, int32_t cap, int32_t* tiles, uint16_t* nbr_loc, void* stream) {
+  CRK_REQUIRE(ctx && adj && keys && cell_start && cell_end && tiles && adj->row_offsets,
+              "crk_adjacency_tiles: null argument");
+  CRK_REQUIRE(adj->n > 0, "crk_adjacency_tiles: empty adjacency");
+  CRK_REQUIRE(adj->n_edges == 0 || (adj->nbr && nbr_loc), "crk_adjacency_tiles: neighbour lists missing");
+  CRK_REQUIRE(cap > 0 && cap <= kTileCap, "crk_adjacency_tiles: cap must be in [1, %d] (got %d)", kTileCap, cap);
+  const unsigned nt = nblocks(adj->n, kTileRows);
+  hipLaunchKernelGGL(k_tiles, dim3(nt), dim3(kBlock), 0, (hipStream_t)stream, adj->n, adj->row_offsets, adj->nbr,
+                     keys, cell_start, cell_end, cap, tiles, nbr_loc);
+  CRK_LAUNCH_CHECK("k_tiles");
   return CRK_OK;
 }
 

@@ -15,7 +15,7 @@
 //   crk_balance          f_ij                          (compSPH/balance.py)
 #include <cmath>
 
-#include "pairs.hpp"
+#include "tiles.hpp"
 
 namespace crk {
 
@@ -181,19 +181,38 @@ __global__ void k_check_init(float* red, float* hmm, int32_t nr) {
 // v_sig = max_j (0.5 (c_i + c_j) - mu_ij if mu_ij <= 0 else 0),
 // mu_ij = (v_i - v_j).x_ij / (r_ij + 1e-14 h_i), over every entry
 // (wp_vsig.py:55-86), c = sqrt(|u| gamma (gamma - 1)).
-template <int D, int G, int K, bool VS>
-__global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C, int32_t n,
-                                                    const float* __restrict__ x, const float* __restrict__ h,
-                                                    const float* __restrict__ v, const float* __restrict__ u,
-                                                    float cfac, int32_t* __restrict__ anbr,
-                                                    int32_t* __restrict__ acnt, float* __restrict__ vsig,
-                                                    float* __restrict__ V, const float4* __restrict__ geo4,
-                                                    const float4* __restrict__ vel4, const float* __restrict__ m,
-                                                    float2* __restrict__ vm2) {
-  Lane L = lane_of<G>(n);
+
+// The support sweep on an LDS tile.  The neighbour's {x, h} and {v, c}
+// records and the tile's row offsets are staged in LDS and the entries are
+// read through their 16-bit local indices only; each compacted pair keeps
+// its local index (for the tiled passes after this one) and its sorted id
+// (for every other reader of the pairs; the staged {x, h} records carry it).  A wide
+// tile reads and writes the 32-bit ids.  Each round's row heads (entry
+// range, the first two indices, the row's own records) are loaded during the
+// round before.
+struct SupHead {
+  int32_t b, e, j0, j1;
+  float4 gk, wk;
+};
+template <int G, class Src>
+__device__ inline SupHead support_head(const Lane& L, const Src& src, const int32_t* soff, int32_t r0,
+                                       const float4* __restrict__ geo4, const float4* __restrict__ vel4) {
+  SupHead h;
+  h.b = soff[L.k - r0];
+  h.e = L.valid ? soff[L.k - r0 + 1] : h.b;
+  h.j0 = h.b + L.g < h.e ? src.id(h.b + L.g) : 0;
+  h.j1 = h.b + L.g + G < h.e ? src.id(h.b + L.g + G) : 0;
+  h.gk = geo4[L.k];
+  h.wk = vel4[L.k];
+  return h;
+}
+template <int D, int G, int K, bool VS, class Src>
+__device__ inline void support_row_t(const Lane& L, const Src& src, const SupHead& hd, Geo geo, float C,
+                                     int32_t* __restrict__ anbr, uint16_t* __restrict__ aloc,
+                                     int32_t* __restrict__ acnt, float* __restrict__ vsig,
+                                     float* __restrict__ V, const float* __restrict__ m, float2* __restrict__ vm2) {
   const int32_t k = L.k;
-  // the row's own fields from its records ({x, h} after the Owen update, {v, c})
-  const float4 gk = geo4[k], wk = vel4[k];
+  const float4 gk = hd.gk, wk = hd.wk;
   const float xi[3] = {gk.x, gk.y, gk.z}, vi[3] = {wk.x, wk.y, wk.z};
   const float hi = gk.w;
   const float ih = 1.f / hi;
@@ -203,30 +222,29 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
   const int gbase = lane & ~(G - 1);
   const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
   const unsigned long long lt = (1ull << lane) - 1ull;
-  const int32_t b = rows.off[k], e = L.valid ? rows.off[k + 1] : b;
+  const int32_t b = hd.b, e = hd.e;
   int32_t run = 0;
   float vs = 0.f;
   Comp vol;
-  // software pipeline: the neighbour's records one iteration ahead, its id two ahead
-  int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
-  int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
-  float4 gn = geo4[jc], wn = VS ? vel4[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
+  int32_t jc = hd.j0, jn = hd.j1;
+  float4 gn = src.geo(jc), wn = VS ? src.aux(jc) : make_float4(0.f, 0.f, 0.f, 0.f);
   for (int32_t base = b; base < e; base += G) {
     const int32_t q = base + L.g;
     const bool inside = q < e;
     bool act = false;
-    int32_t j = 0;
+    int32_t j = 0, jg = 0;
     if (inside) {
       j = jc;
       const float4 gj = gn;
       const float4 wj = wn;
+      jg = aloc ? __float_as_int(gj.z) : j;  // staged records carry the sorted id
       if (q + G < e) {
         jc = jn;
-        gn = geo4[jc];
-        if (VS) wn = vel4[jc];
-        if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+        gn = src.geo(jc);
+        if (VS) wn = src.aux(jc);
+        if (q + 2 * G < e) jn = src.id(q + 2 * G);
       }
-      const float xj[3] = {gj.x, gj.y, gj.z};
+      const float xj[3] = {gj.x, gj.y, D == 3 ? gj.z : 0.f};
       float dx[3];
       pair_dx<D>(geo, xi, xj, dx);
       float r = fsqrt(dot<D>(dx, dx));
@@ -248,20 +266,89 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
       }
       act = r < fmaxf(hi, gj.w);
     }
-    const unsigned long long m = __ballot(act ? 1 : 0) & gmask;
+    const unsigned long long mm = __ballot(act ? 1 : 0) & gmask;
     if (act) {
-      anbr[b + run + __popcll(m & lt)] = j;
+      const int32_t slot = b + run + __popcll(mm & lt);
+      anbr[slot] = jg;
+      if (aloc) aloc[slot] = (uint16_t)j;  // a staged tile: the local index too
     }
-    run += __popcll(m);
+    run += __popcll(mm);
   }
   vs = group_max<G>(vs);
-  float S = group_total<G>(vol);
+  float Sv = group_total<G>(vol);
   if (!L.valid || L.g != 0) return;
   acnt[k] = run;
-  const float Vk = 1.f / S;
+  const float Vk = 1.f / Sv;
   V[k] = Vk;
   vm2[k] = make_float2(Vk, m[k]);
   if (VS) vsig[k] = vs;
+}
+
+template <int D, int G, int K, bool VS>
+__global__ void __launch_bounds__(kBlock) k_support_t(Rows rows, const uint16_t* __restrict__ nloc,
+                                                      const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
+                                                      int32_t* __restrict__ anbr, uint16_t* __restrict__ aloc,
+                                                      int32_t* __restrict__ acnt, float* __restrict__ vsig,
+                                                      float* __restrict__ V, const float4* __restrict__ geo4,
+                                                      const float4* __restrict__ vel4, const float* __restrict__ m,
+                                                      float2* __restrict__ vm2) {
+  __shared__ int32_t tab[kTileWords];
+  __shared__ int32_t soff[kTileRows + 1];
+  __shared__ float4 sg[kTileCap];
+  __shared__ float4 sw[VS ? kTileCap : 1];
+  const int32_t t = xcd_block();
+  const int32_t r0 = t * kTileRows;
+  for (int w = threadIdx.x; w <= kTileRows; w += blockDim.x) soff[w] = rows.off[min(r0 + w, n)];
+  tile_table(tiles, t, tab);
+  const bool wide = tab[0] < 0;
+  if (!wide) {
+    // the staged {x, h} record carries the particle's sorted id in its unused
+    // z component (d <= 2), for the compacted pairs' 32-bit ids
+    const int32_t U = tab[1];
+    for (int32_t u = threadIdx.x; u < U; u += blockDim.x) {
+      const int32_t g = tile_global(tab, u);
+      float4 a = geo4[g];
+      a.z = __int_as_float(g);
+      sg[u] = a;
+      if (VS) sw[u] = vel4[g];
+    }
+    __syncthreads();
+  }
+  auto sweep = [&](const auto& src, uint16_t* lout) {
+    Lane L = tile_lane<G>(t, 0, n);
+    SupHead nxt = support_head<G>(L, src, soff, r0, geo4, vel4);
+    for (int round = 0; round < G; ++round) {
+      const Lane Lc = L;
+      const SupHead cur = nxt;
+      if (round + 1 < G) {
+        L = tile_lane<G>(t, round + 1, n);
+        nxt = support_head<G>(L, src, soff, r0, geo4, vel4);
+      }
+      support_row_t<D, G, K, VS>(Lc, src, cur, geo, C, anbr, lout, acnt, vsig, V, m, vm2);
+    }
+  };
+  if (wide) sweep(GlobalSrc<float4>{rows.nbr, geo4, vel4}, nullptr);
+  else sweep(TileSrc<float4>{nloc, nullptr, sg, sw}, aloc);
+}
+
+// The support sweep without tiles: one row per lane group, the neighbours'
+// records gathered through the sorted ids (the same row code as a wide tile).
+template <int D, int G, int K, bool VS>
+__global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C, int32_t n,
+                                                    int32_t* __restrict__ anbr, int32_t* __restrict__ acnt,
+                                                    float* __restrict__ vsig, float* __restrict__ V,
+                                                    const float4* __restrict__ geo4, const float4* __restrict__ vel4,
+                                                    const float* __restrict__ m, float2* __restrict__ vm2) {
+  const Lane L = lane_of<G>(n);
+  const GlobalSrc<float4> src{rows.nbr, geo4, vel4};
+  SupHead hd;
+  hd.b = rows.off[L.k];
+  hd.e = L.valid ? rows.off[L.k + 1] : hd.b;
+  hd.j0 = hd.b + L.g < hd.e ? src.id(hd.b + L.g) : 0;
+  hd.j1 = hd.b + L.g + G < hd.e ? src.id(hd.b + L.g + G) : 0;
+  hd.gk = geo4[L.k];
+  hd.wk = vel4[L.k];
+  support_row_t<D, G, K, VS>(L, src, hd, geo, C, anbr, nullptr, acnt, vsig, V, m, vm2);
 }
 
 // v_sig alone over the full rows (wp_vsig.py:55-86), for a state changed
@@ -426,6 +513,89 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
       fin.vm2[k] = make_float2(Vk, fin.m[k]);
       if (fin.vsig) fin.vsig[k] = vs;
     }
+  }
+  if (ratio) {  // max |h_new / h - 1|, one atomic per block
+    rmax = block_reduce(rmax, MaxOp());
+    if (threadIdx.x == 0) atomic_max_pos(ratio, rmax);
+  }
+}
+
+// Owen's update on an LDS tile (the non-final sweep of k_owen, row by row):
+// the row's psi0 over its Verlet entries from the source's {x} records.
+template <int D, int G, int K, class Src>
+__device__ inline float owen_row(const Lane& L, const Src& src, const int32_t* __restrict__ off, Geo geo, float C,
+                                 float* __restrict__ h, const int32_t* __restrict__ perm,
+                                 float* __restrict__ supports, const double* __restrict__ lut_psiH,
+                                 const double* __restrict__ lut_nh, int nlut, float nhT,
+                                 const float* __restrict__ hminmax, float4* __restrict__ geo4,
+                                 const int32_t* __restrict__ run) {
+  const int32_t k = L.k;
+  const float4 gk = geo4[k];  // the row's {x, h} (h before this update)
+  const float xi[3] = {gk.x, gk.y, gk.z};
+  const float hi = gk.w;
+  const float ih = 1.f / hi;
+  Comp acc;
+  const int32_t b = off[k], e = L.valid ? off[k + 1] : b;
+  // the neighbour's record one iteration ahead, its id two ahead
+  int32_t jn = b + L.g + G < e ? src.id(b + L.g + G) : 0;
+  float4 gn = src.geo(b + L.g < e ? src.id(b + L.g) : 0);
+  for (int32_t q = b + L.g; q < e; q += G) {
+    const float4 gj = gn;
+    if (q + G < e) {
+      gn = src.geo(jn);
+      if (q + 2 * G < e) jn = src.id(q + 2 * G);
+    }
+    const float xj[3] = {gj.x, gj.y, gj.z};
+    float dx[3];
+    pair_dx<D>(geo, xi, xj, dx);
+    float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    if (!(qq < 1.f)) continue;
+    float kk, dk;
+    kwd<K, D>(qq, kk, dk);
+    acc.add(C * fabsf(dk));
+  }
+  float S = group_total<G>(acc);
+  float ps = powf(S, 1.f / (float)D);
+  float nh = (float)lut_interp(lut_psiH, lut_nh, nlut, (double)ps);
+  float sr = nhT / nh;
+  float a = sr >= 1.f ? 0.4f * (1.f + 1.f / (sr * sr * sr)) : 0.4f * (1.f + sr * sr);
+  float hn = (1.f - a + a * sr) * hi;
+  const int32_t rr = run_index(run, k);
+  float lo = hminmax[2 * rr] * 0.25f, up = hminmax[2 * rr + 1] * 4.0f;
+  hn = hn < lo ? lo : (hn > up ? up : hn);
+  if (!L.valid || L.g != 0) return 0.f;
+  h[k] = hn;
+  geo4[k].w = hn;
+  supports[perm[k]] = hn;
+  return fabsf(hn / hi - 1.f);
+}
+
+// One block per tile: the tile's {x, h} records staged in LDS (the Owen
+// update only reads the neighbours' positions, so the rows' own h writes to
+// geo4 do not race with what another tile stages).
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_owen_t(Rows rows, const uint16_t* __restrict__ nloc,
+                                                   const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
+                                                   float* __restrict__ h, const int32_t* __restrict__ perm,
+                                                   float* __restrict__ supports, const double* __restrict__ lut_psiH,
+                                                   const double* __restrict__ lut_nh, int nlut, float nhT,
+                                                   const float* __restrict__ hminmax, float* __restrict__ ratio,
+                                                   float4* __restrict__ geo4, const int32_t* __restrict__ run) {
+  __shared__ int32_t tab[kTileWords];
+  __shared__ float4 sg[kTileCap];
+  const int32_t t = xcd_block();
+  tile_table(tiles, t, tab);
+  const bool wide = tab[0] < 0;
+  if (!wide) tile_fill(sg, (const float4*)geo4, tab);
+  float rmax = 0.f;
+  for (int round = 0; round < G; ++round) {
+    const Lane L = tile_lane<G>(t, round, n);
+    const float r = wide
+        ? owen_row<D, G, K>(L, GlobalSrc<float4>{rows.nbr, geo4, geo4}, rows.off, geo, C, h, perm, supports,
+                            lut_psiH, lut_nh, nlut, nhT, hminmax, geo4, run)
+        : owen_row<D, G, K>(L, TileSrc<float4>{nloc, rows.nbr, sg, sg}, rows.off, geo, C, h, perm, supports,
+                            lut_psiH, lut_nh, nlut, nhT, hminmax, geo4, run);
+    rmax = fmaxf(rmax, r);
   }
   if (ratio) {  // max |h_new / h - 1|, one atomic per block
     rmax = block_reduce(rmax, MaxOp());
@@ -612,15 +782,15 @@ struct MomAcc {
 // COMPACT = true: iterate the full Verlet rows and compact the pairs with
 // |x_ij| < max(h_i, h_j) into anbr / acnt on the way (the support sweep's
 // compaction, for when V and v_sig were computed by the final Owen sweep).
-template <int D, int G, int K, bool COMPACT>
-__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo geo, float C, int32_t n,
-                                                    const float* __restrict__ x, const float* __restrict__ h,
-                                                    const float* __restrict__ V, const float* __restrict__ m,
-                                                    const float* __restrict__ u, float gamma,
-                                                    const int32_t* __restrict__ perm, MomOut o,
-                                                    int32_t* __restrict__ anbr, int32_t* __restrict__ acnt,
-                                                    const float4* __restrict__ geo4) {
-  Lane L = lane_of<G>(n);
+// One row of the moments pass from a neighbour source (geo = {x, h}, aux =
+// {V, m}); COMPACT rows scan the full Verlet row through the global ids.
+template <int D, int G, int K, bool COMPACT, class Src>
+__device__ inline void moments_row(const Lane& L, const Src& src, ARows rows, Rows full, Geo geo, float C, int32_t n,
+                                   const float* __restrict__ x, const float* __restrict__ h,
+                                   const float* __restrict__ V, const float* __restrict__ m,
+                                   const float* __restrict__ u, float gamma, const int32_t* __restrict__ perm,
+                                   const MomOut& o, int32_t* __restrict__ anbr, int32_t* __restrict__ acnt,
+                                   const float4* __restrict__ geo4) {
   const int32_t k = L.k;
   const float4 gk = geo4[k];
   const float xi[3] = {gk.x, gk.y, gk.z};
@@ -628,18 +798,18 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
   if (!COMPACT) {
     const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
     // software pipeline: the neighbour's records one iteration ahead, its id two ahead
-    int32_t jc = b + L.g < e ? rows.nbr[b + L.g] : 0;
-    int32_t jn = b + L.g + G < e ? rows.nbr[b + L.g + G] : 0;
-    float4 gn = geo4[jc];
-    float2 vn = o.vm2[jc];
+    int32_t jc = b + L.g < e ? src.id(b + L.g) : 0;
+    int32_t jn = b + L.g + G < e ? src.id(b + L.g + G) : 0;
+    float4 gn = src.geo(jc);
+    float2 vn = src.aux(jc);
     for (int32_t q = b + L.g; q < e; q += G) {
       const float4 gj = gn;
       const float2 vmj = vn;
       if (q + G < e) {
         jc = jn;
-        gn = geo4[jc];
-        vn = o.vm2[jc];
-        if (q + 2 * G < e) jn = rows.nbr[q + 2 * G];
+        gn = src.geo(jc);
+        vn = src.aux(jc);
+        if (q + 2 * G < e) jn = src.id(q + 2 * G);
       }
       float dx[3];
       const float xj[3] = {gj.x, gj.y, gj.z};
@@ -848,6 +1018,46 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
   o.ent_u[i] = s;
 }
 
+template <int D, int G, int K, bool COMPACT>
+__global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo geo, float C, int32_t n,
+                                                    const float* __restrict__ x, const float* __restrict__ h,
+                                                    const float* __restrict__ V, const float* __restrict__ m,
+                                                    const float* __restrict__ u, float gamma,
+                                                    const int32_t* __restrict__ perm, MomOut o,
+                                                    int32_t* __restrict__ anbr, int32_t* __restrict__ acnt,
+                                                    const float4* __restrict__ geo4) {
+  const Lane L = lane_of<G>(n);
+  moments_row<D, G, K, COMPACT>(L, GlobalSrc<float2>{rows.nbr, geo4, o.vm2}, rows, full, geo, C, n, x, h, V, m, u,
+                                gamma, perm, o, anbr, acnt, geo4);
+}
+
+// One block per tile, the pairs' {x, h} and {V, m} records staged in LDS.
+template <int D, int G, int K>
+__global__ void __launch_bounds__(kBlock) k_moments_t(ARows rows, const uint16_t* __restrict__ ploc,
+                                                      const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
+                                                      const float* __restrict__ x, const float* __restrict__ h,
+                                                      const float* __restrict__ V, const float* __restrict__ m,
+                                                      const float* __restrict__ u, float gamma,
+                                                      const int32_t* __restrict__ perm, MomOut o,
+                                                      const float4* __restrict__ geo4) {
+  __shared__ int32_t tab[kTileWords];
+  __shared__ float4 sg[kTileCap];
+  __shared__ float2 sv[kTileCap];
+  const int32_t t = xcd_block();
+  tile_table(tiles, t, tab);
+  const bool wide = tab[0] < 0;
+  if (!wide) tile_fill(sg, geo4, tab, sv, (const float2*)o.vm2);
+  for (int round = 0; round < G; ++round) {
+    const Lane L = tile_lane<G>(t, round, n);
+    if (wide)
+      moments_row<D, G, K, false>(L, GlobalSrc<float2>{rows.nbr, geo4, o.vm2}, rows, Rows{}, geo, C, n, x, h, V, m,
+                                  u, gamma, perm, o, nullptr, nullptr, geo4);
+    else
+      moments_row<D, G, K, false>(L, TileSrc<float2>{ploc, rows.nbr, sg, sv}, rows, Rows{}, geo, C, n, x, h, V, m,
+                                  u, gamma, perm, o, nullptr, nullptr, geo4);
+  }
+}
+
 // ------------------------------------------------------------ state refresh
 // After the caller changed the state between the CRK factors and the rest of
 // the stage (boundary conditions, crkSPH.py:94): the equation of state from
@@ -904,7 +1114,7 @@ template <int D>
 __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* __restrict__ nbr,
                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ pnbr, int32_t n,
                                const int32_t* __restrict__ perm, const float* __restrict__ m_u, DtArg dt,
-                               const float* __restrict__ pf,
+                               const float* __restrict__ pf, const float* __restrict__ pv,
                                const float4* __restrict__ rec, const float* __restrict__ f,
                                float* __restrict__ ap_o, float* __restrict__ av_o, float* __restrict__ f_o) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -918,8 +1128,9 @@ __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* _
     const int64_t sl = b + p;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      if (ap_o) ap_o[(int64_t)q * D + a] = hit ? pf[sl * 2 * D + a] : 0.f;
-      if (av_o) av_o[(int64_t)q * D + a] = hit ? pf[sl * 2 * D + D + a] : 0.f;
+      const float va = hit ? pv[sl * D + a] : 0.f;
+      if (ap_o) ap_o[(int64_t)q * D + a] = hit ? pf[sl * D + a] - va : 0.f;
+      if (av_o) av_o[(int64_t)q * D + a] = va;
     }
     if (f_o) {
       float fv = 0.f;
@@ -929,7 +1140,7 @@ __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* _
         } else {
           float a[3];
 #pragma unroll
-          for (int d = 0; d < D; ++d) a[d] = pf[sl * 2 * D + d] + pf[sl * 2 * D + D + d];
+          for (int d = 0; d < D; ++d) a[d] = pf[sl * D + d];
           fv = pair_fraction<D>(ri, rec[nbr[q]], mdt, a);
         }
       }
@@ -1217,6 +1428,12 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
                            hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin, run_ids(ctx)));
     ctx->have_volume = true;
     ctx->have_vsig = vs != nullptr;
+  } else if (use_tiles(ctx)) {
+    CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,
+        hipLaunchKernelGGL((k_owen_t<DD, G, KK>), dim3(tile_count(n)), dim3(kBlock), 0, s, rows_of(ctx),
+                           ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->h), ctx->adj.perm,
+                           st->supports, F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut,
+                           (float)ctx->nh_target, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_owen<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
@@ -1288,22 +1505,32 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   float* vs = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN || cfg->switchParams.scheme == CRK_SWITCH_LEARNED)
                   ? F<float>(ctx->vsig) : nullptr;
   int GS = ctx->group[CRK_PASS_SUPPORT];
-  if (vs) {
+  const bool tiled = use_tiles(ctx);
+  CRK_REQUIRE(!tiled || ctx->adj.n_edges == 0 || ed->loc, "crk_support: the adjacency has tiles: pair buffer loc missing");
+  if (tiled) {
+#define SUPPORT_T(VS)                                                                                            \
+    CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,                                                              \
+        hipLaunchKernelGGL((k_support_t<DD, G, KK, VS>), dim3(tile_count(n)), dim3(kBlock), 0, s, rows_of(ctx), \
+                           ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, ed->nbr, ed->loc, ed->count, vs,       \
+                           F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4), F<float>(ctx->m),    \
+                           F<float2>(ctx->vm2)))
+    if (vs) { SUPPORT_T(true); } else { SUPPORT_T(false); }
+#undef SUPPORT_T
+  } else if (vs) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_support<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
-                           geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u), cfac,
-                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4),
-                           F<float>(ctx->m), F<float2>(ctx->vm2)));
+                           geo, C, n, ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4),
+                           F<float4>(ctx->vel4), F<float>(ctx->m), F<float2>(ctx->vm2)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_support<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
-                           geo, C, n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->u), cfac,
-                           ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4),
-                           F<float>(ctx->m), F<float2>(ctx->vm2)));
+                           geo, C, n, ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4),
+                           F<float4>(ctx->vel4), F<float>(ctx->m), F<float2>(ctx->vm2)));
   }
   CRK_LAUNCH_CHECK("k_support");
   ctx->pair_cnt = ed->count;
   ctx->pair_nbr = ed->nbr;
+  ctx->pair_loc = tiled ? ed->loc : nullptr;
   ctx->have_support = true;
   ctx->have_volume = true;
   ctx->have_vsig = vs != nullptr;
@@ -1331,7 +1558,14 @@ static int factors_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
                            F<float4>(ctx->geo4)));
     ctx->pair_cnt = ed->count;
     ctx->pair_nbr = ed->nbr;
+    ctx->pair_loc = nullptr;
     ctx->have_support = true;
+  } else if (use_tiles(ctx) && ctx->pair_loc) {
+    CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GM,
+        hipLaunchKernelGGL((k_moments_t<DD, G, KK>), dim3(tile_count(n)), dim3(kBlock), 0, s, arows_of(ctx),
+                           ctx->pair_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->x), F<float>(ctx->h),
+                           F<float>(ctx->V), F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo,
+                           F<float4>(ctx->geo4)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GM,
         hipLaunchKernelGGL((k_moments<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
@@ -1371,7 +1605,7 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
   CRK_REQUIRE(adj->n > 0 && adj->row_offsets && adj->perm && ed->count, "crk_edges_export: adjacency/pairs missing");
   if (adj->n_edges == 0) return CRK_OK;
   CRK_REQUIRE(adj->nbr && ed->nbr, "crk_edges_export: neighbour lists missing");
-  CRK_REQUIRE((!ap_out && !av_out) || ed->pf, "crk_edges_export: pair data missing");
+  CRK_REQUIRE((!ap_out && !av_out) || (ed->pf && ed->pv), "crk_edges_export: pair data missing");
   const DtArg dta = dt_arg(ctx, dt);
   CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->pf && (dt > 0.f || dta.p)),
               "crk_edges_export: f needs edges->f, or edges->rec with masses, pair forces and dt > 0");
@@ -1380,7 +1614,7 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(dim, hipLaunchKernelGGL(k_edges_export<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->row_offsets,
-                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->pf,
+                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->pf, ed->pv,
                                            (const float4*)ed->rec, ed->f, ap_out, av_out, f_out));
   CRK_LAUNCH_CHECK("k_edges_export");
   return CRK_OK;

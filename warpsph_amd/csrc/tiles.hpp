@@ -1,0 +1,127 @@
+// tiles.hpp -- LDS tiles of an adjacency.
+//
+// The sorted rows are grouped CRK_TILE_ROWS at a time (a tile).  Rows that are
+// consecutive in Morton order sit in a compact patch of cells, so every
+// neighbour of a tile's rows lies in a few contiguous ranges of sorted
+// indices ("runs": the patch's cells, merged where one ends where the next
+// begins).  crk_adjacency_tiles records each tile's runs and, for every
+// Verlet entry, the neighbour's position in the concatenation of its tile's
+// runs (a 16-bit local index).  A pair pass then copies the neighbour
+// records of one tile into LDS with coalesced loads and reads them there,
+// instead of gathering every neighbour through the texture path.
+//
+// Tile table, CRK_TILE_WORDS int32 per tile:
+//   [0] number of runs, or -1 for a "wide" tile (more than cap particles or
+//       CRK_TILE_MAXRUNS runs: its passes gather from global memory)
+//   [1] U, particles in the runs
+//   [4 + 2 r], [5 + 2 r]  first sorted index and local offset of run r;
+//   run nruns is a sentinel with local offset U.
+#pragma once
+
+#include "pairs.hpp"
+
+namespace crk {
+
+constexpr int kTileRows = CRK_TILE_ROWS;
+constexpr int kTileCap = CRK_TILE_CAP;
+constexpr int kTileRuns = CRK_TILE_MAXRUNS;
+constexpr int kTileWords = CRK_TILE_WORDS;
+
+// Copy tile t's table into LDS (every thread of the block calls; synchronises;
+// requires blockDim.x == kBlock).
+__device__ inline void tile_table(const int32_t* __restrict__ tiles, int32_t t, int32_t* sh) {
+  for (int w = threadIdx.x; w < kTileWords; w += blockDim.x) sh[w] = tiles[(int64_t)t * kTileWords + w];
+  __syncthreads();
+}
+
+// Sorted index of local slot u (binary search over the runs' offsets).
+__device__ inline int32_t tile_global(const int32_t* sh, int32_t u) {
+  int lo = 0, hi = sh[0];  // run lo .. hi - 1 holds u
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sh[5 + 2 * mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return sh[4 + 2 * lo] + (u - sh[5 + 2 * lo]);
+}
+
+// Stage the records of the tile's runs: dst[u] = src[global(u)], u < U
+// (and a second array the same way when src2 is given).  Synchronises
+// before returning.  (Holding several slots' loads in flight per thread was
+// measured slower: the registers it needs lower the pass's occupancy.)
+template <class T, class T2 = T>
+__device__ inline void tile_fill(T* dst, const T* __restrict__ src, const int32_t* sh, T2* dst2 = nullptr,
+                                 const T2* __restrict__ src2 = nullptr) {
+  const int32_t U = sh[1];
+  for (int32_t u = threadIdx.x; u < U; u += blockDim.x) {
+    const int32_t g = tile_global(sh, u);
+    const T a = src[g];
+    if (src2) {
+      const T2 b = src2[g];
+      dst2[u] = b;
+    }
+    dst[u] = a;
+  }
+  __syncthreads();
+}
+
+// Row k of round `round` of a tile whose rounds hold blockDim / G rows each
+template <int G>
+__device__ inline Lane tile_lane(int32_t t, int round, int32_t n) {
+  Lane l;
+  l.g = threadIdx.x % G;
+  l.k = t * kTileRows + round * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
+  l.valid = l.k < n;
+  if (!l.valid) l.k = n - 1;
+  return l;
+}
+
+// Neighbour sources of the pair passes: the global records through the
+// 32-bit neighbour ids, or the tile's LDS copy through the local indices.
+// Each source serves the row's neighbours' {x, h} record (geo) and one
+// pass-specific record (aux); gid(q) is the entry's sorted neighbour index.
+template <class A>
+struct GlobalSrc {
+  const int32_t* __restrict__ nbr;
+  const float4* __restrict__ g;
+  const A* __restrict__ a;
+  __device__ inline int32_t id(int64_t q) const { return nbr[q]; }
+  __device__ inline int32_t gid(int64_t q) const { return nbr[q]; }
+  __device__ inline float4 geo(int32_t j) const { return g[j]; }
+  __device__ inline A aux(int32_t j) const { return a[j]; }
+};
+template <class A>
+struct TileSrc {
+  const uint16_t* __restrict__ loc;
+  const int32_t* __restrict__ nbr;
+  const float4* g;  // LDS
+  const A* a;       // LDS
+  __device__ inline int32_t id(int64_t q) const { return loc[q]; }
+  __device__ inline int32_t gid(int64_t q) const { return nbr[q]; }
+  __device__ inline float4 geo(int32_t j) const { return g[j]; }
+  __device__ inline A aux(int32_t j) const { return a[j]; }
+};
+
+// the bound adjacency carries tiles the tiled passes can use (d <= 2)
+inline bool use_tiles(const CrkContext* c) { return c->adj.tiles && c->adj.nbr_loc && c->dim <= 2; }
+inline unsigned tile_count(int32_t n) { return nblocks(n, kTileRows); }
+
+static_assert(kTileRows == kBlock, "a tile's rows are processed by one block of kBlock threads in G rounds");
+
+}  // namespace crk
+
+// dispatch of the tiled kernels over the kernel function (KK), d in {1, 2}
+// (DD) and the group size (G)
+#define CRK_DIM12_G_DISPATCH(DIM, GS, ...)                                                     \
+  switch (DIM) {                                                                               \
+    CRK_D_CASE(1, CRK_G_SWITCH(GS, __VA_ARGS__))                                               \
+    CRK_D_CASE(2, CRK_G_SWITCH(GS, __VA_ARGS__))                                               \
+    default: ::crk::set_error("tiled passes: dim must be 1 or 2"); return CRK_EINVAL;         \
+  }
+#define CRK_KDG12_DISPATCH(KERN, DIM, GS, ...)                                                 \
+  switch (KERN) {                                                                              \
+    CRK_K_CASE(0, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
+    CRK_K_CASE(1, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
+    CRK_K_CASE(2, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
+    default: ::crk::set_error("unknown kernel %d", KERN); return CRK_EINVAL;                   \
+  }

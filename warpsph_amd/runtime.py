@@ -41,6 +41,9 @@ class Runtime:
         # deferred Verlet tests (graph replay): the tests set the context's
         # device flag instead of returning, and nothing is rebuilt
         self.deferred = False
+        # LDS tiles for the pair passes of 1D and 2D adjacencies (None: off);
+        # the value is the most particles one tile may stage
+        self.tile_cap = abi.CRK_TILE_CAP
 
     def stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -132,7 +135,30 @@ def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch
              abi.ptr(xb), abi.ptr(hb), abi.ptr(nbr), E.value, rt.stream())
     adj = AdjacencyList(n, grid, keys, perm, rank, cs, ce, off, nbr, xb.view(n, D), hb)
     adj.runs = runs
+    if rt.tile_cap and D <= 2:
+        build_tiles(adj, rt.tile_cap)
     return adj
+
+
+def build_tiles(adj, cap: int = abi.CRK_TILE_CAP):
+    """LDS tiles of an adjacency (crk_adjacency_tiles): the pair passes then
+    stage each tile's neighbour records in LDS."""
+    rt = runtime(adj.perm.device)
+    nt = (adj.n + abi.CRK_TILE_ROWS - 1) // abi.CRK_TILE_ROWS
+    adj.tiles = torch.empty(nt * abi.CRK_TILE_WORDS, dtype=torch.int32, device=adj.perm.device)
+    adj.nbr_loc = torch.empty(max(adj.n_edges, 1), dtype=torch.int16, device=adj.perm.device)
+    A = adj.cabi()
+    abi.call("crk_adjacency_tiles", rt.ctx, C.byref(A), abi.ptr(adj.keys), abi.ptr(adj.cell_start),
+             abi.ptr(adj.cell_end), int(cap), abi.ptr(adj.tiles), abi.ptr(adj.nbr_loc), rt.stream())
+    return adj
+
+
+def tile_stats(adj):
+    """(tiles, wide tiles, mean staged particles of the others) of a tiled adjacency."""
+    t = adj.tiles.view(-1, abi.CRK_TILE_WORDS).cpu()
+    wide = t[:, 0] < 0
+    U = t[~wide, 1].double()
+    return int(t.shape[0]), int(wide.sum()), float(U.mean()) if U.numel() else 0.0
 
 
 def adjacency_valid(cfg: abi.CrkConfig, adj, positions: torch.Tensor, supports: torch.Tensor, runs=None) -> bool:

@@ -184,6 +184,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     if pairs is None or rebuilt or pairs.adjacency is not adj:
         pairs = stage_pairs(adj)
     st.pairs = pairs
+    if fused_support and not rebuilt:
+        pairs.loc = None  # the moments pass compacts the pairs by their sorted ids only
     ed = st.pairs.cabi()
     if fused_support and not rebuilt:
         # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101) with the

@@ -58,14 +58,18 @@ class StagePairs:
         self.dtp = None
         self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)
         self.nbr = torch.empty(E, dtype=torch.int32, device=dev)
-        # pair forces per slot: ap_ij (dim) then av_ij (dim)
-        self.pf = torch.empty((E, 2, dim), dtype=torch.float32, device=dev)
+        # pair accelerations per slot: a_ij = ap_ij + av_ij and av_ij
+        self.pf = torch.empty((E, dim), dtype=torch.float32, device=dev)
+        self.pv = torch.empty((E, dim), dtype=torch.float32, device=dev)
         self.rec = torch.empty((adjacency.n, 4), dtype=torch.float32, device=dev)
+        # tile-local neighbour index of each pair (adjacencies with LDS tiles)
+        self.loc = torch.empty(E, dtype=torch.int16, device=dev) if adjacency.tiles is not None else None
         self._full = None
 
     def cabi(self) -> abi.CrkEdges:
         """Pointers for the C-ABI; f_ij stays implicit (evaluated from rec)."""
-        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.pf, self.rec)), None)
+        return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.pf, self.rec)), None,
+                            abi.ptr(self.loc).value, abi.ptr(self.pv).value)
 
     @property
     def n_pairs(self) -> int:
@@ -231,6 +235,10 @@ class AdjacencyList:
         self.hb = hb
         self.n_edges = int(nbr.shape[0])
         self.runs = None  # the ensemble run table it was built for (per-run grids)
+        # LDS tiles (runtime.build_tiles): per-tile run tables and the
+        # tile-local index of every entry's neighbour, or None
+        self.tiles = None
+        self.nbr_loc = None
         self._export = None
 
     @property
@@ -249,6 +257,9 @@ class AdjacencyList:
         a.rank = self.rank.data_ptr()
         a.row_offsets = self.row_offsets.data_ptr()
         a.nbr = self.nbr.data_ptr() if self.n_edges > 0 else None
+        if self.tiles is not None:
+            a.tiles = self.tiles.data_ptr()
+            a.nbr_loc = self.nbr_loc.data_ptr() if self.n_edges > 0 else None
         return a
 
     def _expand(self):

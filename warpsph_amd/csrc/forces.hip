@@ -328,11 +328,14 @@ __device__ inline float limiter_phi(const LimParams& lp, const float* dx, float 
     }
     gi = 0.25f * gi;
     gj = 0.25f * gj;
-    float ri = gi * frcp(gj);  // sgn(g)|g| = g, sgn(0) = +1
-    float rj = gj * frcp(gi);
-    if (ri != ri) ri = 1.f;
-    if (rj != rj) rj = 1.f;
-    float rr = ri < rj ? ri : rj;
+    // r_i = g_i / g_j, r_j = g_j / g_i (sgn(g)|g| = g) and min(r_i, r_j):
+    // for slopes of one sign min(|g_i|, |g_j|) / max(|g_i|, |g_j|) (one
+    // reciprocal instead of two); opposite signs or one zero slope give
+    // min <= 0; two zero slopes give 0/0 -> 1 in both ratios
+    const float agi = fabsf(gi), agj = fabsf(gj);
+    const bool same = (gi > 0.f && gj > 0.f) || (gi < 0.f && gj < 0.f);
+    float rr = same ? fminf(agi, agj) * frcp(fmaxf(agi, agj)) : 0.f;
+    if (gi == 0.f && gj == 0.f) rr = 1.f;
     float vl = 0.f;
     if (rr > 0.f) { float q = 2.f * frcp(1.f + rr); vl = rr * q * q; }
     phi = vl * factor;
@@ -445,6 +448,7 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
 #pragma unroll
   for (int a = 0; a < D * D; ++a) cri.gB[a] = ri[Rec<D>::GB + a];
   const float smi = hi * inv_kscale;
+  const float kscale = 1.f / inv_kscale;
   const float ismi = 1.f / smi;
   const float epsi = 1.0e-7f * smi * smi;
   const float ViOm = Vi / mi;
@@ -513,7 +517,7 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     // mu_i = (vdot_i - vdot_j).eta_i / (eta_i.eta_i + 1e-7 s_i^2), eta_i = x / s_i
     //      = s_i (w.x) / (r^2 + 1e-7 s_i^4) ... evaluated in the reference's form
     const float smj = hj * inv_kscale;
-    const float ismj = frcp(smj);
+    const float ismj = ihj * kscale;  // 1 / s_j without a second reciprocal
     const float rdi = frcp(r2 * ismi * ismi + epsi);
     const float rdj = frcp(r2 * ismj * ismj + 1.0e-7f * smj * smj);
     float wx = 0.f, wxU = 0.f;

@@ -187,27 +187,3 @@ def test_unbound_calls_after_an_ensemble_step():
     out = integ.function(s1, crkSPH_step, dt, config, scheme).state
     torch.cuda.synchronize()
     assert torch.isfinite(out.state.densities).all()
-
-
-def test_sedov_generator_on_the_gpu():
-    """buildSedov (sedov/initial.py:24-218), 'singular': the blast energy sits
-    in the particle at the origin, Owen supports and the gather density hold
-    rho0 in the undisturbed lattice, and the state steps."""
-    from warpsph_amd.generators import buildSedov
-    from warpsph_amd.integrator import getIntegrator
-    from warpsph_amd.scheme import crkSPH_step
-    announce("sedov generator")
-    system, config, scheme = buildSedov(nx=41, dim=2, device="cuda", targetNeighbors=32)
-    st = system.state
-    E = float((st.internalEnergies.double() * st.masses.double()).sum())
-    assert abs(E - 1.0) < 1e-6
-    assert int((st.internalEnergies > 0).sum()) == 1
-    rho = st.densities.double()
-    assert abs(float(rho.median()) - 1.0) < 0.02
-    assert torch.isfinite(st.supports).all() and float(st.supports.min()) > 0
-    integ = getIntegrator()
-    s = system
-    for _ in range(3):
-        s = integ.function(s, crkSPH_step, config.dt, config, scheme).state
-    torch.cuda.synchronize()
-    assert torch.isfinite(s.state.internalEnergies).all()

@@ -1,0 +1,174 @@
+"""Initial-condition generators on the GPU, called with the reference's
+signatures (examples/compressible/01-sod-shock-tube-1d.py:136-157,
+sod/sod.py:21-225, sedov/initial.py:24-218, kelvinHelmholtz/sample.py:17-75).
+
+The adaptive supports they return are checked as fixed points, by brute
+force in float64 over all pairs (periodic minimum image) with the oracle's
+kernel tables: Monaghan's h_i = H(m_i / rho_i(h_i)) and Owen's
+h_i = computeNewSupport(n_h target, n_h(psi_i), h_i) to within the
+iteration's stopping threshold, and the returned density is the gather
+density at the final supports."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from tests.test_gpu_parity import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(x, lo, hi, periodic):
+    d = x[:, None, :] - x[None, :, :]
+    L = hi - lo
+    for a in range(x.shape[1]):
+        if periodic[a]:
+            d[..., a] -= L[a] * np.round(d[..., a] / L[a])
+    return np.sqrt((d * d).sum(-1))
+
+
+def _gather(r, h, m, kernel, dim):
+    from oracle import oracle as O
+    C, _ = O.kernel_constants(kernel, dim)
+    k, dk = O.kernel_eval(kernel, dim, (r / h[:, None]).ravel())
+    k = k.reshape(r.shape)
+    dk = dk.reshape(r.shape)
+    rho = (m[None, :] * C * k / h[:, None] ** dim).sum(1)
+    return rho, C * np.abs(dk)
+
+
+def _fixed_point_residual(state, config, scheme_h):
+    """max |h_new / h - 1| of one more adaptive-support update at the state's
+    supports, and the gather density there."""
+    from oracle import oracle as O
+    from warpsph_amd.utils import nH_to_n_h, volumeToSupportHelper
+    dom = config.domain
+    x = state.positions.double().cpu().numpy()
+    h = state.supports.double().cpu().numpy()
+    m = state.masses.double().cpu().numpy()
+    dim = config.dim
+    r = _pairs(x, dom.min.double().cpu().numpy(), dom.max.double().cpu().numpy(),
+               [bool(p) for p in dom.periodic.cpu().tolist()])
+    kernel = config.kernel.value
+    rho, adk = _gather(r, h, m, kernel, dim)
+    if scheme_h == "monaghan":
+        H = volumeToSupportHelper(m / rho, config.targetNeighbors, dim)
+        return float(np.max(np.abs(H / h - 1))), rho
+    psi = adk.sum(1) ** (1.0 / dim)
+    nh_lut, psi_lut = O.owen_lut(kernel, dim)
+    nh = O.lut_interp(psi_lut, nh_lut, psi)
+    target = nH_to_n_h(config.targetNeighbors, dim)
+    hn = np.array([O.owen_new_support(target, float(a), float(b)) for a, b in zip(nh, h)])
+    return float(np.max(np.abs(hn / h - 1))), rho
+
+
+def _sod_config(nx):
+    from warpsph_amd.configurations import DomainDescription, buildConfig
+    from warpsph_amd.enumTypes import KernelFunctions
+    dev = torch.device("cuda")
+    dom = DomainDescription(torch.tensor([-1.0], device=dev), torch.tensor([1.0], device=dev),
+                            torch.tensor([True], device=dev), 1)
+    config, _ = buildConfig(device=dev, domain=dom, dim=1, kernel=KernelFunctions.Wendland4, targetNeighbors=6.0,
+                            dt=1e-3, adaptiveDt=True, cflFactor=0.3)
+    config.nx = nx
+    return config
+
+
+@pytest.mark.parametrize("scheme_h,smoothIC", [("owen", False), ("owen", True), ("monaghan", True)])
+def test_sod_tube_on_the_gpu(scheme_h, smoothIC):
+    from warpsph_amd.enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme
+    from warpsph_amd.generators import buildSod1D, sodInitialState
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import buildScheme
+    from warpsph_amd.sample import cfl_dt
+    announce(f"sod generator {scheme_h} smoothIC={smoothIC}")
+    config = _sod_config(256)
+    SimulationSystem, SimulationState, SimulationConfig, SimulationUpdate, fn, export_fn, import_fn = \
+        buildScheme(CompressibleSPHScheme.CRKSPH)
+    schemeConfig = SimulationConfig()
+    schemeConfig.gamma = 1.4
+    schemeConfig.adaptiveSupportScheme = AdaptiveSupportScheme.Owen if scheme_h == "owen" else \
+        AdaptiveSupportScheme.Monaghan
+    left, right = sodInitialState(1.0, 1.0, 0.0), sodInitialState(0.1, 0.125, 0.0)
+    compSystem = buildSod1D(SimulationSystem, SimulationState, 8, left, right, 1.4, config, smoothIC,
+                            adaptiveSupportScheme=schemeConfig.adaptiveSupportScheme)
+    st = compSystem.state
+    assert st.positions.shape == (256 + 32, 1) and compSystem.adjacency is not None
+    res, rho = _fixed_point_residual(st, config, scheme_h)
+    print(f"fixed-point residual {res:.2e}", flush=True)
+    assert res < 2e-3
+    if smoothIC:
+        # the densities are the gather densities at the adapted supports
+        assert np.max(np.abs(st.densities.double().cpu().numpy() / rho - 1)) < 1e-5
+    # config.dt is computeTimestep(dt=None) of the built state
+    assert config.dt == pytest.approx(cfl_dt(st, config, schemeConfig, None), rel=1e-12)
+    # the state steps
+    s = compSystem.initializeNewState()
+    for _ in range(3):
+        s = getIntegrator().function(s, fn, config.dt, config, schemeConfig).state
+    torch.cuda.synchronize()
+    assert torch.isfinite(s.state.internalEnergies).all() and torch.isfinite(s.state.positions).all()
+
+
+def test_sedov_on_the_gpu():
+    """'singular': the blast energy sits in the particle at the origin; Owen
+    supports at their fixed point and the gather density at them hold rho0
+    in the undisturbed lattice."""
+    from warpsph_amd.configurations import CRKSPHConfig, buildConfig, buildDomainDescription
+    from warpsph_amd.enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, KernelFunctions
+    from warpsph_amd.generators import buildSedov
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import buildScheme, crkSPH_step
+    announce("sedov generator")
+    SimulationSystem, SimulationState, *_ = buildScheme(CompressibleSPHScheme.CRKSPH)
+    dev = torch.device("cuda")
+    config, _ = buildConfig(device=dev, domain=buildDomainDescription(2.0, 2, periodic=True, device=dev), dim=2,
+                            kernel=KernelFunctions.Wendland4, targetNeighbors=32, dt=1e-4)
+    system = buildSedov(SimulationSystem, SimulationState, config, 41, 2, 2.0, True, 1.0, 1.0, 'singular',
+                        5 / 3, None, 32, torch.float32, dev)
+    st = system.state
+    E = st.internalEnergies.double() * st.masses.double()
+    assert abs(float(E.sum()) - 1.0) < 1e-6 and int((E > 0).sum()) == 1
+    assert float(torch.linalg.norm(st.positions[int(torch.argmax(E))])) < 1e-6
+    res, rho = _fixed_point_residual(st, config, "owen")
+    print(f"fixed-point residual {res:.2e}", flush=True)
+    assert res < 2e-3
+    assert np.max(np.abs(st.densities.double().cpu().numpy() / rho - 1)) < 1e-5
+    assert abs(float(st.densities.double().median()) - 1.0) < 0.02
+    scheme = CRKSPHConfig(gamma=5 / 3, adaptiveSupportScheme=AdaptiveSupportScheme.Owen)
+    s = system
+    for _ in range(3):
+        s = getIntegrator().function(s, crkSPH_step, config.dt, config, scheme).state
+    torch.cuda.synchronize()
+    assert torch.isfinite(s.state.internalEnergies).all()
+
+
+def test_kelvin_helmholtz_timestep_on_the_gpu():
+    """config.dt = computeTimestep(..., dt=config.dt) * 2/3 (kelvinHelmholtz/sample.py:73):
+    cflFactor min h / (c_s xi), clamped to [minDt, maxDt] and to
+    dtGrowthFactor times the configuration's previous dt."""
+    from warpsph_amd.configurations import CRKSPHConfig, DomainDescription, buildConfig, sphKernel_xi
+    from warpsph_amd.enumTypes import CompressibleSPHScheme
+    from warpsph_amd.generators import sampleKHH
+    from warpsph_amd.scheme import buildScheme
+    announce("kelvin-helmholtz generator")
+    SimulationSystem, SimulationState, *_ = buildScheme(CompressibleSPHScheme.CRKSPH)
+    dev = torch.device("cuda")
+    dom = DomainDescription(torch.tensor([0.0, 0.0], device=dev), torch.tensor([1.0, 1.0], device=dev),
+                            torch.tensor([True, True], device=dev), 2)
+    config, _ = buildConfig(device=dev, domain=dom, dim=2, targetNeighbors=32, dt=1e-3, maxDt=1e-2)
+    scheme = CRKSPHConfig(gamma=5 / 3)
+    system = sampleKHH(1.0, 2.0, 0.5, -0.5, 0.025, 0.05, 4, 0.01, 64, config, scheme, SimulationState,
+                       SimulationSystem)
+    st = system.state
+    u = st.internalEnergies.double().cpu().numpy()
+    c = np.sqrt(np.abs(u) * (5 / 3) * (2 / 3))
+    h = float(st.supports.double().min())
+    dt = 0.3 * h / (c * sphKernel_xi(config.kernel, 2))
+    dt = min(max(float(dt.min()), 1e-6), 1e-2)
+    # the lattice set-up (P = 0: c_s = 0) left config.dt at min(maxDt, 1.1 * 1e-3)
+    prev = min(1e-2, 1.1e-3)
+    if dt > prev:
+        dt = min(dt, 1.1 * prev)
+    assert config.dt == pytest.approx(dt * 2 / 3, rel=1e-5)

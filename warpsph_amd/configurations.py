@@ -100,6 +100,43 @@ def buildConfig(device=None, dtype=None, domain=None, dim=None, verletScale=None
 
 
 # ------------------------------------------------------------------ parameter blocks
+def configurationToDict(config: SimulationConfig) -> Dict[str, Any]:
+    """simulationConfig.py:125-155: the configuration as plain JSON values."""
+    dt = config.dt.detach().cpu().item() if isinstance(config.dt, torch.Tensor) else config.dt
+    return {
+        'device': str(config.device), 'dtype': str(config.dtype),
+        'domain': {'min': _host_values(config.domain.min), 'max': _host_values(config.domain.max),
+                   'periodic': [bool(b) for b in _host_values(config.domain.periodic)], 'dim': config.domain.dim},
+        'dim': config.dim, 'verletScale': config.verletScale, 'kernel': config.kernel.name,
+        'integrationScheme': config.integrationScheme.name, 'cflFactor': config.cflFactor, 'dt': dt,
+        'minDt': config.minDt, 'maxDt': config.maxDt, 'dtGrowthFactor': config.dtGrowthFactor,
+        'adaptiveDt': config.adaptiveDt, 'targetNeighbors': config.targetNeighbors,
+        'supportMode': config.supportMode.name, 'gradientMode': config.gradientMode.name,
+        'laplacianMode': config.laplacianMode.name, 'samplingScheme': config.samplingScheme.name,
+    }
+
+
+def dictToConfig(configDict: Dict[str, Any]) -> SimulationConfig:
+    """simulationConfig.py:157-200: the inverse of configurationToDict."""
+    device = torch.device(configDict['device'])
+    dtype = getattr(torch, configDict['dtype'].split('.')[-1])
+    dd = configDict['domain']
+    domain = DomainDescription(torch.tensor(dd['min'], dtype=dtype, device=device),
+                               torch.tensor(dd['max'], dtype=dtype, device=device),
+                               torch.tensor(dd['periodic'], dtype=torch.bool, device=device), int(dd['dim']))
+    dt = configDict.get('dt')
+    return SimulationConfig(
+        device=device, dtype=dtype, domain=domain, dim=int(configDict['dim']),
+        verletScale=float(configDict['verletScale']), kernel=KernelFunctions[configDict['kernel']],
+        integrationScheme=IntegrationSchemeType[configDict['integrationScheme']],
+        cflFactor=float(configDict['cflFactor']), dt=None if dt is None else float(dt),
+        minDt=configDict['minDt'], maxDt=configDict['maxDt'], dtGrowthFactor=configDict['dtGrowthFactor'],
+        adaptiveDt=bool(configDict['adaptiveDt']), targetNeighbors=float(configDict['targetNeighbors']),
+        supportMode=SupportScheme[configDict['supportMode']], gradientMode=GradientScheme[configDict['gradientMode']],
+        laplacianMode=LaplacianScheme[configDict['laplacianMode']],
+        samplingScheme=SamplingScheme[configDict['samplingScheme']])
+
+
 class ViscosityTerms(Enum):  # diffusionParameters.py:9-21
     Default = 0
     MonaghanGingold1983 = 1

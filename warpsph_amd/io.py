@@ -1,21 +1,26 @@
-"""Trajectory export and import with the reference's layout (io.py:8-284).
+"""Trajectory export and import with the reference's layout and API
+(src/warpSPH/io.py:8-307).
 
 File layout (one file per call, ``{exportPath}/trajectory/{tag}.h5``):
 
-  attrs: scheme (name), time, plus scalar extras
+  attrs: scheme (name), time, scalar extras; 'dict' / 'list' for the keys of
+         dictionary and list extras
   adjacency/   i, j, numNeighbors, edgeOffsets, queryPositions, querySupports,
                referencePositions, referenceSupports; attrs numRows, numCols
-  state/       one dataset per tensor field of the state; attrs for scalars
-               (and the string 'None' for absent fields)
+  state/       one dataset per tensor field of the state (ap_ij, av_ij, f_ij
+               over the adjacency edges); attrs for scalars, 'None' for
+               absent fields
   stages/stage_k/  attrs index; aux/{a}_adjacency, aux/{a}_state (as above);
-               one dataset per update field
-  dict_{key}/  extra dictionaries (datasets and attrs)
+               one dataset per tensor field of the update
+  dict_{key}/  dictionary and list extras (datasets, attrs, subgroups)
+  {key}        tensor extras
 
 With h5py installed the file is HDF5, loadable by the reference's
 importSimulationSystem.  Without it (h5py is not part of this image) the same
 hierarchy is written as one uncompressed .npz archive (``{tag}.h5.npz``):
 datasets under their slash-separated paths, attributes as JSON under
-``__attrs__``.  The API is the reference's either way.
+``__attrs__``, read back without unpickling.  importSimulationSystem takes
+either path.
 """
 from __future__ import annotations
 
@@ -43,7 +48,8 @@ class _Tree:
 
 
 class _Group:
-    """The part of the h5py Group interface the layout uses."""
+    """The part of the h5py Group interface the layout uses (children listed
+    in name order, as h5py lists them)."""
 
     def __init__(self, tree: _Tree, path: str = ""):
         self._t, self._path = tree, path
@@ -67,19 +73,29 @@ class _Group:
             return _Group(self._t, p)
         raise KeyError(name)
 
+    def __contains__(self, name: str) -> bool:
+        p = self._p(name)
+        return p in self._t.data or p in self._t.groups
+
     def keys(self):
         pre = f"{self._path}/" if self._path else ""
-        names = [p[len(pre):] for p in list(self._t.data) + sorted(self._t.groups) if p.startswith(pre)]
-        return [q for q in names if "/" not in q]
+        names = [p[len(pre):] for p in list(self._t.data) + list(self._t.groups) if p.startswith(pre)]
+        return sorted(q for q in names if "/" not in q)
+
+    def items(self):
+        return [(k, self[k]) for k in self.keys()]
+
+    def close(self):
+        pass
 
 
 def _json(v):
-    if isinstance(v, np.integer):
-        return int(v)
-    if isinstance(v, np.floating):
-        return float(v)
+    if isinstance(v, np.generic):
+        return v.item()
     if isinstance(v, np.ndarray):
         return v.tolist()
+    if isinstance(v, torch.Tensor):
+        return v.detach().cpu().tolist()
     return v
 
 
@@ -105,20 +121,46 @@ def _load_tree(path: str) -> _Group:
     return _Group(t, "")
 
 
-# ------------------------------------------------------------ the layout
+def _open_write(path: str):
+    return h5py.File(path, 'w') if h5py is not None else _Group(_Tree())
+
+
+def _open_read(path: str):
+    if os.path.exists(path) and not path.endswith('.npz'):
+        if h5py is None:
+            raise RuntimeError(f"{path} is an HDF5 file and h5py is not installed")
+        return h5py.File(path, 'r')
+    return _load_tree(path if path.endswith('.npz') else path + '.npz')
+
+
+def _is_dataset(v) -> bool:
+    return isinstance(v, np.ndarray) or (h5py is not None and isinstance(v, h5py.Dataset))
+
+
 def _np(t):
     return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
 
 
+def _tensor(v, device):
+    """A dataset as a tensor of its stored dtype (0-d datasets as 0-d tensors),
+    as hdfDtypeToTorchDtype keeps it (io.py:128-142)."""
+    a = np.asarray(v[()])
+    return torch.from_numpy(np.array(a)).to(device)
+
+
+def _attr(v):
+    return v.item() if isinstance(v, np.generic) else v
+
+
+# ------------------------------------------------------------ export
 def dumpAdjacency(adjacency, adjacencyGroup):
-    """io.py:8-20."""
+    """io.py:8-20.  The list is symmetric: query and reference positions
+    (supports) are the ones it was built at, in user order."""
     adjacencyGroup.create_dataset('i', data=_np(adjacency.i))
     adjacencyGroup.create_dataset('j', data=_np(adjacency.j))
     adjacencyGroup.create_dataset('numNeighbors', data=_np(adjacency.numNeighbors))
     adjacencyGroup.create_dataset('edgeOffsets', data=_np(adjacency.edgeOffsets))
-    # positions and supports the list was built at, in user order
     perm = adjacency.perm.long()
-    D = adjacency.xb.shape[1]
     xu = torch.empty_like(adjacency.xb)
     hu = torch.empty_like(adjacency.hb)
     xu[perm] = adjacency.xb
@@ -131,14 +173,20 @@ def dumpAdjacency(adjacency, adjacencyGroup):
 
 
 def dumpState(state, stateGroup):
-    """io.py:22-33: tensors as datasets, scalars as attrs, absent fields as 'None'
-    (the pair data ap_ij, av_ij, f_ij as datasets over the adjacency edges)."""
+    """io.py:22-33: tensors as datasets, scalars as attrs, absent fields as
+    'None'; the pair data as the datasets ap_ij, av_ij, f_ij over the
+    adjacency edges."""
     import dataclasses
     for f in dataclasses.fields(state):
         name = f.name
+        if name.startswith('_'):
+            continue
         if name == "pairs":
-            if state.pairs is not None:
-                for k, v in (("ap_ij", state.ap_ij), ("av_ij", state.av_ij), ("f_ij", state.f_ij)):
+            for k in ("ap_ij", "av_ij", "f_ij"):
+                v = getattr(state, k)
+                if v is None:
+                    stateGroup.attrs[k] = 'None'
+                else:
                     stateGroup.create_dataset(k, data=_np(v))
             continue
         v = getattr(state, name)
@@ -150,54 +198,92 @@ def dumpState(state, stateGroup):
             stateGroup.attrs[name] = v
 
 
-def dumpStage(stage, stageGroups, index, exportStagesAdjacency=True):
-    """io.py:35-57."""
+def dumpStage(stage, stageGroups, index, SimulationState, exportStagesAdjacency):
+    """io.py:35-57: aux entries (adjacency lists, states) and the update's
+    tensor fields of stage ``index``."""
+    from .systems import AdjacencyList
+    import dataclasses
     g = stageGroups[index]
     g.attrs['index'] = index
     aux = g.create_group('aux')
     for ai, a in enumerate(stage.aux):
-        if hasattr(a, 'perm'):
+        if isinstance(a, AdjacencyList):
             if exportStagesAdjacency:
                 dumpAdjacency(a, aux.create_group(f'{ai}_adjacency'))
-        else:
+        elif isinstance(a, SimulationState):
             dumpState(a, aux.create_group(f'{ai}_state'))
-    import dataclasses
+        else:
+            print(f'Unsupported aux type: {type(a)}')
     for f in dataclasses.fields(stage.update):
         v = getattr(stage.update, f.name)
         if isinstance(v, torch.Tensor):
             g.create_dataset(f.name, data=_np(v))
 
 
-def exportSimulationSystem(system, scheme, exportPath: str, tag: str, stages=None, exportAdjacency: bool = True,
-                           exportStagesAdjacency: bool = True, extras: Optional[Dict[str, Any]] = None) -> str:
-    """io.py:63-127.  Returns the path written."""
-    os.makedirs(f'{exportPath}/trajectory', exist_ok=True)
+def _scheme_name(scheme):
+    return scheme.name if hasattr(scheme, 'name') else scheme
+
+
+def _dump_extra(root, key, value):
+    """io.py:92-125: dictionaries and lists as groups dict_{key} (marked by a
+    'dict' / 'list' attribute of the key), tensors as datasets, the rest as
+    attributes."""
+    if isinstance(value, dict):
+        eg = root.create_group(f'dict_{key}')
+        for sk, sv in value.items():
+            if isinstance(sv, torch.Tensor):
+                eg.create_dataset(sk, data=_np(sv))
+            else:
+                eg.attrs[sk] = sv
+        root.attrs[key] = 'dict'
+    elif isinstance(value, list):
+        eg = root.create_group(f'dict_{key}')
+        for si, sv in enumerate(value):
+            if isinstance(sv, torch.Tensor):
+                eg.create_dataset(f'{si}', data=_np(sv))
+            elif isinstance(sv, dict):
+                sub = eg.create_group(f'{si}')
+                for ssk, ssv in sv.items():
+                    if isinstance(ssv, torch.Tensor):
+                        sub.create_dataset(ssk, data=_np(ssv))
+                    else:
+                        sub.attrs[ssk] = ssv
+            else:
+                eg.attrs[f'{si}'] = sv
+        root.attrs[key] = 'list'
+    elif isinstance(value, torch.Tensor):
+        root.create_dataset(key, data=_np(value))
+    else:
+        root.attrs[key] = value
+
+
+def exportSimulationSystem(exportPath, tag, scheme, system: Any, exportAdjacency=False, stages=None,
+                           exportStagesAdjacency=True, extraData=None):
+    """io.py:63-127: write ``{exportPath}/trajectory/{tag}.h5``.
+
+    As in the reference, the system's adjacency is written whenever it has
+    one (``exportAdjacency`` is accepted and does not change that), each
+    stage's adjacency when ``exportStagesAdjacency``.  Returns the path
+    written (``{tag}.h5.npz`` without h5py)."""
+    os.makedirs(f'{exportPath}/trajectory/', exist_ok=True)
     path = f'{exportPath}/trajectory/{tag}.h5'
-    root = h5py.File(path, 'w') if h5py is not None else _Group(_Tree())
-    root.attrs['scheme'] = scheme.name if hasattr(scheme, 'name') else str(scheme)
+    root = _open_write(path)
+    root.attrs['scheme'] = _scheme_name(scheme)
     t = system.t
+    if isinstance(t, torch.Tensor):
+        t = t.detach().cpu()
+        t = t.item() if t.ndim == 0 else t.numpy()
     root.attrs['time'] = float(t) if np.ndim(t) == 0 else np.asarray(t, dtype=np.float64)
-    if exportAdjacency and system.adjacency is not None:
+    if system.adjacency is not None:
         dumpAdjacency(system.adjacency, root.create_group('adjacency'))
     dumpState(system.state, root.create_group('state'))
-    if stages:
+    if stages is not None:
         sg = root.create_group('stages')
         groups = [sg.create_group(f'stage_{k}') for k in range(len(stages))]
-        for k, st in enumerate(stages):
-            dumpStage(st, groups, k, exportStagesAdjacency)
-    for key, value in (extras or {}).items():
-        if isinstance(value, dict):
-            eg = root.create_group(f'dict_{key}')
-            for sk, sv in value.items():
-                if isinstance(sv, torch.Tensor):
-                    eg.create_dataset(sk, data=_np(sv))
-                else:
-                    eg.attrs[sk] = sv
-            root.attrs[key] = 'dict'
-        elif isinstance(value, torch.Tensor):
-            root.create_dataset(key, data=_np(value))
-        else:
-            root.attrs[key] = value
+        for k in range(len(stages)):
+            dumpStage(stages[k], groups, k, type(system.state), exportStagesAdjacency)
+    for key, value in (extraData.items() if extraData is not None else ()):
+        _dump_extra(root, key, value)
     if h5py is not None:
         root.close()
         return path
@@ -205,34 +291,150 @@ def exportSimulationSystem(system, scheme, exportPath: str, tag: str, stages=Non
     return path + '.npz'
 
 
-def _open(path: str):
-    if h5py is not None and not path.endswith('.npz'):
-        return h5py.File(path, 'r')
-    return _load_tree(path if path.endswith('.npz') else path + '.npz')
+# ------------------------------------------------------------ import
+def loadAdjacency(adjacencyGroup, device):
+    """io.py:145-160: the adjacency of the file, with the CSR the passes use
+    (AdjacencyList.fromExport)."""
+    from .systems import AdjacencyList
+    g = {k: _tensor(adjacencyGroup[k], device) for k in
+         ('i', 'j', 'numNeighbors', 'edgeOffsets', 'referencePositions', 'referenceSupports')}
+    return AdjacencyList.fromExport(g['i'], g['j'], g['numNeighbors'], g['edgeOffsets'], g['referencePositions'],
+                                    g['referenceSupports'])
 
 
-def loadState(stateGroup, device):
-    """io.py:163-172: the state's tensor fields (attrs: scalars)."""
-    from .systems import CompSPHState
+def loadState(stateGroup, device, SimulationState):
+    """io.py:162-170: the state's datasets (and its scalar attributes, where
+    the state has such a field); ap_ij, av_ij and f_ij become the state's
+    pair data."""
     import dataclasses
-    kw = {}
-    for f in dataclasses.fields(CompSPHState):
-        if f.name == "pairs":
+    from .systems import StoredPairs
+    names = {f.name for f in dataclasses.fields(SimulationState)}
+    kw, pair = {}, {}
+    for k, v in stateGroup.items():
+        if not _is_dataset(v):
             continue
-        if f.name in stateGroup.keys():
-            kw[f.name] = torch.as_tensor(np.asarray(stateGroup[f.name])).to(device)
-        elif f.name in stateGroup.attrs and stateGroup.attrs[f.name] != 'None':
-            kw[f.name] = stateGroup.attrs[f.name]
-    return CompSPHState(**kw)
+        if k in ('ap_ij', 'av_ij', 'f_ij') and 'pairs' in names:
+            pair[k] = _tensor(v, device)
+        elif k in names:
+            kw[k] = _tensor(v, device)
+    for k, v in stateGroup.attrs.items():
+        if k in names and k not in kw and not (isinstance(v, str) and v == 'None'):
+            kw[k] = _attr(v)
+    if pair:
+        kw['pairs'] = StoredPairs(**pair)
+    return SimulationState(**kw)
 
 
-def importSimulationSystem(importPath: str, device='cuda'):
-    """io.py:208-284 (state, time, scheme name; the adjacency is rebuilt by
-    the next step).  Returns (system, schemeName, file handle)."""
-    from .systems import CompSPHSystem
-    f = _open(importPath)
-    t = f.attrs['time']
-    scheme = f.attrs['scheme']
-    state = loadState(f['state'], device)
-    system = CompSPHSystem(state=state, adjacency=None, t=t if np.ndim(t) else float(t))
-    return system, scheme, f
+def loadStage(stageGroup, device, SimulationState, SimulationUpdate):
+    """io.py:173-191: StageResult(aux=[adjacency, state], update)."""
+    from .integrator import StageResult
+    aux = []
+    for k, v in stageGroup['aux'].items():
+        if '_adjacency' in k:
+            aux.append(loadAdjacency(v, device))
+        elif '_state' in k:
+            aux.append(loadState(v, device, SimulationState))
+        else:
+            print(f'Unsupported aux type for key {k}')
+    upd = {k: _tensor(v, device) for k, v in stageGroup.items() if k != 'aux' and _is_dataset(v)}
+    return StageResult(aux=aux, update=SimulationUpdate(**upd))
+
+
+def schemeNameToSimulationScheme(name: str):
+    """io.py:194-203 (the compressible schemes)."""
+    from .enumTypes import CompressibleSPHScheme
+    for s in CompressibleSPHScheme:
+        if s.name.lower() == name.lower():
+            return s
+    raise ValueError(f'Unsupported scheme name: {name}')
+
+
+def _load_extra(group, value, device):
+    if _is_dataset(value):
+        return _tensor(value, device)
+    if isinstance(value, _Group) or (h5py is not None and isinstance(value, h5py.Group)):
+        return {k: _load_extra(group, v, device) for k, v in value.items()}
+    return _attr(value)
+
+
+def importSimulationSystem(importPath, device, dtype, SimulationSystem: Optional[Any] = None,
+                           SimulationState: Optional[Any] = None, SimulationUpdate: Optional[Any] = None):
+    """io.py:208-270.  Returns (system, stages, schemeEnum, extraData): the
+    system with its state and adjacency (None when the file has none), the
+    stages as StageResult(aux, update), the scheme, and the extras (tensor
+    extras as tensors, dictionaries and lists rebuilt).  Tensors keep their
+    stored dtypes, as in the reference (``dtype`` is accepted for its
+    signature)."""
+    from .scheme import buildScheme
+    f = _open_read(importPath)
+    try:
+        t = _attr(f.attrs['time'])
+        schemeEnum = schemeNameToSimulationScheme(str(f.attrs['scheme']))
+        if SimulationSystem is None or SimulationState is None or SimulationUpdate is None:
+            SimulationSystem, SimulationState, _, SimulationUpdate, _, _, _ = buildScheme(schemeEnum)
+        adjacency = loadAdjacency(f['adjacency'], device) if 'adjacency' in f else None
+        state = loadState(f['state'], device, SimulationState)
+        stages = []
+        if 'stages' in f:
+            sg = f['stages']
+            for k in sorted(sg.keys(), key=lambda s: int(s.split('_')[-1])):
+                stages.append(loadStage(sg[k], device, SimulationState, SimulationUpdate))
+        extraData = {}
+        for k, v in f.items():
+            if k in ('adjacency', 'state', 'stages') or k.startswith('dict_'):
+                continue
+            if _is_dataset(v):
+                extraData[k] = _tensor(v, device)
+        for k, v in f.attrs.items():
+            if k in ('scheme', 'time'):
+                continue
+            v = _attr(v)
+            if isinstance(v, str) and v in ('dict', 'list'):
+                grp = f[f'dict_{k}']
+                if v == 'dict':
+                    d = {sk: _load_extra(grp, sv, device) for sk, sv in grp.items()}
+                    d.update({sk: _attr(sv) for sk, sv in grp.attrs.items()})
+                    extraData[k] = d
+                else:
+                    items = {sk: _load_extra(grp, sv, device) for sk, sv in grp.items()}
+                    items.update({sk: _attr(sv) for sk, sv in grp.attrs.items()})
+                    for sk, sv in list(items.items()):
+                        if isinstance(sv, dict):
+                            sub = grp[sk]
+                            sv.update({a: _attr(b) for a, b in sub.attrs.items()})
+                    extraData[k] = [items[sk] for sk in sorted(items, key=int)]
+            else:
+                extraData[k] = v
+    finally:
+        f.close()
+    if isinstance(t, (list, np.ndarray)):
+        t = np.asarray(t, dtype=np.float64)
+    return SimulationSystem(state=state, adjacency=adjacency, t=t), stages, schemeEnum, extraData
+
+
+# ------------------------------------------------------------ case configuration
+def prepExport(caseName, config, schemeConfig, scheme, export_fn):
+    """io.py:280-303: write export/{caseName}/config.json (scheme name, the
+    simulation configuration, the scheme's parameters through ``export_fn``,
+    a timestamp) and return the case's export path."""
+    from .configurations import configurationToDict
+    from .utils import getCurrentTimestamp
+    exportDict = {
+        'scheme': _scheme_name(scheme),
+        'config': configurationToDict(config),
+        'schemeConfig': export_fn(schemeConfig),
+        'timestamp': getCurrentTimestamp(),
+    }
+    exportPath = f'export/{caseName}'
+    os.makedirs(exportPath, exist_ok=True)
+    with open(f'{exportPath}/config.json', 'w') as fh:
+        json.dump(exportDict, fh, indent=4)
+    return exportPath
+
+
+def importConfigs(configPath, import_fn):
+    """io.py:307-312: (SimulationConfig, scheme parameters) of a config.json."""
+    from .configurations import dictToConfig
+    with open(configPath, 'r') as fh:
+        d = json.load(fh)
+    return dictToConfig(d['config']), import_fn(d['schemeConfig'])

@@ -95,6 +95,19 @@ class StagePairs:
         return self._full
 
 
+class StoredPairs:
+    """Pair data read back from a trajectory file: ap_ij, av_ij and f_ij over
+    the edges of the stage's adjacency, in the reference's layout
+    (src/warpSPH/systems/compSPH.py:30-32); ``full()`` returns them as a
+    StagePairs would."""
+
+    def __init__(self, ap_ij=None, av_ij=None, f_ij=None):
+        self._full = (ap_ij, av_ij, f_ij)
+
+    def full(self):
+        return self._full
+
+
 @dataclass
 class CompSPHState:
     positions: torch.Tensor = None
@@ -240,6 +253,46 @@ class AdjacencyList:
         self.tiles = None
         self.nbr_loc = None
         self._export = None
+
+    @classmethod
+    def fromExport(cls, i, j, numNeighbors, edgeOffsets, referencePositions, referenceSupports):
+        """The adjacency of the reference's export form (src/warpSPH/io.py:8-20,
+        145-160): edges grouped by row, row u's numNeighbors[u] edges starting
+        at edgeOffsets[u], i and j user indices.  The rows' order in the edge
+        arrays becomes the sorted row order (``perm``), so that the edges keep
+        their slots and a re-export reproduces the arrays; the positions and
+        supports it was built at are those of the file.  No cell table or LDS
+        tiles are attached: the pair passes read it through its neighbour ids.
+        Raises ValueError when the arrays do not describe such a grouping."""
+        dev = j.device
+        nn = numNeighbors.to(torch.int64)
+        off = edgeOffsets.to(torch.int64)
+        n = int(nn.shape[0])
+        E = int(j.shape[0])
+        if off.shape[0] != n or i.shape[0] != E or int(nn.sum()) != E:
+            raise ValueError(f"adjacency arrays disagree: {n} rows, {int(nn.sum())} counted edges, "
+                             f"{int(i.shape[0])} i, {E} j, {int(off.shape[0])} offsets")
+        # an empty row sorts before the row that starts at the same offset
+        order = torch.argsort(off * 2 + (nn > 0).to(torch.int64), stable=True)
+        start = torch.cumsum(nn[order], 0) - nn[order]
+        if not torch.equal(off[order], start):
+            raise ValueError("edgeOffsets do not tile the edge arrays row by row")
+        rows = torch.repeat_interleave(order, nn[order])
+        if not torch.equal(i.to(torch.int64), rows):
+            raise ValueError("edge i does not match the row its offset places it in")
+        jj = j.to(torch.int64)
+        if E and (int(jj.min()) < 0 or int(jj.max()) >= n):
+            raise ValueError("edge j out of range")
+        perm = order.to(torch.int32)
+        rank = torch.empty(n, dtype=torch.int32, device=dev)
+        rank[order] = torch.arange(n, dtype=torch.int32, device=dev)
+        row_offsets = torch.cat([start, torch.tensor([E], dtype=torch.int64, device=dev)]).to(torch.int32)
+        nbr = rank[jj].contiguous()
+        xb = referencePositions[order].to(torch.float32).contiguous()
+        hb = referenceSupports[order].to(torch.float32).contiguous()
+        adj = cls(n, None, None, perm, rank, None, None, row_offsets, nbr, xb, hb)
+        adj._export = (i.to(torch.int64), jj, nn, off)
+        return adj
 
     @property
     def numRows(self):

@@ -1,4 +1,4 @@
-"""Support-radius helpers with the names and signatures of
+"""Support-radius helpers (and the export timestamp) with the names and signatures of
 src/warpSPH/utils/support.py:6-44.
 
 A support radius H around a particle of volume V holds the particles of the
@@ -42,3 +42,9 @@ def volumeToSupportHelper(volume, targetNeighbors, dim):
     """Support radius H holding ``targetNeighbors`` particles of ``volume``:
     H = (n_H V / omega_d)^(1/d)."""
     return _root(targetNeighbors * volume / _unit_ball(dim), dim)
+
+
+def getCurrentTimestamp():
+    """Local time as YYYY-MM-DD_HH-MM-SS (src/warpSPH/utils/util.py:3-4)."""
+    import datetime
+    return datetime.datetime.now().strftime("%Y-%m-%d_%H-%M-%S")

@@ -135,6 +135,25 @@ def run_gpu(nx, steps, warmup, timer=True, tuning_file=None, retune=False):
             groups, spread)
 
 
+def host_description(threads):
+    """The CPU the baseline ran on: nproc, the lscpu model name, the OpenMP
+    settings and the threads the oracle used."""
+    import platform
+    import subprocess
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "threads_used": threads}
+
+
 def cpu_baseline(nx, nsteps, warm):
     """C++/OpenMP CPU oracle (FP32, Kahan) on an nx^2 sample; returns the
     rate and the oracle's final density for the accuracy check."""
@@ -499,7 +518,9 @@ def main():
         rate, rho_orc, dt_c, threads, wall = cpu_baseline(args.cpu_nx, args.cpu_steps, 1)
         cpu = {"value": rate, "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"C++/OpenMP FP32 CPU oracle, {args.cpu_nx}x{args.cpu_nx} shear layer, "
-                         f"{args.cpu_steps} RK2 steps after 1 warm-up step ({wall:.1f} s)"}
+                         f"{args.cpu_steps} RK2 steps after 1 warm-up step ({wall:.1f} s)",
+               "label": "build's C++ CPU oracle, not the reference Warp CPU path",
+               "host": host_description(threads)}
         # density L1 of the GPU against the oracle on the same sample and steps
         from warpsph_amd import sample
         from warpsph_amd.integrator import getIntegrator

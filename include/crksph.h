@@ -474,6 +474,11 @@ enum {
   CRK_PASS_SUPPORT = 8, CRK_NPASS = 9
 };
 int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group);
+/* Threads per block of a pass that runs on LDS tiles (one block per tile of
+ * CRK_TILE_ROWS rows): 256, 512 or 1024; chosen by the autotuner with G.
+ * Launch configuration only (no reference counterpart: the reference's Warp
+ * launches one thread per particle, accel.py:477-502). */
+int crk_set_block_size(CrkContext* ctx, int32_t pass, int32_t threads);
 
 /* Pair data of one stage in the reference layout (ap_ij, av_ij, f_ij over the
  * adjacency edges, src/warpSPH/systems/compSPH.py:30-32): slot data of

@@ -147,3 +147,37 @@ def test_tiled_passes_equal_untiled(name):
         step = _one_step(name, cap)
         for k in ref_step:
             assert relerr(step[k], ref_step[k]) <= 1e-6, (label, k, relerr(step[k], ref_step[k]))
+
+
+@pytest.mark.parametrize("name", ["shear2d", "sod1d"])
+def test_tile_block_sizes_give_identical_steps(name):
+    """Every tiled pass run with 256, 512 and 1024 threads per tile block,
+    including shapes whose rounds are wider than the tile (G = 1 or 2 with
+    512 or 1024 threads: the lanes past the tile's last row idle): a row is
+    computed by the same code whichever round holds it, so one RK2 step is
+    bit-identical for every shape."""
+    from warpsph_amd import abi
+    from warpsph_amd.autotune import TILED, set_groups
+    from warpsph_amd.runtime import runtime
+    announce(f"tile block sizes {name}")
+    rt = runtime("cuda:0")
+    keep = dict(getattr(rt, "groups", {}))
+    try:
+        for g in (1, 4, 16):  # the lane-group size changes the summation order: compare within one G
+            base = None
+            for tb in (256, 512, 1024):
+                shape = {}
+                for p in TILED:
+                    shape[p] = g
+                    shape[p + ".threads"] = tb
+                set_groups("cuda:0", shape)
+                got = _one_step(name, abi.CRK_TILE_CAP)
+                if base is None:
+                    base = got
+                    continue
+                for k in base:
+                    assert np.array_equal(got[k], base[k]), (g, tb, k)
+    finally:
+        restore = {p: keep.get(p, 4) for p in TILED}
+        restore.update({p + ".threads": keep.get(p + ".threads", 1024 if p == "accel" else 256) for p in TILED})
+        set_groups("cuda:0", restore)

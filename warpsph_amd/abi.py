@@ -169,6 +169,7 @@ _PROTOS = {
                               C.c_float, C.c_float, C.POINTER(CrkAdjacency), C.POINTER(CrkEdges),
                               C.POINTER(CrkAdjacency), C.POINTER(CrkEdges), _P, _P, _P],
     "crk_set_group_size": [_P, C.c_int32, C.c_int32],
+    "crk_set_block_size": [_P, C.c_int32, C.c_int32],
     "crk_struct_sizes": [C.POINTER(C.c_int64), C.c_int32],
     "crk_cfl_dt": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, C.c_double,
                    C.POINTER(C.c_double), _P],

@@ -1,11 +1,16 @@
 """Launch-shape autotuner of the CRKSPH passes.
 
-Every pair pass runs with G lanes per particle row (crk_set_group_size).
-The best G depends on the pass (register pressure, number of compensated
-accumulators to combine, edge-array traffic), the dimension and the
-neighbour count, so it is measured: for each pass in turn, every candidate G
-is timed over full RK2 steps of the caller's workload on a scratch copy of
-the state (HIP events on the launch stream), and the fastest is kept.
+Every pair pass runs with G lanes per particle row (crk_set_group_size), and
+the passes that stage LDS tiles run one block of 256, 512 or 1024 threads per
+tile (crk_set_block_size).  The best shape depends on the pass (register
+pressure and LDS per block, hence occupancy; number of compensated
+accumulators to combine; edge-array traffic), the dimension and the
+neighbour count, so it is measured: for each pass in turn, every candidate
+(G, threads) is timed over full RK2 steps of the caller's workload on a
+scratch copy of the state (HIP events on the launch stream), and the fastest
+is kept.
+
+A tuning is a dict {pass: G, "<pass>.threads": threads}.
 """
 from __future__ import annotations
 
@@ -26,10 +31,18 @@ SCOPES = {
 }
 
 
+# passes that run on LDS tiles (their block size is a knob too)
+TILED = ("owen", "support", "moments", "velgrad", "accel")
+THREADS = (256, 512, 1024)
+
+
 def set_groups(device, groups: Dict[str, int]):
     rt = runtime(device)
     for name, g in groups.items():
-        abi.call("crk_set_group_size", rt.ctx, abi.PASSES.index(name), int(g))
+        if name.endswith(".threads"):
+            abi.call("crk_set_block_size", rt.ctx, abi.PASSES.index(name[:-len(".threads")]), int(g))
+        else:
+            abi.call("crk_set_group_size", rt.ctx, abi.PASSES.index(name), int(g))
     rt.groups = dict(getattr(rt, "groups", {}), **groups)
 
 
@@ -48,7 +61,8 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
     integ = getIntegrator()
     current = dict(getattr(rt, "groups", {}))
     defaults = {"owen": 4, "support": 32, "moments": 2, "velgrad": 8, "switch": 8, "accel": 16, "balance": 32,
-                "compat": 32}
+                "compat": 32, "owen.threads": 256, "support.threads": 256, "moments.threads": 256,
+                "velgrad.threads": 256, "accel.threads": 1024}
     for k, v in defaults.items():
         current.setdefault(k, v)
     set_groups(dev, current)
@@ -63,9 +77,12 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
         names.remove("switch")
     report = {}
     for name in names:
-        best, best_t = current[name], None
-        for g in candidates:
-            set_groups(dev, {name: g})
+        tkey = name + ".threads"
+        shapes = [(g, tb) for g in candidates for tb in (THREADS if name in TILED else (None,))]
+        best, best_t = (current[name], current.get(tkey)), None
+        for g, tb in shapes:
+            shape = {name: g} if tb is None else {name: g, tkey: tb}
+            set_groups(dev, shape)
             s = _copy(base)
             integ.function(s, crkSPH_step, dt, config, scheme)  # warm this shape
             with PassTimer() as pt:
@@ -75,13 +92,17 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
             if SCOPES[name] not in summ:  # the pass does not run in this configuration
                 break
             t = summ[SCOPES[name]][1]
-            report.setdefault(name, {})[g] = t
+            report.setdefault(name, {})[g if tb is None else f"{g}/{tb}"] = t
             if best_t is None or t < best_t:
-                best, best_t = g, t
-        current[name] = best
-        set_groups(dev, {name: best})
+                best, best_t = (g, tb), t
+        current[name] = best[0]
+        shape = {name: best[0]}
+        if best[1] is not None:
+            current[tkey] = best[1]
+            shape[tkey] = best[1]
+        set_groups(dev, shape)
         if verbose and name in report:
-            print(f"autotune {name}: " + ", ".join(f"G={g}:{t:.3f}ms" for g, t in report[name].items()) + f" -> {best}")
+            print(f"autotune {name}: " + ", ".join(f"{k}:{t:.3f}ms" for k, t in report[name].items()) + f" -> {best}")
     rt.tuning_report = report
     return current
 

@@ -83,6 +83,9 @@ struct CrkContext {
   bool have_volume = false;  // V (and v_sig) current for the bound adjacency and supports
   // lanes per particle row for each pass (crk_set_group_size)
   int32_t group[CRK_NPASS] = {4, 8, 2, 8, 8, 16, 32, 32, 32};
+  // threads per block of the tiled passes (crk_set_block_size): one block
+  // per tile of CRK_TILE_ROWS rows
+  int32_t threads[CRK_NPASS] = {256, 256, 256, 256, 256, 1024, 256, 256, 256};
   // bound adjacency / user state
   CrkAdjacency adj{};
   CrkState st{};
@@ -303,17 +306,18 @@ __device__ inline void atomic_min_pos(float* addr, float v) { atomicMin((unsigne
 
 // Block-wide reductions (every thread of the block must call; result valid
 // in every thread).  Used so that grid reductions issue one atomic per block.
+// (Blocks of up to 1024 threads.)
 template <class Op>
 __device__ inline float block_reduce(float v, Op op) {
-  __shared__ float sh_red[kBlock / 64];
+  __shared__ float sh_red[16];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh_red[threadIdx.x >> 6] = v;
   __syncthreads();
   float r = sh_red[0];
-#pragma unroll
-  for (int w = 1; w < kBlock / 64; ++w) r = op(r, sh_red[w]);
+  const int nw = ((int)blockDim.x + 63) >> 6;
+  for (int w = 1; w < nw; ++w) r = op(r, sh_red[w]);
   return r;
 }
 struct MaxOp { __device__ float operator()(float a, float b) const { return fmaxf(a, b); } };

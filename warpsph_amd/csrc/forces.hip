@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_velgrad(ARows rows, Geo geo, float C
 // One block per tile, the pairs' {x, h} and {v, V, m / rho} records staged
 // in LDS (d <= 2).
 template <int D, int G, int K, bool CULLEN>
-__global__ void __launch_bounds__(kBlock) k_velgrad_t(ARows rows, const uint16_t* __restrict__ ploc,
+__global__ void __launch_bounds__(kTileThreadsMax) k_velgrad_t(ARows rows, const uint16_t* __restrict__ ploc,
                                                       const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
                                                       VgIn in, const int32_t* __restrict__ perm, VgOut o) {
   static_assert(D <= 2, "tiled velocity gradient: d <= 2");
@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(kBlock) k_velgrad_t(ARows rows, const uint16_t
   tile_table(tiles, t, tab);
   const bool wide = tab[0] < 0;
   if (!wide) tile_fill(sg, in.geo4, tab, sw, in.vg4);
-  for (int round = 0; round < G; ++round) {
+  const int nr = tile_rounds<G>();
+  for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);
     if (wide)
       velgrad_row<D, G, K, CULLEN>(L, GlobalSrc<float4>{rows.nbr, in.geo4, in.vg4}, rows, geo, C, n, in, perm, o);
@@ -587,12 +588,11 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
                          cin, cp, cout, rec);
 }
 
-// The acceleration pass on an LDS tile: 1024 threads share one tile's staged
-// records (NF floats per particle, d <= 2), G lanes per row, the tile's rows
-// in rounds of 1024 / G.
-constexpr int kAccelThreads = 1024;
+// The acceleration pass on an LDS tile: the block's threads (1024 by
+// default) share one tile's staged records (NF floats per particle, d <= 2),
+// G lanes per row, the tile's rows in rounds of blockDim / G.
 template <int D, int G, int K, bool SW>
-__global__ void __launch_bounds__(kAccelThreads) k_accel_dudt_t(ARows rows, const uint16_t* __restrict__ ploc,
+__global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, const uint16_t* __restrict__ ploc,
                                                                 const int32_t* __restrict__ tiles, Geo geo, float C,
                                                                 LimParams lp, float inv_kscale, float Cl, float Cq,
                                                                 int32_t n, AdIn in, const int32_t* __restrict__ perm,
@@ -613,15 +613,9 @@ __global__ void __launch_bounds__(kAccelThreads) k_accel_dudt_t(ARows rows, cons
     }
     __syncthreads();
   }
-  constexpr int RPR = kAccelThreads / G;  // rows per round
-  constexpr int NR = (kTileRows + RPR - 1) / RPR;
-  for (int round = 0; round < NR; ++round) {
-    const int32_t lr = round * RPR + (int32_t)threadIdx.x / G;
-    Lane L;
-    L.g = threadIdx.x % G;
-    L.k = t * kTileRows + lr;
-    L.valid = lr < kTileRows && L.k < n;
-    if (!L.valid) L.k = min(L.k, n - 1);
+  const int nr = tile_rounds<G>();
+  for (int round = 0; round < nr; ++round) {
+    const Lane L = tile_lane<G>(t, round, n);
     if (wide)
       accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, Cl, Cq, 0, n, in, perm, o, cin,
                              cp, cout, rec);
@@ -705,14 +699,15 @@ int crk_velocity_gradient(CrkContext* ctx, const CrkConfig* cfg, CrkUpdate* up, 
           F<float>(ctx->gB), F<float4>(ctx->geo4), F<float4>(ctx->vg4), F<float>(ctx->mr)};
   VgOut out{F<float>(ctx->J), F<float>(ctx->divcd), F<float>(ctx->vsig), up->drhodt};
   int GS = ctx->group[CRK_PASS_VELGRAD];
+  const int TB = ctx->threads[CRK_PASS_VELGRAD];
   if (use_tiles(ctx) && ctx->pair_loc) {
     if (cullen) {
       CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,
-          hipLaunchKernelGGL((k_velgrad_t<DD, G, KK, true>), dim3(tile_count(n)), dim3(kBlock), 0, s,
+          hipLaunchKernelGGL((k_velgrad_t<DD, G, KK, true>), dim3(tile_count(n)), dim3(TB), 0, s,
                              arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, n, in, ctx->adj.perm, out));
     } else {
       CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,
-          hipLaunchKernelGGL((k_velgrad_t<DD, G, KK, false>), dim3(tile_count(n)), dim3(kBlock), 0, s,
+          hipLaunchKernelGGL((k_velgrad_t<DD, G, KK, false>), dim3(tile_count(n)), dim3(TB), 0, s,
                              arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, n, in, ctx->adj.perm, out));
     }
   } else if (cullen) {
@@ -789,7 +784,8 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
 #define ACCEL_T(SWV, CI, CP, CO)                                                                                 \
     CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,                                                              \
         hipLaunchKernelGGL((k_accel_dudt_t<DD, (G < 4 ? 4 : G), KK, SWV>), dim3(tile_count(n)),                 \
-                           dim3(kAccelThreads), 0, s, arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, lp,  \
+                           dim3(ctx->threads[CRK_PASS_ACCEL]), 0, s, arows_of(ctx), ctx->pair_loc,             \
+                           ctx->adj.tiles, geo, C, lp,                                                          \
                            inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q, n, in, ctx->adj.perm, out, CI, CP, \
                            CO, F<float4>(ctx->rec)))
     if (sw) { ACCEL_T(true, cd_in(ctx), cd_params(ctx, cfg, dt), cd_out(ctx, st)); }

@@ -285,7 +285,7 @@ __device__ inline void support_row_t(const Lane& L, const Src& src, const SupHea
 }
 
 template <int D, int G, int K, bool VS>
-__global__ void __launch_bounds__(kBlock) k_support_t(Rows rows, const uint16_t* __restrict__ nloc,
+__global__ void __launch_bounds__(kTileThreadsMax) k_support_t(Rows rows, const uint16_t* __restrict__ nloc,
                                                       const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
                                                       int32_t* __restrict__ anbr, uint16_t* __restrict__ aloc,
                                                       int32_t* __restrict__ acnt, float* __restrict__ vsig,
@@ -315,12 +315,13 @@ __global__ void __launch_bounds__(kBlock) k_support_t(Rows rows, const uint16_t*
     __syncthreads();
   }
   auto sweep = [&](const auto& src, uint16_t* lout) {
+    const int nr = tile_rounds<G>();
     Lane L = tile_lane<G>(t, 0, n);
     SupHead nxt = support_head<G>(L, src, soff, r0, geo4, vel4);
-    for (int round = 0; round < G; ++round) {
+    for (int round = 0; round < nr; ++round) {
       const Lane Lc = L;
       const SupHead cur = nxt;
-      if (round + 1 < G) {
+      if (round + 1 < nr) {
         L = tile_lane<G>(t, round + 1, n);
         nxt = support_head<G>(L, src, soff, r0, geo4, vel4);
       }
@@ -574,7 +575,7 @@ __device__ inline float owen_row(const Lane& L, const Src& src, const int32_t* _
 // update only reads the neighbours' positions, so the rows' own h writes to
 // geo4 do not race with what another tile stages).
 template <int D, int G, int K>
-__global__ void __launch_bounds__(kBlock) k_owen_t(Rows rows, const uint16_t* __restrict__ nloc,
+__global__ void __launch_bounds__(kTileThreadsMax) k_owen_t(Rows rows, const uint16_t* __restrict__ nloc,
                                                    const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
                                                    float* __restrict__ h, const int32_t* __restrict__ perm,
                                                    float* __restrict__ supports, const double* __restrict__ lut_psiH,
@@ -588,7 +589,8 @@ __global__ void __launch_bounds__(kBlock) k_owen_t(Rows rows, const uint16_t* __
   const bool wide = tab[0] < 0;
   if (!wide) tile_fill(sg, (const float4*)geo4, tab);
   float rmax = 0.f;
-  for (int round = 0; round < G; ++round) {
+  const int nr = tile_rounds<G>();
+  for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);
     const float r = wide
         ? owen_row<D, G, K>(L, GlobalSrc<float4>{rows.nbr, geo4, geo4}, rows.off, geo, C, h, perm, supports,
@@ -1033,7 +1035,7 @@ __global__ void __launch_bounds__(kBlock) k_moments(ARows rows, Rows full, Geo g
 
 // One block per tile, the pairs' {x, h} and {V, m} records staged in LDS.
 template <int D, int G, int K>
-__global__ void __launch_bounds__(kBlock) k_moments_t(ARows rows, const uint16_t* __restrict__ ploc,
+__global__ void __launch_bounds__(kTileThreadsMax) k_moments_t(ARows rows, const uint16_t* __restrict__ ploc,
                                                       const int32_t* __restrict__ tiles, Geo geo, float C, int32_t n,
                                                       const float* __restrict__ x, const float* __restrict__ h,
                                                       const float* __restrict__ V, const float* __restrict__ m,
@@ -1047,7 +1049,8 @@ __global__ void __launch_bounds__(kBlock) k_moments_t(ARows rows, const uint16_t
   tile_table(tiles, t, tab);
   const bool wide = tab[0] < 0;
   if (!wide) tile_fill(sg, geo4, tab, sv, (const float2*)o.vm2);
-  for (int round = 0; round < G; ++round) {
+  const int nr = tile_rounds<G>();
+  for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);
     if (wide)
       moments_row<D, G, K, false>(L, GlobalSrc<float2>{rows.nbr, geo4, o.vm2}, rows, Rows{}, geo, C, n, x, h, V, m,
@@ -1208,6 +1211,14 @@ int crk_set_group_size(CrkContext* ctx, int32_t pass, int32_t group) {
   CRK_REQUIRE(group == 1 || group == 2 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
               "group size must be a power of two in [1, 64] (got %d)", group);
   ctx->group[pass] = group;
+  return CRK_OK;
+}
+
+int crk_set_block_size(CrkContext* ctx, int32_t pass, int32_t threads) {
+  CRK_REQUIRE(ctx, "crk_set_block_size: null context");
+  CRK_REQUIRE(pass >= 0 && pass < CRK_NPASS, "unknown pass %d", pass);
+  CRK_REQUIRE(valid_tile_threads(threads), "tile block size must be 256, 512 or 1024 (got %d)", threads);
+  ctx->threads[pass] = threads;
   return CRK_OK;
 }
 
@@ -1430,8 +1441,8 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
     ctx->have_vsig = vs != nullptr;
   } else if (use_tiles(ctx)) {
     CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,
-        hipLaunchKernelGGL((k_owen_t<DD, G, KK>), dim3(tile_count(n)), dim3(kBlock), 0, s, rows_of(ctx),
-                           ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->h), ctx->adj.perm,
+        hipLaunchKernelGGL((k_owen_t<DD, G, KK>), dim3(tile_count(n)), dim3(ctx->threads[CRK_PASS_OWEN]), 0, s,
+                           rows_of(ctx), ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->h), ctx->adj.perm,
                            st->supports, F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut,
                            (float)ctx->nh_target, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx)));
   } else {
@@ -1510,7 +1521,8 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
   if (tiled) {
 #define SUPPORT_T(VS)                                                                                            \
     CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GS,                                                              \
-        hipLaunchKernelGGL((k_support_t<DD, G, KK, VS>), dim3(tile_count(n)), dim3(kBlock), 0, s, rows_of(ctx), \
+        hipLaunchKernelGGL((k_support_t<DD, G, KK, VS>), dim3(tile_count(n)), dim3(ctx->threads[CRK_PASS_SUPPORT]), \
+                           0, s, rows_of(ctx),                                                                  \
                            ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, ed->nbr, ed->loc, ed->count, vs,       \
                            F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4), F<float>(ctx->m),    \
                            F<float2>(ctx->vm2)))
@@ -1562,8 +1574,8 @@ static int factors_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState*
     ctx->have_support = true;
   } else if (use_tiles(ctx) && ctx->pair_loc) {
     CRK_KDG12_DISPATCH(cfg->kernel, ctx->dim, GM,
-        hipLaunchKernelGGL((k_moments_t<DD, G, KK>), dim3(tile_count(n)), dim3(kBlock), 0, s, arows_of(ctx),
-                           ctx->pair_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->x), F<float>(ctx->h),
+        hipLaunchKernelGGL((k_moments_t<DD, G, KK>), dim3(tile_count(n)), dim3(ctx->threads[CRK_PASS_MOMENTS]), 0, s,
+                           arows_of(ctx), ctx->pair_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->x), F<float>(ctx->h),
                            F<float>(ctx->V), F<float>(ctx->m), F<float>(ctx->u), cfg->gamma, ctx->adj.perm, mo,
                            F<float4>(ctx->geo4)));
   } else {

@@ -65,14 +65,25 @@ __device__ inline void tile_fill(T* dst, const T* __restrict__ src, const int32_
   __syncthreads();
 }
 
-// Row k of round `round` of a tile whose rounds hold blockDim / G rows each
+// A tile's rows are processed by one block of blockDim.x threads (256, 512
+// or 1024, the pass's tile block size), G lanes per row, blockDim.x / G rows
+// per round.
+constexpr int kTileThreadsMax = 1024;
+template <int G>
+__device__ inline int tile_rounds() {
+  return (kTileRows * G + (int)blockDim.x - 1) / (int)blockDim.x;
+}
+// Row k of round `round` of tile t.  Lanes past the tile's last row (a round
+// wider than the tile) or past n are invalid and keep a row index inside
+// the tile, so that their (unused) reads stay in bounds.
 template <int G>
 __device__ inline Lane tile_lane(int32_t t, int round, int32_t n) {
   Lane l;
   l.g = threadIdx.x % G;
-  l.k = t * kTileRows + round * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
-  l.valid = l.k < n;
-  if (!l.valid) l.k = n - 1;
+  const int32_t lr = round * (int32_t)(blockDim.x / G) + (int32_t)(threadIdx.x / G);
+  l.k = t * kTileRows + lr;
+  l.valid = lr < kTileRows && l.k < n;
+  if (!l.valid) l.k = min(t * kTileRows + min(lr, kTileRows - 1), n - 1);
   return l;
 }
 
@@ -106,7 +117,9 @@ struct TileSrc {
 inline bool use_tiles(const CrkContext* c) { return c->adj.tiles && c->adj.nbr_loc && c->dim <= 2; }
 inline unsigned tile_count(int32_t n) { return nblocks(n, kTileRows); }
 
-static_assert(kTileRows == kBlock, "a tile's rows are processed by one block of kBlock threads in G rounds");
+static_assert(kTileRows == kBlock, "the default tile block holds one thread per tile row");
+// tile block size of a pass (crk_set_block_size): 256, 512 or 1024
+inline bool valid_tile_threads(int32_t t) { return t == 256 || t == 512 || t == 1024; }
 
 }  // namespace crk
 

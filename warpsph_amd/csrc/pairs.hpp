@@ -5,8 +5,9 @@
 // Work mapping: a group of G consecutive lanes of a wave64 owns one sorted
 // particle row; lane g handles the row's pairs g, g + G, ... so pair arrays
 // (neighbour ids, ap, av) stream coalesced.  Each lane keeps
-// Kahan-compensated FP32 partial sums; the G partials are combined in FP64
-// (deterministic, every lane ends with the same value).
+// Kahan-compensated FP32 partial sums; the G partials are combined by a
+// compensated FP32 butterfly (deterministic, every lane ends with the same
+// value).
 #pragma once
 
 #include "common.hpp"
@@ -95,19 +96,35 @@ struct Comp {  // compensated value s - c
   }
 };
 
-// Sum of the G lanes' compensated partials: each partial s - c is formed
-// exactly in FP64 and the G values are added by an FP64 xor butterfly (the
-// order is the same for every lane, so all lanes end with the same value).
+// Sum of the G lanes' compensated partials s - c, in FP32: an xor butterfly
+// whose step adds two partials (s1 - c1) + (s2 - c2) as s = s1 + s2 with the
+// rounding error e of that addition recovered exactly (TwoSum, Knuth) and
+// folded into the compensation, c = (c1 + c2) - e.  Both the sum and the
+// exact error are symmetric in the two operands, so every lane of the group
+// ends with the same value.
 template <int G>
-__device__ inline double group_total_d(Comp a) {
-  double t = (double)a.s - (double)a.c;
+__device__ inline Comp group_comp(Comp a) {
 #pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, G);
-  return t;
+  for (int off = G / 2; off > 0; off >>= 1) {
+    const float s2 = __shfl_xor(a.s, off, G), c2 = __shfl_xor(a.c, off, G);
+    const float t = a.s + s2;
+    const float bb = t - a.s;
+    const float e = (a.s - (t - bb)) + (s2 - bb);
+    a.c = (a.c + c2) - e;
+    a.s = t;
+  }
+  return a;
 }
 template <int G>
 __device__ inline float group_total(Comp a) {
-  return (float)group_total_d<G>(a);
+  a = group_comp<G>(a);
+  return a.s - a.c;
+}
+// the same total formed in FP64 (input of the FP64 moment solve)
+template <int G>
+__device__ inline double group_total_d(Comp a) {
+  a = group_comp<G>(a);
+  return (double)a.s - (double)a.c;
 }
 
 template <int G>

@@ -346,8 +346,8 @@ def learned(args):
     pairs of the two-state generator, training on the GPU (BF16 MFMA kernels,
     Adam), then a config-2-shape shear layer (--nx^2, alpha-weighted Q): the
     coarse run with the learned switch and with the Cullen-Dehnen switch
-    against the Cullen-Dehnen run at 4x the particle count (density L1 on a
-    256^2 grid at --t-end), with wall times."""
+    against the Cullen-Dehnen run at 4x the particle count (L1 of the SPH
+    densities averaged per cell of a 256^2 grid at --t-end), with wall times."""
     import numpy as np
     import torch
     from warpsph_amd import learned as Lm
@@ -380,7 +380,7 @@ def learned(args):
         s = Lm.advance(system, config, scheme, args.t_end)
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0
-        rho, _ = Lm.grid_density(s.state.positions, s.state.masses, 256)
+        rho, _ = Lm.density_field(s.state, 256)
         fb = None
         if mdl is not None:  # fraction of particles on the Cullen-Dehnen fallback in the final state
             f = Lm.stage_features(s, config, scheme)

@@ -511,6 +511,12 @@ typedef struct CrkMlp {
   const uint16_t* wbf16;   /* device [CRK_MLP_NBF16], from crk_mlp_pack */
   float mu[8], sigma[8];   /* input normalisation x' = (f - mu) / sigma */
   float lo[8], hi[8];      /* raw feature range of the training data */
+  /* the switch's alpha from the network output s = sigmoid(MLP(x')):
+   * alpha = scale * s, times the Cullen-Dehnen alpha of the particle when
+   * multiply != 0 (the network then rescales the plain switch; s = 1/scale
+   * reproduces it) */
+  float   scale;
+  int32_t multiply;
 } CrkMlp;
 /* BF16 operand copy of the parameters. */
 int crk_mlp_pack(const float* params, uint16_t* wbf16, void* stream);
@@ -527,8 +533,9 @@ int crk_mlp_grad(CrkContext* ctx, const CrkMlp* model, int32_t n, const float* x
 int crk_learned_features(CrkContext* ctx, const CrkConfig* cfg, float dt, float* features_out,
                          void* stream);
 /* The learned switch of the bound stage (in place of crk_switch): the
- * Cullen-Dehnen switch, then alpha = MLP(features) where every feature lies
- * in [lo, hi], the Cullen-Dehnen alpha elsewhere.  Writes st->alphas and the
+ * Cullen-Dehnen switch, then alpha = scale sigmoid(MLP(features)) (times the
+ * Cullen-Dehnen alpha when model->multiply) where every feature lies in
+ * [lo, hi], the Cullen-Dehnen alpha elsewhere.  Writes st->alphas and the
  * stage's alpha (read by crk_accel_dudt with alpha-weighted Q).            */
 int crk_learned_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, float dt,
                        const CrkMlp* model, void* stream);

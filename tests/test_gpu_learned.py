@@ -119,17 +119,21 @@ def test_learned_switch_falls_back_to_cullen_dehnen():
     assert torch.equal(u_cd.dvdt, u_l.dvdt) and torch.equal(u_cd.dudt, u_l.dudt)
 
 
-def test_learned_switch_uses_the_network():
-    """Every feature in range: alpha = MLP((features - mu) / sigma), checked
-    against the oracle's network on the exported features."""
+@pytest.mark.parametrize("multiply", [False, True])
+def test_learned_switch_uses_the_network(multiply):
+    """Every feature in range: alpha = scale MLP((features - mu) / sigma),
+    times the Cullen-Dehnen alpha of the same stage when the model
+    multiplies, checked against the oracle's network on the exported
+    features."""
     from oracle import mlp_oracle as M
     from warpsph_amd import abi
     from warpsph_amd.configurations import toCrkConfig
     from warpsph_amd.runtime import runtime
-    announce("learned switch network")
+    announce(f"learned switch network multiply={multiply}")
     model, p = _model(11)
     model.mu[:] = 0.0
     model.sigma[:] = 2.0
+    model.scale, model.multiply = (2.0, True) if multiply else (1.0, False)
     u_l, st_l, system, config, scheme, dt = _stage(model)
     rt = runtime("cuda:0")
     n = st_l.numParticles
@@ -139,11 +143,30 @@ def test_learned_switch_uses_the_network():
     f = feats.cpu().numpy()
     assert np.all(np.isfinite(f))
     ref, _ = M.forward(p, (f - model.mu) / model.sigma)
-    got = st_l.alphas.cpu().numpy()
-    assert np.max(np.abs(got - ref)) < 2e-3
-    # and it changes the dynamics against the Cullen-Dehnen stage
     u_cd, st_cd, *_ = _stage()
+    if multiply:
+        a_cd = st_cd.alphas.cpu().numpy().astype(np.float64)
+        ref = 2.0 * ref * a_cd
+        tol = 2e-3 * 2.0 * max(1.0, float(np.max(a_cd)))
+    else:
+        tol = 2e-3
+    got = st_l.alphas.cpu().numpy()
+    assert np.max(np.abs(got - ref)) < tol
+    # and it changes the dynamics against the Cullen-Dehnen stage
     assert not torch.equal(u_cd.dvdt, u_l.dvdt)
+
+
+def test_constant_network_scales_the_cullen_dehnen_switch():
+    """LearnedSwitch.constant(m): alpha = m alpha_CD on every particle."""
+    from warpsph_amd.learned import LearnedSwitch
+    announce("constant multiplier network")
+    u_cd, st_cd, *_ = _stage()
+    for m in (0.5, 1.0, 1.5):
+        _, st_m, *_ = _stage(LearnedSwitch.constant(m, device="cuda"))
+        a_cd = st_cd.alphas.cpu().numpy().astype(np.float64)
+        got = st_m.alphas.cpu().numpy().astype(np.float64)
+        # zero weights: the output is the FP32 bias alone, sigmoid(logit(m / 2))
+        assert np.max(np.abs(got - m * a_cd)) <= 1e-5 * m * np.max(a_cd), m
 
 
 def test_feature_gradients_against_fp64_assembly():
@@ -186,13 +209,14 @@ def test_feature_gradients_against_fp64_assembly():
     assert np.all(f[:, 7] > 0.3) and np.all(f[:, 7] < 3.0)
 
 
-def test_learned_switch_gain_on_a_held_out_pair():
+def test_learned_switch_gain_on_held_out_pairs():
     """What the trained network gains: train on solver-produced pairs of the
-    two-state generator (64^2 / 128^2), then run a held-out pair's coarse
-    state with the learned and with the Cullen-Dehnen switch and compare the
-    density L1 against its 4x-resolution run.  The printed figures are the
-    test's result; it fails only if the network is much worse than the plain
-    switch or barely used."""
+    two-state generator (64^2 / 128^2, three seeds), then run four held-out
+    seeds' coarse states with the learned and with the Cullen-Dehnen switch
+    and compare the density field (SPH densities averaged per cell of a 16^2
+    grid) against each seed's 4x-resolution run at t = 0.09.  The network
+    rescales the Cullen-Dehnen alpha; it must lower the error on every
+    held-out seed and by more than 3 % on average."""
     from warpsph_amd import learned as Lm
     from warpsph_amd.ensemble import two_state_run
     from warpsph_amd.enumTypes import ViscositySwitch
@@ -203,28 +227,36 @@ def test_learned_switch_gain_on_a_held_out_pair():
         feats.append(f)
         targs.append(t)
     model, hist = Lm.fit(torch.cat(feats), torch.cat(targs), epochs=40, batch=4096, lr=3e-3)
-    coarse, config, scheme = two_state_run(64, 2999, device="cuda")
-    fine, _, _ = two_state_run(128, 2999, device="cuda")
-    res = {}
-    for name, sw, mdl, sysm in (("fine", ViscositySwitch.CullenDehnen2010, None, fine),
-                                ("cd", ViscositySwitch.CullenDehnen2010, None, coarse),
-                                ("learned", ViscositySwitch.Learned, model, coarse)):
-        sc = Lm._with_switch(scheme, sw)
-        sc.viscositySwitchParams.model = mdl
-        s = Lm.advance(sysm.initializeNewState(), config, sc, 0.09)
-        res[name] = Lm.grid_density(s.state.positions, s.state.masses, 16)[0]
-        if mdl is not None:
-            f = Lm.stage_features(s, config, sc)
-            lo = torch.as_tensor(mdl.lo, device=f.device)
-            hi = torch.as_tensor(mdl.hi, device=f.device)
-            used = float(((f >= lo) & (f <= hi)).all(dim=1).float().mean())
+    assert model.multiply and model.scale == 2.0
 
     def l1(a, b):
         return float((a - b).abs().mean() / b.abs().mean())
 
-    l1_cd, l1_ml = l1(res["cd"], res["fine"]), l1(res["learned"], res["fine"])
-    print(f"loss {hist[0]:.4f} -> {hist[-1]:.4f}; density L1 vs 4x: Cullen-Dehnen {l1_cd:.4f}, learned {l1_ml:.4f}, "
-          f"gain {(l1_cd - l1_ml) / l1_cd:+.1%}; network used on {used:.1%} of the particles")
+    gains, used = [], []
+    for seed in (2999, 2998, 2997, 2996):
+        coarse, config, scheme = two_state_run(64, seed, device="cuda")
+        fine, _, _ = two_state_run(128, seed, device="cuda")
+        res = {}
+        for name, sw, mdl, sysm in (("fine", ViscositySwitch.CullenDehnen2010, None, fine),
+                                    ("cd", ViscositySwitch.CullenDehnen2010, None, coarse),
+                                    ("learned", ViscositySwitch.Learned, model, coarse)):
+            sc = Lm._with_switch(scheme, sw)
+            sc.viscositySwitchParams.model = mdl
+            s = Lm.advance(sysm.initializeNewState(), config, sc, 0.09)
+            res[name] = Lm.density_field(s.state, 16)[0]
+            if mdl is not None:
+                f = Lm.stage_features(s, config, sc)
+                lo = torch.as_tensor(mdl.lo, device=f.device)
+                hi = torch.as_tensor(mdl.hi, device=f.device)
+                used.append(float(((f >= lo) & (f <= hi)).all(dim=1).float().mean()))
+        l1_cd, l1_ml = l1(res["cd"], res["fine"]), l1(res["learned"], res["fine"])
+        gains.append((l1_cd - l1_ml) / l1_cd)
+        print(f"seed {seed}: density L1 vs 4x: Cullen-Dehnen {l1_cd:.5f}, learned {l1_ml:.5f}, "
+              f"gain {gains[-1]:+.1%}", flush=True)
+    mean_gain = sum(gains) / len(gains)
+    print(f"loss {hist[0]:.4f} -> {hist[-1]:.4f}; mean gain {mean_gain:+.1%}; network used on "
+          f"{min(used):.1%} .. {max(used):.1%} of the particles", flush=True)
     assert hist[-1] < hist[0]
-    assert used > 0.9
-    assert l1_ml <= 1.10 * l1_cd
+    assert min(used) > 0.9
+    assert all(g > 0.0 for g in gains), gains
+    assert mean_gain > 0.03, mean_gain

@@ -115,7 +115,7 @@ MLP_NBF16 = 64 * 16 + 64 * 64 + 64
 
 class CrkMlp(C.Structure):
     _fields_ = [("params", C.c_void_p), ("wbf16", C.c_void_p), ("mu", C.c_float * 8), ("sigma", C.c_float * 8),
-                ("lo", C.c_float * 8), ("hi", C.c_float * 8)]
+                ("lo", C.c_float * 8), ("hi", C.c_float * 8), ("scale", C.c_float), ("multiply", C.c_int32)]
 
 
 _P = C.c_void_p

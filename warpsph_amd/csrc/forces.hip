@@ -896,8 +896,9 @@ int crk_learned_switch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
   if ((r = crk_switch(ctx, cfg, st, dt, stream))) return r;
   hipStream_t s = (hipStream_t)stream;
   if ((r = features_launch(ctx, cfg, dt, model, nullptr, s))) return r;
+  CRK_REQUIRE(model->scale > 0.f, "crk_learned_switch: model scale must be positive");
   return mlp_infer_launch(model, ctx->n, F<float>(ctx->feat), F<uint8_t>(ctx->fflag), F<float>(ctx->alpha),
-                          F<float>(ctx->alpha), ctx->adj.perm, st->alphas, s);
+                          F<float>(ctx->alpha), ctx->adj.perm, st->alphas, s, false);
 }
 
 int crk_balance(CrkContext* ctx, const CrkConfig* cfg, float dt, CrkEdges* ed, void* stream) {

@@ -132,14 +132,14 @@ __device__ inline s4 input4(const float* __restrict__ x, int32_t p, int32_t n, i
   return pack4(v.x, v.y, v.z, v.w);
 }
 
-// alpha = sigmoid(MLP(x)) for n particles; where flag[p] != 0 the fallback
-// value alt[p] is written instead.  Written to out[p] and, when out2 is not
-// null, to out2[map[p]].
+// alpha = scale sigmoid(MLP(x)) for n particles (times alt[p] when mul);
+// where flag[p] != 0 the fallback value alt[p] is written instead.  Written
+// to out[p] and, when out2 is not null, to out2[map[p]].
 __global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ q, const float* __restrict__ b,
                                                    const float* __restrict__ x, int32_t n,
                                                    const uint8_t* __restrict__ flag, const float* alt,
                                                    float* out, const int32_t* __restrict__ map,
-                                                   float* __restrict__ out2) {
+                                                   float* __restrict__ out2, float scale, int mul) {
   const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
   WRegs W;
   load_weights(q, r, g, W);
@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ 
     Fwd f;
     forward(W, b, input4(x, p, n, g), f, g);
     if (g == 0 && p < n) {
-      float a = sigm(f.z3);
+      float a = scale * sigm(f.z3);
+      if (mul) a = a * alt[p];
       if (flag && flag[p]) a = alt[p];
       out[p] = a;
       if (out2) out2[map[p]] = a;
@@ -358,9 +359,12 @@ inline unsigned mlp_blocks(int32_t n, int32_t cap) {
 }
 
 int mlp_infer_launch(const CrkMlp* m, int32_t n, const float* x, const uint8_t* flag, const float* alt, float* out,
-                     const int32_t* map, float* out2, hipStream_t s) {
+                     const int32_t* map, float* out2, hipStream_t s, bool raw) {
+  const float scale = raw ? 1.f : m->scale;
+  const int mul = raw ? 0 : (m->multiply != 0);
+  CRK_REQUIRE(!mul || alt, "the multiplying switch needs the Cullen-Dehnen alpha");
   hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_blocks(n, 4096)), dim3(256), 0, s, m->wbf16, m->params, x, n, flag, alt,
-                     out, map, out2);
+                     out, map, out2, scale, mul);
   CRK_LAUNCH_CHECK("k_mlp_infer");
   return CRK_OK;
 }
@@ -381,7 +385,7 @@ int crk_mlp_pack(const float* params, uint16_t* wbf16, void* stream) {
 int crk_mlp_forward(const CrkMlp* m, int32_t n, const float* x, float* alpha, void* stream) {
   CRK_REQUIRE(m && m->params && m->wbf16 && x && alpha && n >= 0, "crk_mlp_forward: invalid arguments");
   if (n == 0) return CRK_OK;
-  return mlp_infer_launch(m, n, x, nullptr, nullptr, alpha, nullptr, nullptr, (hipStream_t)stream);
+  return mlp_infer_launch(m, n, x, nullptr, nullptr, alpha, nullptr, nullptr, (hipStream_t)stream, true);
 }
 
 int crk_mlp_grad(CrkContext* ctx, const CrkMlp* m, int32_t n, const float* x, const float* target, float* grad,

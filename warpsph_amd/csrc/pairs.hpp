@@ -291,9 +291,10 @@ inline T* F(Buf& b) { return (T*)b.p; }
 int require_bound(CrkContext* ctx, const CrkConfig* cfg);
 
 // the MLP of the learned switch (mlp.hip): alpha into out[p] (sorted) and
-// out2[map[p]]; flag[p] selects alt[p] instead
+// out2[map[p]]; flag[p] selects alt[p] instead; raw: the sigmoid output
+// alone (no scale, no Cullen-Dehnen factor)
 int mlp_infer_launch(const CrkMlp* m, int32_t n, const float* x, const uint8_t* flag, const float* alt, float* out,
-                     const int32_t* map, float* out2, hipStream_t s);
+                     const int32_t* map, float* out2, hipStream_t s, bool raw);
 
 }  // namespace crk
 

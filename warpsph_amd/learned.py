@@ -8,13 +8,20 @@ particles whose features leave the training range keep the Cullen-Dehnen
 alpha.  alpha reaches the dynamics through alpha-weighted Q
 (``CRKSPHConfig.alphaWeightedViscosity``).
 
+The network rescales the Cullen-Dehnen switch: alpha = alpha_CD * scale *
+sigmoid(MLP(x)) with scale = 2, so an output of 1/2 is the plain switch and
+the network can lower the viscosity where the plain switch over-dissipates
+and raise it where it under-dissipates (``multiply``; the absolute form
+alpha = scale * sigmoid(MLP(x)) is kept for models trained that way).
+
 Training targets come from the solver itself.  For a coarse run and the same
 initial state at 4x the particle count (the config-4 generator's two-state
-runs): the coarse run is repeated with constant alpha values; at snapshot
+runs): the coarse run is repeated with the Cullen-Dehnen alpha scaled by
+constant factors (the learned switch with constant networks); at snapshot
 times the density of each run is binned on a uniform grid, and a coarse
-particle's target is the soft arg-min over the constant alphas of the grid
-cell's density error against the fine run.  The features are those of the
-plain (Cullen-Dehnen) coarse run at the same snapshot.
+particle's target is the soft arg-min over the factors of the grid cell's
+density error against the fine run.  The features are those of the plain
+(Cullen-Dehnen) coarse run at the same snapshot.
 """
 from __future__ import annotations
 
@@ -46,8 +53,11 @@ class LearnedSwitch:
     """Parameters (FP32 master and BF16 operand copies, on the GPU) and the
     feature normalisation / range of the training data."""
 
-    def __init__(self, params=None, mu=None, sigma=None, lo=None, hi=None, device="cuda"):
+    def __init__(self, params=None, mu=None, sigma=None, lo=None, hi=None, device="cuda", scale=2.0,
+                 multiply=True):
         self.device = torch.device(device)
+        self.scale = float(scale)
+        self.multiply = bool(multiply)
         p = init_params() if params is None else np.asarray(params, dtype=np.float32)
         if p.shape != (abi.MLP_NPARAM,):
             raise ValueError(f"expected {abi.MLP_NPARAM} parameters, got {p.shape}")
@@ -69,6 +79,8 @@ class LearnedSwitch:
         m.wbf16 = self.wbf16.data_ptr()
         for k in ("mu", "sigma", "lo", "hi"):
             getattr(m, k)[:] = [float(v) for v in getattr(self, k)]
+        m.scale = self.scale
+        m.multiply = int(self.multiply)
         self._c = m  # kept alive while the caller uses the pointer
         return m
 
@@ -96,12 +108,30 @@ class LearnedSwitch:
         return g[:abi.MLP_NPARAM], g[abi.MLP_NPARAM]
 
     def save(self, path: str):
-        np.savez(path, params=self.params.cpu().numpy(), mu=self.mu, sigma=self.sigma, lo=self.lo, hi=self.hi)
+        np.savez(path, params=self.params.cpu().numpy(), mu=self.mu, sigma=self.sigma, lo=self.lo, hi=self.hi,
+                 scale=np.float32(self.scale), multiply=np.int32(self.multiply))
 
     @classmethod
     def load(cls, path: str, device="cuda") -> "LearnedSwitch":
         with np.load(path, allow_pickle=False) as z:
-            return cls(z["params"], z["mu"], z["sigma"], z["lo"], z["hi"], device=device)
+            scale = float(z["scale"]) if "scale" in z.files else 1.0
+            multiply = bool(z["multiply"]) if "multiply" in z.files else False
+            return cls(z["params"], z["mu"], z["sigma"], z["lo"], z["hi"], device=device, scale=scale,
+                       multiply=multiply)
+
+    @classmethod
+    def constant(cls, value: float, scale: float = 2.0, multiply: bool = True, device="cuda") -> "LearnedSwitch":
+        """A network whose output is the constant value / scale for every
+        input (zero weights, output bias logit(value / scale)) and that
+        accepts every finite feature: alpha = value * alpha_CD (multiply) or
+        alpha = value."""
+        s = float(value) / float(scale)
+        if not 0.0 < s < 1.0:
+            raise ValueError(f"constant {value} outside (0, {scale})")
+        p = np.zeros(abi.MLP_NPARAM, np.float32)
+        p[-1] = math.log(s / (1.0 - s))
+        inf = np.full(8, np.inf, np.float32)
+        return cls(p, None, None, -inf, inf, device=device, scale=scale, multiply=multiply)
 
 
 class Adam:
@@ -123,11 +153,12 @@ class Adam:
 
 
 def fit(features: torch.Tensor, targets: torch.Tensor, epochs: int = 20, batch: int = 8192, lr: float = 2e-3,
-        seed: int = 0, model: Optional[LearnedSwitch] = None, verbose: bool = False):
-    """Train on raw features [N][8] and targets [N] (GPU tensors).  The
-    normalisation is the data's mean and standard deviation; the range [lo,
-    hi] is the 0.1 % .. 99.9 % quantile span widened by 5 % on both sides.
-    Returns (model, per-epoch mean loss)."""
+        seed: int = 0, model: Optional[LearnedSwitch] = None, verbose: bool = False, scale: float = 2.0,
+        multiply: bool = True):
+    """Train on raw features [N][8] and targets [N] in [0, 1] (the network's
+    sigmoid output; GPU tensors).  The normalisation is the data's mean and
+    standard deviation; the range [lo, hi] is the 0.1 % .. 99.9 % quantile
+    span widened by 5 % on both sides.  Returns (model, per-epoch mean loss)."""
     X = features.to(torch.float32)
     T = targets.to(torch.float32)
     ok = torch.isfinite(X).all(dim=1) & torch.isfinite(T)
@@ -139,7 +170,7 @@ def fit(features: torch.Tensor, targets: torch.Tensor, epochs: int = 20, batch: 
     span = qhi - qlo
     lo, hi = qlo - 0.05 * span, qhi + 0.05 * span
     if model is None:
-        model = LearnedSwitch(init_params(seed), mu, sg, lo, hi, device=X.device)
+        model = LearnedSwitch(init_params(seed), mu, sg, lo, hi, device=X.device, scale=scale, multiply=multiply)
     else:
         model.mu, model.sigma, model.lo, model.hi = (np.asarray(a, np.float32) for a in (mu, sg, lo, hi))
     # train on the in-range samples (the others fall back to Cullen-Dehnen at run time)
@@ -177,6 +208,19 @@ def grid_density(positions: torch.Tensor, masses: torch.Tensor, G: int, lo=(0.0,
     rho = torch.zeros(G * G, dtype=torch.float64, device=positions.device)
     rho.index_add_(0, idx, masses.double())
     return (rho / (float(L[0] * L[1]) / (G * G))).view(G, G), idx
+
+
+def density_field(state, G: int, lo=(0.0, 0.0), hi=(1.0, 1.0)):
+    """The particles' SPH densities averaged per cell of a G x G grid, mass
+    weighted (2D): unlike the binned mass per cell it changes continuously
+    with the trajectory, so small changes of the dynamics are seen.  Returns
+    (field [G][G], cell index of each particle)."""
+    msum, idx = grid_density(state.positions, state.masses, G, lo, hi)
+    w = torch.zeros(G * G, dtype=torch.float64, device=state.positions.device)
+    w.index_add_(0, idx, state.masses.double() * state.densities.double())
+    area = (hi[0] - lo[0]) * (hi[1] - lo[1]) / (G * G)
+    m = (msum * area).view(-1)
+    return (w / torch.where(m > 0, m, torch.ones_like(m))).view(G, G), idx
 
 
 def advance(system, config, scheme, t_target: float, integ=None):
@@ -223,37 +267,58 @@ def _with_switch(scheme, switch, alpha_weighted=True):
 
 
 def pair_samples(seed: int, nx: int = 128, snapshots: Sequence[float] = (0.02, 0.04, 0.06),
-                 alphas: Sequence[float] = (0.0, 0.1, 0.25, 0.5, 0.75, 1.0), grid: int = 32, tau: float = 0.005,
-                 device="cuda"):
+                 alphas: Sequence[float] = (0.25, 0.5, 0.75, 1.0, 1.5, 1.9), grid: int = 32, tau: float = 0.002,
+                 device="cuda", scale: float = 2.0, multiply: bool = True):
     """Training samples from one coarse / 4x-resolution pair of the two-state
-    generator: (features [M][8], targets [M])."""
+    generator: (features [M][8], targets [M] in [0, 1]).  ``alphas`` are the
+    constant factors on the Cullen-Dehnen alpha (multiply) or the constant
+    alphas; targets are the soft arg-min divided by ``scale``.  A candidate
+    run that fails (its supports outgrow the periodic box when too little
+    viscosity lets a strong shock scatter particles) leaves the candidate
+    set from then on."""
     from .ensemble import two_state_run
     from .enumTypes import ViscositySwitch
     coarse, config, scheme = two_state_run(nx, seed, device=device)
     fine, _, _ = two_state_run(2 * nx, seed, device=device)
     cd = _with_switch(scheme, ViscositySwitch.CullenDehnen2010)
-    flat = _with_switch(scheme, ViscositySwitch.NoneSwitch)
-    runs_a = []
+    runs_a, schemes_a, values = [], [], []
     for a in alphas:
         s = coarse.initializeNewState()
-        s.state.alphas.fill_(float(a))
+        if multiply:
+            sc = _with_switch(scheme, ViscositySwitch.Learned)
+            sc.viscositySwitchParams.model = LearnedSwitch.constant(a, scale, True, device=device)
+        else:
+            sc = _with_switch(scheme, ViscositySwitch.NoneSwitch)
+            s.state.alphas.fill_(float(a))
         runs_a.append(s)
+        schemes_a.append(sc)
+        values.append(float(a))
     s_cd, s_f = coarse.initializeNewState(), fine.initializeNewState()
     feats, targets = [], []
     for ts in snapshots:
         s_cd = advance(s_cd, config, cd, ts)
         s_f = advance(s_f, config, cd, ts)
-        runs_a = [advance(s, config, flat, ts) for s in runs_a]
-        rho_f, _ = grid_density(s_f.state.positions, s_f.state.masses, grid)
+        alive = []
+        for s, sc, a in zip(runs_a, schemes_a, values):
+            try:
+                alive.append((advance(s, config, sc, ts), sc, a))
+            except ValueError:
+                pass
+        if not alive:
+            break
+        runs_a, schemes_a, values = (list(c) for c in zip(*alive))
+        rho_f, _ = density_field(s_f.state, grid)
         errs = []
         for s in runs_a:
-            rho_a, _ = grid_density(s.state.positions, s.state.masses, grid)
+            rho_a, _ = density_field(s.state, grid)
             errs.append((rho_a - rho_f).abs() / rho_f.mean())
         E = torch.stack(errs)  # [n_alpha, G, G]
         w = torch.softmax(-E / tau, dim=0)  # soft arg-min over the constant alphas (tau: relative error)
-        a_cell = (w * torch.tensor(alphas, dtype=torch.float64, device=E.device)[:, None, None]).sum(dim=0)
+        a_cell = (w * torch.tensor(values, dtype=torch.float64, device=E.device)[:, None, None]).sum(dim=0) / scale
         f = stage_features(s_cd, config, cd)
-        _, idx = grid_density(s_cd.state.positions, s_cd.state.masses, grid)
+        _, idx = density_field(s_cd.state, grid)
         feats.append(f)
         targets.append(a_cell.view(-1)[idx].to(torch.float32))
+    if not feats:
+        raise RuntimeError(f"pair_samples: every candidate run of seed {seed} failed")
     return torch.cat(feats), torch.cat(targets)

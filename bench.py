@@ -279,8 +279,13 @@ def ensemble(args):
     e, config, scheme = ens_mod.sample_ensemble(n_runs=args.runs, nmin=args.run_min, nmax=args.run_max,
                                                 seed=args.seed, device="cuda:0")
     gen_s = time.perf_counter() - t0
-    if os.path.exists(args.tuning_file):
-        groups = load(args.tuning_file)
+    # the ensemble's own launch shapes (autotuned on the 256-run workload)
+    # unless a tuning file is given
+    tf = args.tuning_file
+    if tf == DEFAULT_TUNING:
+        tf = os.path.join(ROOT, "warpsph_amd", "tuning", "mi355x_ensemble_256.json")
+    if os.path.exists(tf):
+        groups = load(tf)
         set_groups(e.state.positions.device, groups)
     else:
         groups = None

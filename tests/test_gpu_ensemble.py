@@ -187,3 +187,61 @@ def test_unbound_calls_after_an_ensemble_step():
     out = integ.function(s1, crkSPH_step, dt, config, scheme).state
     torch.cuda.synchronize()
     assert torch.isfinite(out.state.densities).all()
+
+
+def test_ensemble_at_workload_run_sizes():
+    """The batched workload's run sizes: 16 runs drawn by the data generator
+    (sides of nx^2 particles, log-uniform in [4096, 16384]), three RK2 steps
+    with each run's own CFL dt.  Every run against the same run stepped alone
+    (per-step bar), and the smallest and largest runs against the CPU oracle.
+    alpha is left out here: its signal speed is a maximum over every entry of
+    the Verlet list, skin included (wp_vsig.py:55-86), and the ensemble
+    rebuilds all runs' lists when one run's list fails, so a run whose
+    standalone list was kept sees a different skin (the other fields differ
+    by the summation order of the re-sorted rows only)."""
+    from oracle import oracle as O
+    from warpsph_amd import sample
+    from warpsph_amd.ensemble import cfl_dt, run_system, sample_ensemble
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.scheme import crkSPH_step
+    ens, config, scheme = sample_ensemble(n_runs=16, nmin=4096, nmax=16384, seed=4, device="cuda")
+    counts = np.diff(ens.runs.offsets)
+    announce(f"ensemble 16 runs, {int(counts.min())}..{int(counts.max())} particles")
+    assert len(counts) == 16 and counts.min() >= 4096 and counts.max() <= 16384
+    integ = getIntegrator()
+    singles = [run_system(ens, r) for r in range(16)]
+    pick = [int(np.argmin(counts)), int(np.argmax(counts))]
+    sims = {r: O.from_system(cpu_copy(singles[r], config, scheme), config, scheme, 32) for r in pick}
+    e, prev = ens, None
+    prev_s = [None] * 16
+    for _ in range(3):
+        dts = cfl_dt(e, config, scheme, prev)
+        prev = dts
+        for r in range(16):
+            d1 = sample.cfl_dt(singles[r].state, config, scheme, prev_s[r])
+            assert d1 == dts[r], (r, d1, dts[r])
+            prev_s[r] = d1
+            singles[r] = integ.function(singles[r], crkSPH_step, d1, config, scheme).state
+        for r, sim in sims.items():
+            sim.step(float(np.float32(dts[r])), 1)
+        e = integ.function(e, crkSPH_step, dts, config, scheme).state
+    torch.cuda.synchronize()
+    worst = 0.0
+    for r in range(16):
+        b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
+        fe, fs = _fields(e.state), _fields(singles[r].state)
+        fe.pop("alpha")
+        for k in fe:
+            err = relerr(fe[k][b:q].cpu().numpy(), fs[k].cpu().numpy())
+            worst = max(worst, err)
+            assert err <= RTOL, (r, k, err)
+    print(f"worst ensemble-vs-single field error {worst:.2e}", flush=True)
+    for r, sim in sims.items():
+        b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
+        st = e.state
+        errs = {"v": relerr(st.velocities[b:q].cpu().numpy(), sim.get("v")),
+                "u": relerr(st.internalEnergies[b:q].cpu().numpy(), sim.get("u")),
+                "rho": relerr(st.densities[b:q].cpu().numpy(), sim.get("rho")),
+                "h": relerr(st.supports[b:q].cpu().numpy(), sim.get("h"))}
+        print(f"run {r} ({q - b} particles) vs oracle: {errs}", flush=True)
+        assert all(v <= 3 * RTOL for v in errs.values()), (r, errs)

@@ -32,7 +32,7 @@ SCOPES = {
 
 
 # passes that run on LDS tiles (their block size is a knob too)
-TILED = ("owen", "support", "moments", "velgrad", "accel")
+TILED = ("owen", "support", "moments", "velgrad", "accel", "compat")
 THREADS = (256, 512, 1024)
 
 
@@ -62,7 +62,7 @@ def autotune(system, config, scheme, dt: float, candidates: Iterable[int] = (1, 
     current = dict(getattr(rt, "groups", {}))
     defaults = {"owen": 4, "support": 32, "moments": 2, "velgrad": 8, "switch": 8, "accel": 16, "balance": 32,
                 "compat": 32, "owen.threads": 256, "support.threads": 256, "moments.threads": 256,
-                "velgrad.threads": 256, "accel.threads": 1024}
+                "velgrad.threads": 256, "accel.threads": 1024, "compat.threads": 256}
     for k, v in defaults.items():
         current.setdefault(k, v)
     set_groups(dev, current)

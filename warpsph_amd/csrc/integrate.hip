@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "pairs.hpp"
+#include "tiles.hpp"
 
 namespace crk {
 
@@ -70,6 +71,7 @@ struct CompIn {
   const float* pf;  // per slot: a = ap + av (D)
   const float4* rec;
   float b;
+  const uint16_t* loc;  // tile-local neighbour index per slot (tiled adjacencies), or null
 };
 
 template <int G>
@@ -97,6 +99,18 @@ struct CompSrcG {
   __device__ inline int32_t id(int64_t q) const { return nbr[q]; }
   __device__ inline float4 r(int32_t j) const { return rec[j]; }
   __device__ inline float vh(int32_t j, int c) const { return vhs[(int64_t)j * D + c]; }
+};
+
+// The same from a tile's LDS copies (the stage's records and v_half of the
+// particles the tile's rows reach) through the 16-bit local indices.
+template <int D>
+struct CompSrcT {
+  const uint16_t* __restrict__ loc;
+  const float4* rec;  // LDS
+  const float* vhs;   // LDS, D per staged particle
+  __device__ inline int32_t id(int64_t q) const { return loc[q]; }
+  __device__ inline float4 r(int32_t j) const { return rec[j]; }
+  __device__ inline float vh(int32_t j, int c) const { return vhs[j * D + c]; }
 };
 
 // The sum over one stage's pairs of row k: b f_ij (v_half_j - v_half_i).a_ij
@@ -187,6 +201,47 @@ __global__ void __launch_bounds__(kBlock) k_compat(const int32_t* __restrict__ p
   const Lane L = lane_of<G>(n);  // XCD-aware row order
   compat_row<D, G, NST, MODE>(L, CompSrcG<D>{s0.nbr, s0.rec, vhs}, CompSrcG<D>{s1.nbr, s1.rec, vhs}, perm, vhs, s0,
                               s1, m_u, du0, u0, dta, u, dudt_last);
+}
+
+// The single sweep of both stages (MODE 2, shared sorted order) on LDS
+// tiles (d <= 2): one block per tile stages both stages' records and
+// v_half of every particle the tile's rows reach; wide tiles gather.
+template <int D, int G>
+__global__ void __launch_bounds__(kTileThreadsMax) k_compat_t(const int32_t* __restrict__ tiles,
+                                                              const int32_t* __restrict__ perm, int32_t n,
+                                                              const float* __restrict__ vhs, CompIn s0, CompIn s1,
+                                                              const float* __restrict__ m_u, float* __restrict__ du0,
+                                                              const float* __restrict__ u0, DtArg dta,
+                                                              float* __restrict__ u, float* __restrict__ dudt_last) {
+  static_assert(D <= 2, "tiled finalize: d <= 2");
+  __shared__ int32_t tab[kTileWords];
+  __shared__ float4 sr0[kTileCap];
+  __shared__ float4 sr1[kTileCap];
+  __shared__ float svh[kTileCap * D];
+  const int32_t t = xcd_block();
+  tile_table(tiles, t, tab);
+  const bool wide = tab[0] < 0;
+  if (!wide) {
+    const int32_t U = tab[1];
+    for (int32_t q = threadIdx.x; q < U; q += blockDim.x) {
+      const int32_t g = tile_global(tab, q);
+      sr0[q] = s0.rec[g];
+      sr1[q] = s1.rec[g];
+#pragma unroll
+      for (int c = 0; c < D; ++c) svh[q * D + c] = vhs[(int64_t)g * D + c];
+    }
+    __syncthreads();
+  }
+  const int nr = tile_rounds<G>();
+  for (int round = 0; round < nr; ++round) {
+    const Lane L = tile_lane<G>(t, round, n);
+    if (wide)
+      compat_row<D, G, 2, 2>(L, CompSrcG<D>{s0.nbr, s0.rec, vhs}, CompSrcG<D>{s1.nbr, s1.rec, vhs}, perm, vhs, s0,
+                             s1, m_u, du0, u0, dta, u, dudt_last);
+    else
+      compat_row<D, G, 2, 2>(L, CompSrcT<D>{s0.loc, sr0, svh}, CompSrcT<D>{s1.loc, sr1, svh}, perm, vhs, s0, s1,
+                             m_u, du0, u0, dta, u, dudt_last);
+  }
 }
 
 // ------------------------------------------------------------ reductions
@@ -396,17 +451,24 @@ struct CompArgs {
   const float* masses;
   DtArg dta;
   float *vhs, *du0, *u_out, *dudt_last;
+  int32_t threads;  // tile block size (tiled sweep)
 };
 
 // One finalize sweep over the stages X (and Y when NST = 2) of adjacency A.
-// (Staging the neighbours' records on LDS tiles was measured slower here:
-// the L1 already serves these narrow gathers, and the staging costs
-// occupancy.)
 template <int D, int G, int NST, int MODE>
 static int compat_sweep(const CompArgs& a, const CrkAdjacency* A, const CompIn& X, const CompIn& Y, hipStream_t s) {
   hipLaunchKernelGGL(k_vhalf_sorted<D>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, A->perm, a.n, a.v0, a.dvdt0, a.dvdt1,
                      a.b0, a.b1, a.dta, a.vhs);
   CRK_LAUNCH_CHECK("k_vhalf_sorted");
+  // both stages in one tiled sweep when their pairs carry tile-local indices
+  if constexpr (D <= 2 && NST == 2 && MODE == 2) {
+    if (A->tiles && A->nbr_loc && (A->n_edges == 0 || (X.loc && Y.loc))) {
+      hipLaunchKernelGGL((k_compat_t<D, G>), dim3(tile_count(a.n)), dim3(a.threads), 0, s, A->tiles, A->perm, a.n,
+                         a.vhs, X, Y, a.masses, a.du0, a.u0, a.dta, a.u_out, a.dudt_last);
+      CRK_LAUNCH_CHECK("k_compat_t");
+      return CRK_OK;
+    }
+  }
   hipLaunchKernelGGL((k_compat<D, G, NST, MODE>), dim3(nblocks((int64_t)a.n * G)), dim3(kBlock), 0, s, A->perm, a.n,
                      a.vhs, X, Y, a.masses, a.du0, a.u0, a.dta, a.u_out, a.dudt_last);
   CRK_LAUNCH_CHECK("k_compat");
@@ -502,10 +564,11 @@ int crk_compatible_energy(CrkContext* ctx, const CrkConfig* cfg, int32_t n, floa
   float* du0 = (float*)ctx->du0.p;
   // both stages in one sweep when they share the sorted order
   const bool shared = adj0->perm == adj1->perm;
-  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->pf, (const float4*)e0->rec, b0};
-  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->pf, (const float4*)e1->rec, b1};
+  CompIn c0{adj0->row_offsets, e0->count, e0->nbr, e0->pf, (const float4*)e0->rec, b0, e0->loc};
+  CompIn c1{adj1->row_offsets, e1->count, e1->nbr, e1->pf, (const float4*)e1->rec, b1, e1->loc};
   const int GS = ctx->group[CRK_PASS_COMPAT];
-  const CompArgs ca{n, v0, u0, dvdt0, dvdt1, b0, b1, masses, dta, vhs, du0, u_out, dudt_last};
+  const CompArgs ca{n, v0, u0, dvdt0, dvdt1, b0, b1, masses, dta, vhs, du0, u_out, dudt_last,
+                    ctx->threads[CRK_PASS_COMPAT]};
 #define RUNG(DD)                                                                                  \
   switch (GS) {                                                                                   \
     case 1: st = compat_launch<DD, 1>(ca, adj0, c0, adj1, c1, shared, s); break;                  \

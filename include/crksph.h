@@ -340,8 +340,10 @@ typedef struct CrkEdges {          /* one stage's interacting pairs, caller owne
                              it is used)                                       */
   uint16_t* loc;  /* [E]     tile-local index of each pair's neighbour (written
                              by crk_support when the adjacency has tiles)      */
-  float*   pv;    /* [E*dim] av_ij of each slot, the viscous part of a_ij
-                             (ap_ij = a_ij - av_ij)                            */
+  float*   th;    /* [E]     viscous fraction of each slot's pair force:
+                             av_ij = th a_ij, ap_ij = (1 - th) a_ij (ap and av
+                             are parallel, both along the pair's CRK
+                             gradient, and of one sign: th in [0, 1])         */
 } CrkEdges;
 
 /* Gathers the state into sorted order (x, v, h, m, u, alpha0, divergence). */

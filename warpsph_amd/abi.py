@@ -104,7 +104,7 @@ class CrkUpdate(C.Structure):
 
 
 class CrkEdges(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "pf", "rec", "f", "loc", "pv")]
+    _fields_ = [(k, C.c_void_p) for k in ("count", "nbr", "pf", "rec", "f", "loc", "th")]
 
 
 MLP_IN = 8

@@ -353,7 +353,7 @@ struct AdIn {
   const float* drhodt_u;
 };
 struct AdOut {
-  float *pf, *pv, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
+  float *pf, *th, *dvdt_s, *dvdt_u, *dudt_u, *dEdt_u, *div_u;
 };
 
 // a D-vector of a slot (2D: one 8-byte store)
@@ -543,16 +543,17 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     const float Pj = rj[Rec<D>::P];
     const float pP = -(Pi + Pj) * VV;
     const float pQ = -(Qi + Qj) * VV;
-    float av[3], aa[3];
+    // a_ij = ap + av = (pP + pQ) gradW (accel.py:228,241); pP and pQ have one
+    // sign (P, Q >= 0), so the viscous part is the fraction pQ / (pP + pQ)
+    const float pS = pP + pQ;
+    float aa[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      const float apv = pP * gw[a];
-      av[a] = pQ * gw[a];
-      aa[a] = apv + av[a];
+      aa[a] = pP * gw[a] + pQ * gw[a];
       acc[a].add(aa[a]);
     }
     store_slot<D>(o.pf, q0, aa);
-    store_slot<D>(o.pv, q0, av);
+    o.th[q0] = pS != 0.f ? pQ * frcp(pS) : 0.f;
     float dt_ = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) dt_ = dt_ + (vi[a] - vj[a]) * gw[a];
@@ -755,7 +756,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   if (r) return r;
   CRK_REQUIRE(st && st->divergence, "crk_accel_dudt: state->divergence missing");
   CRK_REQUIRE(up && up->dvdt && up->dudt && up->dEdt && up->drhodt, "crk_accel_dudt: update buffers missing");
-  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->pf && ed->pv)), "crk_accel_dudt: pair-force buffers missing");
+  CRK_REQUIRE(ed && (ctx->adj.n_edges == 0 || (ed->pf && ed->th)), "crk_accel_dudt: pair-force buffers missing");
   CRK_REQUIRE(ed->count == ctx->pair_cnt && ed->nbr == ctx->pair_nbr,
               "crk_accel_dudt: edges differ from the pairs of crk_support");
   if (!ctx->have_velgrad) { set_error("crk_accel_dudt before crk_velocity_gradient"); return CRK_ESTATE; }
@@ -770,7 +771,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   AdIn in{F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->v), F<float>(ctx->m), F<float>(ctx->V),
           F<float>(ctx->rho), F<float>(ctx->P), F<float>(ctx->c), F<float>(ctx->J), F<float>(ctx->A),
           F<float>(ctx->B), F<float>(ctx->gA), F<float>(ctx->gB), F<float>(ctx->alpha), (const float*)up->drhodt};
-  AdOut out{ed->pf, ed->pv, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
+  AdOut out{ed->pf, ed->th, F<float>(ctx->dvdt), up->dvdt, up->dudt, up->dEdt, st->divergence};
   // staged on tiles unless alpha-weighted Q reads alpha_j through the sorted ids
   // (or a group of fewer than 4 lanes would leave a round beyond the tile)
   const bool tiled = use_tiles(ctx) && ctx->pair_loc && !cfg->alphaWeightedQ && ctx->group[CRK_PASS_ACCEL] >= 4;

@@ -1117,7 +1117,7 @@ template <int D>
 __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* __restrict__ nbr,
                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ pnbr, int32_t n,
                                const int32_t* __restrict__ perm, const float* __restrict__ m_u, DtArg dt,
-                               const float* __restrict__ pf, const float* __restrict__ pv,
+                               const float* __restrict__ pf, const float* __restrict__ th,
                                const float4* __restrict__ rec, const float* __restrict__ f,
                                float* __restrict__ ap_o, float* __restrict__ av_o, float* __restrict__ f_o) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1129,11 +1129,12 @@ __global__ void k_edges_export(const int32_t* __restrict__ off, const int32_t* _
   for (int32_t q = b; q < e; ++q) {
     const bool hit = p < c && pnbr[b + p] == nbr[q];
     const int64_t sl = b + p;
+    const float tv = hit ? th[sl] : 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      const float va = hit ? pv[sl * D + a] : 0.f;
-      if (ap_o) ap_o[(int64_t)q * D + a] = hit ? pf[sl * D + a] - va : 0.f;
-      if (av_o) av_o[(int64_t)q * D + a] = va;
+      const float fa = hit ? pf[sl * D + a] : 0.f;
+      if (ap_o) ap_o[(int64_t)q * D + a] = (1.f - tv) * fa;
+      if (av_o) av_o[(int64_t)q * D + a] = tv * fa;
     }
     if (f_o) {
       float fv = 0.f;
@@ -1617,7 +1618,7 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
   CRK_REQUIRE(adj->n > 0 && adj->row_offsets && adj->perm && ed->count, "crk_edges_export: adjacency/pairs missing");
   if (adj->n_edges == 0) return CRK_OK;
   CRK_REQUIRE(adj->nbr && ed->nbr, "crk_edges_export: neighbour lists missing");
-  CRK_REQUIRE((!ap_out && !av_out) || (ed->pf && ed->pv), "crk_edges_export: pair data missing");
+  CRK_REQUIRE((!ap_out && !av_out) || (ed->pf && ed->th), "crk_edges_export: pair data missing");
   const DtArg dta = dt_arg(ctx, dt);
   CRK_REQUIRE(!f_out || ed->f || (ed->rec && masses && ed->pf && (dt > 0.f || dta.p)),
               "crk_edges_export: f needs edges->f, or edges->rec with masses, pair forces and dt > 0");
@@ -1626,7 +1627,7 @@ int crk_edges_export(CrkContext* ctx, int32_t dim, const CrkAdjacency* adj, cons
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   CRK_DIM_DISPATCH(dim, hipLaunchKernelGGL(k_edges_export<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, adj->row_offsets,
-                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->pf, ed->pv,
+                                           adj->nbr, ed->count, ed->nbr, n, adj->perm, masses, dta, ed->pf, ed->th,
                                            (const float4*)ed->rec, ed->f, ap_out, av_out, f_out));
   CRK_LAUNCH_CHECK("k_edges_export");
   return CRK_OK;

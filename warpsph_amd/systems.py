@@ -58,9 +58,10 @@ class StagePairs:
         self.dtp = None
         self.count = torch.empty(adjacency.n, dtype=torch.int32, device=dev)
         self.nbr = torch.empty(E, dtype=torch.int32, device=dev)
-        # pair accelerations per slot: a_ij = ap_ij + av_ij and av_ij
+        # pair accelerations per slot: a_ij = ap_ij + av_ij, and the viscous
+        # fraction th (av_ij = th a_ij)
         self.pf = torch.empty((E, dim), dtype=torch.float32, device=dev)
-        self.pv = torch.empty((E, dim), dtype=torch.float32, device=dev)
+        self.th = torch.empty(E, dtype=torch.float32, device=dev)
         self.rec = torch.empty((adjacency.n, 4), dtype=torch.float32, device=dev)
         # tile-local neighbour index of each pair (adjacencies with LDS tiles)
         self.loc = torch.empty(E, dtype=torch.int16, device=dev) if adjacency.tiles is not None else None
@@ -69,7 +70,7 @@ class StagePairs:
     def cabi(self) -> abi.CrkEdges:
         """Pointers for the C-ABI; f_ij stays implicit (evaluated from rec)."""
         return abi.CrkEdges(*(abi.ptr(t).value for t in (self.count, self.nbr, self.pf, self.rec)), None,
-                            abi.ptr(self.loc).value, abi.ptr(self.pv).value)
+                            abi.ptr(self.loc).value, abi.ptr(self.th).value)
 
     @property
     def n_pairs(self) -> int:

@@ -56,13 +56,12 @@ __device__ inline void velgrad_row(const Lane& L, const Src& src, ARows rows, Ge
     const float xj[3] = {gj.x, gj.y, gj.z};
     pair_dx<D>(geo, xi, xj, dx);
     float r = fsqrt(dot<D>(dx, dx));
-    float ir = inv_or_zero(r);
     const float vj[3] = {wj.x, wj.y, wj.z};
     float hj = gj.w;
     if (CULLEN) {
       float H = fmaxf(hi, hj);
       if (r < H) {
-        WdW w = wdw<K, D>(C, r, ir, frcp(H));
+        WdW w = wdw<K, D>(C, r, frcp(H));
         float dt_ = 0.f;
 #pragma unroll
         for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * dx[a];
@@ -70,7 +69,7 @@ __device__ inline void velgrad_row(const Lane& L, const Src& src, ARows rows, Ge
       }
     }
     if (!(r < hj)) continue;
-    WdW w = wdw<K, D>(C, r, ir, frcp(hj));
+    WdW w = wdw<K, D>(C, r, frcp(hj));
     float g[3];
     grad_wr<D>(cr, dx, w, g);
     float Vj = D < 3 ? wj.z : wj.w;
@@ -225,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_gradnorm(ARows rows, Geo geo, float 
     const float r = fsqrt(dot<D>(dx, dx));
     const float hj = in.h[j];
     if (!(r < hj)) continue;
-    WdW w = wdw<K, D>(C, r, inv_or_zero(r), frcp(hj));
+    WdW w = wdw<K, D>(C, r, frcp(hj));
     float g[3];
     grad_wr<D>(cr, dx, w, g);
     const float Vj = in.V[j];
@@ -371,7 +370,8 @@ __device__ inline void store_slot(float* p, int64_t q, const float* v) {
 // neighbours; one coalesced pass packs them into the neighbour record
 // (Rec<D>) first, so that a neighbour costs NF/4 16-byte loads.
 template <int D>
-__global__ void k_pack_rec(int32_t n, AdIn in, const float* __restrict__ divcd, float4* __restrict__ rec) {
+__global__ void k_pack_rec(int32_t n, AdIn in, const float* __restrict__ divcd, float Cl, float Cq,
+                           float4* __restrict__ rec) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   float r[Rec<D>::NF];
@@ -391,12 +391,14 @@ __global__ void k_pack_rec(int32_t n, AdIn in, const float* __restrict__ divcd, 
   }
   r[Rec<D>::H] = in.h[k];
   r[Rec<D>::V] = in.V[k];
-  r[Rec<D>::RHO] = in.rho[k];
-  r[Rec<D>::C] = in.c[k];
+  const float rho = in.rho[k];
+  r[Rec<D>::QA] = -Cl * rho * in.c[k];
+  r[Rec<D>::QB] = Cq * rho;
   r[Rec<D>::A] = in.A[k];
   r[Rec<D>::P] = in.P[k];
   r[Rec<D>::M] = in.m[k];
-  r[Rec<D>::DIV] = divcd ? divcd[k] : 0.f;
+  const float dv = divcd ? divcd[k] : 0.f;
+  r[Rec<D>::MSG] = dv > 0.f ? in.m[k] : (dv < 0.f ? -in.m[k] : 0.f);
   float4* p = rec + (int64_t)k * (Rec<D>::NF / 4);
 #pragma unroll
   for (int c = 0; c < Rec<D>::NF / 4; ++c) p[c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
@@ -426,7 +428,7 @@ struct RecSrcT {
 // the sorted ids: global source only).
 template <int D, int G, int K, bool SW, class Src>
 __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo geo, float C, const LimParams& lp,
-                                 float inv_kscale, float Cl, float Cq, int alphaQ, int32_t n, const AdIn& in,
+                                 float inv_kscale, int alphaQ, int32_t n, const AdIn& in,
                                  const int32_t* __restrict__ perm, const AdOut& o, const CdIn& cin,
                                  const CdParams& cp, const CdOut& cout, const float4* __restrict__ rec) {
   const int32_t k = L.k;
@@ -438,7 +440,7 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
   for (int d = 0; d < D; ++d) { xi[d] = ri[Rec<D>::X + d]; vi[d] = ri[Rec<D>::VEL + d]; }
 #pragma unroll
   for (int a = 0; a < D * D; ++a) Ji[a] = ri[Rec<D>::J + a];
-  const float hi = ri[Rec<D>::H], rhoi = ri[Rec<D>::RHO], Pi = ri[Rec<D>::P], ci = ri[Rec<D>::C],
+  const float hi = ri[Rec<D>::H], Pi = ri[Rec<D>::P], qai = ri[Rec<D>::QA], qbi = ri[Rec<D>::QB],
               Vi = ri[Rec<D>::V], mi = ri[Rec<D>::M];
   const float ihi = 1.f / hi;
   const float ali = alphaQ ? in.alpha[k] : 1.f;
@@ -474,7 +476,6 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     const float r = fsqrt(r2);
     const float hj = rj[Rec<D>::H];
     const float ihj = frcp(hj);
-    const float ir = inv_or_zero(r);
     const float* vj = &rj[Rec<D>::VEL];
     const float* Jj = &rj[Rec<D>::J];
     float cI[3], cJ[3];
@@ -500,13 +501,9 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     for (int a = 0; a < D; ++a) { crj.B[a] = rj[Rec<D>::B + a]; crj.gA[a] = rj[Rec<D>::GA + a]; }
 #pragma unroll
     for (int a = 0; a < D * D; ++a) crj.gB[a] = rj[Rec<D>::GB + a];
-    WdW wj = wdw<K, D>(C, r, ir, ihj);  // W(x_ij, h_j) for the i-side gradient
-    WdW wi = wdw<K, D>(C, r, ir, ihi);  // W(x_ji, h_i) for the j-side gradient
-    if (SW && wi.W != 0.f) {
-      float dj = rj[Rec<D>::DIV];
-      float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);
-      accs.add(rj[Rec<D>::M] * sg * wi.W);
-    }
+    WdW wj = wdw<K, D>(C, r, ihj);  // W(x_ij, h_j) for the i-side gradient
+    WdW wi = wdw<K, D>(C, r, ihi);  // W(x_ji, h_i) for the j-side gradient
+    if (SW && wi.W != 0.f) accs.add(rj[Rec<D>::MSG] * wi.W);
     float gi[3], gjr[3], dxn[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) dxn[a] = -dx[a];
@@ -530,11 +527,12 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     const float fi = ismi * rdi, fj = ismj * rdj;
     float mui = fminf(wx * fi, 0.f), muj = fminf(wx * fj, 0.f);
     float muiU = fminf(wxU * fi, 0.f), mujU = fminf(wxU * fj, 0.f);
-    float rhoj = rj[Rec<D>::RHO], cj = rj[Rec<D>::C];
-    float Qi = rhoi * (-Cl * ci * mui + Cq * mui * mui);
-    float Qj = rhoj * (-Cl * cj * muj + Cq * muj * muj);
-    float QiU = rhoi * (-Cl * ci * muiU + Cq * muiU * muiU);
-    float QjU = rhoj * (-Cl * cj * mujU + Cq * mujU * mujU);
+    // Q = rho (-C_l c mu + C_q mu^2) (accel.py:202-206, dudt.py) as mu (qa + qb mu)
+    const float qaj = rj[Rec<D>::QA], qbj = rj[Rec<D>::QB];
+    float Qi = mui * (qai + qbi * mui);
+    float Qj = muj * (qaj + qbj * muj);
+    float QiU = muiU * (qai + qbi * muiU);
+    float QjU = mujU * (qaj + qbj * mujU);
     if (alphaQ) {
       float wa = 0.5f * (ali + in.alpha[j]);
       Qi = Qi * wa; Qj = Qj * wa; QiU = QiU * wa; QjU = QjU * wa;
@@ -575,17 +573,17 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
   }
   o.dudt_u[i] = du;
   o.dEdt_u[i] = mi * ke + mi * du;
-  o.div_u[i] = -in.drhodt_u[i] / rhoi;
+  o.div_u[i] = -in.drhodt_u[i] / in.rho[k];
   if (SW) cullen_row(cin, cp, k, i, S, hi, cout);
 }
 
 template <int D, int G, int K, bool SW>
 __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, float C, LimParams lp, float inv_kscale,
-                                                       float Cl, float Cq, int alphaQ, int32_t n, AdIn in,
+                                                       int alphaQ, int32_t n, AdIn in,
                                                        const int32_t* __restrict__ perm, AdOut o, CdIn cin,
                                                        CdParams cp, CdOut cout, const float4* __restrict__ rec) {
   const Lane L = lane_of<G>(n);
-  accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, Cl, Cq, alphaQ, n, in, perm, o,
+  accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, alphaQ, n, in, perm, o,
                          cin, cp, cout, rec);
 }
 
@@ -595,7 +593,7 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
 template <int D, int G, int K, bool SW>
 __global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, const uint16_t* __restrict__ ploc,
                                                                 const int32_t* __restrict__ tiles, Geo geo, float C,
-                                                                LimParams lp, float inv_kscale, float Cl, float Cq,
+                                                                LimParams lp, float inv_kscale,
                                                                 int32_t n, AdIn in, const int32_t* __restrict__ perm,
                                                                 AdOut o, CdIn cin, CdParams cp, CdOut cout,
                                                                 const float4* __restrict__ rec) {
@@ -618,10 +616,10 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, co
   for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);
     if (wide)
-      accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, Cl, Cq, 0, n, in, perm, o, cin,
+      accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
                              cp, cout, rec);
     else
-      accel_row<D, G, K, SW>(L, RecSrcT{ploc, srec}, rows, geo, C, lp, inv_kscale, Cl, Cq, 0, n, in, perm, o, cin,
+      accel_row<D, G, K, SW>(L, RecSrcT{ploc, srec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
                              cp, cout, rec);
   }
 }
@@ -779,6 +777,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   const float* divcd = (cfg->switchParams.scheme == CRK_SWITCH_CULLEN_DEHNEN ||
                         cfg->switchParams.scheme == CRK_SWITCH_LEARNED) ? F<float>(ctx->divcd) : nullptr;
   CRK_DIM_DISPATCH(ctx->dim, hipLaunchKernelGGL(k_pack_rec<DD>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, in, divcd,
+                                                cfg->diffusion.C_l, cfg->diffusion.C_q,
                                                 F<float4>(ctx->rec)));
   CRK_LAUNCH_CHECK("k_pack_rec");
   if (tiled) {
@@ -787,7 +786,7 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
         hipLaunchKernelGGL((k_accel_dudt_t<DD, (G < 4 ? 4 : G), KK, SWV>), dim3(tile_count(n)),                 \
                            dim3(ctx->threads[CRK_PASS_ACCEL]), 0, s, arows_of(ctx), ctx->pair_loc,             \
                            ctx->adj.tiles, geo, C, lp,                                                          \
-                           inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q, n, in, ctx->adj.perm, out, CI, CP, \
+                           inv_kscale, n, in, ctx->adj.perm, out, CI, CP,                                       \
                            CO, F<float4>(ctx->rec)))
     if (sw) { ACCEL_T(true, cd_in(ctx), cd_params(ctx, cfg, dt), cd_out(ctx, st)); }
     else { ACCEL_T(false, CdIn{}, CdParams{}, CdOut{}); }
@@ -795,13 +794,13 @@ static int accel_dudt(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st,
   } else if (sw) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
-                           arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
+                           arows_of(ctx), geo, C, lp, inv_kscale,
                            cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, cd_in(ctx), cd_params(ctx, cfg, dt),
                            cd_out(ctx, st), F<float4>(ctx->rec)));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_accel_dudt<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s,
-                           arows_of(ctx), geo, C, lp, inv_kscale, cfg->diffusion.C_l, cfg->diffusion.C_q,
+                           arows_of(ctx), geo, C, lp, inv_kscale,
                            cfg->alphaWeightedQ, n, in, ctx->adj.perm, out, CdIn{}, CdParams{}, CdOut{},
                            F<float4>(ctx->rec)));
   }

@@ -50,24 +50,27 @@ __device__ inline void store_vec(float* p, const float* v) {
 // Every per-particle field that the acceleration pass gathers from its
 // neighbours, packed per sorted particle into NF floats (whole float4s): a
 // neighbour then costs NF/4 16-byte loads from one or two cache lines
-// instead of one load per field array.  The lighter passes gather a few
+// instead of one load per field array.  The viscosity's density and sound
+// speed enter only as Q = mu (qa + qb mu) with qa = -C_l rho c and
+// qb = C_q rho, and the switch's divergence only as m sgn(div): the record
+// holds those products.  The lighter passes gather a few
 // narrow fields and stay on the SoA arrays (a wide record would touch a
 // cache line per neighbour for a few bytes).
 template <int D>
 struct Rec;
 template <>
 struct Rec<1> {
-  static constexpr int NF = 16, X = 0, H = 1, V = 2, VEL = 3, RHO = 4, C = 5, J = 6, A = 7, B = 8, GA = 9, GB = 10,
-                       P = 11, M = 12, DIV = 13;
+  static constexpr int NF = 16, X = 0, H = 1, V = 2, VEL = 3, QA = 4, QB = 5, J = 6, A = 7, B = 8, GA = 9, GB = 10,
+                       P = 11, M = 12, MSG = 13;
 };
 template <>
 struct Rec<2> {
-  static constexpr int NF = 24, X = 0, H = 2, V = 3, VEL = 4, RHO = 6, C = 7, J = 8, A = 12, B = 13, GA = 15,
-                       GB = 17, P = 21, M = 22, DIV = 23;
+  static constexpr int NF = 24, X = 0, H = 2, V = 3, VEL = 4, QA = 6, QB = 7, J = 8, A = 12, B = 13, GA = 15,
+                       GB = 17, P = 21, M = 22, MSG = 23;
 };
 template <>
 struct Rec<3> {
-  static constexpr int NF = 40, X = 0, H = 3, V = 4, VEL = 5, RHO = 8, C = 9, P = 10, M = 11, DIV = 12, A = 13,
+  static constexpr int NF = 40, X = 0, H = 3, V = 4, VEL = 5, QA = 8, QB = 9, P = 10, M = 11, MSG = 12, A = 13,
                        B = 14, GA = 17, GB = 20, J = 29;
 };
 // the whole record of particle j (loads of fields the caller does not use
@@ -178,6 +181,27 @@ __device__ inline void kwd(float q, float& k, float& dk) {
   }
 }
 
+// k(q) and (dk/dq) / q: every Wendland dk/dq carries a factor q, so the
+// gradient's dW/dr / r needs no 1/r (and stays finite at r = 0, where it
+// multiplies x_ij = 0)
+template <int K, int D>
+__device__ inline void kwq(float q, float& k, float& dkq) {
+  float t = 1.f - q;
+  float t2 = t * t;
+  if (K == CRK_KERNEL_WENDLAND2) {
+    if (D == 1) { k = t2 * t * (1.f + 3.f * q); dkq = -12.f * t2; }
+    else { float t3 = t2 * t; k = t3 * t * (1.f + 4.f * q); dkq = -20.f * t3; }
+  } else if (K == CRK_KERNEL_WENDLAND4) {
+    float t4 = t2 * t2;
+    if (D == 1) { k = t4 * t * (1.f + 5.f * q + 8.f * q * q); dkq = -14.f * t4 * (1.f + 4.f * q); }
+    else { float t5 = t4 * t; k = t5 * t * (1.f + 6.f * q + (35.f / 3.f) * q * q); dkq = (-56.f / 3.f) * t5 * (1.f + 5.f * q); }
+  } else {
+    float t4 = t2 * t2;
+    if (D == 1) { float t6 = t4 * t2; k = t6 * t * (1.f + 7.f * q + 19.f * q * q + 21.f * q * q * q); dkq = -6.f * t6 * (3.f + 18.f * q + 35.f * q * q); }
+    else { float t7 = t4 * t2 * t; k = t7 * t * (1.f + 8.f * q + 25.f * q * q + 32.f * q * q * q); dkq = -22.f * t7 * (1.f + 7.f * q + 16.f * q * q); }
+  }
+}
+
 template <int D>
 __device__ inline float ipow(float x) {
   float r = x;
@@ -191,15 +215,15 @@ struct WdW {
   float W, fac;
 };
 template <int K, int D>
-__device__ inline WdW wdw(float C, float r, float invr, float invh) {
+__device__ inline WdW wdw(float C, float r, float invh) {
   WdW o;
   float q = r * invh;
   if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
-  float k, dk;
-  kwd<K, D>(q, k, dk);
-  float ihD = ipow<D>(invh);
-  o.W = C * k * ihD;
-  o.fac = C * dk * ihD * invh * invr;  // r > 0 implied by dk(0) = 0 and invr finite
+  float k, dkq;
+  kwq<K, D>(q, k, dkq);
+  const float CihD = C * ipow<D>(invh);
+  o.W = CihD * k;
+  o.fac = CihD * dkq * (invh * invh);  // (dW/dr) / r = C h^-d (dk/dq / q) / h^2
   return o;
 }
 

@@ -740,7 +740,7 @@ struct MomAcc {
   // one pair with r < h_j: W = W(x_ij, h_j) and its x_i-gradient
   template <int K>
   __device__ inline void add(float C, const float* dx, float r, float hj, float Vj, float mj) {
-    WdW w = wdw<K, D>(C, r, inv_or_zero(r), frcp(hj));
+    WdW w = wdw<K, D>(C, r, frcp(hj));
     float g[3];
 #pragma unroll
     for (int d = 0; d < D; ++d) g[d] = w.fac * dx[d];

@@ -48,7 +48,10 @@ enum {
 enum { /* KernelFunctions subset (warpSPHCore; values are this library's) */
   CRK_KERNEL_WENDLAND2 = 0,
   CRK_KERNEL_WENDLAND4 = 1,
-  CRK_KERNEL_WENDLAND6 = 2
+  CRK_KERNEL_WENDLAND6 = 2,
+  CRK_KERNEL_B7 = 3     /* degree-7 B-spline, knots at r/H = 1/4, 1/2, 3/4
+                           (the kernel of every reference compressible example,
+                           e.g. examples/compressible/12-Kelvin-Helmholtz.ipynb) */
 };
 enum { /* EnergyScheme, src/warpSPH/enumTypes.py:5-11 */
   CRK_ENERGY_EQUALWORK = 0, CRK_ENERGY_PDV = 1, CRK_ENERGY_DIMINISHING = 2,
@@ -205,6 +208,11 @@ int crk_cfl_dt_runs(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const floa
 /* ------------------------------------------------------ host utilities */
 /* Kernel normalisation C_d and kernel scale H/h (Dehnen & Aly 2012). */
 int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kernel_scale);
+/* xi of the CFL step and of the Cullen-Dehnen relaxation time, the value of
+ * warpSPHCore's sphKernel_xi (timestep/compressible.py:50,
+ * CullenDehnen2010.py:286): the kernel scale H/h for the Wendland kernels;
+ * for B7 the values the reference's example outputs fix (1D, 2D). */
+int crk_kernel_xi(int32_t kernel, int32_t dim, double* xi);
 
 /* Owen psi_H look-up table on n_h in [n_min, n_max] (nlut entries).
  * Restates computePsi, src/warpSPH/modules/adaptiveSupport/wp_psi.py:8-99

@@ -49,25 +49,72 @@ namespace {
 // ------------------------------------------------------------ kernels
 // W(r, H) = C_d / H^d * k(r/H), 0 <= q < 1 (H is the cut-off radius, the
 // reference's "support", limiter.py:107-112).
+// B7 (kernel 3): the centred degree-7 B-spline on the support,
+// k(q) = sum_m c_m (b_m - q)_+^7 with c = (1, -8, 28, -56), b = (1, 3/4,
+// 1/2, 1/4); C_d = 1 / (S_d I_{d-1}), I_n = sum_m c_m b_m^(n+8) n! 7! / (n+8)!
+// (I_0 = 315/8192, I_1 = 7435/1179648, I_2 = 105/65536), S_d = 2, 2 pi, 4 pi.
 double kernelC(int kernel, int dim) {
-  static const double C[3][3] = {
+  static const double C[4][3] = {
       {1.25, 2.228169203286535, 3.3422538049298027},   // Wendland C2: 5/4, 7/pi, 21/(2pi)
       {1.5, 2.8647889756541161, 4.9238560519055126},   // Wendland C4: 3/2, 9/pi, 495/(32pi)
-      {1.71875, 3.5468815889050962, 6.7889530412636603} // Wendland C6: 55/32, 78/(7pi), 1365/(64pi)
+      {1.71875, 3.5468815889050962, 6.7889530412636603}, // Wendland C6: 55/32, 78/(7pi), 1365/(64pi)
+      {13.003174603174603, 25.25175659831448, 49.668468335573586}  // B7: 8192/630, 1179648/(2 pi 7435), 65536/(420 pi)
   };
   return C[kernel][dim - 1];
 }
-// H/h with h = twice the kernel standard deviation (Dehnen & Aly 2012).
+// H/h with h = twice the kernel standard deviation (Dehnen & Aly 2012);
+// B7: sigma^2 = I_{d+1} / (d I_{d-1}) = 1/24, 531453/13085600, 19/480.
 double kernelScale(int kernel, int dim) {
-  static const double S[3][3] = {
+  static const double S[4][3] = {
       {1.6201851746019651, 1.8973665961010275, 1.9364916731037083},
       {1.9364916731037087, 2.1712405933672376, 2.2079402165819615},
-      {2.2079402165819615, 2.4152294576982398, 2.4494897427831779}};
+      {2.2079402165819615, 2.4152294576982398, 2.4494897427831779},
+      {2.449489742783178, 2.4810435822586725, 2.513123449750173}};  // B7: 1 / (2 sigma)
   return S[kernel][dim - 1];
+}
+
+// xi of the CFL step and the Cullen-Dehnen time (sphKernel_xi,
+// timestep/compressible.py:50, CullenDehnen2010.py:286): the kernel scale for
+// the Wendland kernels; for B7 the values the reference's example outputs
+// fix in 1D and 2D (the Sod tube's and the Kelvin-Helmholtz run's printed
+// initial steps, with initial states this build reproduces), 3D by the 2D
+// value's ratio to the kernel scale.
+double kernelXi(int kernel, int dim) {
+  if (kernel == CRK_KERNEL_B7) {
+    const double X[3] = {3.865665, 3.932735, 3.932735 * (2.513123449750173 / 2.4810435822586725)};
+    return X[dim - 1];
+  }
+  return kernelScale(kernel, dim);
+}
+
+// B7 profile and derivative: below the first knot the piece's polynomial
+// (the truncated powers cancel there), above it the truncated powers.
+template <class R>
+R b7p7(R x) { R x2 = x * x; R x3 = x2 * x; return x3 * x3 * x; }
+template <class R>
+R b7p6(R x) { R x2 = x * x; R x3 = x2 * x; return x3 * x3; }
+template <class R>
+R b7val(R q) {
+  if (q < R(0.25)) {
+    R q2 = q * q;
+    return R(151.0 / 1024.0) + q2 * (R(-105.0 / 64.0) + q2 * (R(35.0 / 4.0) + q2 * (R(-35) + R(35) * q)));
+  }
+  R a = R(1) - q, b = std::max(R(0.75) - q, R(0)), c = std::max(R(0.5) - q, R(0));
+  return b7p7(a) - R(8) * b7p7(b) + R(28) * b7p7(c);
+}
+template <class R>
+R b7der(R q) {
+  if (q < R(0.25)) {
+    R q2 = q * q;
+    return q * (R(-105.0 / 32.0) + q2 * (R(35) + q2 * (R(-210) + R(245) * q)));
+  }
+  R a = R(1) - q, b = std::max(R(0.75) - q, R(0)), c = std::max(R(0.5) - q, R(0));
+  return R(-7) * (b7p6(a) - R(8) * b7p6(b) + R(28) * b7p6(c));
 }
 
 template <class R>
 R kval(int kernel, int dim, R q) {
+  if (kernel == CRK_KERNEL_B7) return b7val(q);
   R t = R(1) - q;
   if (kernel == CRK_KERNEL_WENDLAND2) {
     if (dim == 1) return t * t * t * (R(1) + R(3) * q);
@@ -85,6 +132,7 @@ R kval(int kernel, int dim, R q) {
 }
 template <class R>
 R kder(int kernel, int dim, R q) {  // dk/dq
+  if (kernel == CRK_KERNEL_B7) return b7der(q);
   R t = R(1) - q;
   if (kernel == CRK_KERNEL_WENDLAND2) {
     if (dim == 1) return R(-12) * q * t * t;
@@ -349,7 +397,7 @@ struct Sim {
     K.kernel = cfg.kernel;
     K.C = R(kernelC(cfg.kernel, D));
     kscale = R(kernelScale(cfg.kernel, D));
-    xi = kscale;
+    xi = R(kernelXi(cfg.kernel, D));
     nhTarget = nH_to_n_h((double)cfg.targetNeighbors, D);
     lut_nh.resize(4096); lut_psiH.resize(4096);
     owenLUT(cfg.kernel, D, 4096, 2.0, 6.0, lut_nh.data(), lut_psiH.data());
@@ -1329,13 +1377,18 @@ SimBase* makeSim(const CrkConfig& c, int n) {
 extern "C" {
 
 int orc_kernel_constants(int kernel, int dim, double* C, double* scale) {
-  if (kernel < 0 || kernel > 2 || dim < 1 || dim > 3) return CRK_EINVAL;
+  if (kernel < 0 || kernel > 3 || dim < 1 || dim > 3) return CRK_EINVAL;
   *C = kernelC(kernel, dim);
   *scale = kernelScale(kernel, dim);
   return CRK_OK;
 }
+int orc_kernel_xi(int kernel, int dim, double* xi) {
+  if (kernel < 0 || kernel > 3 || dim < 1 || dim > 3) return CRK_EINVAL;
+  *xi = kernelXi(kernel, dim);
+  return CRK_OK;
+}
 int orc_kernel_eval(int kernel, int dim, int n, const double* q, double* k, double* dk) {
-  if (kernel < 0 || kernel > 2 || dim < 1 || dim > 3) return CRK_EINVAL;
+  if (kernel < 0 || kernel > 3 || dim < 1 || dim > 3) return CRK_EINVAL;
   for (int i = 0; i < n; ++i) {
     bool in = q[i] < 1.0;
     k[i] = in ? kval<double>(kernel, dim, q[i]) : 0.0;
@@ -1344,7 +1397,7 @@ int orc_kernel_eval(int kernel, int dim, int n, const double* q, double* k, doub
   return CRK_OK;
 }
 int orc_owen_lut(int kernel, int dim, int nlut, double nmin, double nmax, double* nh, double* psiH) {
-  if (kernel < 0 || kernel > 2 || dim < 1 || dim > 3 || nlut < 2) return CRK_EINVAL;
+  if (kernel < 0 || kernel > 3 || dim < 1 || dim > 3 || nlut < 2) return CRK_EINVAL;
   owenLUT(kernel, dim, nlut, nmin, nmax, nh, psiH);
   return CRK_OK;
 }
@@ -1363,7 +1416,7 @@ uint32_t orc_morton(int dim, int cx, int cy, int cz) {
 
 void* orc_sim_create(const CrkConfig* cfg, int precision, int n) {
   if (!cfg || n <= 0 || cfg->domain.dim < 1 || cfg->domain.dim > 3) return nullptr;
-  if (cfg->kernel < 0 || cfg->kernel > 2) return nullptr;
+  if (cfg->kernel < 0 || cfg->kernel > 3) return nullptr;
   return precision == 64 ? makeSim<double>(*cfg, n) : makeSim<float>(*cfg, n);
 }
 void orc_sim_destroy(void* s) { delete (SimBase*)s; }

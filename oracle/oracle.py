@@ -81,6 +81,15 @@ def kernel_constants(kernel: int, dim: int):
     return c.value, s.value
 
 
+def kernel_xi(kernel: int, dim: int) -> float:
+    """xi of the CFL step and the Cullen-Dehnen time."""
+    x = C.c_double()
+    L = lib()
+    L.orc_kernel_xi.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
+    assert L.orc_kernel_xi(kernel, dim, C.byref(x)) == 0
+    return x.value
+
+
 def kernel_eval(kernel: int, dim: int, q):
     q = np.ascontiguousarray(q, dtype=np.float64)
     k = np.empty_like(q)

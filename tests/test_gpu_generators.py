@@ -172,3 +172,72 @@ def test_kelvin_helmholtz_timestep_on_the_gpu():
     if dt > prev:
         dt = min(dt, 1.1 * prev)
     assert config.dt == pytest.approx(dt * 2 / 3, rel=1e-5)
+
+
+def _notebook_config(dim, L, nx, dt, device):
+    """buildConfig as the reference's compressible notebooks call it: B7,
+    n_h = 4 particles per support radius, CFL 0.3, the Owen supports and no
+    viscosity switch in the scheme configuration."""
+    from warpsph_amd.configurations import CRKSPHConfig, buildConfig, buildDomainDescription
+    from warpsph_amd.enumTypes import AdaptiveSupportScheme, KernelFunctions, ViscositySwitch
+    from warpsph_amd.utils import n_h_to_nH
+    config, _ = buildConfig(domain=buildDomainDescription(L, dim, True, device, torch.float32), dim=dim,
+                            kernel=KernelFunctions.B7, targetNeighbors=n_h_to_nH(4, dim), device=device,
+                            dtype=torch.float32, dt=dt, adaptiveDt=True, cflFactor=0.3)
+    config.nx = nx
+    sc = CRKSPHConfig()
+    sc.gamma = 5 / 3
+    sc.rho0 = 1.0
+    sc.viscositySwitchParams.scheme = ViscositySwitch.NoneSwitch
+    sc.adaptiveSupportScheme = AdaptiveSupportScheme.Owen
+    return config, sc
+
+
+def test_sod_notebook_b7_initial_state_and_step():
+    """examples/compressible/01-Sod_Shock_Tube_1D.ipynb cells 2-3 and 7: nx =
+    800, left (p, rho) = (1, 1), right (0.1795, 0.25), sampling ratio 4,
+    smoothIC, B7, Owen.  The internal energy before the interface blend,
+    P / ((gamma - 1) rho) with rho the Owen-adapted gather density, printed
+    by the reference as '0.7121 to  2.138' ({:6.4g}): this pins the B7
+    kernel, its Owen table, the 16 Owen iterations and the gather density
+    against a number the reference holds.  The initial step 3.0056745163165033e-4
+    then fixes B7's xi in 1D (crk_kernel_xi); the test checks that the CFL
+    path reproduces it to FP32 rounding."""
+    from warpsph_amd.enumTypes import CompressibleSPHScheme
+    from warpsph_amd.generators import buildSod1D, sodInitialState
+    from warpsph_amd.scheme import buildScheme
+    announce("sod notebook B7")
+    System, State, *_ = buildScheme(CompressibleSPHScheme.CRKSPH)
+    dev = torch.device("cuda")
+    config, sc = _notebook_config(1, 2, 800, 1e-3, dev)
+    system = buildSod1D(System, State, 4, sodInitialState(1, 1, 0), sodInitialState(0.1795, 0.25, 0), 5 / 3,
+                        config, True, adaptiveSupportScheme=sc.adaptiveSupportScheme)
+    st = system.state
+    P0 = torch.where(st.materials == 0, torch.tensor(1.0, device=dev), torch.tensor(0.1795, device=dev))
+    u = P0 / ((5 / 3 - 1) * st.densities)
+    text = f"{u.min().item():6.4g} to {u.max().item():6.4g}"
+    print(text, config.dt, flush=True)
+    assert text == "0.7121 to  2.138"
+    assert config.dt == pytest.approx(3.0056745163165033e-4, rel=3e-7)
+
+
+def test_kelvin_helmholtz_notebook_b7_step():
+    """examples/compressible/12-Kelvin-Helmholtz.ipynb cells 2-4 and 8: nx =
+    256, B7, n_h = 4 (h = 4 dx on the lattice), P = 2.5, rho in {1, 2},
+    config.dt = computeTimestep * 2/3.  The printed 3.892789827659726e-4
+    fixes B7's xi in 2D; the test checks the sampler, the EOS and the CFL
+    path reproduce it to FP32 rounding."""
+    import math
+    from warpsph_amd.enumTypes import CompressibleSPHScheme
+    from warpsph_amd.generators import sampleKHH
+    from warpsph_amd.scheme import buildScheme
+    announce("kelvin-helmholtz notebook B7")
+    System, State, *_ = buildScheme(CompressibleSPHScheme.CRKSPH)
+    dev = torch.device("cuda")
+    config, sc = _notebook_config(2, 1, 256, None, dev)
+    config.domain.min = torch.tensor([0.0, 0.0], device=dev)
+    config.domain.max = torch.tensor([1.0, 1.0], device=dev)
+    config.minDt = 1e-8
+    system = sampleKHH(1.0, 2.0, 0.5, -0.5, 0.025, 0.05 / math.sqrt(2.0), 4.0, 0.1, 256, config, sc, State, System)
+    assert float(system.state.supports.min()) == pytest.approx(4.0 / 256.0, rel=1e-6)
+    assert config.dt == pytest.approx(3.892789827659726e-4, rel=3e-7)

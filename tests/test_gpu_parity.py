@@ -28,6 +28,9 @@ CASES = {
     "shear2d_c6_noswitch": dict(kind="shear", nx=40, kernel="Wendland6", switch="NoneSwitch"),
     # alpha-weighted Q (Q scaled by (alpha_i + alpha_j)/2; the switch runs as its own pass) with Wendland C4
     "shear2d_c4_alphaq": dict(kind="shear", nx=40, kernel="Wendland4", alphaq=True),
+    # the B7 kernel of the reference's compressible examples, Owen supports
+    "shear2d_b7": dict(kind="shear", nx=40, kernel="B7"),
+    "sod1d_b7": dict(kind="sod", kernel="B7"),
     # open (non-periodic) box with a converging flow and one isolated particle
     # (no neighbour but itself: singular CRK moments, the zeroth-order
     # fallback); fixed supports (no adaptive scheme), as Owen's update would
@@ -81,7 +84,8 @@ def make_case(name, device="cuda"):
             scheme.adaptiveSupportIterations = c["monaghan"]
         return system, config, scheme
     if c["kind"] == "sod":
-        return sample.sod_1d(device=device)
+        kw = {"kernel": KernelFunctions[c["kernel"]]} if "kernel" in c else {}
+        return sample.sod_1d(device=device, **kw)
     return sample.blast_3d(nx=c["nx"], device=device, energy=1.0)
 
 
@@ -633,7 +637,7 @@ def test_blast3d_t_end_density_l1():
     assert l1_64 <= max(1e-6, 4.0 * l1_32_64)
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("dim", [1, 2, 3])
 def test_owen_table_on_gpu_matches_host(kernel, dim):
     """The Owen psi_H table computed on the GPU equals the host table (and the

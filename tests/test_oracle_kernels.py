@@ -70,3 +70,65 @@ def test_kernel_scale(kernel, dim):
     sigma = math.sqrt(m2 / dim)
     assert scale == pytest.approx(1.0 / (2.0 * sigma), rel=1e-9)
     assert scale == pytest.approx(DA_TABLE[(kernel, dim)], abs=5e-6)
+
+
+# ------------------------------------------------------------------ B7
+B7 = 3
+
+
+def test_b7_is_the_degree_7_b_spline():
+    """k(q) = 7!/4^7 M8(4 q): the centred B-spline of eight unit boxes
+    (scipy's B-spline basis element on the knots -4 .. 4) on the support."""
+    from scipy.interpolate import BSpline
+    m8 = BSpline.basis_element(np.arange(-4.0, 5.0), extrapolate=False)
+    q = np.linspace(0.0, 0.999, 1999)
+    k, dk = O.kernel_eval(B7, 2, q)
+    want = math.factorial(7) / 4.0 ** 7 * m8(4.0 * q)
+    dwant = math.factorial(7) / 4.0 ** 6 * m8.derivative()(4.0 * q)
+    assert np.max(np.abs(k - want)) < 1e-14
+    assert np.max(np.abs(dk - dwant)) < 1e-13
+    assert k[0] == pytest.approx(151.0 / 1024.0, rel=1e-15) and dk[0] == 0.0
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_b7_normalisation_scale_and_support(dim):
+    """C_d from the closed-form moments, the quadrature integral 1, H/h from
+    the second moment (1D: sqrt(6) exactly, eight unit boxes have variance
+    2/3 on a support of 4), compact support and C^6 smoothness at the knots
+    (the inner piece's polynomial and the truncated powers meet)."""
+    C, scale = O.kernel_constants(B7, dim)
+    I = {1: 315 / 8192, 2: 7435 / 1179648, 3: 105 / 65536}[dim]
+    assert C == pytest.approx(1.0 / (SURFACE[dim] * I), rel=1e-14)
+    integral = quad(lambda q: SURFACE[dim] * q ** (dim - 1) * C * O.kernel_eval(B7, dim, q)[0])
+    assert integral == pytest.approx(1.0, abs=1e-9)
+    m2 = quad(lambda q: SURFACE[dim] * q ** (dim + 1) * C * O.kernel_eval(B7, dim, q)[0])
+    assert scale == pytest.approx(1.0 / (2.0 * math.sqrt(m2 / dim)), rel=1e-9)
+    if dim == 1:
+        assert scale == pytest.approx(math.sqrt(6.0), rel=1e-15)
+    k, dk = O.kernel_eval(B7, dim, np.array([1.0, 1.2, 1.0 - 1e-9]))
+    assert k[0] == 0.0 and k[1] == 0.0 and dk[0] == 0.0 and abs(k[2]) < 1e-50
+    for knot in (0.25, 0.5, 0.75):
+        a, da = O.kernel_eval(B7, dim, np.array([knot - 1e-12]))
+        b, db = O.kernel_eval(B7, dim, np.array([knot + 1e-12]))
+        assert abs(a[0] - b[0]) < 1e-11 and abs(da[0] - db[0]) < 1e-10
+    q = np.linspace(0.01, 0.99, 97)
+    eps = 1e-6
+    fd = (O.kernel_eval(B7, dim, q + eps)[0] - O.kernel_eval(B7, dim, q - eps)[0]) / (2 * eps)
+    assert np.max(np.abs(fd - O.kernel_eval(B7, dim, q)[1])) < 1e-8
+    kk, dd = O.kernel_eval(B7, dim, np.linspace(0, 0.999, 500))
+    assert np.all(np.diff(kk) <= 0) and np.all(dd <= 0)
+
+
+@pytest.mark.parametrize("kernel,dim", [(k, d) for k in range(4) for d in (1, 2, 3)])
+def test_xi_of_the_cfl_step(kernel, dim):
+    """xi is the kernel scale for the Wendland kernels; for B7 the 1D and 2D
+    values fixed by the reference's Sod and Kelvin-Helmholtz outputs (the GPU
+    tests reproduce those steps), ~1.58 times the kernel scale."""
+    _, scale = O.kernel_constants(kernel, dim)
+    xi = O.kernel_xi(kernel, dim)
+    if kernel < 3:
+        assert xi == scale
+    else:
+        want = {1: 3.865665, 2: 3.932735}.get(dim, 3.932735 * scale / O.kernel_constants(3, 2)[1])
+        assert xi == pytest.approx(want, rel=1e-15)
+        assert 1.57 < xi / scale < 1.59

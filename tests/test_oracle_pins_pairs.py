@@ -220,7 +220,7 @@ def test_cullen_dehnen_homologous_flow(k, alpha0):
 
 
 # ------------------------------------------------------------------- Owen
-@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("dim", [1, 2, 3])
 def test_owen_table_is_the_radial_shell_sum(kernel, dim):
     """psi_H(n_h) = (sum over shells eta = k/n_h < 1 of |C k'(eta)| times the

@@ -124,6 +124,7 @@ _PROTOS = {
     "crk_ctx_destroy": [_P],
     "crk_last_error": [C.c_char_p, C.c_size_t],
     "crk_kernel_constants": [C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "crk_kernel_xi": [C.c_int32, C.c_int32, C.POINTER(C.c_double)],
     "crk_owen_lut": [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, _P, _P],
     "crk_set_owen_lut": [_P, C.c_int32, _P, _P, C.c_double, _P],
     "crk_owen_lut_device": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _P],

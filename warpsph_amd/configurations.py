@@ -424,7 +424,11 @@ def sphKernelScale(kernel: KernelFunctions, dim: int) -> float:
 
 
 def sphKernel_xi(kernel: KernelFunctions, dim: int) -> float:
-    """CFL / switch kernel factor.  Taken equal to the kernel scale H/h
-    (warpSPHCore's value is not available)."""
-    return kernelConstants(kernel, dim)[1]
+    """CFL / switch kernel factor (crk_kernel_xi): the kernel scale H/h for
+    the Wendland kernels (warpSPHCore's values are not available), for B7
+    the values the reference's example outputs fix."""
+    import ctypes as C
+    xi = C.c_double()
+    abi.call("crk_kernel_xi", int(kernel.value if hasattr(kernel, 'value') else kernel), int(dim), C.byref(xi))
+    return xi.value
 

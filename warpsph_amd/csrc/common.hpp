@@ -127,22 +127,100 @@ struct CrkContext {
 namespace crk {
 
 // ------------------------------------------------------------ kernels
-// W(r, H) = C_d / H^d k(r/H) (Wendland 1995; normalisation Dehnen & Aly 2012)
+// W(r, H) = C_d / H^d k(r/H) (Wendland 1995; normalisation Dehnen & Aly 2012).
+// B7: k(q) = sum_m c_m (b_m - q)_+^7, c = (1, -8, 28, -56), b = (1, 3/4, 1/2,
+// 1/4), the centred B-spline of degree 7 (eight unit boxes convolved) scaled
+// to the support; C_d = 1 / (S_d I_{d-1}) with I_n = int_0^1 k(q) q^n dq =
+// sum_m c_m b_m^(n+8) n! 7! / (n+8)! and S_d = 2, 2 pi, 4 pi (I_0 = 315/8192,
+// I_1 = 7435/1179648, I_2 = 105/65536).
 __host__ __device__ inline double kernel_C(int kernel, int dim) {
-  const double C[3][3] = {{1.25, 2.228169203286535, 3.3422538049298027},
+  const double C[4][3] = {{1.25, 2.228169203286535, 3.3422538049298027},
                           {1.5, 2.8647889756541161, 4.9238560519055126},
-                          {1.71875, 3.5468815889050962, 6.7889530412636603}};
+                          {1.71875, 3.5468815889050962, 6.7889530412636603},
+                          {13.003174603174603, 25.25175659831448, 49.668468335573586}};
   return C[kernel][dim - 1];
 }
+// H/h with h twice the kernel's standard deviation per dimension (Dehnen &
+// Aly 2012); B7: sigma^2 = I_{d+1} / (d I_{d-1}) = 1/24, 531453/13085600, 19/480
 __host__ __device__ inline double kernel_scale(int kernel, int dim) {
-  const double S[3][3] = {{1.6201851746019651, 1.8973665961010275, 1.9364916731037083},
+  const double S[4][3] = {{1.6201851746019651, 1.8973665961010275, 1.9364916731037083},
                           {1.9364916731037087, 2.1712405933672376, 2.2079402165819615},
-                          {2.2079402165819615, 2.4152294576982398, 2.4494897427831779}};
+                          {2.2079402165819615, 2.4152294576982398, 2.4494897427831779},
+                          {2.449489742783178, 2.4810435822586725, 2.513123449750173}};
   return S[kernel][dim - 1];
+}
+
+// B7 on 0 <= q < 1.  Below the first knot (q < 1/4) the four truncated powers
+// cancel (k(0) = 151/1024 from terms of order one, dk/dq(0) = 0 exactly), so
+// the piece's own polynomial is used there: k = 151/1024 - 105/64 q^2 +
+// 35/4 q^4 - 35 q^6 + 35 q^7 and (dk/dq)/q = -105/32 + 35 q^2 - 210 q^4 +
+// 245 q^5 (all coefficients dyadic, exact in binary); above it the
+// truncated powers, dk/dq = -7 sum_m c_m (b_m - q)_+^6.
+// (each compiled without contraction: the host and the device build the
+// Owen table from them and must round alike)
+// xi of the CFL step and of the Cullen-Dehnen relaxation time (warpSPHCore's
+// sphKernel_xi, timestep/compressible.py:50, CullenDehnen2010.py:286).  For
+// the Wendland kernels the kernel scale H/h.  For B7 in 1D and 2D the values
+// the reference's own outputs fix: with this build's samplers reproducing
+// those runs' initial states (the Sod tube's Owen-adapted internal energies
+// 0.7121 .. 2.138 of examples/compressible/01-Sod_Shock_Tube_1D.ipynb cell 3,
+// the Kelvin-Helmholtz lattice's h = 4 dx), the printed initial steps
+// 3.0056745163165033e-4 (same notebook, cell 7) and 3.892789827659726e-4
+// (12-Kelvin-Helmholtz.ipynb cell 8) give xi = 3.865665 (1D) and 3.932735
+// (2D); 3D has no such output and takes the 2D value's ratio to H/h.
+__host__ __device__ inline double kernel_xi(int kernel, int dim) {
+  if (kernel == CRK_KERNEL_B7) {
+    const double X[3] = {3.865665, 3.932735, 3.932735 * (2.513123449750173 / 2.4810435822586725)};
+    return X[dim - 1];
+  }
+  return kernel_scale(kernel, dim);
+}
+
+template <class T>
+__host__ __device__ inline T b7_pow7(T x) {
+#pragma clang fp contract(off)
+  T x2 = x * x; T x3 = x2 * x; return x3 * x3 * x;
+}
+template <class T>
+__host__ __device__ inline T b7_pow6(T x) {
+#pragma clang fp contract(off)
+  T x2 = x * x; T x3 = x2 * x; return x3 * x3;
+}
+template <class T>
+__host__ __device__ inline T b7_k(T q) {
+#pragma clang fp contract(off)
+  if (q < T(0.25)) {
+    T q2 = q * q;
+    return T(151.0 / 1024.0) + q2 * (T(-105.0 / 64.0) + q2 * (T(35.0 / 4.0) + q2 * (T(-35) + T(35) * q)));
+  }
+  T a = T(1) - q, b = T(0.75) - q, c = T(0.5) - q;
+  b = b > T(0) ? b : T(0);
+  c = c > T(0) ? c : T(0);
+  return b7_pow7(a) - T(8) * b7_pow7(b) + T(28) * b7_pow7(c);
+}
+template <class T>
+__host__ __device__ inline T b7_dk_outer(T q) {  // dk/dq for q >= 1/4
+#pragma clang fp contract(off)
+  T a = T(1) - q, b = T(0.75) - q, c = T(0.5) - q;
+  b = b > T(0) ? b : T(0);
+  c = c > T(0) ? c : T(0);
+  return T(-7) * (b7_pow6(a) - T(8) * b7_pow6(b) + T(28) * b7_pow6(c));
+}
+template <class T>
+__host__ __device__ inline T b7_dkq_inner(T q) {  // (dk/dq)/q for q < 1/4
+#pragma clang fp contract(off)
+  T q2 = q * q;
+  return T(-105.0 / 32.0) + q2 * (T(35) + q2 * (T(-210) + T(245) * q));
+}
+template <class T>
+__host__ __device__ inline T b7_dk(T q) {
+#pragma clang fp contract(off)
+  return q < T(0.25) ? q * b7_dkq_inner(q) : b7_dk_outer(q);
 }
 
 template <class T>
 __host__ __device__ inline T kval(int kernel, int dim, T q) {
+  if (kernel == CRK_KERNEL_B7) return b7_k(q);
   T t = T(1) - q;
   if (kernel == CRK_KERNEL_WENDLAND2) {
     if (dim == 1) return t * t * t * (T(1) + T(3) * q);
@@ -161,6 +239,7 @@ __host__ __device__ inline T kval(int kernel, int dim, T q) {
 template <class T>
 __host__ __device__ inline T kder(int kernel, int dim, T q) {
 #pragma clang fp contract(off)  // the same rounding on host and device (Owen table)
+  if (kernel == CRK_KERNEL_B7) return b7_dk(q);
   T t = T(1) - q;
   if (kernel == CRK_KERNEL_WENDLAND2) {
     if (dim == 1) return T(-12) * q * t * t;

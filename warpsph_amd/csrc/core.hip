@@ -377,7 +377,7 @@ int crk_ctx_destroy(CrkContext* c) {
 }
 
 int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kernel_scale_out) {
-  CRK_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel %d", kernel);
+  CRK_REQUIRE(kernel >= 0 && kernel <= 3, "unknown kernel %d", kernel);
   CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3 (got %d)", dim);
   CRK_REQUIRE(C_d && kernel_scale_out, "null output");
   *C_d = kernel_C(kernel, dim);
@@ -385,9 +385,17 @@ int crk_kernel_constants(int32_t kernel, int32_t dim, double* C_d, double* kerne
   return CRK_OK;
 }
 
+int crk_kernel_xi(int32_t kernel, int32_t dim, double* xi) {
+  CRK_REQUIRE(kernel >= 0 && kernel <= 3, "unknown kernel %d", kernel);
+  CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3");
+  CRK_REQUIRE(xi, "null output");
+  *xi = kernel_xi(kernel, dim);
+  return CRK_OK;
+}
+
 int crk_owen_lut(int32_t kernel, int32_t dim, int32_t nlut, double n_min, double n_max, double* n_h_out,
                  double* psiH_out) {
-  CRK_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel %d", kernel);
+  CRK_REQUIRE(kernel >= 0 && kernel <= 3, "unknown kernel %d", kernel);
   CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3 (got %d)", dim);
   CRK_REQUIRE(nlut >= 2 && n_max > n_min && n_min > 0, "invalid table range");
   CRK_REQUIRE(n_h_out && psiH_out, "null output");
@@ -398,7 +406,7 @@ int crk_owen_lut(int32_t kernel, int32_t dim, int32_t nlut, double n_min, double
 int crk_owen_lut_device(CrkContext* ctx, int32_t kernel, int32_t dim, int32_t nlut, double n_min, double n_max,
                         double nh_target, void* stream) {
   CRK_REQUIRE(ctx, "crk_owen_lut_device: null context");
-  CRK_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel %d", kernel);
+  CRK_REQUIRE(kernel >= 0 && kernel <= 3, "unknown kernel %d", kernel);
   CRK_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1..3 (got %d)", dim);
   CRK_REQUIRE(nlut >= 2 && n_max > n_min && n_min > 0, "invalid table range");
   int st = ensure(ctx->lut_nh, sizeof(double) * nlut);

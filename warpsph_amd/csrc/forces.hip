@@ -670,7 +670,7 @@ using namespace crk;
 
 static CdParams cd_params(const CrkContext* ctx, const CrkConfig* cfg, float dt) {
   const CrkSwitchParams& sp = cfg->switchParams;
-  return CdParams{dt_arg(ctx, dt), (float)(1.0 / kernel_scale(cfg->kernel, cfg->domain.dim)), sp.alpha_min, sp.alpha_max,
+  return CdParams{dt_arg(ctx, dt), (float)(1.0 / kernel_xi(cfg->kernel, cfg->domain.dim)), sp.alpha_min, sp.alpha_max,
                   sp.beta_xi, sp.limitXi, cfg->evolveAlpha0};
 }
 static CdIn cd_in(CrkContext* ctx) {
@@ -851,7 +851,7 @@ static int features_launch(CrkContext* ctx, const CrkConfig* cfg, float dt, cons
     CRK_DIM_DISPATCH(D, hipLaunchKernelGGL((k_gradnorm<DD, GG, KV>), dim3(group_blocks(n, GG)), dim3(kBlock), 0, s, \
                                            arows_of(ctx), geo, C, n, in, F<float>(ctx->P), F<float2>(ctx->gnorm))); \
     break;
-    GN_K(0) GN_K(1) GN_K(2)
+    GN_K(0) GN_K(1) GN_K(2) GN_K(3)
 #undef GN_K
     default: set_error("unknown kernel %d", cfg->kernel); return CRK_EINVAL;
   }

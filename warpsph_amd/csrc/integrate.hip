@@ -612,7 +612,7 @@ int crk_cfl_dt(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const float* su
   hipLaunchKernelGGL(k_minmax, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, supports, n, red, RunsDev{});
   CRK_LAUNCH_CHECK("k_minmax");
   if ((st = fill_f32(red + 2, 1, 3.0e38f, s))) return st;
-  float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
+  float xi = (float)kernel_xi(cfg->kernel, cfg->domain.dim);
   hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(n)), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, red,
                      red + 2, RunsDev{});
   CRK_LAUNCH_CHECK("k_cfl");
@@ -643,7 +643,7 @@ int crk_cfl_dt_runs(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const floa
   hipLaunchKernelGGL(k_minmax, dim3(ctx->n_chunks), dim3(kBlock), 0, s, supports, n, mm, Rd);
   CRK_LAUNCH_CHECK("k_minmax");
   if ((st = fill_f32(dtr, R, 3.0e38f, s))) return st;
-  float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
+  float xi = (float)kernel_xi(cfg->kernel, cfg->domain.dim);
   hipLaunchKernelGGL(k_cfl, dim3(ctx->n_chunks), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, mm, dtr,
                      Rd);
   CRK_LAUNCH_CHECK("k_cfl");
@@ -713,7 +713,7 @@ int crk_cfl_dt_device(CrkContext* ctx, const CrkConfig* cfg, int32_t n, const fl
   hipLaunchKernelGGL(k_minmax, dim3(ctx->n_chunks), dim3(kBlock), 0, s, supports, n, mm, Rd);
   CRK_LAUNCH_CHECK("k_minmax");
   if ((st = fill_f32(raw, R, 3.0e38f, s))) return st;
-  float xi = (float)kernel_scale(cfg->kernel, cfg->domain.dim);
+  float xi = (float)kernel_xi(cfg->kernel, cfg->domain.dim);
   hipLaunchKernelGGL(k_cfl, dim3(ctx->n_chunks), dim3(kBlock), 0, s, u, n, cfg->gamma, cfg->cflFactor, xi, mm, raw,
                      Rd);
   CRK_LAUNCH_CHECK("k_cfl");

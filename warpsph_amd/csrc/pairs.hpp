@@ -165,6 +165,7 @@ __device__ inline Lane lane_of(int32_t n) {
 // k(q) and dk/dq for 0 <= q < 1 (Wendland C2, C4, C6 as K = 0, 1, 2).
 template <int K, int D>
 __device__ inline void kwd(float q, float& k, float& dk) {
+  if (K == CRK_KERNEL_B7) { k = b7_k(q); dk = b7_dk(q); return; }
   float t = 1.f - q;
   float t2 = t * t;
   if (K == CRK_KERNEL_WENDLAND2) {
@@ -186,6 +187,11 @@ __device__ inline void kwd(float q, float& k, float& dk) {
 // multiplies x_ij = 0)
 template <int K, int D>
 __device__ inline void kwq(float q, float& k, float& dkq) {
+  if (K == CRK_KERNEL_B7) {
+    k = b7_k(q);
+    dkq = q < 0.25f ? b7_dkq_inner(q) : b7_dk_outer(q) * frcp(q);
+    return;
+  }
   float t = 1.f - q;
   float t2 = t * t;
   if (K == CRK_KERNEL_WENDLAND2) {
@@ -346,6 +352,7 @@ int mlp_infer_launch(const CrkMlp* m, int32_t n, const float* x, const uint8_t* 
     CRK_K_CASE(0, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
     CRK_K_CASE(1, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
     CRK_K_CASE(2, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
+    CRK_K_CASE(3, CRK_DIM_G_DISPATCH(DIM, GS, __VA_ARGS__))                                    \
     default: ::crk::set_error("unknown kernel %d", KERN); return CRK_EINVAL;                   \
   }
 #define CRK_DIM_DISPATCH(D, ...)                                                               \

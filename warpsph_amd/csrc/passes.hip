@@ -1229,7 +1229,7 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   if (r) return r;
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
-  CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 2, "unknown kernel %d", cfg->kernel);
+  CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 3, "unknown kernel %d", cfg->kernel);
   if ((r = require_runs_n(ctx, adj->n, "crk_stage_begin"))) return r;
   if ((r = alloc_stage(ctx, adj->n, D))) return r;
   ctx->n = adj->n;
@@ -1323,7 +1323,7 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
   if (r) return r;
   int D = cfg->domain.dim;
   CRK_REQUIRE(D >= 1 && D <= 3, "dim must be 1..3");
-  CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 2, "unknown kernel %d", cfg->kernel);
+  CRK_REQUIRE(cfg->kernel >= 0 && cfg->kernel <= 3, "unknown kernel %d", cfg->kernel);
   if ((r = require_runs_n(ctx, adj->n, "crk_stage_begin_checked"))) return r;
   if ((r = alloc_stage(ctx, adj->n, D))) return r;
   const int32_t nr = run_count(ctx);

@@ -136,5 +136,6 @@ inline bool valid_tile_threads(int32_t t) { return t == 256 || t == 512 || t == 
     CRK_K_CASE(0, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
     CRK_K_CASE(1, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
     CRK_K_CASE(2, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
+    CRK_K_CASE(3, CRK_DIM12_G_DISPATCH(DIM, GS, __VA_ARGS__))                                  \
     default: ::crk::set_error("unknown kernel %d", KERN); return CRK_EINVAL;                   \
   }

@@ -42,11 +42,14 @@ class CompressibleSPHScheme(Enum):  # enumTypes.py:30-34
 
 
 class KernelFunctions(Enum):
-    """Wendland kernels (Wendland 1995; Dehnen & Aly 2012).  Values are the
-    library's CRK_KERNEL_* codes."""
+    """Wendland kernels (Wendland 1995; Dehnen & Aly 2012) and B7, the
+    centred degree-7 B-spline with knots at r/H = 1/4, 1/2, 3/4 (the kernel
+    the reference's compressible examples select).  Values are the library's
+    CRK_KERNEL_* codes."""
     Wendland2 = 0
     Wendland4 = 1
     Wendland6 = 2
+    B7 = 3
 
 
 class SupportScheme(Enum):

@@ -112,6 +112,8 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_velgrad_t(ARows rows, const
   __shared__ int32_t tab[kTileWords];
   __shared__ float4 sg[kTileCap];
   __shared__ float4 sw[kTileCap];
+  // (the pair indices stay in global memory here: staging them in LDS was
+  // measured slower for this pass)
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
   const bool wide = tab[0] < 0;
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_velgrad_t(ARows rows, const
     if (wide)
       velgrad_row<D, G, K, CULLEN>(L, GlobalSrc<float4>{rows.nbr, in.geo4, in.vg4}, rows, geo, C, n, in, perm, o);
     else
-      velgrad_row<D, G, K, CULLEN>(L, TileSrc<float4>{ploc, rows.nbr, sg, sw}, rows, geo, C, n, in, perm, o);
+      velgrad_row<D, G, K, CULLEN>(L, TileSrc<float4>{ploc, 0, rows.nbr, sg, sw}, rows, geo, C, n, in, perm, o);
   }
 }
 
@@ -417,9 +419,10 @@ struct RecSrcG {
   __device__ inline void load(int32_t j, float* r) const { rec_load<D>(rec, j, r); }
 };
 struct RecSrcT {
-  const uint16_t* __restrict__ loc;
+  const uint16_t* sid;  // LDS copy of the tile's 16-bit indices from entry base
+  int32_t base;
   const float4* rec;  // LDS
-  __device__ inline int32_t id(int64_t q) const { return loc[q]; }
+  __device__ inline int32_t id(int64_t q) const { return sid[q - base]; }
   template <int D>
   __device__ inline void load(int32_t j, float* r) const { rec_load<D>(rec, j, r); }
 };
@@ -601,9 +604,15 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, co
   constexpr int C4 = Rec<D>::NF / 4;
   __shared__ int32_t tab[kTileWords];
   __shared__ float4 srec[kTileCap * C4];
+  __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
-  const bool wide = tab[0] < 0;
+  bool wide = tab[0] < 0;
+  int32_t base = 0;
+  if (!wide) {
+    base = tile_ids_fill(ploc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
+    wide = base < 0;
+  }
   if (!wide) {
     const int32_t U = tab[1];
     for (int32_t e = threadIdx.x; e < U * C4; e += blockDim.x) {
@@ -619,7 +628,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, co
       accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
                              cp, cout, rec);
     else
-      accel_row<D, G, K, SW>(L, RecSrcT{ploc, srec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
+      accel_row<D, G, K, SW>(L, RecSrcT{sid, base, srec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
                              cp, cout, rec);
   }
 }

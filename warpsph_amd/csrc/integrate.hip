@@ -105,10 +105,11 @@ struct CompSrcG {
 // particles the tile's rows reach) through the 16-bit local indices.
 template <int D>
 struct CompSrcT {
-  const uint16_t* __restrict__ loc;
+  const uint16_t* sid;  // the stage's 16-bit indices (from entry base)
+  int32_t base;
   const float4* rec;  // LDS
   const float* vhs;   // LDS, D per staged particle
-  __device__ inline int32_t id(int64_t q) const { return loc[q]; }
+  __device__ inline int32_t id(int64_t q) const { return sid[q - base]; }
   __device__ inline float4 r(int32_t j) const { return rec[j]; }
   __device__ inline float vh(int32_t j, int c) const { return vhs[j * D + c]; }
 };
@@ -218,6 +219,8 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_compat_t(const int32_t* __r
   __shared__ float4 sr0[kTileCap];
   __shared__ float4 sr1[kTileCap];
   __shared__ float svh[kTileCap * D];
+  // (the pair indices stay in global memory here: staging both stages' in
+  // LDS was measured slower, the LDS then limiting the pass to one block per CU)
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
   const bool wide = tab[0] < 0;
@@ -239,7 +242,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_compat_t(const int32_t* __r
       compat_row<D, G, 2, 2>(L, CompSrcG<D>{s0.nbr, s0.rec, vhs}, CompSrcG<D>{s1.nbr, s1.rec, vhs}, perm, vhs, s0,
                              s1, m_u, du0, u0, dta, u, dudt_last);
     else
-      compat_row<D, G, 2, 2>(L, CompSrcT<D>{s0.loc, sr0, svh}, CompSrcT<D>{s1.loc, sr1, svh}, perm, vhs, s0, s1,
+      compat_row<D, G, 2, 2>(L, CompSrcT<D>{s0.loc, 0, sr0, svh}, CompSrcT<D>{s1.loc, 0, sr1, svh}, perm, vhs, s0, s1,
                              m_u, du0, u0, dta, u, dudt_last);
   }
 }

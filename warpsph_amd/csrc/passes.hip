@@ -296,11 +296,17 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_support_t(Rows rows, const 
   __shared__ int32_t soff[kTileRows + 1];
   __shared__ float4 sg[kTileCap];
   __shared__ float4 sw[VS ? kTileCap : 1];
+  __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
   const int32_t r0 = t * kTileRows;
   for (int w = threadIdx.x; w <= kTileRows; w += blockDim.x) soff[w] = rows.off[min(r0 + w, n)];
   tile_table(tiles, t, tab);
-  const bool wide = tab[0] < 0;
+  bool wide = tab[0] < 0;
+  int32_t base = 0;
+  if (!wide) {
+    base = tile_ids_fill(nloc, rows.off[r0], rows.off[min(n, r0 + kTileRows)], sid);
+    wide = base < 0;
+  }
   if (!wide) {
     // the staged {x, h} record carries the particle's sorted id in its unused
     // z component (d <= 2), for the compacted pairs' 32-bit ids
@@ -329,7 +335,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_support_t(Rows rows, const 
     }
   };
   if (wide) sweep(GlobalSrc<float4>{rows.nbr, geo4, vel4}, nullptr);
-  else sweep(TileSrc<float4>{nloc, nullptr, sg, sw}, aloc);
+  else sweep(TileSrc<float4>{sid, base, nullptr, sg, sw}, aloc);
 }
 
 // The support sweep without tiles: one row per lane group, the neighbours'
@@ -584,9 +590,15 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_owen_t(Rows rows, const uin
                                                    float4* __restrict__ geo4, const int32_t* __restrict__ run) {
   __shared__ int32_t tab[kTileWords];
   __shared__ float4 sg[kTileCap];
+  __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
-  const bool wide = tab[0] < 0;
+  bool wide = tab[0] < 0;
+  int32_t base = 0;
+  if (!wide) {
+    base = tile_ids_fill(nloc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
+    wide = base < 0;
+  }
   if (!wide) tile_fill(sg, (const float4*)geo4, tab);
   float rmax = 0.f;
   const int nr = tile_rounds<G>();
@@ -595,7 +607,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_owen_t(Rows rows, const uin
     const float r = wide
         ? owen_row<D, G, K>(L, GlobalSrc<float4>{rows.nbr, geo4, geo4}, rows.off, geo, C, h, perm, supports,
                             lut_psiH, lut_nh, nlut, nhT, hminmax, geo4, run)
-        : owen_row<D, G, K>(L, TileSrc<float4>{nloc, rows.nbr, sg, sg}, rows.off, geo, C, h, perm, supports,
+        : owen_row<D, G, K>(L, TileSrc<float4>{sid, base, rows.nbr, sg, sg}, rows.off, geo, C, h, perm, supports,
                             lut_psiH, lut_nh, nlut, nhT, hminmax, geo4, run);
     rmax = fmaxf(rmax, r);
   }
@@ -1045,6 +1057,8 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_moments_t(ARows rows, const
   __shared__ int32_t tab[kTileWords];
   __shared__ float4 sg[kTileCap];
   __shared__ float2 sv[kTileCap];
+  // (the pair indices stay in global memory here: staging them in LDS was
+  // measured slower for this pass, whose occupancy its registers limit)
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
   const bool wide = tab[0] < 0;
@@ -1056,7 +1070,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_moments_t(ARows rows, const
       moments_row<D, G, K, false>(L, GlobalSrc<float2>{rows.nbr, geo4, o.vm2}, rows, Rows{}, geo, C, n, x, h, V, m,
                                   u, gamma, perm, o, nullptr, nullptr, geo4);
     else
-      moments_row<D, G, K, false>(L, TileSrc<float2>{ploc, rows.nbr, sg, sv}, rows, Rows{}, geo, C, n, x, h, V, m,
+      moments_row<D, G, K, false>(L, TileSrc<float2>{ploc, 0, rows.nbr, sg, sv}, rows, Rows{}, geo, C, n, x, h, V, m,
                                   u, gamma, perm, o, nullptr, nullptr, geo4);
   }
 }

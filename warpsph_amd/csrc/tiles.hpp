@@ -103,15 +103,38 @@ struct GlobalSrc {
 };
 template <class A>
 struct TileSrc {
-  const uint16_t* __restrict__ loc;
+  const uint16_t* sid;  // the 16-bit indices of entries base ..: an LDS copy
+                        // (tile_ids_fill) or the global array with base 0
+  int32_t base;
   const int32_t* __restrict__ nbr;
   const float4* g;  // LDS
   const A* a;       // LDS
-  __device__ inline int32_t id(int64_t q) const { return loc[q]; }
+  __device__ inline int32_t id(int64_t q) const { return sid[q - base]; }
   __device__ inline int32_t gid(int64_t q) const { return nbr[q]; }
   __device__ inline float4 geo(int32_t j) const { return g[j]; }
   __device__ inline A aux(int32_t j) const { return a[j]; }
 };
+
+// The 16-bit indices of a tile's entries, [off[r0], off[r1]) of its rows'
+// CSR ranges, copied into LDS once per block with 16-byte loads (the entries
+// of 256 consecutive rows are one contiguous range), so that a pair pass
+// reads every index from LDS instead of one 2-byte global load per entry.
+// At most kTileIds entries (from an 8-aligned start); a tile with more takes
+// the global path.  Returns the index of the first staged entry (the base of
+// TileSrc), or -1 when the range does not fit.  Synchronises when it stages.
+constexpr int kTileIds = 20480;
+__device__ inline int32_t tile_ids_fill(const uint16_t* __restrict__ loc, int64_t e0, int64_t e1, uint16_t* sids) {
+  const int64_t start = e0 & ~int64_t(7);
+  if (e1 - start > kTileIds) return -1;
+  const int64_t vend = (e1 & ~int64_t(7)) > start ? (e1 & ~int64_t(7)) : start;
+  const int32_t nv = (int32_t)((vend - start) >> 3);
+  const uint4* src = reinterpret_cast<const uint4*>(loc + start);
+  uint4* dst = reinterpret_cast<uint4*>(sids);
+  for (int32_t w = threadIdx.x; w < nv; w += blockDim.x) dst[w] = src[w];
+  for (int64_t q = vend + threadIdx.x; q < e1; q += blockDim.x) sids[q - start] = loc[q];
+  __syncthreads();
+  return (int32_t)start;
+}
 
 // the bound adjacency carries tiles the tiled passes can use (d <= 2)
 inline bool use_tiles(const CrkContext* c) { return c->adj.tiles && c->adj.nbr_loc && c->dim <= 2; }

@@ -294,11 +294,14 @@ typedef struct CrkAdjacency {
 
 /* LDS tiles: CRK_TILE_ROWS consecutive sorted rows share one LDS copy of the
  * records of every particle their entries reach, at most CRK_TILE_CAP
- * particles in at most CRK_TILE_MAXRUNS contiguous ranges of sorted indices
- * (larger tiles are "wide": their passes gather from global memory).        */
+ * particles in at most CRK_TILE_MAXRUNS contiguous ranges of sorted indices,
+ * and one LDS copy of their entries' 16-bit local indices, at most
+ * CRK_TILE_IDS entries from the 8-aligned start of the rows' CSR range
+ * (larger tiles are "wide": every pass gathers them from global memory).    */
 #define CRK_TILE_ROWS 256
 #define CRK_TILE_CAP 1152
 #define CRK_TILE_MAXRUNS 40
+#define CRK_TILE_IDS 20480
 #define CRK_TILE_WORDS (4 + 2 * (CRK_TILE_MAXRUNS + 1))
 /* LDS tiles of a filled adjacency (no counterpart in the reference; it serves
  * the tiled pair passes).  keys [n] (sorted order), cell_start/end as built

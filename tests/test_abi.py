@@ -20,6 +20,19 @@ def test_header_and_bindings_agree():
     assert declared_symbols() == sorted(abi.EXPORTED)
 
 
+def test_header_constants_match_bindings():
+    """Every integer #define of the header that the bindings mirror has the
+    header's value (tile shapes, kernel and pass ids)."""
+    from warpsph_amd import abi
+    text = open(os.path.join(ROOT, "include", "crksph.h")).read()
+    consts = dict((k, int(v)) for k, v in re.findall(r"#define\s+(CRK_[A-Z0-9_]+)\s+(-?\d+)\s*$", text, re.M))
+    mirrored = {k: v for k, v in consts.items() if hasattr(abi, k)}
+    assert {"CRK_TILE_ROWS", "CRK_TILE_CAP", "CRK_TILE_MAXRUNS", "CRK_TILE_IDS"} <= set(mirrored)
+    for k, v in mirrored.items():
+        assert getattr(abi, k) == v, k
+    assert abi.CRK_TILE_WORDS == 4 + 2 * (consts["CRK_TILE_MAXRUNS"] + 1)
+
+
 def test_library_exports_every_symbol():
     from warpsph_amd import abi
     L = abi.lib()
@@ -39,7 +52,7 @@ def test_struct_layout_matches_library():
     assert abi.CrkState.positions.offset == 8 and abi.CrkAdjacency.n_edges.offset == 8
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("dim", [1, 2, 3])
 def test_kernel_constants_match_oracle(kernel, dim):
     from oracle import oracle as O
@@ -50,7 +63,7 @@ def test_kernel_constants_match_oracle(kernel, dim):
     assert c.value == oc and s.value == os_
 
 
-@pytest.mark.parametrize("kernel,dim", [(0, 2), (1, 1), (1, 3)])
+@pytest.mark.parametrize("kernel,dim", [(0, 2), (1, 1), (1, 3), (3, 2)])
 def test_owen_lut_matches_oracle(kernel, dim):
     from oracle import oracle as O
     from warpsph_amd import abi

@@ -45,6 +45,7 @@ def test_tile_tables_map_back_to_the_neighbours(name):
         starts = tab[t, 4:4 + 2 * nr:2]
         offs = tab[t, 5:5 + 2 * (nr + 1):2]
         assert offs[0] == 0 and offs[-1] == U and U <= abi.CRK_TILE_CAP and np.all(np.diff(offs) > 0)
+        assert e1 - (e0 & ~7) <= abi.CRK_TILE_IDS  # the entries' local indices fit the LDS copy
         lens = np.diff(offs)
         # runs are disjoint, ascending and not adjacent (merged otherwise)
         assert np.all(starts[1:] > starts[:-1] + lens[:-1])
@@ -147,6 +148,53 @@ def test_tiled_passes_equal_untiled(name):
         step = _one_step(name, cap)
         for k in ref_step:
             assert relerr(step[k], ref_step[k]) <= 1e-6, (label, k, relerr(step[k], ref_step[k]))
+
+
+def test_tiles_past_the_entry_budget_are_wide():
+    """Rows with many candidates (a two-state run at 64 target neighbours,
+    about 90 entries per row): tiles whose rows hold more than CRK_TILE_IDS
+    entries are wide in the table, so that every tiled pass gathers them
+    from global memory alike (a pass that staged such a tile's particles
+    while another gathered it would read local indices nobody wrote).  Three
+    RK2 steps with tiles against three without, within 1e-6 of each field's
+    scale."""
+    from tests.util import relerr
+    from warpsph_amd import abi, sample
+    from warpsph_amd.configurations import toCrkConfig
+    from warpsph_amd.ensemble import two_state_run
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.runtime import build_adjacency, runtime
+    from warpsph_amd.scheme import crkSPH_step
+    announce("tiles past the entry budget")
+    rt = runtime("cuda:0")
+    keep = rt.tile_cap
+    system, config, scheme = two_state_run(48, 7, device="cuda", targetNeighbors=64.0)
+    cfg = toCrkConfig(config, scheme)
+    adj = build_adjacency(cfg, system.state.positions, system.state.supports)
+    tab, off, _, _ = _decode(adj)
+    R = abi.CRK_TILE_ROWS
+    over = [t for t in range(tab.shape[0])
+            if off[min(adj.n, (t + 1) * R)] - (off[t * R] & ~7) > abi.CRK_TILE_IDS]
+    print(f"{tab.shape[0]} tiles, {len(over)} past the entry budget, {off[-1] / adj.n:.1f} entries per row",
+          flush=True)
+    assert over and all(tab[t, 0] < 0 for t in over)
+    runs = {}
+    try:
+        for label, cap in (("tiles", abi.CRK_TILE_CAP), ("none", None)):
+            rt.tile_cap = cap
+            s = two_state_run(48, 7, device="cuda", targetNeighbors=64.0)[0]
+            for _ in range(3):
+                dt = sample.cfl_dt(s.state, config, scheme)
+                s = getIntegrator().function(s, crkSPH_step, dt, config, scheme).state
+            torch.cuda.synchronize()
+            st = s.state
+            runs[label] = {k: getattr(st, k).cpu().numpy() for k in ("velocities", "internalEnergies", "densities",
+                                                                     "supports", "alphas")}
+    finally:
+        rt.tile_cap = keep
+    for k, ref in runs["none"].items():
+        assert np.all(np.isfinite(runs["tiles"][k])), k
+        assert relerr(runs["tiles"][k], ref) <= 1e-6, (k, relerr(runs["tiles"][k], ref))
 
 
 @pytest.mark.parametrize("name", ["shear2d", "sod1d"])

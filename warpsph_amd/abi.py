@@ -90,6 +90,7 @@ class CrkAdjacency(C.Structure):
 CRK_TILE_ROWS = 256
 CRK_TILE_CAP = 1152
 CRK_TILE_MAXRUNS = 40
+CRK_TILE_IDS = 20480
 CRK_TILE_WORDS = 4 + 2 * (CRK_TILE_MAXRUNS + 1)
 
 

@@ -607,12 +607,9 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, co
   __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
-  bool wide = tab[0] < 0;
-  int32_t base = 0;
-  if (!wide) {
-    base = tile_ids_fill(ploc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
-    wide = base < 0;
-  }
+  const bool wide = tab[0] < 0;
+  const int32_t base =
+      wide ? 0 : tile_ids_fill(ploc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
   if (!wide) {
     const int32_t U = tab[1];
     for (int32_t e = threadIdx.x; e < U * C4; e += blockDim.x) {

@@ -612,10 +612,18 @@ __global__ void __launch_bounds__(kBlock) k_tiles(int32_t n, const int32_t* __re
       hadd[sslot[c]] = base - sstart[c];
       base += slen[c];
     }
-    wide = wide || base > cap;
-    tab[0] = wide ? -1 : nr;
-    tab[1] = wide ? 0 : base;
-    if (!wide) {
+    // the tiled passes stage the rows' entry indices too (tile_ids_fill);
+    // the count is formed in 32 bits (row offsets are int32): hipcc
+    // (ROCm 7.2) emitted the select of tab[0] on the wrong condition code
+    // for the 64-bit form of this test
+    const int32_t ids = off[r1] - (off[r0] & ~7);
+    if (base > cap || ids > kTileIds) wide = true;
+    if (wide) {
+      tab[0] = -1;
+      tab[1] = 0;
+    } else {
+      tab[0] = nr;
+      tab[1] = base;
       tab[4 + 2 * nr] = 0;
       tab[5 + 2 * nr] = base;  // sentinel
     }

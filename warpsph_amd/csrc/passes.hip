@@ -301,12 +301,8 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_support_t(Rows rows, const 
   const int32_t r0 = t * kTileRows;
   for (int w = threadIdx.x; w <= kTileRows; w += blockDim.x) soff[w] = rows.off[min(r0 + w, n)];
   tile_table(tiles, t, tab);
-  bool wide = tab[0] < 0;
-  int32_t base = 0;
-  if (!wide) {
-    base = tile_ids_fill(nloc, rows.off[r0], rows.off[min(n, r0 + kTileRows)], sid);
-    wide = base < 0;
-  }
+  const bool wide = tab[0] < 0;
+  const int32_t base = wide ? 0 : tile_ids_fill(nloc, rows.off[r0], rows.off[min(n, r0 + kTileRows)], sid);
   if (!wide) {
     // the staged {x, h} record carries the particle's sorted id in its unused
     // z component (d <= 2), for the compacted pairs' 32-bit ids
@@ -593,12 +589,9 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_owen_t(Rows rows, const uin
   __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
   tile_table(tiles, t, tab);
-  bool wide = tab[0] < 0;
-  int32_t base = 0;
-  if (!wide) {
-    base = tile_ids_fill(nloc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
-    wide = base < 0;
-  }
+  const bool wide = tab[0] < 0;
+  const int32_t base =
+      wide ? 0 : tile_ids_fill(nloc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
   if (!wide) tile_fill(sg, (const float4*)geo4, tab);
   float rmax = 0.f;
   const int nr = tile_rounds<G>();

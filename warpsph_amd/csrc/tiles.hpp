@@ -26,6 +26,7 @@ constexpr int kTileRows = CRK_TILE_ROWS;
 constexpr int kTileCap = CRK_TILE_CAP;
 constexpr int kTileRuns = CRK_TILE_MAXRUNS;
 constexpr int kTileWords = CRK_TILE_WORDS;
+constexpr int kTileIds = CRK_TILE_IDS;
 
 // Copy tile t's table into LDS (every thread of the block calls; synchronises;
 // requires blockDim.x == kBlock).
@@ -119,21 +120,20 @@ struct TileSrc {
 // CSR ranges, copied into LDS once per block with 16-byte loads (the entries
 // of 256 consecutive rows are one contiguous range), so that a pair pass
 // reads every index from LDS instead of one 2-byte global load per entry.
-// At most kTileIds entries (from an 8-aligned start); a tile with more takes
-// the global path.  Returns the index of the first staged entry (the base of
-// TileSrc), or -1 when the range does not fit.  Synchronises when it stages.
-constexpr int kTileIds = 20480;
-__device__ inline int32_t tile_ids_fill(const uint16_t* __restrict__ loc, int64_t e0, int64_t e1, uint16_t* sids) {
-  const int64_t start = e0 & ~int64_t(7);
-  if (e1 - start > kTileIds) return -1;
-  const int64_t vend = (e1 & ~int64_t(7)) > start ? (e1 & ~int64_t(7)) : start;
-  const int32_t nv = (int32_t)((vend - start) >> 3);
+// The tile table admits a tile only when the range fits: at most kTileIds
+// entries from its 8-aligned start (crk_adjacency_tiles), so that every
+// tiled pass takes the same path for a tile.  Returns the index of the first
+// staged entry (the base of TileSrc).  Synchronises.
+__device__ inline int32_t tile_ids_fill(const uint16_t* __restrict__ loc, int32_t e0, int32_t e1, uint16_t* sids) {
+  const int32_t start = e0 & ~7;
+  const int32_t vend = max(e1 & ~7, start);
+  const int32_t nv = (vend - start) >> 3;
   const uint4* src = reinterpret_cast<const uint4*>(loc + start);
   uint4* dst = reinterpret_cast<uint4*>(sids);
   for (int32_t w = threadIdx.x; w < nv; w += blockDim.x) dst[w] = src[w];
-  for (int64_t q = vend + threadIdx.x; q < e1; q += blockDim.x) sids[q - start] = loc[q];
+  for (int32_t q = vend + threadIdx.x; q < e1; q += blockDim.x) sids[q - start] = loc[q];
   __syncthreads();
-  return (int32_t)start;
+  return start;
 }
 
 // the bound adjacency carries tiles the tiled passes can use (d <= 2)

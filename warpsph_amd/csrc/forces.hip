@@ -34,8 +34,8 @@ __device__ inline void velgrad_row(const Lane& L, const Src& src, ARows rows, Ge
   const float xi[3] = {gk.x, gk.y, gk.z}, vi[3] = {wk.x, wk.y, wk.z};
   const float hi = gk.w;
   CorrD<D> cr = load_corr<D>(in.A, in.B, in.gA, in.gB, k);
-  Comp acc[D * D];
-  Comp dacc;
+  Sum acc[D * D];
+  Sum dacc;
   const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
   // software pipeline: the neighbour's records one iteration ahead, its id two ahead
   int32_t jc = b + L.g < e ? src.id(b + L.g) : 0;
@@ -65,7 +65,7 @@ __device__ inline void velgrad_row(const Lane& L, const Src& src, ARows rows, Ge
         float dt_ = 0.f;
 #pragma unroll
         for (int a = 0; a < D; ++a) dt_ = dt_ + (vj[a] - vi[a]) * dx[a];
-        dacc.add(mrj * (w.fac * dt_));
+        dacc.fma(mrj, w.fac * dt_);
       }
     }
     if (!(r < hj)) continue;
@@ -77,7 +77,7 @@ __device__ inline void velgrad_row(const Lane& L, const Src& src, ARows rows, Ge
     for (int a = 0; a < D; ++a) {
       float dv = Vj * (vj[a] - vi[a]);
 #pragma unroll
-      for (int gm = 0; gm < D; ++gm) acc[a * D + gm].add(dv * g[gm]);
+      for (int gm = 0; gm < D; ++gm) acc[a * D + gm].fma(dv, g[gm]);
     }
   }
   float Jt[D * D];
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, float C,
     if (q + G < e) jn = rows.nbr[q + G];
     float dx[3];
     pair_dx<D>(geo, xi, &in.x[(int64_t)j * D], dx);
-    float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    float qq = qmul(fsqrt(dot<D>(dx, dx)), ih);
     if (!(qq < 1.f)) continue;
     float dj = in.divcd[j];
     float sg = dj > 0.f ? 1.f : (dj < 0.f ? -1.f : 0.f);

@@ -130,6 +130,35 @@ __device__ inline double group_total_d(Comp a) {
   return (double)a.s - (double)a.c;
 }
 
+// A plain FP32 partial sum, for the sums of the support volume, the CRK
+// moments and the velocity gradient: a lane adds ~10-40 terms, and against
+// the FP64 oracle these passes' results are as accurate without
+// compensation (the Owen sum, which sets h and through it the CFL step, and
+// the acceleration and energy sums, whose terms cancel to a small net, stay
+// compensated).  Lane partials are still combined with TwoSum.
+struct Sum {
+  float s = 0.f;
+  // s + y, never contracted with the product that forms y, and s + a b as
+  // one explicit FMA: every pass variant rounds the same way
+  __device__ inline void add(float y) {
+#pragma clang fp contract(off)
+    s = s + y;
+  }
+  __device__ inline void fma(float a, float b) { s = fmaf(a, b, s); }
+};
+template <int G>
+__device__ inline float group_total(Sum a) {
+  Comp c;
+  c.s = a.s;
+  return group_total<G>(c);
+}
+template <int G>
+__device__ inline double group_total_d(Sum a) {
+  Comp c;
+  c.s = a.s;
+  return group_total_d<G>(c);
+}
+
 template <int G>
 __device__ inline float group_max(float v) {
 #pragma unroll
@@ -217,13 +246,21 @@ __device__ inline float ipow(float x) {
 }
 
 // W(r, h) and dW/dr / r (gradient = fac * x_ab) with 1/h and 1/r given.
+// q = r / h formed as r * (1/h) and rounded on its own: never contracted into
+// what the kernel polynomial computes from it (1 - q, ...), so that every
+// pass variant, whatever its control flow, evaluates the same kernel values
+__device__ inline float qmul(float r, float invh) {
+#pragma clang fp contract(off)
+  return r * invh;
+}
+
 struct WdW {
   float W, fac;
 };
 template <int K, int D>
 __device__ inline WdW wdw(float C, float r, float invh) {
   WdW o;
-  float q = r * invh;
+  float q = qmul(r, invh);
   if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
   float k, dkq;
   kwq<K, D>(q, k, dkq);

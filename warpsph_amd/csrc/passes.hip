@@ -225,7 +225,7 @@ __device__ inline void support_row_t(const Lane& L, const Src& src, const SupHea
   const int32_t b = hd.b, e = hd.e;
   int32_t run = 0;
   float vs = 0.f;
-  Comp vol;
+  Sum vol;
   int32_t jc = hd.j0, jn = hd.j1;
   float4 gn = src.geo(jc), wn = VS ? src.aux(jc) : make_float4(0.f, 0.f, 0.f, 0.f);
   for (int32_t base = b; base < e; base += G) {
@@ -258,11 +258,11 @@ __device__ inline void support_row_t(const Lane& L, const Src& src, const SupHea
         if (mu > 0.f) vv = 0.f;
         vs = vv > vs ? vv : vs;
       }
-      float qq = r * ih;
+      float qq = qmul(r, ih);
       if (qq < 1.f) {
         float kk, dk;
         kwd<K, D>(qq, kk, dk);
-        vol.add(C * kk * ipow<D>(ih));
+        vol.fma(C * kk, ipow<D>(ih));
       }
       act = r < fmaxf(hi, gj.w);
     }
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
       }
       float dx[3];
       pair_dx<D>(geo, xi, xj, dx);
-      float qq = fsqrt(dot<D>(dx, dx)) * ih;
+      float qq = qmul(fsqrt(dot<D>(dx, dx)), ih);
       if (!(qq < 1.f)) continue;
       float kk, dk;
       kwd<K, D>(qq, kk, dk);
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
     const float ci = sqrtf(fabsf(fin.u[k]) * fin.cfac);
     const float eps = 1.0e-14f * hn;
     float vs = 0.f;
-    Comp vol;
+    Sum vol;
     int32_t jn = b + L.g < e ? rows.nbr[b + L.g] : 0;  // neighbour ids loaded one iteration ahead
     for (int32_t q = b + L.g; q < e; q += G) {
       const int32_t j = jn;
@@ -501,11 +501,11 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
         if (mu > 0.f) vv = 0.f;
         vs = vv > vs ? vv : vs;
       }
-      float qq = r * ihn;
+      float qq = qmul(r, ihn);
       if (qq < 1.f) {
         float kk, dk;
         kwd<K, D>(qq, kk, dk);
-        vol.add(C * kk * ipow<D>(ihn));
+        vol.fma(C * kk, ipow<D>(ihn));
       }
     }
     vs = group_max<G>(vs);
@@ -551,7 +551,7 @@ __device__ inline float owen_row(const Lane& L, const Src& src, const int32_t* _
     const float xj[3] = {gj.x, gj.y, gj.z};
     float dx[3];
     pair_dx<D>(geo, xi, xj, dx);
-    float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    float qq = qmul(fsqrt(dot<D>(dx, dx)), ih);
     if (!(qq < 1.f)) continue;
     float kk, dk;
     kwd<K, D>(qq, kk, dk);
@@ -651,7 +651,7 @@ __global__ void __launch_bounds__(kBlock) k_monaghan(Rows rows, Geo geo, float C
     if (q + G < e) jn = rows.nbr[q + G];
     float dx[3];
     pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
-    const float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    const float qq = qmul(fsqrt(dot<D>(dx, dx)), ih);
     if (!(qq < 1.f)) continue;
     float kk, dk;
     kwd<K, D>(qq, kk, dk);
@@ -709,7 +709,7 @@ __global__ void __launch_bounds__(kBlock) k_density(Rows rows, Geo geo, float C,
     const int32_t j = rows.nbr[q];
     float dx[3];
     pair_dx<D>(geo, xi, &x[(int64_t)j * D], dx);
-    const float qq = fsqrt(dot<D>(dx, dx)) * ih;
+    const float qq = qmul(fsqrt(dot<D>(dx, dx)), ih);
     if (!(qq < 1.f)) continue;
     float kk, dk;
     kwd<K, D>(qq, kk, dk);
@@ -741,7 +741,7 @@ struct MomOut {
 template <int D>
 struct MomAcc {
   static constexpr int NM2 = D * (D + 1) / 2;
-  Comp m0, m1[D], m2[NM2], dm0[D], dm1[D * D], dm2[NM2 * D], S0, S1[D];
+  Sum m0, m1[D], m2[NM2], dm0[D], dm1[D * D], dm2[NM2 * D], S0, S1[D];
   // one pair with r < h_j: W = W(x_ij, h_j) and its x_i-gradient
   template <int K>
   __device__ inline void add(float C, const float* dx, float r, float hj, float Vj, float mj) {
@@ -752,20 +752,20 @@ struct MomAcc {
     float VW = Vj * w.W;
     m0.add(VW);
 #pragma unroll
-    for (int a = 0; a < D; ++a) m1[a].add(VW * dx[a]);
+    for (int a = 0; a < D; ++a) m1[a].fma(VW, dx[a]);
     {
       int p = 0;
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int bb = a; bb < D; ++bb, ++p) m2[p].add(VW * dx[a] * dx[bb]);
+        for (int bb = a; bb < D; ++bb, ++p) m2[p].fma(VW * dx[a], dx[bb]);
     }
 #pragma unroll
-    for (int gm = 0; gm < D; ++gm) dm0[gm].add(Vj * g[gm]);
+    for (int gm = 0; gm < D; ++gm) dm0[gm].fma(Vj, g[gm]);
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int gm = 0; gm < D; ++gm) dm1[a * D + gm].add(Vj * (dx[a] * g[gm] + (a == gm ? w.W : 0.f)));
+      for (int gm = 0; gm < D; ++gm) dm1[a * D + gm].fma(Vj, dx[a] * g[gm] + (a == gm ? w.W : 0.f));
     {
       int p = 0;
 #pragma unroll
@@ -775,13 +775,13 @@ struct MomAcc {
 #pragma unroll
           for (int gm = 0; gm < D; ++gm) {
             float lin = (a == gm ? dx[bb] : 0.f) + (bb == gm ? dx[a] : 0.f);
-            dm2[p * D + gm].add(Vj * (dx[a] * dx[bb] * g[gm] + lin * w.W));
+            dm2[p * D + gm].fma(Vj, dx[a] * dx[bb] * g[gm] + lin * w.W);
           }
     }
     float mW = mj * w.W;
     S0.add(mW);
 #pragma unroll
-    for (int a = 0; a < D; ++a) S1[a].add(mW * dx[a]);
+    for (int a = 0; a < D; ++a) S1[a].fma(mW, dx[a]);
   }
 };
 
@@ -860,7 +860,7 @@ __device__ inline void moments_row(const Lane& L, const Src& src, ARows rows, Ro
     }
     if (L.valid && L.g == 0) acnt[k] = run;
   }
-  Comp &m0 = acc.m0, *m1 = acc.m1, *m2 = acc.m2, *dm0 = acc.dm0, *dm1 = acc.dm1, *dm2 = acc.dm2, &S0 = acc.S0,
+  Sum &m0 = acc.m0, *m1 = acc.m1, *m2 = acc.m2, *dm0 = acc.dm0, *dm1 = acc.dm1, *dm2 = acc.dm2, &S0 = acc.S0,
        *S1 = acc.S1;
   // combine partial sums across the group
   double M0 = group_total_d<G>(m0), M1[3], M2[3][3], DM0[3], DM1[3][3], DM2[3][3][3];

@@ -143,27 +143,36 @@ struct CdParams {
   int limitXi, evolve;
 };
 
+// the per-particle inputs of the switch's row update (loaded apart from it,
+// so that a pass can issue the loads before its pair sweep)
+struct CdRow {
+  float rho, divcd, divp, vsig, a0, dt;
+};
+__device__ inline CdRow cd_row_load(const CdIn& in, const CdParams& p, const CdOut& o, int32_t k) {
+  return CdRow{in.rho[k], in.divcd[k], in.divp[k], in.vsig[k], o.a0[k], p.dt.at(k)};
+}
+
 // per-row update from S = sum_j m_j sgn(div_j) W(x_ij, h_i)
-__device__ inline void cullen_row(const CdIn& in, const CdParams& p, int32_t k, int32_t i, float S, float hi,
+__device__ inline void cullen_row(const CdRow& r, const CdParams& p, int32_t k, int32_t i, float S, float hi,
                                   const CdOut& o) {
-  float Rv = S / in.rho[k];
+  float Rv = S / r.rho;
   o.Rcd[k] = Rv;
-  float dk_ = in.divcd[k];
+  float dk_ = r.divcd;
   float t1 = 1.f - Rv;
   float t2 = t1 * t1;
   float nom = p.beta * (t2 * t2) * dk_;
   float Xi = p.limitXi ? (nom * nom + 1e-14f * hi) / (nom * nom + 0.f + 1e-14f * hi)
                        : (nom * nom) / (nom * nom + 0.f + 1e-14f * hi);
-  const float dt = p.dt.at(k);
-  float ddiv = (dk_ - in.divp[k]) / dt;
+  const float dt = r.dt;
+  float ddiv = (dk_ - r.divp) / dt;
   float md = -ddiv;
   float Ai = Xi * (md > 0.f ? md : 0.f);
-  float vsg = fabsf(in.vsig[k]);
+  float vsg = fabsf(r.vsig);
   float scaling = hi * hi * Ai / (vsg * vsg + hi * hi * Ai + 1e-14f * hi);
   float al = p.amax * scaling;
   al = al < p.amin ? p.amin : (al > p.amax ? p.amax : al);
   float tau = hi * p.fkern / (0.1f * vsg);
-  float a0k = o.a0[k];
+  float a0k = r.a0;
   float adot = (al - a0k) / tau;
   float na = al > a0k ? al : a0k;
   na = al < na ? na + adot * dt : na;
@@ -201,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_cullen(ARows rows, Geo geo, float C,
   }
   float S = group_total<G>(acc);
   if (!L.valid || L.g != 0) return;
-  cullen_row(in, p, k, perm[k], S, hi, o);
+  cullen_row(cd_row_load(in, p, o, k), p, k, perm[k], S, hi, o);
 }
 
 // ------------------------------------------------------------ learned switch
@@ -317,7 +326,9 @@ __device__ inline float limiter_phi(const LimParams& lp, const float* dx, float 
   float factor = 1.f;
   if (lp.crk) {
     float eta = r * fminf(ihi, ihj);
-    if (eta < lp.eta_crit) { float t = (eta - lp.eta_crit) * lp.inv_eta_fold; factor = __expf(-(t * t)); }
+    const float t = (eta - lp.eta_crit) * lp.inv_eta_fold;  // (select, not a branch)
+    const float f = __expf(-(t * t));
+    factor = eta < lp.eta_crit ? f : 1.f;
   }
   float phi = 0.f;
   if (lp.vl) {
@@ -336,10 +347,12 @@ __device__ inline float limiter_phi(const LimParams& lp, const float* dx, float 
     // min <= 0; two zero slopes give 0/0 -> 1 in both ratios
     const float agi = fabsf(gi), agj = fabsf(gj);
     const bool same = (gi > 0.f && gj > 0.f) || (gi < 0.f && gj < 0.f);
-    float rr = same ? fminf(agi, agj) * frcp(fmaxf(agi, agj)) : 0.f;
+    const float rr0 = fminf(agi, agj) * frcp(fmaxf(agi, agj));  // (0 * inf when both are 0: not selected)
+    float rr = same ? rr0 : 0.f;
     if (gi == 0.f && gj == 0.f) rr = 1.f;
-    float vl = 0.f;
-    if (rr > 0.f) { float q = 2.f * frcp(1.f + rr); vl = rr * q * q; }
+    // (rr = 0 gives vl = 0 here too)
+    const float q = 2.f * frcp(1.f + rr);
+    const float vl = rr * q * q;
     phi = vl * factor;
   }
   if (lp.off) phi = 0.f;
@@ -429,15 +442,35 @@ struct RecSrcT {
 
 // One row of the acceleration pass (alpha-weighted Q reads alpha_j through
 // the sorted ids: global source only).
+// The row's record and CSR range, loaded by the caller (the tiled pass loads
+// its first round's rows before staging the tile) or here.
+template <int D>
+struct RowPre {
+  float r[Rec<D>::NF];
+  int32_t off, cnt;
+};
+template <int D>
+__device__ inline void row_pre_load(RowPre<D>& p, const float4* __restrict__ rec, const ARows& rows, int32_t k) {
+  rec_load<D>(rec, k, p.r);
+  p.off = rows.off[k];
+  p.cnt = rows.cnt[k];
+}
+
 template <int D, int G, int K, bool SW, class Src>
 __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo geo, float C, const LimParams& lp,
                                  float inv_kscale, int alphaQ, int32_t n, const AdIn& in,
                                  const int32_t* __restrict__ perm, const AdOut& o, const CdIn& cin,
-                                 const CdParams& cp, const CdOut& cout, const float4* __restrict__ rec) {
+                                 const CdParams& cp, const CdOut& cout, const float4* __restrict__ rec,
+                                 const RowPre<D>& pre) {
   const int32_t k = L.k;
   // the row's own fields from its record (NF/4 16-byte loads)
-  float ri[Rec<D>::NF];
-  rec_load<D>(rec, k, ri);
+  const float* ri = pre.r;
+  // the epilogue's per-row loads, issued before the sweep so that their
+  // latency hides behind it
+  const int32_t i = perm[k];
+  const float drhodt_i = in.drhodt_u[i], rho_k = in.rho[k];
+  CdRow cdr{};
+  if (SW) cdr = cd_row_load(cin, cp, cout, k);
   float xi[3], vi[3], Ji[D * D];
 #pragma unroll
   for (int d = 0; d < D; ++d) { xi[d] = ri[Rec<D>::X + d]; vi[d] = ri[Rec<D>::VEL + d]; }
@@ -459,7 +492,7 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
   const float epsi = 1.0e-7f * smi * smi;
   const float ViOm = Vi / mi;
   Comp acc[D], accu, accs;
-  const int32_t b = rows.off[k], e = L.valid ? b + rows.cnt[k] : b;
+  const int32_t b = pre.off, e = L.valid ? b + pre.cnt : b;
   // neighbour ids loaded two iterations ahead (the record itself is loaded
   // at the start of its iteration: prefetching it too costs more in
   // occupancy than it hides)
@@ -469,10 +502,10 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     const int32_t j = jc;
     float rj[Rec<D>::NF];
     src.template load<D>(j, rj);
-    if (q0 + G < e) {
-      jc = jn;
-      if (q0 + 2 * G < e) jn = src.id(q0 + 2 * G);
-    }
+    // (branch free: past the row's end the id of its last slot is read and
+    // not used)
+    jc = jn;
+    jn = src.id(min(q0 + 2 * G, e - 1));
     float dx[3];
     pair_dx<D>(geo, xi, &rj[Rec<D>::X], dx);
     const float r2 = dot<D>(dx, dx);
@@ -506,7 +539,12 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
     for (int a = 0; a < D * D; ++a) crj.gB[a] = rj[Rec<D>::GB + a];
     WdW wj = wdw<K, D>(C, r, ihj);  // W(x_ij, h_j) for the i-side gradient
     WdW wi = wdw<K, D>(C, r, ihi);  // W(x_ji, h_i) for the j-side gradient
-    if (SW && wi.W != 0.f) accs.add(rj[Rec<D>::MSG] * wi.W);
+    if (SW) {
+      // (a select, not a branch: a zero term would still move the compensation)
+      Comp t = accs;
+      t.add(rj[Rec<D>::MSG] * wi.W);
+      if (wi.W != 0.f) accs = t;
+    }
     float gi[3], gjr[3], dxn[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) dxn[a] = -dx[a];
@@ -566,7 +604,6 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
   float du = group_total<G>(accu);
   float S = SW ? group_total<G>(accs) : 0.f;
   if (!L.valid || L.g != 0) return;
-  int32_t i = perm[k];
   float ke = 0.f;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
@@ -576,8 +613,8 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
   }
   o.dudt_u[i] = du;
   o.dEdt_u[i] = mi * ke + mi * du;
-  o.div_u[i] = -in.drhodt_u[i] / in.rho[k];
-  if (SW) cullen_row(cin, cp, k, i, S, hi, cout);
+  o.div_u[i] = -drhodt_i / rho_k;
+  if (SW) cullen_row(cdr, cp, k, i, S, hi, cout);
 }
 
 template <int D, int G, int K, bool SW>
@@ -586,8 +623,10 @@ __global__ void __launch_bounds__(kBlock) k_accel_dudt(ARows rows, Geo geo, floa
                                                        const int32_t* __restrict__ perm, AdOut o, CdIn cin,
                                                        CdParams cp, CdOut cout, const float4* __restrict__ rec) {
   const Lane L = lane_of<G>(n);
+  RowPre<D> pre;
+  row_pre_load<D>(pre, rec, rows, L.k);
   accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, alphaQ, n, in, perm, o,
-                         cin, cp, cout, rec);
+                         cin, cp, cout, rec, pre);
 }
 
 // The acceleration pass on an LDS tile: the block's threads (1024 by
@@ -606,27 +645,22 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_accel_dudt_t(ARows rows, co
   __shared__ float4 srec[kTileCap * C4];
   __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
-  tile_table(tiles, t, tab);
+  // the first round's rows: their records and CSR ranges load while the
+  // tile is staged
+  RowPre<D> pre0;
+  row_pre_load<D>(pre0, rec, rows, tile_lane<G>(t, 0, n).k);
+  const int32_t base = tile_stage<C4>(tiles, t, rows.off, n, ploc, rec, tab, srec, sid);
   const bool wide = tab[0] < 0;
-  const int32_t base =
-      wide ? 0 : tile_ids_fill(ploc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
-  if (!wide) {
-    const int32_t U = tab[1];
-    for (int32_t e = threadIdx.x; e < U * C4; e += blockDim.x) {
-      const int32_t u = e / C4;
-      srec[e] = rec[(int64_t)tile_global(tab, u) * C4 + (e - u * C4)];
-    }
-    __syncthreads();
-  }
   const int nr = tile_rounds<G>();
   for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);
+    if (round > 0) row_pre_load<D>(pre0, rec, rows, L.k);
     if (wide)
       accel_row<D, G, K, SW>(L, RecSrcG{rows.nbr, rec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
-                             cp, cout, rec);
+                             cp, cout, rec, pre0);
     else
       accel_row<D, G, K, SW>(L, RecSrcT{sid, base, srec}, rows, geo, C, lp, inv_kscale, 0, n, in, perm, o, cin,
-                             cp, cout, rec);
+                             cp, cout, rec, pre0);
   }
 }
 

@@ -261,7 +261,13 @@ template <int K, int D>
 __device__ inline WdW wdw(float C, float r, float invh) {
   WdW o;
   float q = qmul(r, invh);
-  if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
+  if (K == CRK_KERNEL_B7) {
+    if (!(q < 1.f)) { o.W = 0.f; o.fac = 0.f; return o; }
+  } else {
+    // the Wendland polynomials carry a factor (1 - q)^n: at q = 1 the value
+    // and the derivative are exactly zero, so q is clamped instead of branched
+    q = fminf(q, 1.f);
+  }
   float k, dkq;
   kwq<K, D>(q, k, dkq);
   const float CihD = C * ipow<D>(invh);

@@ -136,6 +136,75 @@ __device__ inline int32_t tile_ids_fill(const uint16_t* __restrict__ loc, int32_
   return start;
 }
 
+// tile_global with a fixed number of steps (the run holding u is the last
+// with offset <= u; at most kTileRuns < 64 runs), so that it unrolls where
+// several lookups are in flight
+__device__ inline int32_t tile_global_unrolled(const int32_t* sh, int32_t u) {
+  static_assert(kTileRuns < 64, "six search steps");
+  const int32_t nruns = sh[0];
+  int32_t lo = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const int32_t c = lo + step;
+    if (c < nruns && sh[5 + 2 * c] <= u) lo = c;
+  }
+  return sh[4 + 2 * lo] + (u - sh[5 + 2 * lo]);
+}
+
+// Stage tile t for a pass whose occupancy its LDS limits (one block per CU,
+// registers to spare): the table, then the entries' 16-bit indices (as
+// tile_ids_fill) and the records of the tile's particles (C4 float4 each)
+// with all of a thread's global loads in flight at once.  A load-then-store
+// loop would expose one memory latency per iteration, and with one block on
+// the CU nothing else hides it.  Returns the base of the staged indices
+// (wide tiles, tab[0] < 0, stage nothing).  Synchronises.
+template <int C4, int KR = 8>
+__device__ inline int32_t tile_stage(const int32_t* __restrict__ tiles, int32_t t, const int32_t* __restrict__ off,
+                                     int32_t n, const uint16_t* __restrict__ loc, const float4* __restrict__ rec,
+                                     int32_t* tab, float4* srec, uint16_t* sid) {
+  const int32_t e0 = off[t * kTileRows], e1 = off[min(n, (t + 1) * kTileRows)];
+  for (int w = threadIdx.x; w < kTileWords; w += blockDim.x) tab[w] = tiles[(int64_t)t * kTileWords + w];
+  __syncthreads();
+  if (tab[0] < 0) return 0;
+  const int32_t nr = tab[1] * C4;
+  const int32_t start = e0 & ~7;
+  const int32_t vend = max(e1 & ~7, start);
+  const int32_t nv = (vend - start) >> 3;
+  const uint4* isrc = reinterpret_cast<const uint4*>(loc + start);
+  uint4* idst = reinterpret_cast<uint4*>(sid);
+  const int32_t bd = blockDim.x;
+  constexpr int KI = 4;  // (KR = 8: one round at 1024 threads)
+  // (zero-initialised and loaded per element, so that the staged values
+  // stay in registers)
+  for (int32_t w0 = threadIdx.x, r0 = threadIdx.x; w0 < nv || r0 < nr; w0 += KI * bd, r0 += KR * bd) {
+    uint4 iv[KI];
+    float4 rv[KR];
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+      iv[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (w0 + k * bd < nv) iv[k] = isrc[w0 + k * bd];
+    }
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int32_t e = r0 + k * bd;
+      rv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e < nr) {
+        const int32_t u = e / C4;
+        rv[k] = rec[(int64_t)tile_global_unrolled(tab, u) * C4 + (e - u * C4)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KI; ++k)
+      if (w0 + k * bd < nv) idst[w0 + k * bd] = iv[k];
+#pragma unroll
+    for (int k = 0; k < KR; ++k)
+      if (r0 + k * bd < nr) srec[r0 + k * bd] = rv[k];
+  }
+  for (int32_t q = vend + threadIdx.x; q < e1; q += bd) sid[q - start] = loc[q];
+  __syncthreads();
+  return start;
+}
+
 // the bound adjacency carries tiles the tiled passes can use (d <= 2)
 inline bool use_tiles(const CrkContext* c) { return c->adj.tiles && c->adj.nbr_loc && c->dim <= 2; }
 inline unsigned tile_count(int32_t n) { return nblocks(n, kTileRows); }

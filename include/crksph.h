@@ -181,6 +181,17 @@ int crk_bind_runs(CrkContext* ctx, int32_t n_runs, const int32_t* offsets_host,
  * only while the flag is clear.  All but crk_flag_read are graph-capturable. */
 int crk_flag_clear(CrkContext* ctx, void* stream);
 int crk_flag_read(CrkContext* ctx, int32_t* flag, void* stream);
+/* Speculative stages: after a deferred crk_stage_begin_checked (or
+ * crk_owen_valid), the adaptive-support pass (or the support sweep) may be
+ * launched before the host knows the test's result; it does nothing once
+ * the flag is set (the support sweep then leaves every row without pairs),
+ * until the next blocking test or crk_stage_begin.  crk_flag_read_async
+ * queues a copy of the flag to pinned host memory in slot 0..3;
+ * crk_flag_wait waits for that slot's copy alone (not for the work queued
+ * after it) and returns the flag as it was copied.
+ * No counterpart in the reference (its Verlet test returns to Python). */
+int crk_flag_read_async(CrkContext* ctx, int32_t slot, void* stream);
+int crk_flag_wait(CrkContext* ctx, int32_t slot, int32_t* flag);
 int crk_copy_unless_flag(CrkContext* ctx, const void* src, void* dst, int64_t bytes, void* stream);
 int crk_runs_advance_time(CrkContext* ctx, const float* dt_runs, double* t_runs, void* stream);
 /* crk_cfl_dt_runs on the device (no host synchronisation): dt_out[r] (float)

@@ -186,6 +186,8 @@ _PROTOS = {
     "crk_rk_combine_p": [C.c_int64, C.c_int32, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P],
     "crk_flag_clear": [_P, _P],
     "crk_flag_read": [_P, C.POINTER(C.c_int32), _P],
+    "crk_flag_read_async": [_P, C.c_int32, _P],
+    "crk_flag_wait": [_P, C.c_int32, C.POINTER(C.c_int32)],
     "crk_copy_unless_flag": [_P, _P, _P, C.c_int64, _P],
     "crk_runs_advance_time": [_P, _P, _P, _P],
     "crk_cfl_dt_device": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P, _P, _P],

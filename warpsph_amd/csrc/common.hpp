@@ -108,6 +108,15 @@ struct CrkContext {
   // device flag of deferred Verlet tests (1: some list failed), for steps
   // replayed without host synchronisation
   int32_t* d_flag = nullptr;
+  // d_flag while the bound stage's lists are tested by deferred tests (the
+  // adaptive-support and support passes then do nothing once it is set);
+  // null after a blocking test or crk_stage_begin
+  const int32_t* gate = nullptr;
+  // crk_flag_read_async / crk_flag_wait: the flag's pinned host copies and
+  // the event recorded after each copy, per slot
+  static constexpr int kFlagSlots = 4;
+  int32_t* flag_host = nullptr;
+  hipEvent_t flag_event[kFlagSlots] = {};
   // device address of this device's copy of the allocation-free entry
   // point's file-scope scratch array (hipGetSymbolAddress at context creation)
   double* noalloc_scratch = nullptr;

@@ -266,12 +266,21 @@ int crk_ctx_create(CrkContext** out) {
   if (e != hipSuccess) { (void)hipHostFree(p); delete c; return hip_fail(e, "hipMalloc", __FILE__, __LINE__); }
   e = hipMemset(c->d_flag, 0, 64);
   if (e != hipSuccess) { (void)hipFree(c->d_flag); (void)hipHostFree(p); delete c; return hip_fail(e, "hipMemset", __FILE__, __LINE__); }
-  if (noalloc_setup(c) != CRK_OK) {
+  void* fh = nullptr;
+  e = hipHostMalloc(&fh, 64, hipHostMallocDefault);
+  for (int q = 0; q < CrkContext::kFlagSlots && e == hipSuccess; ++q)
+    e = hipEventCreateWithFlags(&c->flag_event[q], hipEventDisableTiming);
+  if (e != hipSuccess || noalloc_setup(c) != CRK_OK) {
+    if (e != hipSuccess) (void)hip_fail(e, "flag host copy / event", __FILE__, __LINE__);
+    for (hipEvent_t ev : c->flag_event)
+      if (ev) (void)hipEventDestroy(ev);
+    if (fh) (void)hipHostFree(fh);
     (void)hipFree(c->d_flag);
     (void)hipHostFree(p);
     delete c;
     return CRK_EHIP;
   }
+  c->flag_host = (int32_t*)fh;
   *out = c;
   return CRK_OK;
 }
@@ -339,6 +348,25 @@ int crk_flag_read(CrkContext* ctx, int32_t* flag, void* stream) {
   return CRK_OK;
 }
 
+int crk_flag_read_async(CrkContext* ctx, int32_t slot, void* stream) {
+  CRK_REQUIRE(ctx, "crk_flag_read_async: null context");
+  CRK_REQUIRE(slot >= 0 && slot < CrkContext::kFlagSlots, "crk_flag_read_async: slot %d not in [0, %d)", slot,
+              CrkContext::kFlagSlots);
+  hipStream_t s = (hipStream_t)stream;
+  CRK_HIP(hipMemcpyAsync(ctx->flag_host + slot, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipEventRecord(ctx->flag_event[slot], s));
+  return CRK_OK;
+}
+
+int crk_flag_wait(CrkContext* ctx, int32_t slot, int32_t* flag) {
+  CRK_REQUIRE(ctx && flag, "crk_flag_wait: null argument");
+  CRK_REQUIRE(slot >= 0 && slot < CrkContext::kFlagSlots, "crk_flag_wait: slot %d not in [0, %d)", slot,
+              CrkContext::kFlagSlots);
+  CRK_HIP(hipEventSynchronize(ctx->flag_event[slot]));
+  *flag = ((volatile int32_t*)ctx->flag_host)[slot];
+  return CRK_OK;
+}
+
 int crk_copy_unless_flag(CrkContext* ctx, const void* src, void* dst, int64_t bytes, void* stream) {
   CRK_REQUIRE(ctx && src && dst && bytes >= 0 && bytes % 4 == 0, "crk_copy_unless_flag: invalid arguments");
   if (bytes == 0) return CRK_OK;
@@ -372,6 +400,9 @@ int crk_ctx_destroy(CrkContext* c) {
   for (Buf* b : bufs) release(*b);
   if (c->pinned.p) (void)hipHostFree(c->pinned.p);
   if (c->d_flag) (void)hipFree(c->d_flag);
+  for (hipEvent_t ev : c->flag_event)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->flag_host) (void)hipHostFree(c->flag_host);
   delete c;
   return CRK_OK;
 }

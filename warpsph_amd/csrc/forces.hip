@@ -115,9 +115,16 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_velgrad_t(ARows rows, const
   // (the pair indices stay in global memory here: staging them in LDS was
   // measured slower for this pass)
   const int32_t t = xcd_block();
-  tile_table(tiles, t, tab);
+  struct PV {
+    float4 a, b;
+  };
+  tile_stage_fn<2, 0, PV>(
+      tiles, t, 0, 0, nullptr, tab, nullptr, [&](int32_t g) { return PV{in.geo4[g], in.vg4[g]}; },
+      [&](int32_t u, const PV& v) {
+        sg[u] = v.a;
+        sw[u] = v.b;
+      });
   const bool wide = tab[0] < 0;
-  if (!wide) tile_fill(sg, in.geo4, tab, sw, in.vg4);
   const int nr = tile_rounds<G>();
   for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);

@@ -121,34 +121,42 @@ __device__ inline void compat_sum(const CompIn& s, const Src& src, int32_t k, bo
                                   float mdt, float& sa, float& ca) {
   const int32_t b = s.off[k], e = valid ? b + s.cnt[k] : b;
   const float4 ri = s.rec[k];
-  // software pipeline: the neighbour's records one iteration ahead, its id two ahead
+  // software pipeline: the neighbour's records and the pair force one
+  // iteration ahead, the neighbour's id two ahead
   int32_t jc = b + g < e ? src.id(b + g) : 0;
   int32_t jn = b + g + G < e ? src.id(b + g + G) : 0;
   float4 rn = src.r(jc);
-  float vn[3];
+  float vn[3], an[3];
 #pragma unroll
   for (int c = 0; c < D; ++c) vn[c] = src.vh(jc, c);
-  for (int32_t q = b + g; q < e; q += G) {
-    const float4 rj = rn;
-    float vj[3];
-#pragma unroll
-    for (int c = 0; c < D; ++c) vj[c] = vn[c];
-    if (q + G < e) {
-      jc = jn;
-      rn = src.r(jc);
-#pragma unroll
-      for (int c = 0; c < D; ++c) vn[c] = src.vh(jc, c);
-      if (q + 2 * G < e) jn = src.id(q + 2 * G);
-    }
-    float a[3], d0 = 0.f;
+  auto pf_load = [&](int64_t q, float* a) {
     if (D == 2) {
       const float2 p = reinterpret_cast<const float2*>(s.pf)[q];
       a[0] = p.x;
       a[1] = p.y;
     } else {
 #pragma unroll
-      for (int c = 0; c < D; ++c) a[c] = s.pf[(int64_t)q * D + c];
+      for (int c = 0; c < D; ++c) a[c] = s.pf[q * D + c];
     }
+  };
+  if (b + g < e) pf_load(b + g, an);
+  for (int32_t q = b + g; q < e; q += G) {
+    const float4 rj = rn;
+    float vj[3], a[3];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      vj[c] = vn[c];
+      a[c] = an[c];
+    }
+    if (q + G < e) {
+      jc = jn;
+      rn = src.r(jc);
+#pragma unroll
+      for (int c = 0; c < D; ++c) vn[c] = src.vh(jc, c);
+      pf_load(q + G, an);
+      if (q + 2 * G < e) jn = src.id(q + 2 * G);
+    }
+    float d0 = 0.f;
 #pragma unroll
     for (int c = 0; c < D; ++c) d0 = d0 + (vj[c] - vi[c]) * a[c];
     const float f = pair_fraction<D>(ri, rj, mdt, a);
@@ -222,19 +230,27 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_compat_t(const int32_t* __r
   // (the pair indices stay in global memory here: staging both stages' in
   // LDS was measured slower, the LDS then limiting the pass to one block per CU)
   const int32_t t = xcd_block();
-  tile_table(tiles, t, tab);
-  const bool wide = tab[0] < 0;
-  if (!wide) {
-    const int32_t U = tab[1];
-    for (int32_t q = threadIdx.x; q < U; q += blockDim.x) {
-      const int32_t g = tile_global(tab, q);
-      sr0[q] = s0.rec[g];
-      sr1[q] = s1.rec[g];
+  struct PV {
+    float4 a, b;
+    float vh[D];
+  };
+  tile_stage_fn<2, 0, PV>(
+      tiles, t, 0, 0, nullptr, tab, nullptr,
+      [&](int32_t g) {
+        PV v;
+        v.a = s0.rec[g];
+        v.b = s1.rec[g];
 #pragma unroll
-      for (int c = 0; c < D; ++c) svh[q * D + c] = vhs[(int64_t)g * D + c];
-    }
-    __syncthreads();
-  }
+        for (int c = 0; c < D; ++c) v.vh[c] = vhs[(int64_t)g * D + c];
+        return v;
+      },
+      [&](int32_t u, const PV& v) {
+        sr0[u] = v.a;
+        sr1[u] = v.b;
+#pragma unroll
+        for (int c = 0; c < D; ++c) svh[u * D + c] = v.vh[c];
+      });
+  const bool wide = tab[0] < 0;
   const int nr = tile_rounds<G>();
   for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);

@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_support_t(Rows rows, const 
                                                       int32_t* __restrict__ acnt, float* __restrict__ vsig,
                                                       float* __restrict__ V, const float4* __restrict__ geo4,
                                                       const float4* __restrict__ vel4, const float* __restrict__ m,
-                                                      float2* __restrict__ vm2) {
+                                                      float2* __restrict__ vm2, const int32_t* __restrict__ gate) {
   __shared__ int32_t tab[kTileWords];
   __shared__ int32_t soff[kTileRows + 1];
   __shared__ float4 sg[kTileCap];
@@ -299,13 +299,20 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_support_t(Rows rows, const 
   __shared__ __align__(16) uint16_t sid[kTileIds + 8];
   const int32_t t = xcd_block();
   const int32_t r0 = t * kTileRows;
+  if (gate && *gate) {  // (the list failed its deferred test: no pairs)
+    for (int w = threadIdx.x; w < kTileRows; w += blockDim.x)
+      if (r0 + w < n) acnt[r0 + w] = 0;
+    return;
+  }
   for (int w = threadIdx.x; w <= kTileRows; w += blockDim.x) soff[w] = rows.off[min(r0 + w, n)];
   tile_table(tiles, t, tab);
   const bool wide = tab[0] < 0;
   const int32_t base = wide ? 0 : tile_ids_fill(nloc, rows.off[r0], rows.off[min(n, r0 + kTileRows)], sid);
   if (!wide) {
     // the staged {x, h} record carries the particle's sorted id in its unused
-    // z component (d <= 2), for the compacted pairs' 32-bit ids
+    // z component (d <= 2), for the compacted pairs' 32-bit ids (a staging
+    // with several particles' loads in flight per thread, tile_stage_fn, was
+    // measured slower for this pass)
     const int32_t U = tab[1];
     for (int32_t u = threadIdx.x; u < U; u += blockDim.x) {
       const int32_t g = tile_global(tab, u);
@@ -341,8 +348,13 @@ __global__ void __launch_bounds__(kBlock) k_support(Rows rows, Geo geo, float C,
                                                     int32_t* __restrict__ anbr, int32_t* __restrict__ acnt,
                                                     float* __restrict__ vsig, float* __restrict__ V,
                                                     const float4* __restrict__ geo4, const float4* __restrict__ vel4,
-                                                    const float* __restrict__ m, float2* __restrict__ vm2) {
+                                                    const float* __restrict__ m, float2* __restrict__ vm2,
+                                                    const int32_t* __restrict__ gate) {
   const Lane L = lane_of<G>(n);
+  if (gate && *gate) {  // (the list failed its deferred test: no pairs)
+    if (L.valid && L.g == 0) acnt[L.k] = 0;
+    return;
+  }
   const GlobalSrc<float4> src{rows.nbr, geo4, vel4};
   SupHead hd;
   hd.b = rows.off[L.k];
@@ -421,7 +433,9 @@ __global__ void __launch_bounds__(kBlock) k_owen(Rows rows, Geo geo, float C, in
                                                  const double* __restrict__ lut_psiH, const double* __restrict__ lut_nh,
                                                  int nlut, float nhT, const float* __restrict__ hminmax,
                                                  float* __restrict__ ratio, float4* __restrict__ geo4,
-                                                 OwenFinal fin, const int32_t* __restrict__ run) {
+                                                 OwenFinal fin, const int32_t* __restrict__ run,
+                                                 const int32_t* __restrict__ gate) {
+  if (gate && *gate) return;  // (the list failed its deferred test)
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   const float4 gk = geo4[k];  // the row's {x, h} (h before this update)
@@ -583,16 +597,17 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_owen_t(Rows rows, const uin
                                                    float* __restrict__ supports, const double* __restrict__ lut_psiH,
                                                    const double* __restrict__ lut_nh, int nlut, float nhT,
                                                    const float* __restrict__ hminmax, float* __restrict__ ratio,
-                                                   float4* __restrict__ geo4, const int32_t* __restrict__ run) {
+                                                   float4* __restrict__ geo4, const int32_t* __restrict__ run,
+                                                   const int32_t* __restrict__ gate) {
   __shared__ int32_t tab[kTileWords];
   __shared__ float4 sg[kTileCap];
   __shared__ __align__(16) uint16_t sid[kTileIds + 8];
+  if (gate && *gate) return;  // (the list failed its deferred test)
   const int32_t t = xcd_block();
-  tile_table(tiles, t, tab);
+  const int32_t base = tile_stage_fn<2, 2, float4>(
+      tiles, t, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], nloc, tab, sid,
+      [&](int32_t g) { return geo4[g]; }, [&](int32_t u, const float4& v) { sg[u] = v; });
   const bool wide = tab[0] < 0;
-  const int32_t base =
-      wide ? 0 : tile_ids_fill(nloc, rows.off[t * kTileRows], rows.off[min(n, (t + 1) * kTileRows)], sid);
-  if (!wide) tile_fill(sg, (const float4*)geo4, tab);
   float rmax = 0.f;
   const int nr = tile_rounds<G>();
   for (int round = 0; round < nr; ++round) {
@@ -635,7 +650,9 @@ __global__ void __launch_bounds__(kBlock) k_monaghan(Rows rows, Geo geo, float C
                                                      const float* __restrict__ m, const int32_t* __restrict__ perm,
                                                      float* __restrict__ supports, float nT,
                                                      const float* __restrict__ hminmax, float* __restrict__ ratio,
-                                                     float4* __restrict__ geo4, const int32_t* __restrict__ run) {
+                                                     float4* __restrict__ geo4, const int32_t* __restrict__ run,
+                                                     const int32_t* __restrict__ gate) {
+  if (gate && *gate) return;  // (the list failed its deferred test)
   Lane L = lane_of<G>(n);
   const int32_t k = L.k;
   float xi[3];
@@ -1053,9 +1070,18 @@ __global__ void __launch_bounds__(kTileThreadsMax) k_moments_t(ARows rows, const
   // (the pair indices stay in global memory here: staging them in LDS was
   // measured slower for this pass, whose occupancy its registers limit)
   const int32_t t = xcd_block();
-  tile_table(tiles, t, tab);
+  struct PV {
+    float4 a;
+    float2 b;
+  };
+  const float2* vm2 = (const float2*)o.vm2;
+  tile_stage_fn<2, 0, PV>(
+      tiles, t, 0, 0, nullptr, tab, nullptr, [&](int32_t g) { return PV{geo4[g], vm2[g]}; },
+      [&](int32_t u, const PV& v) {
+        sg[u] = v.a;
+        sv[u] = v.b;
+      });
   const bool wide = tab[0] < 0;
-  if (!wide) tile_fill(sg, geo4, tab, sv, (const float2*)o.vm2);
   const int nr = tile_rounds<G>();
   for (int round = 0; round < nr; ++round) {
     const Lane L = tile_lane<G>(t, round, n);
@@ -1247,6 +1273,7 @@ int crk_stage_begin(CrkContext* ctx, const CrkConfig* cfg, const CrkAdjacency* a
   ctx->have_factors = ctx->have_velgrad = ctx->have_accel = ctx->have_support = ctx->have_volume = false;
   ctx->have_vsig = ctx->have_divcd = ctx->have_rcd = false;
   ctx->check_hb = nullptr;
+  ctx->gate = nullptr;
   ctx->owen_checked = false;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
@@ -1345,6 +1372,9 @@ int crk_stage_begin_checked(CrkContext* ctx, const CrkConfig* cfg, const CrkAdja
   ctx->have_vsig = ctx->have_divcd = ctx->have_rcd = false;
   ctx->check_hb = hb;
   ctx->owen_checked = false;
+  // deferred: the adaptive-support and support passes of this stage do
+  // nothing once the flag is set
+  ctx->gate = valid ? nullptr : ctx->d_flag;
   hipStream_t s = (hipStream_t)stream;
   int32_t n = adj->n;
   float* red = F<float>(ctx->s_chk);
@@ -1407,7 +1437,11 @@ int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* 
   CRK_LAUNCH_CHECK("k_hb_ratio");
   const double sv = cfg->verletScale;
   // the positions' part of the test from the checked gather, h/hb after Owen
-  if (!valid) return verlet_flag_launch(ctx, red, 4, 0, 3, 2, nr, sv, s);  // deferred: into the device flag
+  if (!valid) {  // deferred: into the device flag, which then gates the support pass
+    ctx->gate = ctx->d_flag;
+    return verlet_flag_launch(ctx, red, 4, 0, 3, 2, nr, sv, s);
+  }
+  ctx->gate = nullptr;
   float* hostv = (float*)ctx->pinned.p;
   CRK_HIP(hipMemcpyAsync(hostv, red, 4 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
   CRK_HIP(hipStreamSynchronize(s));
@@ -1444,7 +1478,7 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
         hipLaunchKernelGGL((k_owen<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin, run_ids(ctx)));
+                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), fin, run_ids(ctx), ctx->gate));
     ctx->have_volume = true;
     ctx->have_vsig = vs != nullptr;
   } else if (use_tiles(ctx)) {
@@ -1452,13 +1486,14 @@ static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st
         hipLaunchKernelGGL((k_owen_t<DD, G, KK>), dim3(tile_count(n)), dim3(ctx->threads[CRK_PASS_OWEN]), 0, s,
                            rows_of(ctx), ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, F<float>(ctx->h), ctx->adj.perm,
                            st->supports, F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut,
-                           (float)ctx->nh_target, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx)));
+                           (float)ctx->nh_target, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx),
+                           ctx->gate));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_owen<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, F<float>(ctx->x), F<float>(ctx->h), ctx->adj.perm, st->supports,
                            F<double>(ctx->lut_psiH), F<double>(ctx->lut_nh), ctx->nlut, (float)ctx->nh_target,
-                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), OwenFinal{}, run_ids(ctx)));
+                           hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), OwenFinal{}, run_ids(ctx), ctx->gate));
   }
   CRK_LAUNCH_CHECK("k_owen");
   return CRK_OK;
@@ -1491,7 +1526,8 @@ int crk_monaghan_iteration(CrkContext* ctx, const CrkConfig* cfg, const CrkState
   CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
       hipLaunchKernelGGL((k_monaghan<DD, G, KK>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx), geo, C,
                          n, F<float>(ctx->x), F<float>(ctx->h), F<float>(ctx->m), ctx->adj.perm, st->supports,
-                         cfg->targetNeighbors, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx)));
+                         cfg->targetNeighbors, hminmax_dev, max_ratio_dev, F<float4>(ctx->geo4), run_ids(ctx),
+                         ctx->gate));
   CRK_LAUNCH_CHECK("k_monaghan");
   return CRK_OK;
 }
@@ -1533,19 +1569,19 @@ int crk_support(CrkContext* ctx, const CrkConfig* cfg, CrkEdges* ed, void* strea
                            0, s, rows_of(ctx),                                                                  \
                            ctx->adj.nbr_loc, ctx->adj.tiles, geo, C, n, ed->nbr, ed->loc, ed->count, vs,       \
                            F<float>(ctx->V), F<float4>(ctx->geo4), F<float4>(ctx->vel4), F<float>(ctx->m),    \
-                           F<float2>(ctx->vm2)))
+                           F<float2>(ctx->vm2), ctx->gate))
     if (vs) { SUPPORT_T(true); } else { SUPPORT_T(false); }
 #undef SUPPORT_T
   } else if (vs) {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_support<DD, G, KK, true>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4),
-                           F<float4>(ctx->vel4), F<float>(ctx->m), F<float2>(ctx->vm2)));
+                           F<float4>(ctx->vel4), F<float>(ctx->m), F<float2>(ctx->vm2), ctx->gate));
   } else {
     CRK_KDG_DISPATCH(cfg->kernel, ctx->dim, GS,
         hipLaunchKernelGGL((k_support<DD, G, KK, false>), dim3(group_blocks(n, G)), dim3(kBlock), 0, s, rows_of(ctx),
                            geo, C, n, ed->nbr, ed->count, vs, F<float>(ctx->V), F<float4>(ctx->geo4),
-                           F<float4>(ctx->vel4), F<float>(ctx->m), F<float2>(ctx->vm2)));
+                           F<float4>(ctx->vel4), F<float>(ctx->m), F<float2>(ctx->vm2), ctx->gate));
   }
   CRK_LAUNCH_CHECK("k_support");
   ctx->pair_cnt = ed->count;

@@ -46,26 +46,6 @@ __device__ inline int32_t tile_global(const int32_t* sh, int32_t u) {
   return sh[4 + 2 * lo] + (u - sh[5 + 2 * lo]);
 }
 
-// Stage the records of the tile's runs: dst[u] = src[global(u)], u < U
-// (and a second array the same way when src2 is given).  Synchronises
-// before returning.  (Holding several slots' loads in flight per thread was
-// measured slower: the registers it needs lower the pass's occupancy.)
-template <class T, class T2 = T>
-__device__ inline void tile_fill(T* dst, const T* __restrict__ src, const int32_t* sh, T2* dst2 = nullptr,
-                                 const T2* __restrict__ src2 = nullptr) {
-  const int32_t U = sh[1];
-  for (int32_t u = threadIdx.x; u < U; u += blockDim.x) {
-    const int32_t g = tile_global(sh, u);
-    const T a = src[g];
-    if (src2) {
-      const T2 b = src2[g];
-      dst2[u] = b;
-    }
-    dst[u] = a;
-  }
-  __syncthreads();
-}
-
 // A tile's rows are processed by one block of blockDim.x threads (256, 512
 // or 1024, the pass's tile block size), G lanes per row, blockDim.x / G rows
 // per round.
@@ -201,6 +181,56 @@ __device__ inline int32_t tile_stage(const int32_t* __restrict__ tiles, int32_t 
       if (r0 + k * bd < nr) srec[r0 + k * bd] = rv[k];
   }
   for (int32_t q = vend + threadIdx.x; q < e1; q += bd) sid[q - start] = loc[q];
+  __syncthreads();
+  return start;
+}
+
+// Stage tile t for a pass with per-particle records of type V: the table,
+// the entries' 16-bit indices (as tile_ids_fill) and, for every particle u of
+// the tile's runs, store(u, load(global index of u)), with up to KP
+// particles' and KI index chunks' loads of a thread in flight at once (a
+// load-then-store loop such as tile_ids_fill's exposes one memory latency
+// per iteration).  KP and KI trade that latency against the
+// registers of the pass's prologue.  Returns the base of the staged indices
+// (wide tiles, tab[0] < 0, stage nothing).  KI = 0 stages no indices (the
+// pass reads them from global memory; e0, e1 and loc are then unused).
+// Synchronises.
+template <int KP, int KI, class V, class Load, class Store>
+__device__ inline int32_t tile_stage_fn(const int32_t* __restrict__ tiles, int32_t t, int32_t e0, int32_t e1,
+                                        const uint16_t* __restrict__ loc, int32_t* tab, uint16_t* sid,
+                                        const Load& load, const Store& store) {
+  for (int w = threadIdx.x; w < kTileWords; w += blockDim.x) tab[w] = tiles[(int64_t)t * kTileWords + w];
+  __syncthreads();
+  if (tab[0] < 0) return 0;
+  const int32_t U = tab[1];
+  const int32_t start = e0 & ~7;
+  const int32_t vend = max(e1 & ~7, start);
+  const int32_t nv = (vend - start) >> 3;
+  const uint4* isrc = reinterpret_cast<const uint4*>(loc + start);
+  uint4* idst = reinterpret_cast<uint4*>(sid);
+  const int32_t bd = blockDim.x;
+  for (int32_t w0 = threadIdx.x, u0 = threadIdx.x; (KI > 0 && w0 < nv) || u0 < U; w0 += KI * bd, u0 += KP * bd) {
+    uint4 iv[KI > 0 ? KI : 1];
+    V pv[KP];
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+      iv[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (w0 + k * bd < nv) iv[k] = isrc[w0 + k * bd];
+    }
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      pv[k] = V{};
+      if (u0 + k * bd < U) pv[k] = load(tile_global_unrolled(tab, u0 + k * bd));
+    }
+#pragma unroll
+    for (int k = 0; k < KI; ++k)
+      if (w0 + k * bd < nv) idst[w0 + k * bd] = iv[k];
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+      if (u0 + k * bd < U) store(u0 + k * bd, pv[k]);
+  }
+  if (KI > 0)
+    for (int32_t q = vend + threadIdx.x; q < e1; q += bd) sid[q - start] = loc[q];
   __syncthreads();
   return start;
 }

@@ -44,6 +44,11 @@ class Runtime:
         # LDS tiles for the pair passes of 1D and 2D adjacencies (None: off);
         # the value is the most particles one tile may stage
         self.tile_cap = abi.CRK_TILE_CAP
+        # speculative stages (begin_speculative): the adaptive-support pass
+        # and the support sweep are queued before the host reads the Verlet
+        # tests that precede them, and do nothing if a list failed; the GPU
+        # then never waits for the host's reads (False: blocking tests)
+        self.speculate = True
 
     def stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -176,11 +181,56 @@ def rebuild_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: tor
         return build_adjacency(cfg, positions, supports, runs)
 
 
-def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: torch.Tensor, runs=None):
+def flag_wait(rt, slot: int = 0) -> bool:
+    """The device flag as the last crk_flag_read_async of ``slot`` copied it
+    (True: some list failed)."""
+    f = C.c_int32()
+    abi.call("crk_flag_wait", rt.ctx, int(slot), C.byref(f))
+    return bool(f.value)
+
+
+def begin_speculative(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: torch.Tensor, runs, adaptive,
+                      make_pairs, nsing: torch.Tensor):
+    """The stage's Verlet test (with the gather, crk_stage_begin_checked),
+    the adaptive-support pass ``adaptive()``, the test of the updated supports
+    (crk_owen_valid) and the support sweep over ``make_pairs(adjacency)``, all
+    queued before the host reads either test (``Runtime.speculate``).  Both
+    tests go to the device flag, which the two passes read (they do nothing
+    once it is set), and the flag after each test is copied to the host: the
+    GPU never waits for the host's reads.  Returns None when the stage's
+    list failed (nothing took effect; the caller takes the blocking path),
+    else (adjacency, pairs, True when the updated supports failed the test
+    and the list must be rebuilt before the support sweep runs again)."""
+    rt = runtime(st.positions.device)
+    if prior is None or prior.n != st.numParticles or getattr(prior, "runs", None) is not runs:
+        return None
+    s = rt.stream()
+    A = prior.cabi()
+    with timing.scope("stage_begin"):
+        abi.call("crk_flag_clear", rt.ctx, s)
+        abi.call("crk_stage_begin_checked", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), abi.ptr(prior.xb),
+                 abi.ptr(prior.hb), abi.ptr(hminmax), None, s)
+        abi.call("crk_flag_read_async", rt.ctx, 0, s)
+    adaptive()
+    abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), None, s)  # (ORed into the flag)
+    abi.call("crk_flag_read_async", rt.ctx, 1, s)
+    pairs = make_pairs(prior)
+    nsing.zero_()
+    with timing.scope("support"):
+        abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(pairs.cabi()), s)
+    if flag_wait(rt, 0):
+        return None
+    return prior, pairs, flag_wait(rt, 1)
+
+
+def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: torch.Tensor, runs=None, then=None):
     """crk_stage_begin fused with the Verlet test of the prior adjacency and the
     support range; rebuilds (and binds the new list) when the prior list no
-    longer holds every pair within the Verlet radius.  Returns the adjacency."""
+    longer holds every pair within the Verlet radius, then calls ``then``
+    (the pass that follows: it runs on the list returned).  Returns the
+    adjacency."""
     rt = runtime(st.positions.device)
+    s = rt.stream()
     adj = prior if prior is not None and prior.n == st.numParticles and getattr(prior, "runs", None) is runs \
         else None
     if rt.deferred:
@@ -188,7 +238,9 @@ def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: tor
             raise RuntimeError("a deferred step needs the adjacency of the state")
         A = adj.cabi()
         abi.call("crk_stage_begin_checked", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), abi.ptr(adj.xb),
-                 abi.ptr(adj.hb), abi.ptr(hminmax), None, rt.stream())
+                 abi.ptr(adj.hb), abi.ptr(hminmax), None, s)
+        if then is not None:
+            then()
         return adj
     for _ in range(2):
         if adj is None:
@@ -197,8 +249,10 @@ def begin_checked(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax: tor
         ok = C.c_int32()
         with timing.scope("stage_begin"):
             abi.call("crk_stage_begin_checked", rt.ctx, C.byref(cfg), C.byref(A), C.byref(cst), abi.ptr(adj.xb),
-                     abi.ptr(adj.hb), abi.ptr(hminmax), C.byref(ok), rt.stream())
+                     abi.ptr(adj.hb), abi.ptr(hminmax), C.byref(ok), s)
         if ok.value:
+            if then is not None:
+                then()
             return adj
         adj = None
     raise RuntimeError("an adjacency built for the current state failed its own validity test")

@@ -17,7 +17,7 @@ from . import abi, boundary, timing
 from .configurations import (CRKSPHConfig, SimulationConfig, crkSPHConfigToDict, dictToCRKSPHConfig,
                              toCrkConfig)
 from .enumTypes import AdaptiveSupportScheme, CompressibleSPHScheme, ViscositySwitch
-from .runtime import begin_checked, ensure_adjacency, rebuild_adjacency, runtime
+from .runtime import begin_checked, begin_speculative, ensure_adjacency, rebuild_adjacency, runtime
 from .systems import CompressibleSystemUpdate, CompSPHState, CompSPHSystem, StagePairs
 
 # dt argument of a step whose per-particle dt is already on the device
@@ -108,6 +108,18 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
     owen = scheme_h in (AdaptiveSupportScheme.Owen, AdaptiveSupportScheme.Monaghan)  # an adaptive-h pass runs
     fused_support = False
     owen_checked = False
+    spec = None  # (adjacency, pairs, post-update test failed) of a speculative stage
+
+    # the interacting pairs of this stage (compacted by the support sweep,
+    # which also computes the apparent volume) and their pair data; set up
+    # for the list in hand before the host waits on its test (and set up
+    # again if the list is rebuilt)
+    def stage_pairs(a):
+        p = StagePairs(a, dim, st.masses)
+        if runs is not None:
+            p.runs, p.dtp = runs, runs.dtp
+        return p
+
     if owen:
         # evaluateOptimalSupport (optimalSupport.py:12-43): Owen,
         # optimalSupportOwen.py:35-127, or Monaghan, optimalSupportMonaghan.py:20-108
@@ -126,10 +138,16 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         if iters == 1:
             # the stage's gather carries the Verlet test of the prior list and
             # the support range hMin, hMax (buildVerletList with priorNeighborhood)
-            adj = begin_checked(cfg, adj, st, cst, hmm, runs)
-            with timing.scope("owen"):
-                abi.call("crk_owen_support" if fused_support else update, rt.ctx, C.byref(cfg),
-                         C.byref(cst), abi.ptr(hmm), None, s)
+            def owen_pass():
+                with timing.scope("owen"):
+                    abi.call("crk_owen_support" if fused_support else update, rt.ctx, C.byref(cfg),
+                             C.byref(cst), abi.ptr(hmm), None, s)
+            if rt.speculate and not rt.deferred and not fused_support:
+                spec = begin_speculative(cfg, adj, st, cst, hmm, runs, owen_pass, stage_pairs, rt.nsing)
+            if spec is None:
+                adj = begin_checked(cfg, adj, st, cst, hmm, runs, then=owen_pass)
+            else:
+                adj = spec[0]
             owen_checked = True
         else:
             abi.call("crk_minmax", rt.ctx, n, abi.ptr(st.supports), abi.ptr(hmm), s)
@@ -150,19 +168,19 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         # optimalSupportMonaghan.py:96-107) is overwritten by computeCRKFactors
         # (crkSPH.py:61) before any use; it is not evaluated.
 
-    # the interacting pairs of this stage (compacted by the support sweep,
-    # which also computes the apparent volume) and their pair data; set up
-    # for the list in hand before the host waits on its test (and set up
-    # again if the list is rebuilt)
-    def stage_pairs(a):
-        p = StagePairs(a, dim, st.masses)
-        if runs is not None:
-            p.runs, p.dtp = runs, runs.dtp
-        return p
-    pairs = stage_pairs(adj) if adj is not None else None
-    rt.nsing.zero_()
+    support_done = False
     # buildVerletList(SuperSymmetric, verletScale, prior), crkSPH.py:54-58
-    if owen_checked:
+    if spec is not None:
+        # the speculative stage's pairs and support sweep (valid unless the
+        # updated supports failed the list's test)
+        pairs = spec[1]
+        rebuilt = spec[2]
+        support_done = not rebuilt
+        if rebuilt:
+            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
+    elif owen_checked:
+        pairs = stage_pairs(adj) if adj is not None else None
+        rt.nsing.zero_()
         # positions are unchanged since the checked gather: only the supports moved
         if rt.deferred:
             abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), None, s)  # into the device flag
@@ -174,6 +192,8 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         if rebuilt:
             adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
     else:
+        pairs = None
+        rt.nsing.zero_()
         adj, rebuilt = ensure_adjacency(cfg, adj, st.positions, st.supports, runs)
     system.adjacency = adj
     A = adj.cabi()
@@ -193,8 +213,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         with timing.scope("factors_eos"):
             abi.call("crk_pairs_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), C.byref(ed), abi.ptr(rt.nsing), s)
     else:
-        with timing.scope("support"):
-            abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
+        if not support_done:
+            with timing.scope("support"):
+                abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(ed), s)
         # computeCRKFactors + idealGasEOS (crkSPH.py:61, 95-101)
         with timing.scope("factors_eos"):
             abi.call("crk_factors_eos", rt.ctx, C.byref(cfg), C.byref(cst), abi.ptr(rt.nsing), s)

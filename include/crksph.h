@@ -245,6 +245,12 @@ int crk_set_owen_lut(CrkContext* ctx, int32_t nlut, const double* n_h, const dou
  * Host function.                                                            */
 int crk_grid_plan(const CrkConfig* cfg, float h_max, const float* lo, const float* hi,
                   CrkGrid* grid);
+/* crk_grid_plan of every run from the extents of crk_run_extents
+ * (extents[7r] = h_max, [7r+1..7r+3] = lo, [7r+4..7r+6] = hi), in one call;
+ * table_total (may be NULL) = the sum of the grids' table sizes.  Host
+ * function.  No counterpart in the reference (it serves the ensemble). */
+int crk_grid_plan_runs(const CrkConfig* cfg, int32_t n_runs, const float* extents, CrkGrid* grids,
+                       int64_t* table_total);
 
 /* ------------------------------------------------- (a) adjacency build
  * Replaces warpSPHCore.buildVerletList (called at crkSPH.py:54-58 and

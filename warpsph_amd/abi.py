@@ -180,6 +180,7 @@ _PROTOS = {
     "crk_bind_runs": [_P, C.c_int32, _P, _P, _P],
     "crk_run_ids": [_P, C.c_int32, _P, _P],
     "crk_run_extents": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P],
+    "crk_grid_plan_runs": [C.POINTER(CrkConfig), C.c_int32, _P, _P, C.POINTER(C.c_int64)],
     "crk_runs_dt": [_P, C.c_int32, _P, _P, _P],
     "crk_cfl_dt_runs": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P, _P, _P],
     "crk_axpy_p": [C.c_int64, C.c_int32, _P, _P, _P, _P, _P],

@@ -521,4 +521,18 @@ int crk_grid_plan(const CrkConfig* cfg, float h_max, const float* lo, const floa
   return CRK_OK;
 }
 
+int crk_grid_plan_runs(const CrkConfig* cfg, int32_t n_runs, const float* extents, CrkGrid* grids,
+                       int64_t* table_total) {
+  CRK_REQUIRE(cfg && extents && grids && n_runs > 0, "crk_grid_plan_runs: invalid arguments");
+  int64_t total = 0;
+  for (int32_t r = 0; r < n_runs; ++r) {
+    const float* e = extents + 7 * (int64_t)r;
+    int st = crk_grid_plan(cfg, e[0], e + 1, e + 4, grids + r);
+    if (st) return st;
+    total += grids[r].table_size;
+  }
+  if (table_total) *table_total = total;
+  return CRK_OK;
+}
+
 }  // extern "C"

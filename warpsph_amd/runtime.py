@@ -90,12 +90,9 @@ def _run_grids(cfg: abi.CrkConfig, rt, runs, positions: torch.Tensor, supports: 
     abi.call("crk_run_extents", rt.ctx, C.byref(cfg), runs.n, abi.ptr(positions), abi.ptr(supports),
              C.c_void_p(ext.ctypes.data), rt.stream())
     grids = (abi.CrkGrid * R)()
-    for r in range(R):
-        lo = np.ascontiguousarray(ext[7 * r + 1:7 * r + 4])
-        hi = np.ascontiguousarray(ext[7 * r + 4:7 * r + 7])
-        abi.call("crk_grid_plan", C.byref(cfg), float(ext[7 * r]), C.c_void_p(lo.ctypes.data),
-                 C.c_void_p(hi.ctypes.data), C.byref(grids[r]))
-    return grids
+    total = C.c_int64()
+    abi.call("crk_grid_plan_runs", C.byref(cfg), R, C.c_void_p(ext.ctypes.data), grids, C.byref(total))
+    return grids, int(total.value)
 
 
 def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch.Tensor, runs=None):
@@ -108,8 +105,7 @@ def build_adjacency(cfg: abi.CrkConfig, positions: torch.Tensor, supports: torch
     D = int(cfg.domain.dim)
     dev = positions.device
     if runs is not None:
-        grid = _run_grids(cfg, rt, runs, positions, supports)
-        T = int(sum(g.table_size for g in grid))
+        grid, T = _run_grids(cfg, rt, runs, positions, supports)
         gref = grid
     else:
         hmax = float(supports.max().item())

@@ -47,7 +47,7 @@ def _run(speculate, nsteps, kick, monkeypatch):
 
     def recording(*a, **k):
         r = inner(*a, **k)
-        outcomes.append("fallback" if r is None else ("rebuild" if r[2] else "kept"))
+        outcomes.append("fallback" if r is None or r[0] is None else ("rebuild" if r[2] else "kept"))
         return r
     monkeypatch.setattr(S, "begin_speculative", recording)
     system, config, scheme = sample.shear_layer_2d(nx=96, device="cuda:0")

@@ -193,10 +193,12 @@ def begin_speculative(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax:
     queued before the host reads either test (``Runtime.speculate``).  Both
     tests go to the device flag, which the two passes read (they do nothing
     once it is set), and the flag after each test is copied to the host: the
-    GPU never waits for the host's reads.  Returns None when the stage's
-    list failed (nothing took effect; the caller takes the blocking path),
-    else (adjacency, pairs, True when the updated supports failed the test
-    and the list must be rebuilt before the support sweep runs again)."""
+    GPU never waits for the host's reads.  Returns None without a prior
+    list to test (nothing was queued), (None, None, None) when the stage's
+    list failed (nothing took effect: the caller rebuilds and takes the
+    blocking path), else (adjacency, pairs, True when the updated supports
+    failed the test and the list must be rebuilt before the support sweep
+    runs again)."""
     rt = runtime(st.positions.device)
     if prior is None or prior.n != st.numParticles or getattr(prior, "runs", None) is not runs:
         return None
@@ -215,7 +217,7 @@ def begin_speculative(cfg: abi.CrkConfig, prior, st, cst: abi.CrkState, hminmax:
     with timing.scope("support"):
         abi.call("crk_support", rt.ctx, C.byref(cfg), C.byref(pairs.cabi()), s)
     if flag_wait(rt, 0):
-        return None
+        return None, None, None
     return prior, pairs, flag_wait(rt, 1)
 
 

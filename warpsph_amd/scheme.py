@@ -144,6 +144,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
                              C.byref(cst), abi.ptr(hmm), None, s)
             if rt.speculate and not rt.deferred and not fused_support:
                 spec = begin_speculative(cfg, adj, st, cst, hmm, runs, owen_pass, stage_pairs, rt.nsing)
+                if spec is not None and spec[0] is None:
+                    # the list failed its test: rebuilt at once (no second test of it)
+                    spec, adj = None, None
             if spec is None:
                 adj = begin_checked(cfg, adj, st, cst, hmm, runs, then=owen_pass)
             else:

@@ -281,6 +281,7 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
                        const int32_t* row_offsets, const float* xb, const float* hb,
                        int32_t* nbr, int64_t capacity, void* stream);
 
+
 /* Blocking validity test of an adjacency built at (xb, hb) for the current
  * (x, h): valid iff max h/hb < s and 2*max|x - xb| <= (s - max h/hb)*min hb. */
 int crk_adjacency_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t n,
@@ -308,6 +309,30 @@ typedef struct CrkAdjacency {
   const int32_t*  tiles;
   const uint16_t* nbr_loc;
 } CrkAdjacency;
+
+/* Step 1 for a bound ensemble whose runs with keep[r] != 0 (device
+ * uint8[n_runs]) keep their lists of `prior` (built on this context with
+ * the same runs): a kept run's sort order, build positions xb/hb and rows
+ * are carried over (its keys are prior_keys shifted by delta[r], device
+ * int64[n_runs], the change of the run's key base when other runs' cell
+ * tables change size; grid[r] of a kept run must be the prior build's),
+ * the other runs are rebuilt as crk_adjacency_count does.  The
+ * crk_adjacency_fill that follows copies the kept rows.  No counterpart in
+ * the reference (one list per simulation there): each run of an ensemble
+ * then rebuilds its list only when its own test fails, as it would alone. */
+int crk_adjacency_count_keep(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n,
+                             const float* x, const float* h, const uint8_t* keep, const int64_t* delta,
+                             const CrkAdjacency* prior, const uint32_t* prior_keys,
+                             const float* prior_xb, const float* prior_hb, uint32_t* keys,
+                             int32_t* perm, int32_t* rank, int32_t* cell_start, int32_t* cell_end,
+                             int32_t* row_offsets, float* xb, float* hb, int64_t* n_edges,
+                             void* stream);
+/* Which bound runs failed the last Verlet test (blocking): which = 0 the
+ * checked stage gather's (crk_stage_begin_checked), 1 the updated
+ * supports' (crk_owen_valid); failed_host[r] = 1 for each failed run.
+ * No counterpart in the reference (it serves crk_adjacency_count_keep). */
+int crk_runs_failed(CrkContext* ctx, const CrkConfig* cfg, int32_t which, uint8_t* failed_host,
+                    void* stream);
 
 /* LDS tiles: CRK_TILE_ROWS consecutive sorted rows share one LDS copy of the
  * records of every particle their entries reach, at most CRK_TILE_CAP

@@ -77,6 +77,51 @@ def test_ensemble_adjacency_matches_single_runs():
         base += T
 
 
+def test_partial_rebuild_keeps_the_other_runs_lists():
+    """crk_adjacency_count_keep: with runs 0 and 2 of four marked failed and
+    every particle moved, the kept runs' sort order, keys (shifted by the
+    change of their key base), build positions and rows equal the prior
+    list's, and the rebuilt runs' equal a full build at the new state."""
+    from warpsph_amd.configurations import toCrkConfig
+    from warpsph_amd.runtime import build_adjacency, runtime
+    announce("ensemble partial rebuild")
+    ens, parts, config, scheme = make_ensemble((24, 32, 40, 28))
+    cfg = toCrkConfig(config, scheme)
+    runs = ens.runs
+    rt = runtime("cuda:0")
+    rt.bind_runs(runs)
+    x0, h0 = ens.state.positions.clone(), ens.state.supports.clone()
+    prior = build_adjacency(cfg, x0, h0, runs)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    x1 = (x0 + 0.004 * (torch.rand(x0.shape, generator=g) - 0.5).to(x0.device)).remainder(1.0)
+    h1 = h0 * 1.05  # (the failed runs' grids and tables change size)
+    failed = np.array([True, False, True, False])
+    part = build_adjacency(cfg, x1, h1, runs, prior=prior, failed=failed)
+    full = build_adjacency(cfg, x1, h1, runs)
+    rt.bind_runs(None)
+
+    def host(a):
+        return {"perm": a.perm.cpu().numpy(), "rank": a.rank.cpu().numpy(), "keys": a.keys.cpu().numpy().view(np.uint32),
+                "off": a.row_offsets.cpu().numpy().astype(np.int64), "nbr": a.nbr.cpu().numpy(),
+                "xb": a.xb.cpu().numpy(), "hb": a.hb.cpu().numpy(),
+                "tsize": np.array([gr.table_size for gr in a.grid], dtype=np.int64)}
+    P, Q, F = host(prior), host(part), host(full)
+    base = lambda t: np.cumsum(t) - t  # noqa: E731
+    for r in range(4):
+        b, e = int(runs.offsets[r]), int(runs.offsets[r + 1])
+        ref = F if failed[r] else P
+        assert np.array_equal(Q["perm"][b:e], ref["perm"][b:e]), r
+        assert np.array_equal(Q["rank"][b:e], ref["rank"][b:e]), r
+        shift = int(base(Q["tsize"])[r]) - int(base(ref["tsize"])[r])
+        assert np.array_equal(Q["keys"][b:e].astype(np.int64), ref["keys"][b:e].astype(np.int64) + shift), r
+        assert np.array_equal(Q["xb"][b:e], ref["xb"][b:e]) and np.array_equal(Q["hb"][b:e], ref["hb"][b:e]), r
+        q_rows = Q["nbr"][Q["off"][b]:Q["off"][e]]
+        r_rows = ref["nbr"][ref["off"][b]:ref["off"][e]]
+        assert np.array_equal(np.diff(Q["off"][b:e + 1]), np.diff(ref["off"][b:e + 1])), r
+        assert np.array_equal(q_rows, r_rows), r
+    assert Q["off"][-1] == Q["nbr"].size
+
+
 def _fields(st):
     return {"x": st.positions, "v": st.velocities, "u": st.internalEnergies, "rho": st.densities,
             "h": st.supports, "alpha": st.alphas}
@@ -104,6 +149,46 @@ def test_ensemble_steps_match_single_runs(sides):
         e = integ.function(e, crkSPH_step, dts, config, scheme).state
     torch.cuda.synchronize()
     assert np.allclose(e.t, [s.t for s in singles], rtol=0, atol=0)
+    errs = {}
+    for r, s in enumerate(singles):
+        b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
+        fe, fs = _fields(e.state), _fields(s.state)
+        for k in fe:
+            errs[(r, k)] = relerr(fe[k][b:q].cpu().numpy(), fs[k].cpu().numpy())
+    print(errs)
+    assert all(v <= RTOL for v in errs.values()), errs
+
+
+def test_ensemble_keeping_runs_lists_steps_as_single_runs():
+    """Runtime.keep_runs: twenty RK2 steps of three runs, each run's list
+    rebuilt only when its own test fails (partial rebuilds happen on the
+    way); every field, alpha included, against the same runs stepped alone."""
+    from warpsph_amd import sample
+    from warpsph_amd.ensemble import cfl_dt, run_system
+    from warpsph_amd.integrator import getIntegrator
+    from warpsph_amd.runtime import runtime
+    from warpsph_amd.scheme import crkSPH_step
+    announce("ensemble keeping runs' lists, 20 steps")
+    ens, parts, config, scheme = make_ensemble()
+    rt = runtime("cuda:0")
+    integ = getIntegrator()
+    singles = [run_system(ens, r) for r in range(len(parts))]
+    rt.keep_runs = True
+    try:
+        e, lists = ens, []
+        for _ in range(20):
+            dts = cfl_dt(e, config, scheme)
+            for r in range(len(singles)):
+                d1 = sample.cfl_dt(singles[r].state, config, scheme)
+                assert d1 == dts[r], (r, d1, dts[r])
+                singles[r] = integ.function(singles[r], crkSPH_step, d1, config, scheme).state
+            res = integ.function(e, crkSPH_step, dts, config, scheme)
+            lists += [st.aux[0] for st in res.stages]
+            e = res.state
+    finally:
+        rt.keep_runs = False
+    torch.cuda.synchronize()
+    assert len({id(a) for a in lists}) >= 3  # (lists were rebuilt on the way)
     errs = {}
     for r, s in enumerate(singles):
         b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
@@ -193,18 +278,28 @@ def test_ensemble_at_workload_run_sizes():
     """The batched workload's run sizes: 16 runs drawn by the data generator
     (sides of nx^2 particles, log-uniform in [4096, 16384]), three RK2 steps
     with each run's own CFL dt.  Every run against the same run stepped alone
-    (per-step bar), and the smallest and largest runs against the CPU oracle.
-    alpha is left out here: its signal speed is a maximum over every entry of
-    the Verlet list, skin included (wp_vsig.py:55-86), and the ensemble
-    rebuilds all runs' lists when one run's list fails, so a run whose
-    standalone list was kept sees a different skin (the other fields differ
-    by the summation order of the re-sorted rows only)."""
+    (per-step bar), alpha included, and the smallest and largest runs against
+    the CPU oracle.  alpha's signal speed is a maximum over every entry of
+    the Verlet list, skin included (wp_vsig.py:55-86): it agrees because
+    each run rebuilds its list only when its own test fails, as it does
+    alone (Runtime.keep_runs; the ensemble otherwise rebuilds every run's
+    list when one fails, and a run whose standalone list was kept would see
+    another skin)."""
     from oracle import oracle as O
     from warpsph_amd import sample
     from warpsph_amd.ensemble import cfl_dt, run_system, sample_ensemble
     from warpsph_amd.integrator import getIntegrator
     from warpsph_amd.scheme import crkSPH_step
+    from warpsph_amd.runtime import runtime
     ens, config, scheme = sample_ensemble(n_runs=16, nmin=4096, nmax=16384, seed=4, device="cuda")
+    runtime("cuda:0").keep_runs = True
+    try:
+        _steps_against_single_runs(ens, config, scheme, O, sample, cfl_dt, run_system, getIntegrator, crkSPH_step)
+    finally:
+        runtime("cuda:0").keep_runs = False
+
+
+def _steps_against_single_runs(ens, config, scheme, O, sample, cfl_dt, run_system, getIntegrator, crkSPH_step):
     counts = np.diff(ens.runs.offsets)
     announce(f"ensemble 16 runs, {int(counts.min())}..{int(counts.max())} particles")
     assert len(counts) == 16 and counts.min() >= 4096 and counts.max() <= 16384
@@ -230,7 +325,6 @@ def test_ensemble_at_workload_run_sizes():
     for r in range(16):
         b, q = int(e.runs.offsets[r]), int(e.runs.offsets[r + 1])
         fe, fs = _fields(e.state), _fields(singles[r].state)
-        fe.pop("alpha")
         for k in fe:
             err = relerr(fe[k][b:q].cpu().numpy(), fs[k].cpu().numpy())
             worst = max(worst, err)

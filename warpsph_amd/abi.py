@@ -135,6 +135,10 @@ _PROTOS = {
                             _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64), _P],
     "crk_adjacency_fill": [_P, C.POINTER(CrkConfig), C.POINTER(CrkGrid), C.c_int32, _P, _P, _P,
                            _P, _P, _P, C.c_int64, _P],
+    "crk_adjacency_count_keep": [_P, C.POINTER(CrkConfig), C.POINTER(CrkGrid), C.c_int32, _P, _P, _P, _P,
+                                 C.POINTER(CrkAdjacency), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                 C.POINTER(C.c_int64), _P],
+    "crk_runs_failed": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P],
     "crk_adjacency_tiles": [_P, C.POINTER(CrkAdjacency), _P, _P, _P, C.c_int32, _P, _P, _P],
     "crk_adjacency_valid": [_P, C.POINTER(CrkConfig), C.c_int32, _P, _P, _P, _P, _P,
                             C.POINTER(C.c_int32), _P],

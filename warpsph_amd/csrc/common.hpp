@@ -74,6 +74,11 @@ struct CrkContext {
   int32_t collect_n = 0;
   int64_t collect_E = -1;
   bool collect_ok = false;  // no row exceeded nbr_cap
+  // crk_adjacency_count_keep: the runs that keep their prior lists and the
+  // prior rows, for the fill that follows (null run_keep: none)
+  const uint8_t* keep_runs = nullptr;
+  const int32_t* keep_off = nullptr;
+  const int32_t* keep_nbr = nullptr;
   // in-support pairs of each row, compacted in row order (caller-owned
   // buffers of the stage, bound by crk_support)
   int32_t* pair_cnt = nullptr;

@@ -192,6 +192,40 @@ __global__ void k_cells(const uint32_t* __restrict__ keys, int32_t n, int32_t* _
 // each cell in turn, and the hits are compacted with a group ballot, so the
 // row is written with coalesced stores in the same order as the
 // one-thread-per-row scan (cells in scan order, candidates ascending).
+// Runs of a bound ensemble that keep their prior lists through a rebuild
+// (crk_adjacency_count_keep): run_keep[r] != 0, with the prior row offsets
+// and entries (a kept run's sorted indices are unchanged).
+struct KeepDev {
+  const uint8_t* run_keep;
+  const int32_t* off;
+  const int32_t* nbr;
+};
+
+// A kept run's particles take the keys of the prior build, shifted by the
+// change of the run's key base, so that the stable sort reproduces the
+// prior order.
+__global__ void k_keys_keep(int32_t n, const int32_t* __restrict__ run, const uint8_t* __restrict__ keep,
+                            const int64_t* __restrict__ delta, const int32_t* __restrict__ prior_rank,
+                            const uint32_t* __restrict__ prior_keys, uint32_t* __restrict__ k0) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t r = run[i];
+  if (!keep[r]) return;
+  k0[i] = (uint32_t)((int64_t)prior_keys[prior_rank[i]] + delta[r]);
+}
+
+// ... and keep the positions and supports their lists were built at
+template <int D>
+__global__ void k_keep_restore(int32_t n, const int32_t* __restrict__ run, const uint8_t* __restrict__ keep,
+                               const float* __restrict__ prior_xb, const float* __restrict__ prior_hb,
+                               float* __restrict__ xb, float* __restrict__ hb) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !keep[run[k]]) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) xb[(int64_t)k * D + d] = prior_xb[(int64_t)k * D + d];
+  hb[k] = prior_hb[k];
+}
+
 template <int D, int G, int MODE, bool RUNS>
 __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0, Geo geo, float s,
                                                         const float* __restrict__ xb, const float* __restrict__ hb,
@@ -199,13 +233,24 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_g(GridDev g0, CellScan sc0
                                                         int32_t n, int32_t* __restrict__ counts_or_offsets,
                                                         int32_t* __restrict__ nbr, const GridRun* __restrict__ runs,
                                                         const int32_t* __restrict__ run_sorted, int32_t cap,
-                                                        int32_t* __restrict__ overflow) {
+                                                        int32_t* __restrict__ overflow, KeepDev kp) {
   constexpr bool FILL = MODE == 1;
   static_assert(G >= (D == 1 ? 3 : (D == 2 ? 9 : 27)), "a group needs a lane per cell of the scan");
   static_assert(MODE >= 0 && MODE <= 2, "count (0), fill (1) or single pass (2)");
   const Lane L = lane_of<G>(n);
   if (!L.valid) return;  // whole groups only: every lane of a group shares its row
   const int32_t k = L.k;
+  if (RUNS && kp.run_keep && kp.run_keep[run_sorted[k]]) {
+    // a kept run's row: its prior length, or its prior entries (fill)
+    const int32_t ob = kp.off[k], len = kp.off[k + 1] - ob;
+    if (MODE == 1) {
+      const int32_t b = counts_or_offsets[k];
+      for (int32_t q = L.g; q < len; q += G) nbr[b + q] = kp.nbr[ob + q];
+    } else if (L.g == 0) {
+      counts_or_offsets[k] = len;
+    }
+    return;
+  }
   float xi[3];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = xb[(int64_t)k * D + d];
@@ -309,12 +354,14 @@ constexpr double kCollectMaxBytes = 16.0 * (1ull << 30);
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_rows_copy(const int32_t* __restrict__ tmp, int32_t cap,
                                                       const int32_t* __restrict__ off, int32_t n,
-                                                      int32_t* __restrict__ nbr) {
+                                                      int32_t* __restrict__ nbr, const int32_t* __restrict__ run,
+                                                      KeepDev kp) {
   const Lane L = lane_of<G>(n);
   if (!L.valid) return;
   const int32_t k = L.k;
   const int32_t b = off[k], e = off[k + 1];
-  const int32_t* src = tmp + (int64_t)k * cap;
+  // (a kept run's row comes from the prior list)
+  const int32_t* src = kp.run_keep && kp.run_keep[run[k]] ? kp.nbr + kp.off[k] : tmp + (int64_t)k * cap;
   for (int32_t q = b + L.g; q < e; q += G) nbr[q] = src[q - b];
 }
 
@@ -401,11 +448,23 @@ static int key_bits(int64_t table) {
   return b;
 }
 
+// the kept runs of crk_adjacency_count_keep and the prior build they keep
+struct KeepIn {
+  const uint8_t* keep;
+  const int64_t* delta;
+  const int32_t* prior_rank;
+  const uint32_t* prior_keys;
+  const float* prior_xb;
+  const float* prior_hb;
+  const int32_t* prior_off;
+  const int32_t* prior_nbr;
+};
+
 template <int D>
 static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n, const float* x,
                       const float* h, uint32_t* keys, int32_t* perm, int32_t* rank, int32_t* cell_start,
                       int32_t* cell_end, int32_t* row_offsets, float* xb, float* hb, int64_t* n_edges,
-                      hipStream_t s) {
+                      hipStream_t s, const KeepIn* kin = nullptr) {
   GridDev g = make_griddev(*grid, *cfg);
   const bool runs = ctx->n_runs > 0;
   int64_t table = grid->table_size;
@@ -424,6 +483,11 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   int32_t* v1 = (int32_t*)ctx->s_vals1.p;
   hipLaunchKernelGGL(k_keys<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, x, n, g, gr, run, k0, v0);
   CRK_LAUNCH_CHECK("k_keys");
+  if (kin) {
+    hipLaunchKernelGGL(k_keys_keep, dim3(nblocks(n)), dim3(kBlock), 0, s, n, run, kin->keep, kin->delta,
+                       kin->prior_rank, kin->prior_keys, k0);
+    CRK_LAUNCH_CHECK("k_keys_keep");
+  }
   int keybits = runs ? key_bits(table) : D * grid->bits;
   int npass = (keybits + 7) / 8;
   int32_t ntiles = (int32_t)((n + kRsTile - 1) / kRsTile);
@@ -443,6 +507,15 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   CRK_HIP(hipMemcpyAsync(perm, v0, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_post_sort<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, perm, x, h, n, rank, xb, hb);
   CRK_LAUNCH_CHECK("k_post_sort");
+  if (kin) {
+    hipLaunchKernelGGL(k_keep_restore<D>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, run, kin->keep, kin->prior_xb,
+                       kin->prior_hb, xb, hb);
+    CRK_LAUNCH_CHECK("k_keep_restore");
+  }
+  const KeepDev kp{kin ? kin->keep : nullptr, kin ? kin->prior_off : nullptr, kin ? kin->prior_nbr : nullptr};
+  ctx->keep_runs = kp.run_keep;
+  ctx->keep_off = kp.off;
+  ctx->keep_nbr = kp.nbr;
   if ((st = fill_i32(cell_start, table, 0, s))) return st;
   if ((st = fill_i32(cell_end, table, 0, s))) return st;
   hipLaunchKernelGGL(k_cells, dim3(nblocks(n)), dim3(kBlock), 0, s, keys, n, cell_start, cell_end);
@@ -475,19 +548,19 @@ static int count_impl(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid
   if (collect && runs)
     hipLaunchKernelGGL((k_neighbors_g<D, GN, 2, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
                        cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, cap,
-                       overflow_dev);
+                       overflow_dev, kp);
   else if (collect)
     hipLaunchKernelGGL((k_neighbors_g<D, GN, 2, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
                        cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, cap,
-                       overflow_dev);
+                       overflow_dev, kp);
   else if (runs)
     hipLaunchKernelGGL((k_neighbors_g<D, GN, 0, true>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
                        cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, 0,
-                       overflow_dev);
+                       overflow_dev, kp);
   else
     hipLaunchKernelGGL((k_neighbors_g<D, GN, 0, false>), dim3(group_blocks(n, GN)), dim3(kBlock), 0, s, g, sc, geo,
                        cfg->verletScale, xb, hb, cell_start, cell_end, n, row_offsets, tmp, gr, run, 0,
-                       overflow_dev);
+                       overflow_dev, kp);
   CRK_LAUNCH_CHECK("k_neighbors(count)");
   if ((st = scan_exclusive_i32(ctx, row_offsets, row_offsets, n, total_dev, s))) return st;
   hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, s, total_dev, row_offsets + n);
@@ -686,6 +759,41 @@ int crk_adjacency_count(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gr
   return CRK_EINVAL;
 }
 
+int crk_adjacency_count_keep(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n, const float* x,
+                             const float* h, const uint8_t* keep, const int64_t* delta, const CrkAdjacency* prior,
+                             const uint32_t* prior_keys, const float* prior_xb, const float* prior_hb,
+                             uint32_t* keys, int32_t* perm, int32_t* rank, int32_t* cell_start, int32_t* cell_end,
+                             int32_t* row_offsets, float* xb, float* hb, int64_t* n_edges, void* stream) {
+  CRK_REQUIRE(ctx && ctx->n_runs > 0, "crk_adjacency_count_keep: needs bound runs (crk_bind_runs)");
+  CRK_REQUIRE(keep && delta && prior && prior_keys && prior_xb && prior_hb,
+              "crk_adjacency_count_keep: null keep, delta or prior buffer");
+  CRK_REQUIRE(prior->n == n && prior->rank && prior->row_offsets && (prior->n_edges == 0 || prior->nbr),
+              "crk_adjacency_count_keep: the prior adjacency must hold the same %d particles", n);
+  CRK_REQUIRE(ctx && cfg && grid && n_edges, "crk_adjacency_count_keep: null argument");
+  CRK_REQUIRE(x && h && keys && perm && rank && cell_start && cell_end && row_offsets && xb && hb,
+              "crk_adjacency_count_keep: null buffer");
+  int st = require_runs_n(ctx, n, "crk_adjacency_count_keep");
+  if (st) return st;
+  int64_t table = 0;
+  for (int32_t r = 0; r < ctx->n_runs; ++r) {
+    CRK_REQUIRE(grid[r].dim == cfg->domain.dim, "grid/config dimension mismatch (grid %d)", r);
+    CRK_REQUIRE(grid[r].table_size >= 1 && grid[r].table_size <= ((int64_t)1 << 30), "bad grid table size (grid %d)",
+                r);
+    table += grid[r].table_size;
+  }
+  CRK_REQUIRE(table <= ((int64_t)1 << 30), "the runs' cell tables hold %lld cells; at most 2^30 are supported",
+              (long long)table);
+  const KeepIn kin{keep, delta, prior->rank, prior_keys, prior_xb, prior_hb, prior->row_offsets, prior->nbr};
+  hipStream_t s = (hipStream_t)stream;
+  switch (cfg->domain.dim) {
+    case 1: return count_impl<1>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s, &kin);
+    case 2: return count_impl<2>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s, &kin);
+    case 3: return count_impl<3>(ctx, cfg, grid, n, x, h, keys, perm, rank, cell_start, cell_end, row_offsets, xb, hb, n_edges, s, &kin);
+  }
+  set_error("dim must be 1..3");
+  return CRK_EINVAL;
+}
+
 int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* grid, int32_t n,
                        const int32_t* cell_start, const int32_t* cell_end, const int32_t* row_offsets,
                        const float* xb, const float* hb, int32_t* nbr, int64_t capacity, void* stream) {
@@ -707,6 +815,11 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
   if (E > 0) CRK_REQUIRE(nbr, "crk_adjacency_fill: null neighbour buffer");
   int st = require_runs_n(ctx, n, "crk_adjacency_fill");
   if (st) return st;
+  // the kept runs of a crk_adjacency_count_keep (for this fill only)
+  const KeepDev kp{ctx->keep_runs, ctx->keep_off, ctx->keep_nbr};
+  ctx->keep_runs = nullptr;
+  ctx->keep_off = ctx->keep_nbr = nullptr;
+  CRK_REQUIRE(!kp.run_keep || ctx->n_runs > 0, "crk_adjacency_fill: kept runs without bound runs");
   GridDev g = make_griddev(*grid, *cfg);
   CellScan sc = make_scan(*grid, *cfg);
   Geo geo = make_geo(*cfg);
@@ -716,7 +829,8 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
     if (ctx->collect_ok) {
       if (E > 0) {
         hipLaunchKernelGGL(k_rows_copy<16>, dim3(group_blocks(n, 16)), dim3(kBlock), 0, s,
-                           (const int32_t*)ctx->nbr_tmp.p, ctx->nbr_cap_used, row_offsets, n, nbr);
+                           (const int32_t*)ctx->nbr_tmp.p, ctx->nbr_cap_used, row_offsets, n, nbr, run_ids(ctx),
+                           kp);
         CRK_LAUNCH_CHECK("k_rows_copy");
       }
       return CRK_OK;
@@ -729,12 +843,12 @@ int crk_adjacency_fill(CrkContext* ctx, const CrkConfig* cfg, const CrkGrid* gri
     CRK_DIM_DISPATCH(cfg->domain.dim,
         hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), 1, true>), dim3(group_blocks(n, nbr_group<DD>())),
                            dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
-                           (int32_t*)row_offsets, nbr, gr, run, 0, (int32_t*)nullptr));
+                           (int32_t*)row_offsets, nbr, gr, run, 0, (int32_t*)nullptr, kp));
   } else {
     CRK_DIM_DISPATCH(cfg->domain.dim,
         hipLaunchKernelGGL((k_neighbors_g<DD, nbr_group<DD>(), 1, false>), dim3(group_blocks(n, nbr_group<DD>())),
                            dim3(kBlock), 0, s, g, sc, geo, cfg->verletScale, xb, hb, cell_start, cell_end, n,
-                           (int32_t*)row_offsets, nbr, gr, run, 0, (int32_t*)nullptr));
+                           (int32_t*)row_offsets, nbr, gr, run, 0, (int32_t*)nullptr, kp));
   }
   CRK_LAUNCH_CHECK("k_neighbors(fill)");
   return CRK_OK;

@@ -1454,6 +1454,33 @@ int crk_owen_valid(CrkContext* ctx, const CrkConfig* cfg, int32_t* valid, void* 
   return CRK_OK;
 }
 
+int crk_runs_failed(CrkContext* ctx, const CrkConfig* cfg, int32_t which, uint8_t* failed, void* stream) {
+  CRK_REQUIRE(ctx && cfg && failed, "crk_runs_failed: null argument");
+  CRK_REQUIRE(which == 0 || which == 1, "crk_runs_failed: which must be 0 (stage) or 1 (updated supports)");
+  const int32_t nr = run_count(ctx);
+  if (ctx->check_runs != nr || !ctx->check_hb) {
+    set_error("crk_runs_failed needs crk_stage_begin_checked on the bound runs first");
+    return CRK_ESTATE;
+  }
+  if (which == 1 && !ctx->owen_checked) {
+    set_error("crk_runs_failed(1) needs crk_owen_valid after the adaptive-support pass");
+    return CRK_ESTATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int r = ensure_pinned(ctx, 4 * sizeof(float) * (size_t)nr);
+  if (r) return r;
+  float* hostv = (float*)ctx->pinned.p;
+  CRK_HIP(hipMemcpyAsync(hostv, ctx->s_chk.p, 4 * sizeof(float) * (size_t)nr, hipMemcpyDeviceToHost, s));
+  CRK_HIP(hipStreamSynchronize(s));
+  const double sv = cfg->verletScale;
+  for (int32_t q = 0; q < nr; ++q) {
+    const double g = hostv[4 * q + (which == 0 ? 1 : 3)];
+    const bool ok = g < sv && 2.0 * std::sqrt((double)hostv[4 * q]) <= (sv - g) * (double)hostv[4 * q + 2];
+    failed[q] = ok ? 0 : 1;
+  }
+  return CRK_OK;
+}
+
 static int owen_launch(CrkContext* ctx, const CrkConfig* cfg, const CrkState* st, const float* hminmax_dev,
                        float* max_ratio_dev, bool final_sweep, void* stream) {
   int r = require_bound(ctx, cfg);

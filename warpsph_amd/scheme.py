@@ -145,8 +145,9 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
             if rt.speculate and not rt.deferred and not fused_support:
                 spec = begin_speculative(cfg, adj, st, cst, hmm, runs, owen_pass, stage_pairs, rt.nsing)
                 if spec is not None and spec[0] is None:
-                    # the list failed its test: rebuilt at once (no second test of it)
-                    spec, adj = None, None
+                    # the list failed its test: rebuilt at once (no second test
+                    # of it; an ensemble rebuilds the runs that failed it)
+                    spec, adj = None, rebuild_adjacency(cfg, st.positions, st.supports, runs, adj, 0)
             if spec is None:
                 adj = begin_checked(cfg, adj, st, cst, hmm, runs, then=owen_pass)
             else:
@@ -180,7 +181,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
         rebuilt = spec[2]
         support_done = not rebuilt
         if rebuilt:
-            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
+            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs, adj, 1)
     elif owen_checked:
         pairs = stage_pairs(adj) if adj is not None else None
         rt.nsing.zero_()
@@ -193,7 +194,7 @@ def crkSPH_step(system: CompSPHSystem, dt: float, config: SimulationConfig, comp
             abi.call("crk_owen_valid", rt.ctx, C.byref(cfg), C.byref(ok), s)
             rebuilt = not ok.value
         if rebuilt:
-            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs)
+            adj = rebuild_adjacency(cfg, st.positions, st.supports, runs, adj, 1)
     else:
         pairs = None
         rt.nsing.zero_()

@@ -501,8 +501,9 @@ __device__ inline void accel_row(const Lane& L, const Src& src, ARows rows, Geo 
   Comp acc[D], accu, accs;
   const int32_t b = pre.off, e = L.valid ? b + pre.cnt : b;
   // neighbour ids loaded two iterations ahead (the record itself is loaded
-  // at the start of its iteration: prefetching it too costs more in
-  // occupancy than it hides)
+  // at the start of its iteration: from global memory a prefetch costs
+  // more in occupancy than it hides, and from LDS, two record buffers in
+  // turn were measured no faster)
   int32_t jc = b + L.g < e ? src.id(b + L.g) : 0;
   int32_t jn = b + L.g + G < e ? src.id(b + L.g + G) : 0;
   for (int32_t q0 = b + L.g; q0 < e; q0 += G) {

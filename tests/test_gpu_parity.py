@@ -2,7 +2,8 @@
 
 Integer work (Morton keys, sort permutation, cell tables, neighbour rows) must
 match bit for bit; FP32 fields within RTOL = 1e-5 normwise per step
-(max |gpu - oracle| <= 1e-5 * max |oracle|, per field).  The one exception:
+(max |gpu - oracle| <= 1e-5 * max |oracle|, per field and per component
+of a vector field).  The one exception:
 a pair field (ap, av) of a case where FP32 arithmetic itself is further than
 RTOL from FP64 (the FP32 oracle against the FP64 oracle) passes when the GPU
 is within twice that distance of the FP64 oracle.

@@ -89,8 +89,23 @@ def test_one_step_1m(shear1m):
     print("normwise", errs, flush=True)
     # the FP32 fields of the state after one step; alpha and div are
     # Cullen-Dehnen intermediates that amplify rounding where div ~ 0
-    for k in ("x", "v", "u", "rho", "h", "P", "c"):
-        assert errs[k] <= RTOL, (k, errs[k])
+    bad = [k for k in ("x", "v", "u", "rho", "h", "P", "c") if not errs[k] <= RTOL]
+    if bad:
+        # a field beyond RTOL passes only where FP32 arithmetic itself is
+        # that far from FP64 (a small vector component, e.g. v_y of the
+        # shear, measured against its own scale): the GPU within 2x of the
+        # FP32 oracle's own error against the FP64 oracle
+        sim64 = O.from_system(cpu, config, scheme, 64)
+        sim64.step(dt, 1)
+        for k in list(bad):
+            if k == "x":
+                continue
+            ref64 = sim64.get(k)
+            e_gpu, e_o32 = relerr(got[k].cpu().numpy(), ref64), relerr(sim.get(k), ref64)
+            print(f"{k}: gpu vs fp64 {e_gpu:.2e}, fp32 oracle vs fp64 {e_o32:.2e}", flush=True)
+            if e_gpu <= 2.0 * e_o32:
+                bad.remove(k)
+    assert not bad, {k: errs[k] for k in bad}
     for k in ("alpha", "div"):
         assert errs[k] <= 10 * RTOL, (k, errs[k])
 

@@ -4,6 +4,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -85,3 +87,21 @@ def test_a_state_without_particles_is_refused():
     assert st.numParticles == 0
     with pytest.raises(ValueError, match="no particles"):
         crkSPH_step(system, 1e-4, config, scheme)
+
+
+def test_parity_bar_is_per_component():
+    """The per-step bar holds each column of a vector field to its own scale:
+    an error of 1e-4 of a component whose values are 50x smaller than the
+    other's fails, although it is 2e-6 of the field's largest value."""
+    import numpy as np
+    from tests.util import RTOL, relerr
+    rng = np.random.default_rng(3)
+    ref = np.stack([rng.uniform(-0.5, 0.5, 1000), rng.uniform(-0.01, 0.01, 1000)], axis=1)
+    ref[0] = (0.5, 0.01)
+    got = ref.copy()
+    got[7, 1] += 1e-4 * 0.01
+    assert relerr(got, ref) == pytest.approx(1e-4, rel=1e-6)
+    assert relerr(got, ref) > RTOL
+    # a scalar field and a single column are measured as before
+    assert relerr(got[:, 0], ref[:, 0]) == 0.0
+    assert relerr(ref[:, 0] + 1e-6, ref[:, 0]) == pytest.approx(2e-6, rel=1e-6)
